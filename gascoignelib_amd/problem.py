@@ -1,0 +1,130 @@
+"""Host-side description of a multigrid problem in Gascoigne's own terms.
+
+A `Hierarchy` holds, for every multigrid level, what Gascoigne's CPU side hands
+to the GPU path: the assembled block-CSR matrix (ColumnStencil layout, int32
+indices, sorted columns), the mesh-transfer maps to the next coarser level
+(c2f / 2er / 4er / 8er, src/Mesh/gascoignemeshtransfer.h:35-73), the
+hanging-node maps (src/Mesh/hangingindexhandler.h:94-103) and the strong
+Dirichlet node lists (BoundaryIndexHandler::Verteces).
+
+Level numbering follows the solver: level 0 is the coarsest, nlevels-1 the
+finest (src/Solver/stdmultilevelsolver.cc:224-231).  The mesh agent numbers the
+other way round (mesh level 0 = finest); `from_fixture` converts.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HN_KIND = {"Q12d": 12, "Q13d": 13, "Q22d": 22, "Q23d": 23}
+
+
+@dataclass
+class Level:
+    n: int
+    ncomp: int
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    # maps from this level to the next coarser one: (c2f, zweier, vierer, achter)
+    transfer: tuple = None
+    hn_kind: int = 13
+    hn_edges: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
+    hn_faces: np.ndarray = field(default_factory=lambda: np.zeros((0, 10), np.int32))
+    dirichlet: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    dirichlet_comps: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int32))
+    coords: np.ndarray = None
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    @property
+    def size(self):
+        return self.n * self.ncomp
+
+
+class Hierarchy:
+    def __init__(self, levels, dim, name=""):
+        self.levels = list(levels)
+        self.dim = dim
+        self.name = name
+
+    @property
+    def nlevels(self):
+        return len(self.levels)
+
+    def level(self, l):
+        return self.levels[l]
+
+    @property
+    def fine(self):
+        return self.levels[-1]
+
+    @classmethod
+    def from_fixture(cls, path, name=None):
+        """Hierarchy in Gascoigne's own numbering, from a golden fixture written by
+        tests/golden/make_fixtures.py."""
+        z = np.load(path)
+        nl = int(z["nlevels"])
+        dim = int(z["dim"])
+        disc = str(z["disc"])
+        colours = [int(c) for c in z["colours"]]
+        levels = []
+        for s in range(nl):  # solver level s <-> mesh level nl-1-s
+            m = nl - 1 - s
+            p = f"L{m}_"
+            dirs = [z[p + f"bnd{c}"] for c in colours if (p + f"bnd{c}") in z.files]
+            dirichlet = np.concatenate(dirs).astype(np.int32) if dirs else np.zeros(0, np.int32)
+            tr = None
+            if m + 1 < nl:
+                tr = tuple(np.ascontiguousarray(z[p + k], dtype=np.int32)
+                           for k in ("c2f", "zweier", "vierer", "achter"))
+            lev = Level(
+                n=int(z[p + "rowptr"].shape[0] - 1), ncomp=1,
+                rowptr=np.ascontiguousarray(z[p + "rowptr"], dtype=np.int32),
+                col=np.ascontiguousarray(z[p + "col"], dtype=np.int32),
+                val=np.ascontiguousarray(z[p + "val"], dtype=np.float64),
+                transfer=tr, hn_kind=HN_KIND[disc],
+                hn_edges=np.ascontiguousarray(z[p + "hn_edge"], dtype=np.int32).reshape(-1, 4),
+                hn_faces=np.ascontiguousarray(z[p + "hn_face"], dtype=np.int32).reshape(-1, 10),
+                dirichlet=dirichlet, dirichlet_comps=np.zeros(1, np.int32),
+                coords=np.asarray(z[p + "coords"]))
+            levels.append(lev)
+        h = cls(levels, dim, name or str(path))
+        h.fixture = z
+        return h
+
+
+def splitmix64_uniform(seed, n, offset=0):
+    """Uniform doubles in [-1,1) from SplitMix64 at positions offset..offset+n-1:
+    x_k = mix(seed + (k+1)*0x9E3779B97F4A7C15), v = 2*((x>>11)*2^-53) - 1.
+    Position k of the stream is independent of how many values are drawn."""
+    with np.errstate(over="ignore"):
+        k = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)) - 1.0
+
+
+def lattice_rhs(coords, seed, ncomp=1):
+    """Right-hand side f_i uniform in [-1,1), keyed by the node's lattice position
+    on the finest grid so that it does not depend on the numbering."""
+    c = np.asarray(coords, dtype=np.float64)
+    # finest spacing: smallest non-zero coordinate difference
+    h = np.inf
+    for d in range(c.shape[1]):
+        u = np.unique(np.round(c[:, d], 12))
+        if len(u) > 1:
+            h = min(h, float(np.min(np.diff(u))))
+    nmax = int(round(1.0 / h)) + 1
+    idx = np.rint(c / h).astype(np.int64)
+    key = np.zeros(len(c), dtype=np.int64)
+    for d in reversed(range(c.shape[1])):
+        key = key * nmax + idx[:, d]
+    vals = np.empty(len(c) * ncomp)
+    for comp in range(ncomp):
+        v = splitmix64_uniform(seed + comp, int(key.max()) + 1)
+        vals[comp::ncomp] = v[key]
+    return vals

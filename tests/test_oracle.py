@@ -1,0 +1,283 @@
+"""Pins the CPU oracle (oracle/gascoigne_oracle.c).
+
+1. Golden vectors from the compiled reference (tests/golden/*.npz, written by
+   tests/golden/make_fixtures.py from SparseBlockMatrix, MgInterpolatorMatrix,
+   HNStructureQ13d/Q23d and nvector run on Gascoigne's own meshes): bit-exact.
+2. Hand-traced CGInfo / NLInfo status sequences (src/Common/cginfo.cc:202-256,
+   nlinfo.cc:247-356).
+3. Mathematics for the composition that the reference cannot run here
+   (StdMultiLevelSolver / GMRES do not link without Eigen): smoother fixed
+   point, idempotent coarse-grid correction, GMRES residual estimate vs the
+   true preconditioned residual, convergence to the sparse direct solution.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle_ctypes as O
+from gascoignelib_amd.problem import Hierarchy, splitmix64_uniform
+
+FIXTURES = ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada", "q1_3d_cube3_12"]
+
+
+def levels(z):
+    return range(int(z["nlevels"]))
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_seeded_vectors_are_splitmix64(golden, name):
+    z = golden(name)
+    for l in levels(z):
+        x = z[f"L{l}_x"]
+        assert np.array_equal(x, splitmix64_uniform(1000 + l, len(x)))
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_vmult_residual_jacobi_norm_bitexact(golden, name):
+    z = golden(name)
+    for l in levels(z):
+        p = f"L{l}_"
+        A = O.Matrix(z[p + "rowptr"], z[p + "col"], z[p + "val"])
+        assert np.array_equal(np.ctypeslib.as_array(A.s.diag, (A.n,)), z[p + "diag"])
+        x, b = z[p + "x"].copy(), z[p + "b"].copy()
+        y = np.zeros_like(x)
+        A.vmult(y, x, 1.0)
+        assert np.array_equal(y, z[p + "Ax"])
+        h = np.zeros_like(x)
+        A.residual(h, x, b)
+        assert np.array_equal(h, z[p + "res"])
+        A.jacobi(h)
+        assert np.array_equal(h, z[p + "jac"])
+        lib = O.lib()
+        nd = z[p + "norm_dot"]
+        assert lib.or_norm(O.ptr(x), len(x)) == nd[0]
+        assert lib.or_dot(O.ptr(x), O.ptr(b), len(x)) == nd[1]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_transfer_bitexact(golden, name):
+    z = golden(name)
+    nl = int(z["nlevels"])
+    for l in range(nl - 1):
+        p, q = f"L{l}_", f"L{l + 1}_"
+        nc = len(z[q + "x"])
+        T = O.Transfer(nc, z[p + "c2f"], z[p + "zweier"], z[p + "vierer"], z[p + "achter"])
+        assert T.nfine == len(z[p + "x"])
+        uc = np.zeros(nc)
+        T.restrict_zero(uc, z[p + "t_in"].copy())
+        assert np.array_equal(uc, z[p + "restrict"])
+        uf = np.zeros(T.nfine)
+        T.prolongate_add(uf, z[q + "t_in"].copy())
+        assert np.array_equal(uf, z[p + "prolongate"])
+        # R = P^T and prolongation reproduces constants
+        crow, ccol, cval, frow, fcol, fval = T.export()
+        R = sp.csr_matrix((cval, ccol, crow), shape=(nc, T.nfine))
+        P = sp.csr_matrix((fval, fcol, frow), shape=(T.nfine, nc))
+        assert abs(R - P.T).max() == 0
+        assert np.allclose(P @ np.ones(nc), 1.0, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("name,kind", [("q1_3d_8_ada", 13), ("q2_3d_8_ada", 23)])
+def test_hanging_nodes_bitexact(golden, name, kind):
+    z = golden(name)
+    seen = 0
+    for l in levels(z):
+        p = f"L{l}_"
+        if p + "hn_average" not in z.files:
+            continue
+        seen += 1
+        H = O.HN(kind, z[p + "hn_edge"], z[p + "hn_face"])
+        for op in ("average", "distribute", "zero"):
+            u = z[p + "hn_in"].copy()
+            getattr(H, op)(u)
+            assert np.array_equal(u, z[p + "hn_" + op]), op
+    assert seen >= 2
+
+
+@pytest.mark.parametrize("name", ["q1_3d_8_ada", "q2_3d_8_ada"])
+def test_hanging_parents_are_regular(golden, name):
+    """The gather-form device kernels rely on it: no parent of a hanging node
+    hangs itself, and every hanging node has a single entry."""
+    z = golden(name)
+    for l in levels(z):
+        e, f = z[f"L{l}_hn_edge"], z[f"L{l}_hn_face"]
+        hang = set(e[:, 0].tolist()) | set(f[:, 0].tolist())
+        assert len(hang) == len(e) + len(f)
+        parents = set(e[:, 1:].ravel().tolist()) | set(f[:, 1:].ravel().tolist())
+        assert not (hang & parents)
+
+
+def test_gauss_jordan_inverse():
+    rng = np.random.default_rng(5)
+    for N in (1, 2, 3, 4, 5, 8):
+        for trial in range(4):
+            M = rng.standard_normal((N, N))
+            if trial == 1 and N > 1:
+                M[0, 0] = 0.0  # forces a row exchange
+            if trial == 2 and N > 2:
+                M = M[rng.permutation(N)]
+            W = M.copy()
+            O.lib().or_block_gauss_jordan(N, O.ptr(W))
+            assert np.allclose(W @ M, np.eye(N), atol=1e-11)
+
+
+def cg(tol=1e-12, globaltol=1e-12, maxiter=1):
+    return O.cginfo(tol, globaltol, maxiter)
+
+
+def test_cginfo_precondition_sequence():
+    """precondition(): reset; check(0,0) -> converged; LinearMg's check(res,0)
+    re-initialises and stops with 'too many iterations' at prec_maxiter = 1."""
+    info = cg()
+    lib = O.lib()
+    lib.or_cginfo_reset(O.C.byref(info))
+    assert lib.or_cginfo_check(O.C.byref(info), 0.0, 0.0) == 1
+    assert O.CG_STATUS[info.status] == "converged"
+    assert lib.or_cginfo_check(O.C.byref(info), 3.0, 0.0) == 1
+    assert O.CG_STATUS[info.status] == "too many iterations"
+    assert info.firstresidual == 3.0 and info.iteration == 2
+    assert lib.or_cginfo_check(O.C.byref(info), 0.1, 0.0) == 1
+    assert O.CG_STATUS[info.status] == "too many iterations"
+
+
+def test_cginfo_gmres_sequence():
+    lib = O.lib()
+    info = cg(tol=1e-2, globaltol=1e-30, maxiter=100)
+    lib.or_cginfo_reset(O.C.byref(info))
+    assert lib.or_cginfo_check(O.C.byref(info), 1.0, 0) == 0  # first Arnoldi estimate
+    assert info.aimedresidual == 1e-2
+    assert lib.or_cginfo_check(O.C.byref(info), 0.5, 1) == 0
+    assert info.lastrate == 0.5
+    assert lib.or_cginfo_check(O.C.byref(info), 0.009, 2) == 1
+    assert O.CG_STATUS[info.status] == "converged" and info.iteration == 3
+    # an increase does not stop it: check() stores previousresidual = res before
+    # the "residual > previousresidual" test, so "diverging" never fires
+    lib.or_cginfo_reset(O.C.byref(info))
+    lib.or_cginfo_check(O.C.byref(info), 1.0, 0)
+    assert lib.or_cginfo_check(O.C.byref(info), 1.5, 1) == 0
+    assert O.CG_STATUS[info.status] == "running" and info.lastrate == 1.5
+    # maxiter
+    info2 = cg(tol=1e-30, globaltol=1e-30, maxiter=2)
+    lib.or_cginfo_reset(O.C.byref(info2))
+    assert lib.or_cginfo_check(O.C.byref(info2), 1.0, 0) == 0
+    assert lib.or_cginfo_check(O.C.byref(info2), 0.9, 1) == 0
+    assert lib.or_cginfo_check(O.C.byref(info2), 0.8, 2) == 1
+    assert O.CG_STATUS[info2.status] == "too many iterations"
+    # exploded
+    lib.or_cginfo_reset(O.C.byref(info))
+    lib.or_cginfo_check(O.C.byref(info), 1.0, 0)
+    assert lib.or_cginfo_check(O.C.byref(info), 1e16, 1) == 1
+    assert O.CG_STATUS[info.status] == "exploded"
+    # NaN residual is neither below tol nor above breaktol: it keeps running
+    lib.or_cginfo_reset(O.C.byref(info))
+    lib.or_cginfo_check(O.C.byref(info), 1.0, 0)
+    assert lib.or_cginfo_check(O.C.byref(info), float("nan"), 1) == 0
+
+
+def test_nlinfo_sequence():
+    lib = O.lib()
+    nl = O.nlinfo(1e-10, 1e-30, 10)
+    assert lib.or_nlinfo_check(O.C.byref(nl), 0, 1.0, 0.0) == 0
+    assert nl.aimedresidual == 1e-10 and O.NL_STATUS[nl.status] == "running"
+    assert lib.or_nlinfo_check(O.C.byref(nl), 1, 1e-6, 0.5) == 0
+    assert nl.iteration == 1
+    assert lib.or_nlinfo_check(O.C.byref(nl), 2, 1e-11, 0.1) == 1
+    assert O.NL_STATUS[nl.status] == "converged" and nl.iteration == 2
+    nl = O.nlinfo(1e-10, 1e-30, 2)
+    lib.or_nlinfo_check(O.C.byref(nl), 0, 1.0, 0.0)
+    lib.or_nlinfo_check(O.C.byref(nl), 1, 0.5, 0.0)
+    assert lib.or_nlinfo_check(O.C.byref(nl), 2, 0.25, 0.0) == 1
+    assert O.NL_STATUS[nl.status] == "slow_convergence"
+
+
+def rhs(h, seed=7):
+    lev = h.fine
+    f = splitmix64_uniform(seed, lev.n)
+    O.HN(lev.hn_kind, lev.hn_edges, lev.hn_faces).distribute(f)
+    f[lev.dirichlet] = 0.0
+    return f
+
+
+def scipy_matrix(lev):
+    return sp.csr_matrix((lev.val, lev.col, lev.rowptr), shape=(lev.n, lev.n))
+
+
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16"])
+def test_smoother_keeps_exact_solution(golden, name):
+    from conftest import fixture_path
+    h = Hierarchy.from_fixture(fixture_path(name))
+    mg = O.MG(h, iterpre=3, iterpost=0, ndirect=1)
+    lev = h.fine
+    f = rhs(h)
+    u = spla.spsolve(scipy_matrix(lev).tocsc(), f)
+    # LinearMg on the finest level only, coarse level = fine: smooth_exact only
+    info = cg()
+    u2 = u.copy()
+    mg.linear_mg(u2, f, info, coarselevel=h.nlevels - 1)
+    assert np.allclose(u2, u, rtol=0, atol=1e-12 * np.abs(u).max())
+
+
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16", "q1_3d_cube3_12"])
+def test_coarse_grid_correction_is_a_projection(golden, name):
+    """With no smoothing and an exact coarse solve, one LinearMg step is the
+    coarse-grid correction u += P A_c^{-1} R (f - A u).  For Galerkin-consistent
+    transfers (nested Q1: R A P = A_c) it is a projection: a second step changes
+    nothing.  A wrong weight, index or sign in R/P breaks it."""
+    from conftest import fixture_path
+    h = Hierarchy.from_fixture(fixture_path(name))
+    fl = h.nlevels - 1
+    mg = O.MG(h, iterpre=0, iterpost=0, ndirect=h.level(fl - 1).n + 1)
+    f = rhs(h)
+    u = np.zeros(h.fine.n)
+    mg.linear_mg(u, f, cg(), coarselevel=0)
+    assert np.abs(u).max() > 1e-3
+    u2 = u.copy()
+    mg.linear_mg(u2, f, cg(), coarselevel=0)
+    assert np.abs(u2 - u).max() <= 1e-12 * np.abs(u).max()
+
+
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada"])
+def test_gmres_estimate_is_true_preconditioned_residual(golden, name):
+    from conftest import fixture_path
+    h = Hierarchy.from_fixture(fixture_path(name))
+    mg = O.MG(h, iterpre=2, iterpost=2, omega=0.8, ndirect=1)
+    f = rhs(h)
+    A = scipy_matrix(h.fine)
+    for vmax in (2, 4, 7):
+        info = cg(tol=1e-30, globaltol=1e-30, maxiter=100)
+        x = np.zeros(h.fine.n)
+        mg.gmres(x, f, info, vmax=vmax)
+        assert info.iteration == vmax - 1
+        r = f - A @ x
+        r[h.fine.dirichlet] = 0.0
+        br = np.zeros_like(r)
+        mg.precondition(br, r)
+        assert np.linalg.norm(br) == pytest.approx(info.residual, rel=1e-9)
+
+
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada",
+                                  "q1_3d_cube3_12"])
+def test_newton_gmres_mg_reaches_direct_solution(golden, name):
+    from conftest import fixture_path
+    h = Hierarchy.from_fixture(fixture_path(name))
+    mg = O.MG(h, iterpre=2, iterpost=2, omega=0.8, ndirect=1)
+    f = rhs(h)
+    A = scipy_matrix(h.fine)
+    u = np.zeros(h.fine.n)
+    steps0 = mg.mgsteps
+    rc, rep = mg.newton(u, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30))
+    assert rc == 0, O.NL_STATUS[rep.status]
+    ex = spla.spsolve(A.tocsc(), f)
+    lev = h.fine
+    Hn = O.HN(lev.hn_kind, lev.hn_edges, lev.hn_faces)
+    Hn.zero(ex)
+    r = f - A @ u
+    r[lev.dirichlet] = 0.0
+    assert np.abs(r).max() <= 1e-10 * np.abs(f).max() * 1.0001
+    assert np.abs(u - ex).max() <= 1e-7 * np.abs(ex).max()
+    # one V-cycle per preconditioner call (prec_maxiter = 1): GMRES steps + the
+    # initial preconditioning of each cycle; the counter ("--> mgstep") counts
+    # one call per level and V-cycle
+    assert rep.mgsteps == mg.mgsteps - steps0
+    assert rep.mgsteps == (rep.gmres_iterations + rep.newton_iterations) * h.nlevels
