@@ -1,0 +1,219 @@
+"""Benchmark: DOF/s to 1e-10 relative residual on 3D Q1 Poisson, 128^3 cells
+(2,146,689 DOF), mixed-precision vs plain FP64 MG-GMRES on one MI355X, plus
+the residual SpMV's achieved HBM bandwidth against the 8 TB/s peak.
+
+One "step" is one full solve from u = 0: Newton / iterative refinement whose
+correction is one GMRES(10) cycle left-preconditioned by one V(2,2) cycle of
+damped Jacobi (omega = 0.8), dense FP64 direct solve on levels below 1000
+nodes, stopping at ||f - A u||_2 <= 1e-10 ||f - A u0||_2.  Inputs (matrix
+hierarchy, right-hand side) are resident in HBM before the timed region.
+
+usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cells 128]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hip_events():
+    lib = C.CDLL("libamdhip64.so")
+    lib.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    lib.hipEventSynchronize.argtypes = [C.c_void_p]
+    lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    lib.hipEventDestroy.argtypes = [C.c_void_p]
+    return lib
+
+
+def time_kernel(ctx, fn, reps):
+    """average duration of fn() (one kernel launch on ctx's stream), HIP events"""
+    ev = hip_events()
+    a, b = C.c_void_p(), C.c_void_p()
+    ev.hipEventCreate(C.byref(a))
+    ev.hipEventCreate(C.byref(b))
+    for _ in range(3):
+        fn()
+    ctx.sync()
+    ev.hipEventRecord(a, ctx.stream)
+    for _ in range(reps):
+        fn()
+    ev.hipEventRecord(b, ctx.stream)
+    ev.hipEventSynchronize(b)
+    ms = C.c_float()
+    ev.hipEventElapsedTime(C.byref(ms), a, b)
+    ev.hipEventDestroy(a)
+    ev.hipEventDestroy(b)
+    return ms.value / reps * 1e-3
+
+
+def spmv_bytes(n, nnz, sv, sx):
+    """algorithmic bytes of r = b - A x: values + column indices + row pointers,
+    x once, b read, r written"""
+    return nnz * (sv + 4) + (n + 1) * 4 + n * sx * 3
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=128, help="cells per direction (power of 2)")
+    ap.add_argument("--mixed", default="f16", choices=["f32", "f16"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd.problem import lattice_rhs
+    from gascoignelib_amd.structure import uniform_hierarchy
+
+    pre = int(round(np.log2(args.cells)))
+    t0 = time.perf_counter()
+    hier = uniform_hierarchy("cube.inp", 3, pre, drop_coarse=1)
+    t_struct = time.perf_counter() - t0
+    fine = hier.fine
+    ndof = fine.n
+    f = lattice_rhs(fine.coords, seed=2024)
+    f[fine.dirichlet] = 0.0
+    print(f"# hierarchy {args.cells}^3: {ndof} DOF, nnz {fine.nnz}, {hier.nlevels} levels, "
+          f"built in {t_struct:.1f}s", file=sys.stderr, flush=True)
+
+    ctx = H.Context(0)
+    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
+                  ndirect=1000)
+    t0 = time.perf_counter()
+    mg64 = H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common)
+    mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, **common)
+    t_setup = time.perf_counter() - t0
+    F = H.CudaVectorInterface.from_global(ctx, f)
+    U64, UMX = mg64.new_vector(), mgmx.new_vector()
+
+    def solve(mg, U):
+        U.zero()
+        rep = mg.Solve(U, F, H.CGInfo(1e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
+                       vmax=10, norm_kind=1)
+        if H.N.NL_STATUS[rep.status] != "converged":
+            raise RuntimeError(f"solve did not converge: {H.N.NL_STATUS[rep.status]}")
+        return rep
+
+    def timed(mg, U, label):
+        for _ in range(args.warmup):
+            solve(mg, U)
+        ctx.sync()
+        ts = []
+        rep = None
+        for k in range(args.steps):
+            ctx.sync()
+            t = time.perf_counter()
+            rep = solve(mg, U)
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+            print(f"# {label} step {k}: {ts[-1] * 1e3:.1f} ms, newton {rep.newton_iterations}, "
+                  f"gmres {rep.gmres_iterations}", file=sys.stderr, flush=True)
+        return ts, rep
+
+    ts64, rep64 = timed(mg64, U64, "fp64")
+    tsmx, repmx = timed(mgmx, UMX, f"mixed-{args.mixed}")
+    u64, umx = U64.copy_back(), UMX.copy_back()
+
+    # true FP64 residuals and the mixed path's error vs the FP64 solve
+    R = mg64.new_vector()
+    A64 = mg64.fine_matrix
+    def true_rel_res(U):
+        A64.residual(R, U, F)
+        if mg64.fine_mask:
+            mg64.fine_mask.zero(R)
+        return R.norm() / F.norm()
+    rr64, rrmx = true_rel_res(U64), true_rel_res(UMX)
+    err_mx = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+
+    # roofline of the dominant kernel: the FP64 residual SpMV on the finest level
+    X = H.CudaVectorInterface.from_global(ctx, u64)
+    t_spmv = time_kernel(ctx, lambda: A64.residual(R, X, F), args.spmv_reps)
+    b64 = spmv_bytes(ndof, fine.nnz, 8, 8)
+    gbs64 = b64 / t_spmv / 1e9
+    # the mixed V-cycle's finest-level residual (FP32 vectors, stored values)
+    Amx = mgmx.A[-1]  # FP64 handle; the V-cycle's own copy is internal
+    Af = H.CudaCSRMatrixInterface(ctx, fine.rowptr, fine.col, fine.val, 1, args.mixed)
+    Xf = H.CudaVectorInterface.from_global(ctx, u64, 1, "f32")
+    Ff = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
+    Rf = H.CudaVectorInterface(ctx, ndof, 1, "f32")
+    t_spmv_mx = time_kernel(ctx, lambda: Af.residual(Rf, Xf, Ff), args.spmv_reps)
+    bmx = spmv_bytes(ndof, fine.nnz, 2 if args.mixed == "f16" else 4, 4)
+    del Amx
+
+    t64, tmx = float(np.median(ts64)), float(np.median(tsmx))
+    dofs_mx, dofs_64 = ndof / tmx, ndof / t64
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(hier, f)
+
+    line = {
+        "metric": "DOF/s to 1e-10 rel residual (mixed vs FP64 MG-GMRES); SpMV HBM GB/s vs peak",
+        "value": dofs_mx,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmx * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"f64 outer / f32 V-cycle ({args.mixed}-stored matrix values)",
+        "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, Dirichlet 0",
+        "config": {"workload": f"3D Q1 Poisson, unit cube, {args.cells}^3 cells",
+                   "dof": ndof, "nnz": fine.nnz, "levels": hier.nlevels,
+                   "solver": "Newton/IR + GMRES(10) + V(2,2) Jacobi omega=0.8, ndirect=1000"},
+        "fp64": {"dof_per_s": dofs_64, "ms_per_solve": t64 * 1e3,
+                 "newton": rep64.newton_iterations, "gmres": rep64.gmres_iterations,
+                 "true_rel_residual": rr64},
+        "mixed": {"dof_per_s": dofs_mx, "ms_per_solve": tmx * 1e3,
+                  "newton": repmx.newton_iterations, "gmres": repmx.gmres_iterations,
+                  "true_rel_residual": rrmx, "rel_error_vs_fp64": err_mx},
+        "speedup_mixed_vs_fp64": t64 / tmx,
+        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "FP64 residual SpMV r = b - A x, finest level",
+                     "bytes_per_launch": b64, "us_per_launch": t_spmv * 1e6},
+        "spmv_mixed": {"gbs": bmx / t_spmv_mx / 1e9, "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS,
+                       "bytes_per_launch": bmx, "us_per_launch": t_spmv_mx * 1e6},
+        "setup_s": {"structure": t_struct, "device_upload": t_setup},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(hier, f):
+    """The oracle's FP64 Newton/GMRES/MG solve of the same problem on the host
+    cores (OpenMP over the loops the reference parallelises), one solve."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as O
+    om = O.MG(hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
+              ndirect=1000)
+    u = np.zeros_like(f)
+    t = time.perf_counter()
+    rc, rep = om.newton(u, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 50),
+                        norm_kind=1)
+    dt = time.perf_counter() - t
+    return {"value": len(f) / dt, "unit": "DOF/s", "cores": int(O.lib().or_num_threads()),
+            "kind": "port", "sample": f"one full FP64 solve of the same {len(f)}-DOF problem "
+            f"({dt:.1f} s, newton {rep.newton_iterations}, gmres {rep.gmres_iterations})"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1019 @@
+// Runtime of the gascoignelib_amd C ABI: context, vectors, matrices, grid
+// transfer, hanging nodes, Dirichlet masks and the launch wrappers around the
+// kernels of gh_kernels.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "gh_kernels.h"
+
+namespace gh {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+static size_t prec_bytes(int prec) { return prec == GH_F64 ? 8 : prec == GH_F32 ? 4 : 2; }
+
+static unsigned grid_stride(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+int vec_alloc(gh_ctx* ctx, int64_t n, int ncomp, int prec, gh_vec* v) {
+  if (n < 0 || ncomp < 1 || (prec != GH_F64 && prec != GH_F32))
+    return fail(GH_EINVAL, "gh_vec_create: bad shape n=%lld ncomp=%d prec=%d", (long long)n,
+                ncomp, prec);
+  v->ctx = ctx;
+  v->n = n;
+  v->ncomp = ncomp;
+  v->prec = prec;
+  v->size = n * ncomp;
+  v->d = nullptr;
+  size_t bytes = (size_t)std::max<int64_t>(v->size, 1) * prec_bytes(prec);
+  if (hipMalloc(&v->d, bytes) != hipSuccess)
+    return fail(GH_ENOMEM, "gh_vec_create: hipMalloc(%zu) failed", bytes);
+  GH_HIP(hipMemsetAsync(v->d, 0, bytes, ctx->stream));
+  return GH_OK;
+}
+
+void vec_free(gh_vec* v) {
+  if (v->d && v->owned) (void)hipFree(v->d);
+  v->d = nullptr;
+}
+
+int fetch_scalars(gh_ctx* ctx, int first, int count, double* out) {
+  GH_HIP(hipMemcpyAsync(ctx->host_scal + first, ctx->scal + first, sizeof(double) * count,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < count; ++i) out[i] = ctx->host_scal[first + i];
+  return GH_OK;
+}
+
+int reduce_dot(gh_ctx* ctx, int prec, int64_t size, const void* x, const void* y, int slot) {
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_dot<double>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const double*)x, (const double*)y, ctx->partials, ctx->counter,
+                       ctx->scal + slot);
+  else
+    hipLaunchKernelGGL(k_dot<float>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const float*)x, (const float*)y, ctx->partials, ctx->counter,
+                       ctx->scal + slot);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int reduce_norm_l8(gh_ctx* ctx, int prec, int64_t size, const void* x, int slot) {
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_amax<double>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const double*)x, ctx->partials, ctx->counter, ctx->scal + slot);
+  else
+    hipLaunchKernelGGL(k_amax<float>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const float*)x, ctx->partials, ctx->counter, ctx->scal + slot);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// Modified Gram-Schmidt sub-step of GMRES::orthogonalization (gmres.cc:109-120):
+//   vv += (-d) * prev  (d read from a device slot), then partial of vv.next
+//   (or vv.vv when next == nullptr).
+__global__ __launch_bounds__(kBlock) void k_mgs(int64_t n, double* __restrict__ vv,
+                                                const double* __restrict__ prev,
+                                                const double* __restrict__ dslot,
+                                                const double* __restrict__ next,
+                                                double* partials, unsigned* counter,
+                                                double* out) {
+  const double md = prev ? -(*dslot) : 0.0;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double v = vv[i];
+    if (prev) {
+      v += md * prev[i];
+      vv[i] = v;
+    }
+    acc += v * (next ? next[i] : v);
+  }
+  acc = block_reduce<false>(acc);
+  finish_reduction<false>(acc, partials, counter, out);
+}
+
+int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
+             const double* next, int oslot) {
+  hipLaunchKernelGGL(k_mgs, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size, vv, prev,
+                     ctx->scal + dslot, next, ctx->partials, ctx->counter, ctx->scal + oslot);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int vec_axpby(gh_ctx* ctx, int prec, int64_t size, void* y, double a, double d, const void* x) {
+  // mode from (a, x): a==1 -> add; a==0 && x -> equ
+  int mode = (a == 1.0) ? 1 : 0;
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_stride(size)), dim3(kBlock), 0, ctx->stream,
+                       size, (double*)y, a, d, (const double*)x, mode);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_stride(size)), dim3(kBlock), 0, ctx->stream,
+                       size, (float*)y, (float)a, (float)d, (const float*)x, mode);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int vec_scale_to(gh_ctx* ctx, int prec, int64_t size, void* y, double s, const void* x) {
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_stride(size)), dim3(kBlock), 0, ctx->stream,
+                       size, (double*)y, 0.0, s, (const double*)x, 2);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_stride(size)), dim3(kBlock), 0, ctx->stream,
+                       size, (float*)y, 0.f, (float)s, (const float*)x, 2);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int vec_convert(gh_ctx* ctx, int64_t size, int dprec, void* dst, int sprec, const void* src) {
+  unsigned g = grid_stride(size);
+  if (dprec == GH_F64 && sprec == GH_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kBlock), 0, ctx->stream, size,
+                       (double*)dst, (const float*)src);
+  else if (dprec == GH_F32 && sprec == GH_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(kBlock), 0, ctx->stream, size,
+                       (float*)dst, (const double*)src);
+  else if (dprec == sprec) {
+    // a kernel rather than a memcpy node, so that the copy can sit in a captured graph
+    if (dst == src) return GH_OK;
+    if (dprec == GH_F64)
+      hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(kBlock), 0, ctx->stream,
+                         size, (double*)dst, (const double*)src);
+    else
+      hipLaunchKernelGGL((k_convert<float, float>), dim3(g), dim3(kBlock), 0, ctx->stream, size,
+                         (float*)dst, (const float*)src);
+  } else
+    return fail(GH_EINVAL, "vec_convert: unsupported precisions %d <- %d", dprec, sprec);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int vec_fill_zero(gh_ctx* ctx, int prec, int64_t size, void* x) {
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_fill0<double>, dim3(grid_stride(size)), dim3(kBlock), 0, ctx->stream,
+                       size, (double*)x);
+  else
+    hipLaunchKernelGGL(k_fill0<float>, dim3(grid_stride(size)), dim3(kBlock), 0, ctx->stream,
+                       size, (float*)x);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// ------------------------------------------------------------------ SpMV
+template <typename VT, typename XT, typename Acc, class Epi>
+static int launch_csr(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
+  const int64_t n = A->n;
+  const int seg = A->seg;
+  unsigned grid = grid_for(n * seg);
+  const VT* val = (const VT*)A->val;
+#define GH_SEG_CASE(S)                                                                 \
+  case S:                                                                              \
+    hipLaunchKernelGGL((k_csr_seg<S, VT, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0,  \
+                       ctx->stream, n, A->rowptr, A->col, val, x, epi);                \
+    break;
+  switch (seg) {
+    GH_SEG_CASE(1)
+    GH_SEG_CASE(2)
+    GH_SEG_CASE(4)
+    GH_SEG_CASE(8)
+    GH_SEG_CASE(16)
+    GH_SEG_CASE(32)
+    GH_SEG_CASE(64)
+    default:
+      return fail(GH_EINVAL, "bad segment width %d", seg);
+  }
+#undef GH_SEG_CASE
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+template <typename XT, typename Acc, class Epi>
+static int dispatch_val(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
+  if (A->ncomp > 1) {
+    unsigned grid = grid_for(A->n * A->ncomp);
+    if (A->prec == GH_F64)
+      hipLaunchKernelGGL((k_bsr<double, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         A->n, A->ncomp, A->rowptr, A->col, (const double*)A->val, x, epi);
+    else if (A->prec == GH_F32)
+      hipLaunchKernelGGL((k_bsr<float, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         A->n, A->ncomp, A->rowptr, A->col, (const float*)A->val, x, epi);
+    else
+      hipLaunchKernelGGL((k_bsr<__half, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         A->n, A->ncomp, A->rowptr, A->col, (const __half*)A->val, x, epi);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
+  if (A->prec == GH_F64) return launch_csr<double, XT, Acc, Epi>(ctx, A, x, epi);
+  if (A->prec == GH_F32) return launch_csr<float, XT, Acc, Epi>(ctx, A, x, epi);
+  return launch_csr<__half, XT, Acc, Epi>(ctx, A, x, epi);
+}
+
+int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const void* x,
+             double d, double b) {
+  const bool ry = (b != 0.0);
+  if (xprec == GH_F64 && yprec == GH_F64)
+    return dispatch_val<double, double>(ctx, A, (const double*)x,
+                                        EpiSpmv<double, double>{(double*)y, d, b, ry});
+  if (xprec == GH_F64 && yprec == GH_F32)
+    return dispatch_val<double, double>(ctx, A, (const double*)x,
+                                        EpiSpmv<float, double>{(float*)y, d, b, ry});
+  if (xprec == GH_F32 && yprec == GH_F32)
+    return dispatch_val<float, float>(ctx, A, (const float*)x,
+                                      EpiSpmv<float, float>{(float*)y, (float)d, (float)b, ry});
+  if (xprec == GH_F32 && yprec == GH_F64)
+    return dispatch_val<float, float>(ctx, A, (const float*)x,
+                                      EpiSpmv<double, float>{(double*)y, (float)d, (float)b, ry});
+  return fail(GH_EINVAL, "csr_spmv: bad precisions");
+}
+
+int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b) {
+  if (prec == GH_F64)
+    return dispatch_val<double, double>(ctx, A, (const double*)x,
+                                        EpiResidual<double>{(double*)r, (const double*)b});
+  return dispatch_val<float, float>(ctx, A, (const float*)x,
+                                    EpiResidual<float>{(float*)r, (const float*)b});
+}
+
+template <typename T, typename DT>
+static int jacobi_impl(gh_ctx* ctx, const gh_csr* A, T* xn, const T* x, const T* b, T omega,
+                       int smoother, bool x_zero) {
+  const DT* dinv = (const DT*)A->dinv;
+  if (A->ncomp > 1) {
+    if (smoother != 0) return fail(GH_EINVAL, "block matrices support the jacobi smoother only");
+    unsigned grid = grid_for(A->n);
+    if (A->prec == GH_F64)
+      hipLaunchKernelGGL((k_bsr_jacobi<double, T, DT>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         A->n, A->ncomp, A->rowptr, A->col, (const double*)A->val, x, b, dinv,
+                         xn, omega, (int)x_zero);
+    else if (A->prec == GH_F32)
+      hipLaunchKernelGGL((k_bsr_jacobi<float, T, DT>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         A->n, A->ncomp, A->rowptr, A->col, (const float*)A->val, x, b, dinv,
+                         xn, omega, (int)x_zero);
+    else
+      hipLaunchKernelGGL((k_bsr_jacobi<__half, T, DT>), dim3(grid), dim3(kBlock), 0,
+                         ctx->stream, A->n, A->ncomp, A->rowptr, A->col, (const __half*)A->val,
+                         x, b, dinv, xn, omega, (int)x_zero);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
+  if (x_zero) {
+    hipLaunchKernelGGL((k_jacobi_zero<T, DT>), dim3(grid_stride(A->n)), dim3(kBlock), 0,
+                       ctx->stream, A->n, xn, b, dinv, omega, smoother);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
+  if (smoother == 0) return dispatch_val<T, T>(ctx, A, x, EpiJacobi<T, DT>{xn, x, b, dinv, omega});
+  return dispatch_val<T, T>(ctx, A, x, EpiRichardson<T>{xn, x, b, omega});
+}
+
+int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const void* x,
+                     const void* b, double omega, int smoother, bool x_zero) {
+  if (prec == GH_F64) {
+    if (A->prec != GH_F64) return fail(GH_EINVAL, "FP64 sweep needs an FP64 matrix");
+    return jacobi_impl<double, double>(ctx, A, (double*)xnew, (const double*)x,
+                                       (const double*)b, omega, smoother, x_zero);
+  }
+  return jacobi_impl<float, float>(ctx, A, (float*)xnew, (const float*)x, (const float*)b,
+                                   (float)omega, smoother, x_zero);
+}
+
+// ------------------------------------------------------------- transfers
+int transfer_restrict(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, void* uc,
+                      const void* uf, const int32_t* /*unused*/) {
+  unsigned grid = grid_for(T->ncoarse * ncomp);
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_restrict<double>, dim3(grid), dim3(kBlock), 0, ctx->stream, T->ncoarse,
+                       ncomp, T->r_row, T->r_col, (const double*)uf, (double*)uc);
+  else
+    hipLaunchKernelGGL(k_restrict<float>, dim3(grid), dim3(kBlock), 0, ctx->stream, T->ncoarse,
+                       ncomp, T->r_row, T->r_col, (const float*)uf, (float*)uc);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int transfer_prolongate_add(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, void* uf,
+                            const void* uc) {
+  unsigned grid = grid_for(T->nfine * ncomp);
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_prolongate_add<double>, dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       T->nfine, ncomp, T->p_row, T->p_col, (const double*)uc, (double*)uf);
+  else
+    hipLaunchKernelGGL(k_prolongate_add<float>, dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       T->nfine, ncomp, T->p_row, T->p_col, (const float*)uc, (float*)uf);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// op 0 average, 1 distribute, 2 zero
+int hn_apply(gh_ctx* ctx, const gh_hn* H, int ncomp, int prec, void* x, int op) {
+  if (H->ne + H->nf == 0) return GH_OK;
+  if (op == 0) {
+    unsigned grid = grid_for((H->ne + H->nf) * ncomp);
+    if (prec == GH_F64)
+      hipLaunchKernelGGL(k_hn_average<double>, dim3(grid), dim3(kBlock), 0, ctx->stream, H->kind,
+                         H->ne, H->edges, H->nf, H->faces, ncomp, (double*)x);
+    else
+      hipLaunchKernelGGL(k_hn_average<float>, dim3(grid), dim3(kBlock), 0, ctx->stream, H->kind,
+                         H->ne, H->edges, H->nf, H->faces, ncomp, (float*)x);
+  } else {
+    if (op == 1 && H->np > 0) {
+      unsigned grid = grid_for(H->np * ncomp);
+      if (prec == GH_F64)
+        hipLaunchKernelGGL(k_hn_gather<double>, dim3(grid), dim3(kBlock), 0, ctx->stream, H->np,
+                           ncomp, H->d_parent, H->d_row, H->d_src, H->d_w, (double*)x);
+      else
+        hipLaunchKernelGGL(k_hn_gather<float>, dim3(grid), dim3(kBlock), 0, ctx->stream, H->np,
+                           ncomp, H->d_parent, H->d_row, H->d_src, H->d_w, (float*)x);
+      GH_HIP(hipGetLastError());
+    }
+    unsigned grid = grid_for((H->ne + H->nf) * ncomp);
+    if (prec == GH_F64)
+      hipLaunchKernelGGL(k_zero_nodes<double>, dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         H->ne + H->nf, ncomp, H->hanging, (double*)x);
+    else
+      hipLaunchKernelGGL(k_zero_nodes<float>, dim3(grid), dim3(kBlock), 0, ctx->stream,
+                         H->ne + H->nf, ncomp, H->hanging, (float*)x);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int mask_apply(gh_ctx* ctx, const gh_mask* M, int prec, void* x) {
+  const int64_t m = M->nidx * M->ncomps;
+  if (m == 0) return GH_OK;
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_zero_flat<double>, dim3(grid_for(m)), dim3(kBlock), 0, ctx->stream, m,
+                       M->idx, (double*)x);
+  else
+    hipLaunchKernelGGL(k_zero_flat<float>, dim3(grid_for(m)), dim3(kBlock), 0, ctx->stream, m,
+                       M->idx, (float*)x);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+template <typename T>
+static int upload(gh_ctx* ctx, T** dptr, const T* host, size_t count) {
+  *dptr = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (hipMalloc((void**)dptr, bytes) != hipSuccess)
+    return fail(GH_ENOMEM, "hipMalloc(%zu) failed", bytes);
+  if (count) GH_HIP(hipMemcpyAsync(*dptr, host, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return GH_OK;
+}
+
+}  // namespace gh
+
+using namespace gh;
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* gh_last_error(void) { return g_err.c_str(); }
+const char* gh_version(void) { return "gascoigne_hip 0.1 (gfx950)"; }
+
+int gh_ctx_create(int device, gh_ctx** out) {
+  GH_ARG(out, "gh_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  GH_HIP(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(GH_EHIP, "gh_ctx_create: no HIP device");
+  if (device >= 0) {
+    GH_ARG(device < ndev, "gh_ctx_create: device index out of range");
+    GH_HIP(hipSetDevice(device));
+  } else {
+    GH_HIP(hipGetDevice(&device));
+  }
+  gh_ctx* c = new gh_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&c->partials, sizeof(double) * kRedBlocks * 4) != hipSuccess ||
+      hipMalloc((void**)&c->counter, sizeof(unsigned) * 16) != hipSuccess ||
+      hipMalloc((void**)&c->scal, sizeof(double) * kScalars) != hipSuccess ||
+      hipHostMalloc((void**)&c->host_scal, sizeof(double) * kScalars, hipHostMallocDefault) !=
+          hipSuccess) {
+    gh_ctx_destroy(c);
+    return fail(GH_ENOMEM, "gh_ctx_create: allocation failed");
+  }
+  GH_HIP(hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 16, c->stream));
+  GH_HIP(hipMemsetAsync(c->scal, 0, sizeof(double) * kScalars, c->stream));
+  GH_HIP(hipStreamSynchronize(c->stream));
+  *out = c;
+  return GH_OK;
+}
+
+int gh_ctx_destroy(gh_ctx* ctx) {
+  if (!ctx) return GH_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->counter) (void)hipFree(ctx->counter);
+  if (ctx->scal) (void)hipFree(ctx->scal);
+  if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GH_OK;
+}
+
+int gh_ctx_sync(gh_ctx* ctx) {
+  GH_ARG(ctx, "gh_ctx_sync: ctx is NULL");
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+void* gh_ctx_stream(gh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ------------------------------------------------------------------ vectors
+int gh_vec_create(gh_ctx* ctx, int64_t n, int ncomp, int prec, gh_vec** out) {
+  GH_ARG(ctx && out, "gh_vec_create: NULL argument");
+  gh_vec* v = new gh_vec();
+  int rc = vec_alloc(ctx, n, ncomp, prec, v);
+  if (rc) {
+    delete v;
+    return rc;
+  }
+  *out = v;
+  return GH_OK;
+}
+
+int gh_vec_destroy(gh_vec* v) {
+  if (!v) return GH_OK;
+  if (v->ctx && v->ctx->stream) (void)hipStreamSynchronize(v->ctx->stream);
+  vec_free(v);
+  delete v;
+  return GH_OK;
+}
+
+int64_t gh_vec_n(const gh_vec* v) { return v ? v->n : -1; }
+int gh_vec_ncomp(const gh_vec* v) { return v ? v->ncomp : -1; }
+int gh_vec_prec(const gh_vec* v) { return v ? v->prec : -1; }
+void* gh_vec_data(gh_vec* v) { return v ? v->d : nullptr; }
+
+int gh_vec_upload(gh_ctx* ctx, gh_vec* v, const double* host) {
+  GH_ARG(ctx && v && (host || v->size == 0), "gh_vec_upload: NULL argument");
+  if (v->prec == GH_F64) {
+    GH_HIP(hipMemcpyAsync(v->d, host, v->size * 8, hipMemcpyHostToDevice, ctx->stream));
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    return GH_OK;
+  }
+  double* tmp = nullptr;
+  GH_HIP(hipMalloc((void**)&tmp, std::max<int64_t>(v->size, 1) * 8));
+  GH_HIP(hipMemcpyAsync(tmp, host, v->size * 8, hipMemcpyHostToDevice, ctx->stream));
+  int rc = vec_convert(ctx, v->size, GH_F32, v->d, GH_F64, tmp);
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  GH_HIP(hipFree(tmp));
+  return rc;
+}
+
+int gh_vec_download(gh_ctx* ctx, const gh_vec* v, double* host) {
+  GH_ARG(ctx && v && (host || v->size == 0), "gh_vec_download: NULL argument");
+  if (v->prec == GH_F64) {
+    GH_HIP(hipMemcpyAsync(host, v->d, v->size * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    return GH_OK;
+  }
+  double* tmp = nullptr;
+  GH_HIP(hipMalloc((void**)&tmp, std::max<int64_t>(v->size, 1) * 8));
+  GH_TRY(vec_convert(ctx, v->size, GH_F64, tmp, GH_F32, v->d));
+  GH_HIP(hipMemcpyAsync(host, tmp, v->size * 8, hipMemcpyDeviceToHost, ctx->stream));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  GH_HIP(hipFree(tmp));
+  return GH_OK;
+}
+
+int gh_vec_zero(gh_ctx* ctx, gh_vec* v) {
+  GH_ARG(ctx && v, "gh_vec_zero: NULL argument");
+  return vec_fill_zero(ctx, v->prec, v->size, v->d);
+}
+
+static int same_shape(const gh_vec* a, const gh_vec* b) {
+  return a && b && a->n == b->n && a->ncomp == b->ncomp;
+}
+
+int gh_vec_copy(gh_ctx* ctx, gh_vec* dst, const gh_vec* src) {
+  GH_ARG(ctx && same_shape(dst, src), "gh_vec_copy: shape mismatch");
+  return vec_convert(ctx, dst->size, dst->prec, dst->d, src->prec, src->d);
+}
+
+int gh_vec_equ(gh_ctx* ctx, gh_vec* dst, double s, const gh_vec* src) {
+  GH_ARG(ctx && same_shape(dst, src) && dst->prec == src->prec, "gh_vec_equ: shape mismatch");
+  return vec_scale_to(ctx, dst->prec, dst->size, dst->d, s, src->d);
+}
+
+int gh_vec_add(gh_ctx* ctx, gh_vec* dst, double s, const gh_vec* src) {
+  GH_ARG(ctx && same_shape(dst, src) && dst->prec == src->prec, "gh_vec_add: shape mismatch");
+  return vec_axpby(ctx, dst->prec, dst->size, dst->d, 1.0, s, src->d);
+}
+
+int gh_vec_sadd(gh_ctx* ctx, gh_vec* y, double a, double d, const gh_vec* v) {
+  GH_ARG(ctx && same_shape(y, v) && y->prec == v->prec, "gh_vec_sadd: shape mismatch");
+  if (y->prec == GH_F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_stride(y->size)), dim3(kBlock), 0, ctx->stream,
+                       y->size, (double*)y->d, a, d, (const double*)v->d, 0);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_stride(y->size)), dim3(kBlock), 0, ctx->stream,
+                       y->size, (float*)y->d, (float)a, (float)d, (const float*)v->d, 0);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int gh_vec_scal(gh_ctx* ctx, gh_vec* x, double s) {
+  GH_ARG(ctx && x, "gh_vec_scal: NULL argument");
+  return vec_scale_to(ctx, x->prec, x->size, x->d, s, x->d);
+}
+
+int gh_vec_dot(gh_ctx* ctx, const gh_vec* x, const gh_vec* y, double* out) {
+  GH_ARG(ctx && same_shape(x, y) && x->prec == y->prec && out, "gh_vec_dot: shape mismatch");
+  GH_TRY(reduce_dot(ctx, x->prec, x->size, x->d, y->d, 0));
+  return fetch_scalars(ctx, 0, 1, out);
+}
+
+int gh_vec_norm(gh_ctx* ctx, const gh_vec* x, double* out) {
+  GH_ARG(ctx && x && out, "gh_vec_norm: NULL argument");
+  GH_TRY(reduce_dot(ctx, x->prec, x->size, x->d, x->d, 0));
+  double s;
+  GH_TRY(fetch_scalars(ctx, 0, 1, &s));
+  *out = std::sqrt(s);
+  return GH_OK;
+}
+
+int gh_vec_norm_l8(gh_ctx* ctx, const gh_vec* x, double* out) {
+  GH_ARG(ctx && x && out, "gh_vec_norm_l8: NULL argument");
+  GH_TRY(reduce_norm_l8(ctx, x->prec, x->size, x->d, 0));
+  return fetch_scalars(ctx, 0, 1, out);
+}
+
+// ----------------------------------------------------------------- matrices
+static int choose_seg(int64_t n, int64_t nnz) {
+  if (const char* e = getenv("GH_SPMV_SEG")) {
+    int s = atoi(e);
+    if (s == 1 || s == 2 || s == 4 || s == 8 || s == 16 || s == 32 || s == 64) return s;
+  }
+  double avg = n ? (double)nnz / (double)n : 1.0;
+  int s = 1;
+  while (s < 64 && s * 4 < avg) s *= 2;
+  return s;
+}
+
+int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
+                  const double* val, int prec, gh_csr** out) {
+  GH_ARG(ctx && out && rowptr && n >= 0 && ncomp >= 1 && ncomp <= 8,
+         "gh_csr_create: bad arguments (ncomp must be 1..8)");
+  GH_ARG(prec == GH_F64 || prec == GH_F32 || prec == GH_F16, "gh_csr_create: bad precision");
+  *out = nullptr;
+  const int64_t nnz = rowptr[n];
+  GH_ARG(rowptr[0] == 0 && nnz >= 0 && (nnz == 0 || (col && val)), "gh_csr_create: bad rowptr");
+  const int BB = ncomp * ncomp;
+  std::vector<int32_t> diag(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) return fail(GH_EINVAL, "gh_csr_create: rowptr not monotone");
+    diag[i] = -1;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      if (col[p] < 0 || col[p] >= n)
+        return fail(GH_EINVAL, "gh_csr_create: column %d out of range in row %lld", col[p],
+                    (long long)i);
+      if (p > rowptr[i] && col[p] <= col[p - 1])
+        return fail(GH_EINVAL, "gh_csr_create: columns of row %lld not strictly ascending",
+                    (long long)i);
+      if (col[p] == i && diag[i] < 0) diag[i] = p;
+    }
+    if (diag[i] < 0) return fail(GH_EINVAL, "gh_csr_create: row %lld has no diagonal", (long long)i);
+  }
+  gh_csr* A = new gh_csr();
+  A->ctx = ctx;
+  A->n = n;
+  A->ncomp = ncomp;
+  A->nnz = nnz;
+  A->prec = prec;
+  A->seg = choose_seg(n, nnz);
+  A->h_rowptr.assign(rowptr, rowptr + n + 1);
+  A->h_col.assign(col, col + nnz);
+  A->h_val.assign(val, val + nnz * BB);
+  int rc = upload(ctx, &A->rowptr, rowptr, n + 1);
+  if (!rc) rc = upload(ctx, &A->col, col, nnz);
+  double* dval64 = nullptr;
+  if (!rc) rc = upload(ctx, &dval64, val, nnz * BB);
+  int32_t* ddiag = nullptr;
+  if (!rc) rc = upload(ctx, &ddiag, diag.data(), n);
+  int* dsing = nullptr;
+  if (!rc && hipMalloc((void**)&dsing, sizeof(int)) != hipSuccess) rc = fail(GH_ENOMEM, "alloc");
+  if (!rc && hipMemsetAsync(dsing, 0, sizeof(int), ctx->stream) != hipSuccess) rc = GH_EHIP;
+  // inverse diagonal blocks, FP64 then narrowed for the FP32 paths
+  const size_t dsz = (size_t)std::max<int64_t>(n * BB, 1);
+  if (!rc) {
+    if (prec == GH_F64) {
+      if (hipMalloc(&A->dinv, dsz * 8) != hipSuccess) rc = fail(GH_ENOMEM, "alloc dinv");
+      else
+        hipLaunchKernelGGL(k_block_inverse<double>, dim3(grid_for(n)), dim3(kBlock), 0,
+                           ctx->stream, n, ncomp, ddiag, dval64, (double*)A->dinv, dsing);
+    } else {
+      if (hipMalloc(&A->dinv, dsz * 4) != hipSuccess) rc = fail(GH_ENOMEM, "alloc dinv");
+      else
+        hipLaunchKernelGGL(k_block_inverse<float>, dim3(grid_for(n)), dim3(kBlock), 0,
+                           ctx->stream, n, ncomp, ddiag, dval64, (float*)A->dinv, dsing);
+    }
+    if (!rc && hipGetLastError() != hipSuccess) rc = fail(GH_EHIP, "block inverse launch");
+  }
+  if (!rc) {
+    if (prec == GH_F64) {
+      A->val = dval64;
+      dval64 = nullptr;
+    } else {
+      const size_t m = (size_t)std::max<int64_t>(nnz * BB, 1);
+      if (hipMalloc(&A->val, m * (prec == GH_F32 ? 4 : 2)) != hipSuccess)
+        rc = fail(GH_ENOMEM, "alloc val");
+      else if (prec == GH_F32)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(nnz * BB)), dim3(kBlock),
+                           0, ctx->stream, nnz * BB, (float*)A->val, (const double*)dval64);
+      else
+        hipLaunchKernelGGL((k_convert<__half, double>), dim3(grid_stride(nnz * BB)),
+                           dim3(kBlock), 0, ctx->stream, nnz * BB, (__half*)A->val,
+                           (const double*)dval64);
+    }
+  }
+  int sing = 0;
+  if (!rc && hipMemcpyAsync(&sing, dsing, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = GH_EHIP;
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
+  if (dval64) (void)hipFree(dval64);
+  if (ddiag) (void)hipFree(ddiag);
+  if (dsing) (void)hipFree(dsing);
+  if (!rc && sing) rc = fail(GH_ESINGULAR, "gh_csr_create: singular diagonal block");
+  if (rc) {
+    gh_csr_destroy(A);
+    return rc;
+  }
+  *out = A;
+  return GH_OK;
+}
+
+int gh_csr_destroy(gh_csr* A) {
+  if (!A) return GH_OK;
+  if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+  for (void* p : {(void*)A->rowptr, (void*)A->col, A->val, A->dinv, (void*)A->t_rowptr,
+                  (void*)A->t_col, (void*)A->t_pos})
+    if (p) (void)hipFree(p);
+  delete A;
+  return GH_OK;
+}
+
+int64_t gh_csr_n(const gh_csr* A) { return A ? A->n : -1; }
+int64_t gh_csr_nnz(const gh_csr* A) { return A ? A->nnz : -1; }
+int gh_csr_prec(const gh_csr* A) { return A ? A->prec : -1; }
+
+static int vec_matches(const gh_csr* A, const gh_vec* v) {
+  return v && v->n == A->n && v->ncomp == A->ncomp;
+}
+
+int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b) {
+  GH_ARG(ctx && A && vec_matches(A, y) && vec_matches(A, x), "gh_spmv: shape mismatch");
+  GH_ARG(y->d != x->d, "gh_spmv: x and y must not alias");
+  return csr_spmv(ctx, A, y->prec, y->d, x->prec, x->d, d, b);
+}
+
+int gh_residual(gh_ctx* ctx, const gh_csr* A, gh_vec* r, const gh_vec* x, const gh_vec* b) {
+  GH_ARG(ctx && A && vec_matches(A, r) && vec_matches(A, x) && vec_matches(A, b),
+         "gh_residual: shape mismatch");
+  GH_ARG(r->prec == x->prec && x->prec == b->prec, "gh_residual: precision mismatch");
+  GH_ARG(r->d != x->d, "gh_residual: r and x must not alias");
+  if (r->prec == GH_F64 && A->prec != GH_F64)
+    return fail(GH_EINVAL, "gh_residual: FP64 vectors need an FP64 matrix");
+  return csr_residual(ctx, A, r->prec, r->d, x->d, b->d);
+}
+
+int gh_smooth(gh_ctx* ctx, const gh_csr* A, gh_vec* x, const gh_vec* b, gh_vec* h, int niter,
+              double omega, int smoother) {
+  GH_ARG(ctx && A && vec_matches(A, x) && vec_matches(A, b) && vec_matches(A, h),
+         "gh_smooth: shape mismatch");
+  GH_ARG(x->prec == b->prec && b->prec == h->prec, "gh_smooth: precision mismatch");
+  GH_ARG(smoother == 0 || smoother == 1, "gh_smooth: smoother must be 0 (jacobi) or 1 (richardson)");
+  void* cur = x->d;
+  void* nxt = h->d;
+  for (int it = 0; it < niter; ++it) {
+    GH_TRY(csr_jacobi_sweep(ctx, A, x->prec, nxt, cur, b->d, omega, smoother, false));
+    std::swap(cur, nxt);
+  }
+  if (cur != x->d) return vec_convert(ctx, x->size, x->prec, x->d, x->prec, cur);
+  return GH_OK;
+}
+
+int gh_jacobi(gh_ctx* ctx, const gh_csr* A, gh_vec* h) {
+  GH_ARG(ctx && A && vec_matches(A, h), "gh_jacobi: shape mismatch");
+  // h = D^{-1} h, as a sweep from x = 0 with omega = 1 into a temporary
+  gh_vec t;
+  GH_TRY(vec_alloc(ctx, h->n, h->ncomp, h->prec, &t));
+  int rc = csr_jacobi_sweep(ctx, A, h->prec, t.d, nullptr, h->d, 1.0, 0, true);
+  if (!rc) rc = vec_convert(ctx, h->size, h->prec, h->d, h->prec, t.d);
+  (void)hipStreamSynchronize(ctx->stream);
+  vec_free(&t);
+  return rc;
+}
+
+// ---------------------------------------------------------------- transfer
+int gh_transfer_create(gh_ctx* ctx, int64_t ncoarse, const int32_t* c2f, int64_t n2,
+                       const int32_t* zweier, int64_t n4, const int32_t* vierer, int64_t n8,
+                       const int32_t* achter, gh_transfer** out) {
+  GH_ARG(ctx && out && ncoarse >= 0 && c2f, "gh_transfer_create: bad arguments");
+  *out = nullptr;
+  int64_t nfine = -1;
+  for (int64_t i = 0; i < ncoarse; ++i) nfine = std::max<int64_t>(nfine, c2f[i]);
+  for (int64_t i = 0; i < n2; ++i) nfine = std::max<int64_t>(nfine, zweier[3 * i]);
+  for (int64_t i = 0; i < n4; ++i) nfine = std::max<int64_t>(nfine, vierer[5 * i]);
+  for (int64_t i = 0; i < n8; ++i) nfine = std::max<int64_t>(nfine, achter[9 * i]);
+  nfine++;
+  GH_ARG(nfine <= (int64_t)kIdxMask, "gh_transfer_create: too many nodes for packed columns");
+  // weight code of each fine node: I2 -> 1, I4 -> 2, I8 -> 3, else 0
+  // (lookup order of MgInterpolatorMatrix::BasicInit, mginterpolatormatrix.cc:97-121)
+  std::vector<uint8_t> code(nfine, 0);
+  for (int64_t i = n8 - 1; i >= 0; --i) code[achter[9 * i]] = 3;
+  for (int64_t i = n4 - 1; i >= 0; --i) code[vierer[5 * i]] = 2;
+  for (int64_t i = n2 - 1; i >= 0; --i) code[zweier[3 * i]] = 1;
+  std::vector<std::pair<int32_t, int32_t>> F, C;  // (row, col)
+  const int64_t np = ncoarse + 2 * n2 + 4 * n4 + 8 * n8;
+  F.reserve(np);
+  C.reserve(np);
+  for (int64_t i = 0; i < ncoarse; ++i) {
+    GH_ARG(c2f[i] >= 0, "gh_transfer_create: negative c2f entry");
+    F.emplace_back(c2f[i], (int32_t)i);
+    C.emplace_back((int32_t)i, c2f[i]);
+  }
+  auto add = [&](const int32_t* a, int64_t cnt, int w) -> bool {
+    for (int64_t i = 0; i < cnt; ++i)
+      for (int j = 0; j < w; ++j) {
+        int32_t f = a[(w + 1) * i], c = a[(w + 1) * i + 1 + j];
+        if (c < 0 || c >= ncoarse) return false;
+        F.emplace_back(f, c);
+        C.emplace_back(c, f);
+      }
+    return true;
+  };
+  GH_ARG(add(zweier, n2, 2) && add(vierer, n4, 4) && add(achter, n8, 8),
+         "gh_transfer_create: parent index out of range");
+  std::sort(F.begin(), F.end());
+  F.erase(std::unique(F.begin(), F.end()), F.end());
+  std::sort(C.begin(), C.end());
+  C.erase(std::unique(C.begin(), C.end()), C.end());
+  gh_transfer* T = new gh_transfer();
+  T->ctx = ctx;
+  T->ncoarse = ncoarse;
+  T->nfine = nfine;
+  T->h_rrow.assign(ncoarse + 1, 0);
+  T->h_prow.assign(nfine + 1, 0);
+  for (auto& e : C) {
+    T->h_rrow[e.first + 1]++;
+    T->h_rcol.push_back((uint32_t)e.second | ((uint32_t)code[e.second] << 29));
+  }
+  for (auto& e : F) {
+    T->h_prow[e.first + 1]++;
+    T->h_pcol.push_back((uint32_t)e.second | ((uint32_t)code[e.first] << 29));
+  }
+  for (int64_t i = 0; i < ncoarse; ++i) T->h_rrow[i + 1] += T->h_rrow[i];
+  for (int64_t i = 0; i < nfine; ++i) T->h_prow[i + 1] += T->h_prow[i];
+  int rc = upload(ctx, &T->r_row, T->h_rrow.data(), T->h_rrow.size());
+  if (!rc) rc = upload(ctx, &T->r_col, T->h_rcol.data(), T->h_rcol.size());
+  if (!rc) rc = upload(ctx, &T->p_row, T->h_prow.data(), T->h_prow.size());
+  if (!rc) rc = upload(ctx, &T->p_col, T->h_pcol.data(), T->h_pcol.size());
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
+  if (rc) {
+    gh_transfer_destroy(T);
+    return rc;
+  }
+  *out = T;
+  return GH_OK;
+}
+
+int gh_transfer_destroy(gh_transfer* T) {
+  if (!T) return GH_OK;
+  if (T->ctx && T->ctx->stream) (void)hipStreamSynchronize(T->ctx->stream);
+  for (void* p : {(void*)T->r_row, (void*)T->r_col, (void*)T->p_row, (void*)T->p_col})
+    if (p) (void)hipFree(p);
+  delete T;
+  return GH_OK;
+}
+
+int64_t gh_transfer_nfine(const gh_transfer* T) { return T ? T->nfine : -1; }
+
+int gh_restrict_zero(gh_ctx* ctx, const gh_transfer* T, gh_vec* uc, const gh_vec* uf) {
+  GH_ARG(ctx && T && uc && uf && uc->n == T->ncoarse && uf->n == T->nfine &&
+             uc->ncomp == uf->ncomp && uc->prec == uf->prec,
+         "gh_restrict_zero: shape mismatch");
+  return transfer_restrict(ctx, T, uc->ncomp, uc->prec, uc->d, uf->d, nullptr);
+}
+
+int gh_prolongate_add(gh_ctx* ctx, const gh_transfer* T, gh_vec* uf, const gh_vec* uc) {
+  GH_ARG(ctx && T && uc && uf && uc->n == T->ncoarse && uf->n == T->nfine &&
+             uc->ncomp == uf->ncomp && uc->prec == uf->prec,
+         "gh_prolongate_add: shape mismatch");
+  return transfer_prolongate_add(ctx, T, uf->ncomp, uf->prec, uf->d, uc->d);
+}
+
+// ------------------------------------------------------------ hanging nodes
+int gh_hn_create(gh_ctx* ctx, int kind, int64_t n, int64_t ne, const int32_t* edges,
+                 int64_t nf, const int32_t* faces, gh_hn** out) {
+  GH_ARG(ctx && out && (kind == 12 || kind == 13 || kind == 22 || kind == 23),
+         "gh_hn_create: kind must be 12, 13, 22 or 23");
+  GH_ARG(ne >= 0 && nf >= 0 && (ne == 0 || edges) && (nf == 0 || faces),
+         "gh_hn_create: bad lists");
+  GH_ARG(!((kind == 12 || kind == 22) && nf > 0), "gh_hn_create: 2D structures have no faces");
+  *out = nullptr;
+  const bool q2 = (kind == 22 || kind == 23);
+  // hanging set and parent check: the gather-form kernels need regular parents
+  std::vector<uint8_t> hang(n, 0);
+  auto chk = [&](int32_t v) { return v >= 0 && v < n; };
+  for (int64_t k = 0; k < ne; ++k) {
+    GH_ARG(chk(edges[4 * k]), "gh_hn_create: node out of range");
+    GH_ARG(!hang[edges[4 * k]], "gh_hn_create: hanging node listed twice");
+    hang[edges[4 * k]] = 1;
+  }
+  for (int64_t k = 0; k < nf; ++k) {
+    GH_ARG(chk(faces[10 * k]), "gh_hn_create: node out of range");
+    GH_ARG(!hang[faces[10 * k]], "gh_hn_create: hanging node listed twice");
+    hang[faces[10 * k]] = 1;
+  }
+  // contributions (parent, src, w) in the reference's sequential order
+  struct Contr {
+    int32_t p, h;
+    double w;
+  };
+  std::vector<Contr> cs;
+  const double q1w[2] = {0.5, 0.5}, q2w[3] = {0.375, 0.75, -0.125};
+  auto edge_contr = [&]() -> bool {
+    for (int64_t k = 0; k < ne; ++k) {
+      const int32_t* e = edges + 4 * k;
+      for (int j = 0; j < (q2 ? 3 : 2); ++j) {
+        if (!chk(e[1 + j]) || hang[e[1 + j]]) return false;
+        cs.push_back({e[1 + j], e[0], q2 ? q2w[j] : q1w[j]});
+      }
+    }
+    return true;
+  };
+  auto face_contr = [&]() -> bool {
+    for (int64_t k = 0; k < nf; ++k) {
+      const int32_t* f = faces + 10 * k;
+      if (kind == 23) {
+        for (int j = 0; j < 9; ++j) {
+          if (!chk(f[1 + j]) || hang[f[1 + j]]) return false;
+          cs.push_back({f[1 + j], f[0], q2w[j / 3] * q2w[j % 3]});
+        }
+      } else {
+        const int sel[4] = {0, 1, 3, 4};
+        for (int j : sel) {
+          if (!chk(f[1 + j]) || hang[f[1 + j]]) return false;
+          cs.push_back({f[1 + j], f[0], 0.25});
+        }
+      }
+    }
+    return true;
+  };
+  bool ok;
+  if (kind == 23)
+    ok = face_contr() && edge_contr();  // HNStructureQ23d: faces, then edges
+  else
+    ok = edge_contr() && face_contr();  // HNStructureQ13d: edges, then faces
+  GH_ARG(ok, "gh_hn_create: a parent is out of range or itself hanging");
+  // stable grouping by parent keeps the per-parent order
+  std::vector<int64_t> order(cs.size());
+  for (size_t i = 0; i < cs.size(); ++i) order[i] = (int64_t)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return cs[a].p < cs[b].p; });
+  std::vector<int32_t> parent, row(1, 0), src;
+  std::vector<double> w;
+  for (size_t i = 0; i < order.size(); ++i) {
+    const Contr& c = cs[order[i]];
+    if (parent.empty() || parent.back() != c.p) {
+      if (!parent.empty()) row.push_back((int32_t)src.size());
+      parent.push_back(c.p);
+    }
+    src.push_back(c.h);
+    w.push_back(c.w);
+  }
+  if (!parent.empty()) row.push_back((int32_t)src.size());
+  std::vector<int32_t> hanging;
+  for (int64_t k = 0; k < ne; ++k) hanging.push_back(edges[4 * k]);
+  for (int64_t k = 0; k < nf; ++k) hanging.push_back(faces[10 * k]);
+
+  gh_hn* H = new gh_hn();
+  H->ctx = ctx;
+  H->kind = kind;
+  H->n = n;
+  H->ne = ne;
+  H->nf = nf;
+  H->np = (int64_t)parent.size();
+  H->h_edges.assign(edges, edges + 4 * ne);
+  H->h_faces.assign(faces, faces + 10 * nf);
+  int rc = upload(ctx, &H->edges, edges, 4 * ne);
+  if (!rc) rc = upload(ctx, &H->faces, faces, 10 * nf);
+  if (!rc) rc = upload(ctx, &H->d_parent, parent.data(), parent.size());
+  if (!rc) rc = upload(ctx, &H->d_row, row.data(), row.size());
+  if (!rc) rc = upload(ctx, &H->d_src, src.data(), src.size());
+  if (!rc) rc = upload(ctx, &H->d_w, w.data(), w.size());
+  if (!rc) rc = upload(ctx, &H->hanging, hanging.data(), hanging.size());
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
+  if (rc) {
+    gh_hn_destroy(H);
+    return rc;
+  }
+  *out = H;
+  return GH_OK;
+}
+
+int gh_hn_destroy(gh_hn* H) {
+  if (!H) return GH_OK;
+  if (H->ctx && H->ctx->stream) (void)hipStreamSynchronize(H->ctx->stream);
+  for (void* p : {(void*)H->edges, (void*)H->faces, (void*)H->d_parent, (void*)H->d_row,
+                  (void*)H->d_src, (void*)H->d_w, (void*)H->hanging})
+    if (p) (void)hipFree(p);
+  delete H;
+  return GH_OK;
+}
+
+static int hn_check(const gh_hn* H, const gh_vec* x) { return H && x && x->n == H->n; }
+
+int gh_hn_average(gh_ctx* ctx, const gh_hn* H, gh_vec* x) {
+  GH_ARG(ctx && hn_check(H, x), "gh_hn_average: shape mismatch");
+  return hn_apply(ctx, H, x->ncomp, x->prec, x->d, 0);
+}
+int gh_hn_distribute(gh_ctx* ctx, const gh_hn* H, gh_vec* x) {
+  GH_ARG(ctx && hn_check(H, x), "gh_hn_distribute: shape mismatch");
+  return hn_apply(ctx, H, x->ncomp, x->prec, x->d, 1);
+}
+int gh_hn_zero(gh_ctx* ctx, const gh_hn* H, gh_vec* x) {
+  GH_ARG(ctx && hn_check(H, x), "gh_hn_zero: shape mismatch");
+  return hn_apply(ctx, H, x->ncomp, x->prec, x->d, 2);
+}
+
+// ---------------------------------------------------------------- Dirichlet
+int gh_mask_create(gh_ctx* ctx, int64_t n, int ncomp, int64_t nidx, const int32_t* idx,
+                   int ncomps, const int32_t* comps, gh_mask** out) {
+  GH_ARG(ctx && out && n >= 0 && ncomp >= 1 && nidx >= 0 && ncomps >= 0 &&
+             (nidx == 0 || idx) && (ncomps == 0 || comps),
+         "gh_mask_create: bad arguments");
+  *out = nullptr;
+  std::vector<int32_t> flat;
+  flat.reserve(nidx * ncomps);
+  for (int64_t i = 0; i < nidx; ++i) {
+    GH_ARG(idx[i] >= 0 && idx[i] < n, "gh_mask_create: node out of range");
+    for (int k = 0; k < ncomps; ++k) {
+      GH_ARG(comps[k] >= 0 && comps[k] < ncomp, "gh_mask_create: component out of range");
+      flat.push_back((int32_t)(idx[i] * ncomp + comps[k]));
+    }
+  }
+  gh_mask* M = new gh_mask();
+  M->ctx = ctx;
+  M->n = n;
+  M->ncomp = ncomp;
+  M->nidx = nidx;
+  M->ncomps = ncomps;
+  M->h_nodes.assign(idx, idx + nidx);
+  M->h_comps.assign(comps, comps + ncomps);
+  int rc = upload(ctx, &M->idx, flat.data(), flat.size());
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
+  if (rc) {
+    gh_mask_destroy(M);
+    return rc;
+  }
+  *out = M;
+  return GH_OK;
+}
+
+int gh_mask_destroy(gh_mask* M) {
+  if (!M) return GH_OK;
+  if (M->ctx && M->ctx->stream) (void)hipStreamSynchronize(M->ctx->stream);
+  if (M->idx) (void)hipFree(M->idx);
+  delete M;
+  return GH_OK;
+}
+
+int gh_mask_zero(gh_ctx* ctx, const gh_mask* M, gh_vec* x) {
+  GH_ARG(ctx && M && x && x->n == M->n && x->ncomp == M->ncomp, "gh_mask_zero: shape mismatch");
+  return mask_apply(ctx, M, x->prec, x->d);
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+// Internal definitions of the gascoigne_hip library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "gascoigne_hip.h"
+
+namespace gh {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+
+#define GH_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::gh::fail(GH_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,       \
+                        hipGetErrorString(e_));                                   \
+  } while (0)
+
+#define GH_TRY(expr)        \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_ != GH_OK) return rc_; \
+  } while (0)
+
+#define GH_ARG(cond, msg)                                  \
+  do {                                                     \
+    if (!(cond)) return ::gh::fail(GH_EINVAL, "%s", msg); \
+  } while (0)
+
+// launch geometry
+constexpr int kBlock = 256;       // 4 wavefronts of 64
+constexpr int kRedBlocks = 1024;  // fixed grid of every reduction: deterministic sums
+
+inline unsigned grid_for(int64_t work, int per_block = kBlock) {
+  int64_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// number of device scalar slots for reduction results
+constexpr int kScalars = 512;
+
+}  // namespace gh
+
+struct gh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* partials = nullptr;      // kRedBlocks * 4 doubles of block partials
+  unsigned* counter = nullptr;     // last-block tickets (one per slot group)
+  double* scal = nullptr;          // device result slots
+  double* host_scal = nullptr;     // pinned mirror of scal
+  int num_cu = 256;
+};
+
+struct gh_vec {
+  gh_ctx* ctx = nullptr;
+  int64_t n = 0;      // nodes
+  int ncomp = 1;
+  int prec = GH_F64;  // GH_F64 or GH_F32
+  int64_t size = 0;   // n * ncomp
+  void* d = nullptr;
+  bool owned = true;
+};
+
+struct gh_csr {
+  gh_ctx* ctx = nullptr;
+  int64_t n = 0;       // block rows
+  int ncomp = 1;
+  int64_t nnz = 0;     // blocks
+  int prec = GH_F64;   // value storage
+  int32_t* rowptr = nullptr;
+  int32_t* col = nullptr;
+  void* val = nullptr;   // nnz*B*B in prec (double / float / half)
+  void* dinv = nullptr;  // n*B*B inverse diagonal blocks: double (F64) or float
+  int seg = 8;           // lanes per row in the scalar kernels
+  // host copies kept for derived structures (transpose, coarse direct solve)
+  std::vector<int32_t> h_rowptr, h_col;
+  std::vector<double> h_val;
+  // transposed structure for gh_spmv_t (built on first use)
+  int32_t* t_rowptr = nullptr;
+  int32_t* t_col = nullptr;
+  int32_t* t_pos = nullptr;  // position of the transposed block in val
+};
+
+struct gh_transfer {
+  gh_ctx* ctx = nullptr;
+  int64_t ncoarse = 0, nfine = 0;
+  // restriction: coarse rows; columns are fine nodes, weight code in the top
+  // 3 bits (w = 2^-code), sorted ascending (MgInterpolatorMatrix STcoarse)
+  int32_t* r_row = nullptr;
+  uint32_t* r_col = nullptr;
+  // prolongation: fine rows; columns are coarse parents (STfine)
+  int32_t* p_row = nullptr;
+  uint32_t* p_col = nullptr;
+  std::vector<int32_t> h_rrow, h_prow;
+  std::vector<uint32_t> h_rcol, h_pcol;
+};
+
+struct gh_hn {
+  gh_ctx* ctx = nullptr;
+  int kind = 13;
+  int64_t n = 0;
+  int64_t ne = 0, nf = 0;
+  int32_t* edges = nullptr;  // ne x 4
+  int32_t* faces = nullptr;  // nf x 10
+  // distribute in gather form: per parent, contributions in reference order
+  int64_t np = 0;
+  int32_t* d_parent = nullptr;  // np parents
+  int32_t* d_row = nullptr;     // np+1
+  int32_t* d_src = nullptr;     // hanging node per contribution
+  double* d_w = nullptr;        // weight per contribution
+  int32_t* hanging = nullptr;   // ne+nf hanging nodes
+  std::vector<int32_t> h_edges, h_faces;
+};
+
+struct gh_mask {
+  gh_ctx* ctx = nullptr;
+  int64_t n = 0;
+  int ncomp = 1;
+  int64_t nidx = 0;
+  int ncomps = 0;
+  int32_t* idx = nullptr;      // nidx * ncomps flat entries (node*ncomp+comp)
+  std::vector<int32_t> h_nodes, h_comps;
+};
+
+namespace gh {
+// vector helpers shared by the solver translation unit
+int vec_alloc(gh_ctx* ctx, int64_t n, int ncomp, int prec, gh_vec* v);
+void vec_free(gh_vec* v);
+int reduce_dot(gh_ctx* ctx, int prec, int64_t size, const void* x, const void* y, int slot);
+int reduce_norm_l8(gh_ctx* ctx, int prec, int64_t size, const void* x, int slot);
+int fetch_scalars(gh_ctx* ctx, int first, int count, double* out);
+int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const void* x,
+             double d, double b);
+int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b);
+int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const void* x,
+                     const void* b, double omega, int smoother, bool x_zero);
+int vec_axpby(gh_ctx* ctx, int prec, int64_t size, void* y, double a, double d, const void* x);
+int vec_scale_to(gh_ctx* ctx, int prec, int64_t size, void* y, double s, const void* x);
+int vec_convert(gh_ctx* ctx, int64_t size, int dprec, void* dst, int sprec, const void* src);
+int vec_fill_zero(gh_ctx* ctx, int prec, int64_t size, void* x);
+int transfer_restrict(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, void* uc,
+                      const void* uf, const int32_t* zero_rows_mask);
+int transfer_prolongate_add(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, void* uf,
+                            const void* uc);
+int hn_apply(gh_ctx* ctx, const gh_hn* H, int ncomp, int prec, void* x, int op);
+int mask_apply(gh_ctx* ctx, const gh_mask* M, int prec, void* x);
+int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
+             const double* next, int oslot);
+}  // namespace gh

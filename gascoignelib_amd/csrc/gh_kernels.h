@@ -1,0 +1,502 @@
+// Device kernels of the gascoignelib_amd solve path (gfx950, wave64).
+//
+// All kernels are bandwidth kernels: no MFMA.  Arithmetic types:
+//   Acc = double on the FP64 path, float on the mixed path;
+//   matrix values may be stored as double, float or __half.
+#pragma once
+
+#include "gh_internal.h"
+
+namespace gh {
+
+template <typename T> __device__ __forceinline__ T to_acc(double v) { return (T)v; }
+
+__device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); }
+__device__ __forceinline__ float ld(const float* p) { return *p; }
+__device__ __forceinline__ double ld(const double* p) { return *p; }
+
+template <typename T> __device__ __forceinline__ T mul_rn(T a, T b);
+template <> __device__ __forceinline__ double mul_rn<double>(double a, double b) { return __dmul_rn(a, b); }
+template <> __device__ __forceinline__ float mul_rn<float>(float a, float b) { return __fmul_rn(a, b); }
+template <typename T> __device__ __forceinline__ T add_rn(T a, T b);
+template <> __device__ __forceinline__ double add_rn<double>(double a, double b) { return __dadd_rn(a, b); }
+template <> __device__ __forceinline__ float add_rn<float>(float a, float b) { return __fadd_rn(a, b); }
+
+// weight code of packed transfer columns: w = 2^-code (1, 1/2, 1/4, 1/8)
+constexpr uint32_t kIdxMask = 0x1FFFFFFFu;
+template <typename T> __device__ __forceinline__ T code_weight(uint32_t c) {
+  return c == 0 ? (T)1.0 : c == 1 ? (T)0.5 : c == 2 ? (T)0.25 : (T)0.125;
+}
+
+// ------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// block of kBlock threads; result valid in thread 0
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double v) {
+  __shared__ double s[kBlock / 64];
+  v = MAX ? wave_max(v) : wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    r = s[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) r = MAX ? fmax(r, s[w]) : r + s[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Each block stores its partial; the last block to finish folds the partials
+// in a fixed order and writes the result slot.  Same grid every call, so the
+// result is bitwise reproducible.
+template <bool MAX>
+__device__ __forceinline__ void finish_reduction(double part, double* partials,
+                                                 unsigned* counter, double* out) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = part;
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    double a = 0.0;
+    const volatile double* vp = partials;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+      a = MAX ? fmax(a, vp[i]) : a + vp[i];
+    a = block_reduce<MAX>(a);
+    if (threadIdx.x == 0) {
+      *out = a;
+      atomicExch(counter, 0u);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T* __restrict__ x,
+                                                const T* __restrict__ y, double* partials,
+                                                unsigned* counter, double* out) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    acc += (double)x[i] * (double)y[i];
+  acc = block_reduce<false>(acc);
+  finish_reduction<false>(acc, partials, counter, out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amax(int64_t n, const T* __restrict__ x,
+                                                 double* partials, unsigned* counter,
+                                                 double* out) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    acc = fmax(acc, fabs((double)x[i]));
+  acc = block_reduce<true>(acc);
+  finish_reduction<true>(acc, partials, counter, out);
+}
+
+// ----------------------------------------------------------- vector kernels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, T* __restrict__ y, T a, T d,
+                                                  const T* __restrict__ x, int mode) {
+  // mode 0: y = a*y + d*x ; 1: y += d*x ; 2: y = d*x
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    if (mode == 0)
+      y[i] = a * y[i] + d * x[i];
+    else if (mode == 1)
+      y[i] += d * x[i];
+    else
+      y[i] = d * x[i];
+  }
+}
+
+template <typename D, typename S>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t n, D* __restrict__ dst,
+                                                    const S* __restrict__ src) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    dst[i] = (D)src[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill0(int64_t n, T* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = (T)0;
+}
+
+// --------------------------------------------------------------- SpMV (B=1)
+// Rows are mapped to segments of SEG lanes of a wavefront; a segment streams
+// its row's (col,val) pairs with unit stride and folds them with xor shuffles.
+template <typename Y, typename Acc>
+struct EpiSpmv {
+  Y* y;
+  Acc d, b;
+  bool read_y;
+  __device__ __forceinline__ void operator()(int64_t r, Acc s) const {
+    Acc v = d * s;
+    if (read_y) v += b * (Acc)y[r];
+    y[r] = (Y)v;
+  }
+};
+
+template <typename T>
+struct EpiResidual {  // r = b - A x   (MatrixResidual)
+  T* r;
+  const T* b;
+  __device__ __forceinline__ void operator()(int64_t i, T s) const { r[i] = b[i] - s; }
+};
+
+template <typename T, typename DT>
+struct EpiJacobi {  // xn = x + omega * (dinv * (b - A x))   (StdSolver::smooth jacobi)
+  T* xn;
+  const T* x;
+  const T* b;
+  const DT* dinv;
+  T omega;
+  __device__ __forceinline__ void operator()(int64_t i, T s) const {
+    T h = b[i] - s;
+    h = (T)dinv[i] * h;
+    xn[i] = x[i] + omega * h;
+  }
+};
+
+template <typename T>
+struct EpiRichardson {  // xn = x + omega * (b - A x)
+  T* xn;
+  const T* x;
+  const T* b;
+  T omega;
+  __device__ __forceinline__ void operator()(int64_t i, T s) const {
+    xn[i] = x[i] + omega * (b[i] - s);
+  }
+};
+
+template <int SEG, typename VT, typename XT, typename Acc, class Epi>
+__global__ __launch_bounds__(kBlock) void k_csr_seg(int64_t n, const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const VT* __restrict__ val,
+                                                    const XT* __restrict__ x, Epi epi) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = t / SEG;
+  const int lane = threadIdx.x & (SEG - 1);
+  Acc sum = (Acc)0;
+  if (row < n) {
+    const int s = rowptr[row], e = rowptr[row + 1];
+    for (int k = s + lane; k < e; k += SEG) sum += (Acc)ld(val + k) * (Acc)x[col[k]];
+  }
+#pragma unroll
+  for (int off = SEG / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, SEG);
+  if (lane == 0 && row < n) epi(row, sum);
+}
+
+// first Jacobi / Richardson sweep from x = 0: xn = omega * dinv * b
+template <typename T, typename DT>
+__global__ __launch_bounds__(kBlock) void k_jacobi_zero(int64_t n, T* __restrict__ xn,
+                                                        const T* __restrict__ b,
+                                                        const DT* __restrict__ dinv, T omega,
+                                                        int richardson) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    T h = b[i] - (T)0;
+    if (!richardson) h = (T)dinv[i] * h;
+    xn[i] = (T)0 + omega * h;
+  }
+}
+
+// ------------------------------------------------------------ block SpMV (B>1)
+// One thread per block row component: y(i,r) = sum_pos sum_c a(pos)(r,c) x(col,c)
+template <typename VT, typename XT, typename Acc, class Epi>
+__global__ __launch_bounds__(kBlock) void k_bsr(int64_t n, int B, const int32_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ col,
+                                                const VT* __restrict__ val,
+                                                const XT* __restrict__ x, Epi epi) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * B) return;
+  const int64_t i = t / B;
+  const int r = (int)(t - i * B);
+  Acc total = (Acc)0;
+  for (int pos = rowptr[i]; pos < rowptr[i + 1]; ++pos) {
+    const VT* m = val + (int64_t)pos * B * B + (int64_t)r * B;
+    const XT* q = x + (int64_t)col[pos] * B;
+    Acc sum = (Acc)0;
+    for (int c = 0; c < B; ++c) sum += (Acc)ld(m + c) * (Acc)q[c];
+    total += sum;
+  }
+  epi(t, total);
+}
+
+// block Jacobi: xn(i) = x(i) + omega * Dinv_i (b(i) - (A x)(i)); one thread per row
+template <typename VT, typename T, typename DT>
+__global__ __launch_bounds__(kBlock) void k_bsr_jacobi(int64_t n, int B,
+                                                       const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const VT* __restrict__ val,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ b,
+                                                       const DT* __restrict__ dinv,
+                                                       T* __restrict__ xn, T omega, int x_zero) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  T h[8];
+  for (int r = 0; r < B; ++r) h[r] = b[i * B + r];
+  if (!x_zero) {
+    for (int pos = rowptr[i]; pos < rowptr[i + 1]; ++pos) {
+      const VT* m = val + (int64_t)pos * B * B;
+      const T* q = x + (int64_t)col[pos] * B;
+      for (int r = 0; r < B; ++r) {
+        T sum = (T)0;
+        for (int c = 0; c < B; ++c) sum += (T)ld(m + r * B + c) * q[c];
+        h[r] -= sum;
+      }
+    }
+  }
+  const DT* D = dinv + i * B * B;
+  for (int r = 0; r < B; ++r) {
+    T s = (T)0;
+    for (int c = 0; c < B; ++c) s += (T)D[r * B + c] * h[c];
+    xn[i * B + r] = (x_zero ? (T)0 : x[i * B + r]) + omega * s;
+  }
+}
+
+// block-diagonal inverse by Gauss-Jordan with partial pivoting, one thread per
+// node: NodeMatrix::gauss_jordan (src/LinAlg/nodematrix.h:159-213), the device
+// counterpart of invert_device (cuda/cusparsehelper.cu:24-118).
+template <typename DT>
+__global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n, int B,
+                                                          const int32_t* __restrict__ diagpos,
+                                                          const double* __restrict__ val,
+                                                          DT* __restrict__ dinv,
+                                                          int* __restrict__ singular) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double m[64];
+  int p[8];
+  const double* src = val + (int64_t)diagpos[i] * B * B;
+  for (int k = 0; k < B * B; ++k) m[k] = src[k];
+#define M_(a, c) m[(a) * B + (c)]
+  for (int k = 0; k < B; ++k) p[k] = k;
+  for (int j = 0; j < B; ++j) {
+    double mx = fabs(M_(j, j));
+    int r = j;
+    for (int k = j + 1; k < B; ++k)
+      if (fabs(M_(k, j)) > mx) {
+        mx = fabs(M_(k, j));
+        r = k;
+      }
+    if (r > j) {
+      for (int k = 0; k < B; ++k) {
+        double t = M_(j, k);
+        M_(j, k) = M_(r, k);
+        M_(r, k) = t;
+      }
+      int t = p[j];
+      p[j] = p[r];
+      p[r] = t;
+    }
+    if (M_(j, j) == 0.0) {
+      atomicExch(singular, 1);
+      return;
+    }
+    double hr = 1. / M_(j, j);
+    M_(j, j) = hr;
+    for (int k = 0; k < B; ++k) {
+      if (k == j) continue;
+      for (int a = 0; a < B; ++a) {
+        if (a == j) continue;
+        M_(a, k) -= M_(a, j) * M_(j, k) * hr;
+      }
+    }
+    for (int a = 0; a < B; ++a) {
+      M_(a, j) *= hr;
+      M_(j, a) *= -hr;
+    }
+    M_(j, j) = hr;
+  }
+  double hv[8];
+  for (int a = 0; a < B; ++a) {
+    for (int k = 0; k < B; ++k) hv[p[k]] = M_(a, k);
+    for (int k = 0; k < B; ++k) dinv[i * B * B + a * B + k] = (DT)hv[k];
+  }
+#undef M_
+}
+
+// --------------------------------------------------------------- transfers
+// uc(i,c) = sum_pos w * uf(col,c)  (restrict_zero); rows in rowmask != 0 are
+// written as zero (strong Dirichlet rows folded into the restriction).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(int64_t nc, int ncomp,
+                                                     const int32_t* __restrict__ row,
+                                                     const uint32_t* __restrict__ col,
+                                                     const T* __restrict__ uf,
+                                                     T* __restrict__ uc) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nc * ncomp) return;
+  const int64_t i = t / ncomp;
+  const int c = (int)(t - i * ncomp);
+  T acc = (T)0;
+  for (int pos = row[i]; pos < row[i + 1]; ++pos) {
+    const uint32_t e = col[pos];
+    acc = add_rn(acc, mul_rn(code_weight<T>(e >> 29), uf[(int64_t)(e & kIdxMask) * ncomp + c]));
+  }
+  uc[t] = acc;
+}
+
+// uf(i,c) += sum_pos w * uc(col,c)  (prolongate_add, term by term)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prolongate_add(int64_t nf, int ncomp,
+                                                           const int32_t* __restrict__ row,
+                                                           const uint32_t* __restrict__ col,
+                                                           const T* __restrict__ uc,
+                                                           T* __restrict__ uf) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nf * ncomp) return;
+  const int64_t i = t / ncomp;
+  const int c = (int)(t - i * ncomp);
+  T acc = uf[t];
+  for (int pos = row[i]; pos < row[i + 1]; ++pos) {
+    const uint32_t e = col[pos];
+    acc = add_rn(acc, mul_rn(code_weight<T>(e >> 29), uc[(int64_t)(e & kIdxMask) * ncomp + c]));
+  }
+  uf[t] = acc;
+}
+
+// mgstep's coarse-grid correction in one pass (stdmultilevelsolver.cc:513-524):
+//   v = P u_c (rows of hanging / Dirichlet fine nodes removed = HNZero, SetBoundaryVectorZero)
+//   u += mgomega * v
+template <typename T, typename UT>
+__global__ __launch_bounds__(kBlock) void k_prolongate_correct(int64_t nf, int ncomp,
+                                                               const int32_t* __restrict__ row,
+                                                               const uint32_t* __restrict__ col,
+                                                               const T* __restrict__ uc,
+                                                               UT* __restrict__ u, T mgomega) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nf * ncomp) return;
+  const int64_t i = t / ncomp;
+  const int c = (int)(t - i * ncomp);
+  const int s = row[i], e = row[i + 1];
+  if (s == e) return;
+  T v = (T)0;
+  for (int pos = s; pos < e; ++pos) {
+    const uint32_t q = col[pos];
+    v = add_rn(v, mul_rn(code_weight<T>(q >> 29), uc[(int64_t)(q & kIdxMask) * ncomp + c]));
+  }
+  u[t] = (UT)((T)u[t] + mgomega * v);
+}
+
+// -------------------------------------------------------------- hanging nodes
+// HNStructure*::Average, parents are regular nodes (checked at creation), so
+// every hanging node is independent.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hn_average(int kind, int64_t ne, const int32_t* __restrict__ edges,
+                                                       int64_t nf, const int32_t* __restrict__ faces,
+                                                       int ncomp, T* __restrict__ u) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t ntot = (ne + nf) * ncomp;
+  if (t >= ntot) return;
+  const int64_t k = t / ncomp;
+  const int c = (int)(t - k * ncomp);
+  const bool q2 = (kind == 22 || kind == 23);
+  auto U = [&](int32_t node) { return u[(int64_t)node * ncomp + c]; };
+  if (k < ne) {
+    const int32_t* e = edges + 4 * k;
+    T r;
+    if (q2)
+      r = add_rn(add_rn(mul_rn((T)0.375, U(e[1])), mul_rn((T)0.75, U(e[2]))),
+                 mul_rn((T)-0.125, U(e[3])));
+    else
+      r = add_rn(mul_rn((T)0.5, U(e[1])), mul_rn((T)0.5, U(e[2])));
+    u[(int64_t)e[0] * ncomp + c] = r;
+  } else {
+    const int32_t* f = faces + 10 * (k - ne);
+    const int32_t* g = f + 1;
+    T r;
+    if (kind == 23) {
+      const T w[3] = {(T)0.375, (T)0.75, (T)-0.125};
+      T fw[9];
+      for (int a = 0; a < 3; ++a)
+        for (int b2 = 0; b2 < 3; ++b2) fw[3 * a + b2] = (T)((double)w[a] * (double)w[b2]);
+      r = add_rn(add_rn(mul_rn(fw[0], U(g[0])), mul_rn(fw[1], U(g[1]))), mul_rn(fw[2], U(g[2])));
+      r = add_rn(r, add_rn(add_rn(mul_rn(fw[3], U(g[3])), mul_rn(fw[4], U(g[4]))),
+                           mul_rn(fw[5], U(g[5]))));
+      r = add_rn(r, add_rn(add_rn(mul_rn(fw[6], U(g[6])), mul_rn(fw[7], U(g[7]))),
+                           mul_rn(fw[8], U(g[8]))));
+    } else {
+      r = add_rn(add_rn(add_rn(mul_rn((T)0.25, U(g[0])), mul_rn((T)0.25, U(g[1]))),
+                        mul_rn((T)0.25, U(g[3]))),
+                 mul_rn((T)0.25, U(g[4])));
+    }
+    u[(int64_t)f[0] * ncomp + c] = r;
+  }
+}
+
+// HNStructure*::Distribute in gather form: each parent adds its contributions
+// in the order the reference's sequential loop would (edges/faces by key).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hn_gather(int64_t np, int ncomp,
+                                                      const int32_t* __restrict__ parent,
+                                                      const int32_t* __restrict__ row,
+                                                      const int32_t* __restrict__ src,
+                                                      const double* __restrict__ w,
+                                                      T* __restrict__ u) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= np * ncomp) return;
+  const int64_t k = t / ncomp;
+  const int c = (int)(t - k * ncomp);
+  const int64_t p = (int64_t)parent[k] * ncomp + c;
+  T acc = u[p];
+  for (int j = row[k]; j < row[k + 1]; ++j)
+    acc = add_rn(acc, mul_rn((T)w[j], u[(int64_t)src[j] * ncomp + c]));
+  u[p] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_nodes(int64_t m, int ncomp,
+                                                       const int32_t* __restrict__ nodes,
+                                                       T* __restrict__ u) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m * ncomp) return;
+  const int64_t k = t / ncomp;
+  u[(int64_t)nodes[k] * ncomp + (t - k * ncomp)] = (T)0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_flat(int64_t m, const int32_t* __restrict__ idx,
+                                                      T* __restrict__ u) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  u[idx[t]] = (T)0;
+}
+
+// ------------------------------------------------------- dense coarse solve
+// x = Ainv * b with the FP64 inverse of a direct level; one wavefront per row.
+template <typename BT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_dense_solve(int64_t m, const double* __restrict__ Ainv,
+                                                        const BT* __restrict__ b,
+                                                        XT* __restrict__ x) {
+  const int64_t r = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= m) return;
+  double acc = 0.0;
+  for (int64_t c = lane; c < m; c += 64) acc += Ainv[r * m + c] * (double)b[c];
+  acc = wave_sum(acc);
+  if (lane == 0) x[r] = (XT)acc;
+}
+
+}  // namespace gh

@@ -1,0 +1,982 @@
+// Multigrid V/W/F cycle, left-preconditioned GMRES and the Newton/iterative
+// refinement driver of the gascoignelib_amd C ABI.
+//
+// The control flow follows StdMultiLevelSolver::LinearMg / mgstep
+// (src/Solver/stdmultilevelsolver.cc:411-531), GMRES::solve
+// (src/LinAlg/gmres.cc:160-260) and StdMultiLevelSolver::newton
+// (stdmultilevelsolver.cc:548-758); every vector operation runs on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gh_kernels.h"
+
+using namespace gh;
+
+// ===================================================================== info
+extern "C" {
+
+void gh_cginfo_init(gh_cginfo* I, double tol, double globaltol, int maxiter) {
+  std::memset(I, 0, sizeof(*I));
+  I->tol = tol;
+  I->globaltol = globaltol;
+  I->maxiter = maxiter;
+  I->breaktol = 1.e15;
+  gh_cginfo_reset(I);
+}
+
+// CGInfo::reset (src/Common/cginfo.cc:193-199) -> ControlData::reset (:106-115)
+void gh_cginfo_reset(gh_cginfo* I) {
+  I->residual = 0.;
+  I->correction = 0.;
+  I->iteration = 0;
+  I->status = GH_WAITING;
+  I->previousresidual = 1.;
+  I->previouscorrection = 1.;
+  I->rate = 0.;
+  I->lastrate = 0.;
+}
+
+// CGInfo::check (src/Common/cginfo.cc:202-256)
+int gh_cginfo_check(gh_cginfo* I, double resi, double cori) {
+  const double res = std::fabs(resi), cor = std::fabs(cori);
+  I->residual = res;
+  I->correction = cor;
+  if (I->status != GH_RUNNING) {
+    I->firstresidual = res;
+    I->previousresidual = res;
+    I->previouscorrection = cor;
+    I->aimedresidual = res * I->tol;
+    I->status = GH_RUNNING;
+  } else {
+    I->totaliter++;
+    I->lastrate = res / I->previousresidual;
+    I->rate = std::pow(I->residual / I->firstresidual, 1. / (double)std::max(1, I->iteration));
+    I->previousresidual = res;
+  }
+  if (I->residual < std::max(I->aimedresidual, I->globaltol))
+    I->status = GH_CONVERGED;
+  else if (I->iteration >= I->maxiter)
+    I->status = GH_TOO_MANY;
+  else if (I->residual > I->breaktol)
+    I->status = GH_EXPLODED;
+  else if (I->residual > I->previousresidual)
+    I->status = GH_DIVERGING;
+  I->iteration++;
+  if (I->status == GH_RUNNING) return 0;
+  if (I->iteration < I->miniter) return 0;
+  return 1;
+}
+
+// NLInfo (src/Common/nlinfo.cc:100-356)
+void gh_nlinfo_init(gh_nlinfo* I, double tol, double globaltol, int maxiter) {
+  std::memset(I, 0, sizeof(*I));
+  I->tol = tol;
+  I->globaltol = globaltol;
+  I->maxiter = maxiter;
+  I->breaktol = 1.e15;
+  I->maxrelax = 11;
+  I->rho = 0.3;
+  I->linrho = 1.;
+  I->maxresincrease = 1.e3;
+  gh_nlinfo_reset(I);
+}
+
+void gh_nlinfo_reset(gh_nlinfo* I) {
+  I->rate = 0.;
+  I->residual = 0.;
+  I->correction = 0.;
+  I->iteration = 0;
+  I->status = GH_NL_WAITING;
+  I->previousresidual = 1.;
+  I->previouscorrection = 1.;
+  I->relax = 0;
+  I->laststepbad = 0;
+}
+
+int gh_nlinfo_check(gh_nlinfo* I, int iter, double resi, double cori) {
+  const double res = std::fabs(resi), cor = std::fabs(cori);
+  int newiteration = 0;
+  if (I->status == GH_NL_DIVERGED || I->status == GH_NL_STAGNATION) return 1;
+  if (I->status == GH_NL_WAITING) {
+    newiteration = 1;
+  } else if (iter > I->iteration) {
+    newiteration = 1;
+    I->iteration++;
+  }
+  I->status = GH_NL_RUNNING;
+  int thisstepbad = 0;
+  if (!I->iteration) {
+    I->residual = res;
+    I->previousresidual = res;
+    I->correction = cor;
+    I->firstresidual = res;
+    I->aimedresidual = res * I->tol;
+  } else {
+    const double r0 = I->residual, c0 = I->correction;
+    I->residual = res;
+    I->correction = cor;
+    I->rate = std::pow(I->residual / I->firstresidual, 1. / (double)std::max(1, I->iteration));
+    I->lastrate = I->iteration > 1 ? I->residual / I->previousresidual : I->rate;
+    if (I->residual > 2. * I->previousresidual) thisstepbad = 1;
+    if (newiteration) {
+      I->previousresidual = r0;
+      I->previouscorrection = c0;
+    }
+  }
+  if (thisstepbad && I->laststepbad)
+    I->status = GH_NL_DIVERGED;
+  else if (I->residual < std::max(I->aimedresidual, I->globaltol))
+    I->status = GH_NL_CONVERGED;
+  else if (I->iteration >= I->maxiter && I->rate < 1.)
+    I->status = GH_NL_SLOW;
+  else if (I->iteration >= I->maxiter)
+    I->status = GH_NL_STAGNATION;
+  else if (I->residual > I->breaktol)
+    I->status = GH_NL_EXPLODED;
+  else if (I->iteration && I->lastrate > I->maxresincrease)
+    I->status = GH_NL_EXPLODED;
+  I->laststepbad = thisstepbad;
+  if (I->status == GH_NL_RUNNING) return 0;
+  if (I->iteration < I->miniter) return 0;
+  I->totaliter += I->iteration;
+  return 1;
+}
+
+// NLInfo::check_damping (nlinfo.cc:223-238): 0 ok, 1 continue, 2 exploded
+static int nl_check_damping(gh_nlinfo* I, int dampit, double res) {
+  I->residual = std::fabs(res);
+  I->relax = dampit;
+  if (I->residual > 1.e20) return 2;
+  if (!(I->residual <= 1.e20)) return 2;
+  if (I->residual <= I->previousresidual) return 0;
+  return 1;
+}
+
+void gh_mg_params_default(gh_mg_params* p) {
+  p->mgtype = 'V';
+  p->iterpre = 4;
+  p->iterpost = 4;
+  p->iterexact = 10;
+  p->omega = 1.;
+  p->mgomega = 1.;
+  p->smoother = 0;
+  p->ndirect = 1000;
+  p->prec_maxiter = 1;
+  p->prec_tol = 1.e-12;
+  p->prec_globaltol = 1.e-12;
+  p->prec = GH_F64;
+  p->use_graph = 0;
+}
+
+}  // extern "C"
+
+// ================================================================ multigrid
+namespace {
+
+struct MgLevel {
+  gh_csr* A64 = nullptr;   // caller's FP64 matrix
+  gh_csr* A = nullptr;     // matrix of the V-cycle (A64 on the FP64 path)
+  bool own_A = false;
+  gh_transfer* T = nullptr;
+  gh_hn* H = nullptr;
+  gh_mask* M = nullptr;
+  int64_t n = 0, size = 0;
+  int ncomp = 1;
+  // restriction into level l-1 with its strong Dirichlet rows emptied
+  int32_t* r_row = nullptr;
+  uint32_t* r_col = nullptr;
+  // prolongation from level l-1 with hanging / Dirichlet rows of this level emptied
+  int32_t* p_row = nullptr;
+  uint32_t* p_col = nullptr;
+  bool fused_correction = true;  // mask covers every component
+  bool mask_in_restrict = true;  // coarse Dirichlet rows emptied in r_row (set on level l)
+  void* u = nullptr;
+  void* b = nullptr;
+  void* v = nullptr;
+  bool direct = false;
+  double* Ainv = nullptr;
+};
+
+}  // namespace
+
+struct gh_mg {
+  gh_ctx* ctx = nullptr;
+  int nlevels = 0;
+  gh_mg_params P;
+  int wprec = GH_F64;  // working precision of vectors
+  std::vector<MgLevel> L;
+  int64_t mgsteps = 0;
+  // GMRES memory (FP64, finest level)
+  std::vector<double*> mem;
+  double* p = nullptr;
+  double* r = nullptr;
+  double* w = nullptr;
+  // fine-level staging for graph replay / precision change
+  void* fine_b = nullptr;
+  // captured V-cycle (fast path from zero initial guess)
+  hipGraphExec_t graph = nullptr;
+  bool graph_ok = false;
+  int64_t graph_steps = 0;
+  void* graph_result = nullptr;
+};
+
+namespace {
+
+size_t pbytes(int prec) { return prec == GH_F64 ? 8 : 4; }
+
+int dalloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess)
+    return fail(GH_ENOMEM, "hipMalloc(%zu) failed", bytes);
+  return GH_OK;
+}
+
+template <typename T>
+int dupload(gh_ctx* ctx, T** d, const std::vector<T>& h) {
+  GH_TRY(dalloc((void**)d, h.size() * sizeof(T)));
+  if (!h.empty())
+    GH_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return GH_OK;
+}
+
+// dense inverse of a direct level's matrix (LU with partial pivoting, FP64)
+int dense_inverse(const gh_csr* A, std::vector<double>& inv) {
+  const int B = A->ncomp;
+  const int64_t m = A->n * B;
+  std::vector<double> a(m * m, 0.0);
+  for (int64_t i = 0; i < A->n; ++i)
+    for (int32_t pos = A->h_rowptr[i]; pos < A->h_rowptr[i + 1]; ++pos)
+      for (int r = 0; r < B; ++r)
+        for (int c = 0; c < B; ++c)
+          a[(i * B + r) * m + A->h_col[pos] * B + c] += A->h_val[(int64_t)pos * B * B + r * B + c];
+  inv.assign(m * m, 0.0);
+  for (int64_t i = 0; i < m; ++i) inv[i * m + i] = 1.0;
+  for (int64_t k = 0; k < m; ++k) {
+    int64_t p = k;
+    double mx = std::fabs(a[k * m + k]);
+    for (int64_t i = k + 1; i < m; ++i)
+      if (std::fabs(a[i * m + k]) > mx) {
+        mx = std::fabs(a[i * m + k]);
+        p = i;
+      }
+    if (mx == 0.0) return fail(GH_ESINGULAR, "direct coarse level: singular matrix");
+    if (p != k) {
+      for (int64_t j = 0; j < m; ++j) {
+        std::swap(a[k * m + j], a[p * m + j]);
+        std::swap(inv[k * m + j], inv[p * m + j]);
+      }
+    }
+    const double piv = a[k * m + k];
+    for (int64_t i = 0; i < m; ++i) {
+      if (i == k) continue;
+      const double l = a[i * m + k] / piv;
+      if (l == 0.0) continue;
+      for (int64_t j = k; j < m; ++j) a[i * m + j] -= l * a[k * m + j];
+      for (int64_t j = 0; j < m; ++j) inv[i * m + j] -= l * inv[k * m + j];
+    }
+    for (int64_t j = 0; j < m; ++j) inv[k * m + j] /= piv;
+    for (int64_t j = k; j < m; ++j) a[k * m + j] /= piv;
+  }
+  return GH_OK;
+}
+
+// ------------------------------------------------------------- V-cycle pieces
+int sweep(gh_mg* mg, MgLevel& L, int niter, bool& u_zero, void* keep_u) {
+  for (int it = 0; it < niter; ++it) {
+    GH_TRY(csr_jacobi_sweep(mg->ctx, L.A, mg->wprec, L.v, L.u, L.b, mg->P.omega,
+                            mg->P.smoother, u_zero));
+    u_zero = false;
+    std::swap(L.u, L.v);
+  }
+  (void)keep_u;
+  return GH_OK;
+}
+
+int exact(gh_mg* mg, MgLevel& L, bool u_zero) {
+  if (L.direct) {
+    const int64_t m = L.size;
+    unsigned grid = (unsigned)((m + 3) / 4);
+    if (mg->wprec == GH_F64)
+      hipLaunchKernelGGL((k_dense_solve<double, double>), dim3(grid), dim3(kBlock), 0,
+                         mg->ctx->stream, m, L.Ainv, (const double*)L.b, (double*)L.u);
+    else
+      hipLaunchKernelGGL((k_dense_solve<float, float>), dim3(grid), dim3(kBlock), 0,
+                         mg->ctx->stream, m, L.Ainv, (const float*)L.b, (float*)L.u);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
+  bool z = u_zero;
+  GH_TRY(sweep(mg, L, mg->P.iterexact, z, nullptr));
+  if (z) GH_TRY(vec_fill_zero(mg->ctx, mg->wprec, L.size, L.u));  // no sweep ran
+  return GH_OK;
+}
+
+int norm_now(gh_mg* mg, const void* x, int64_t size, double* out) {
+  GH_TRY(reduce_dot(mg->ctx, mg->wprec, size, x, x, 1));
+  double s;
+  GH_TRY(fetch_scalars(mg->ctx, 1, 1, &s));
+  *out = std::sqrt(s);
+  return GH_OK;
+}
+
+int correction(gh_mg* mg, int l) {
+  MgLevel& F = mg->L[l];
+  MgLevel& Cc = mg->L[l - 1];
+  gh_ctx* ctx = mg->ctx;
+  if (Cc.H) GH_TRY(hn_apply(ctx, Cc.H, Cc.ncomp, mg->wprec, Cc.u, 0));  // HNAverage(u_{l-1})
+  if (F.fused_correction) {
+    unsigned grid = grid_for(F.size);
+    if (mg->wprec == GH_F64)
+      hipLaunchKernelGGL((k_prolongate_correct<double, double>), dim3(grid), dim3(kBlock), 0,
+                         ctx->stream, F.n, F.ncomp, F.p_row, F.p_col, (const double*)Cc.u,
+                         (double*)F.u, mg->P.mgomega);
+    else
+      hipLaunchKernelGGL((k_prolongate_correct<float, float>), dim3(grid), dim3(kBlock), 0,
+                         ctx->stream, F.n, F.ncomp, F.p_row, F.p_col, (const float*)Cc.u,
+                         (float*)F.u, (float)mg->P.mgomega);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
+  // general path: v = 0; v += P u; HNZero(v); Dirichlet(v); u += mgomega v
+  GH_TRY(vec_fill_zero(ctx, mg->wprec, F.size, F.v));
+  GH_TRY(transfer_prolongate_add(ctx, F.T, F.ncomp, mg->wprec, F.v, Cc.u));
+  if (F.H) GH_TRY(hn_apply(ctx, F.H, F.ncomp, mg->wprec, F.v, 2));
+  if (F.M) GH_TRY(mask_apply(ctx, F.M, mg->wprec, F.v));
+  return vec_axpby(ctx, mg->wprec, F.size, F.u, 1.0, mg->P.mgomega, F.v);
+}
+
+// StdMultiLevelSolver::mgstep.  need_res: compute the residual norms the
+// reference computes for CGInfo (skipped when one V-cycle is all the caller
+// runs: they cannot change the result).
+int mgstep(gh_mg* mg, int l, int finelevel, int clevel, char& p0, char p, bool u_zero,
+           bool need_res, std::vector<double>& res, std::vector<double>& rw) {
+  mg->mgsteps++;
+  gh_ctx* ctx = mg->ctx;
+  MgLevel& L = mg->L[l];
+  if (l == clevel) {
+    if (p == 'F') p0 = 'V';
+    GH_TRY(exact(mg, L, u_zero));
+    if (l == finelevel && need_res) {
+      GH_TRY(csr_residual(ctx, L.A, mg->wprec, L.v, L.u, L.b));
+      GH_TRY(norm_now(mg, L.v, L.size, &res[l]));
+    }
+    return GH_OK;
+  }
+  bool z = u_zero;
+  GH_TRY(sweep(mg, L, mg->P.iterpre, z, nullptr));
+  if (z) {
+    // no pre-smoothing sweep ran and u is still zero: v = b
+    GH_TRY(vec_convert(ctx, L.size, mg->wprec, L.v, mg->wprec, L.b));
+    GH_TRY(vec_fill_zero(ctx, mg->wprec, L.size, L.u));
+  } else {
+    GH_TRY(csr_residual(ctx, L.A, mg->wprec, L.v, L.u, L.b));
+  }
+  MgLevel& Cc = mg->L[l - 1];
+  {
+    unsigned grid = grid_for(Cc.size);
+    if (mg->wprec == GH_F64)
+      hipLaunchKernelGGL(k_restrict<double>, dim3(grid), dim3(kBlock), 0, ctx->stream, Cc.n,
+                         Cc.ncomp, L.r_row, L.r_col, (const double*)L.v, (double*)Cc.b);
+    else
+      hipLaunchKernelGGL(k_restrict<float>, dim3(grid), dim3(kBlock), 0, ctx->stream, Cc.n,
+                         Cc.ncomp, L.r_row, L.r_col, (const float*)L.v, (float*)Cc.b);
+    GH_HIP(hipGetLastError());
+  }
+  if (Cc.H) GH_TRY(hn_apply(ctx, Cc.H, Cc.ncomp, mg->wprec, Cc.b, 1));  // HNDistribute
+  if (Cc.M && (Cc.H || !L.mask_in_restrict))
+    GH_TRY(mask_apply(ctx, Cc.M, mg->wprec, Cc.b));  // SetBoundaryVectorZero
+  int j = 0;
+  if (p0 == 'V') j = 1;
+  if (p0 == 'W') j = 2;
+  if (p0 == 'F') j = 3;
+  if (j == 0) GH_TRY(vec_fill_zero(ctx, mg->wprec, Cc.size, Cc.u));  // Zero(u_{l-1})
+  for (int i = 0; i < j; ++i)
+    GH_TRY(mgstep(mg, l - 1, finelevel, clevel, p0, p, i == 0, need_res, res, rw));
+  if (l == 0 && p == 'F') p0 = 'W';
+  if (need_res) GH_TRY(norm_now(mg, Cc.u, Cc.size, &rw[l]));
+  GH_TRY(correction(mg, l));
+  z = false;
+  GH_TRY(sweep(mg, L, mg->P.iterpost, z, nullptr));
+  if (need_res) {
+    GH_TRY(csr_residual(ctx, L.A, mg->wprec, L.v, L.u, L.b));
+    GH_TRY(norm_now(mg, L.v, L.size, &res[l]));
+  }
+  return GH_OK;
+}
+
+int coarsest_level(const gh_mg* mg, int finelevel, int coarselevel) {
+  int clevel = coarselevel;
+  for (int level = coarselevel; level < mg->nlevels; ++level)
+    if (mg->L[level].direct) clevel = level;
+  if (clevel > finelevel) clevel = finelevel;
+  return clevel;
+}
+
+// One mgstep on the finest level (the whole of precondition() when
+// prec_maxiter == 1).  The finest level reads the caller's vectors directly
+// when precisions match; otherwise, and when replaying the captured graph, it
+// works on staging buffers.  Level buffer roles are restored afterwards.
+int fast_vcycle(gh_mg* mg, int clevel, void* xdev, int xprec, const void* ydev, int yprec,
+                bool x_zero) {
+  const int fl = mg->nlevels - 1;
+  MgLevel& F = mg->L[fl];
+  gh_ctx* ctx = mg->ctx;
+  std::vector<std::pair<void*, void*>> S(mg->nlevels);
+  for (int l = 0; l < mg->nlevels; ++l) S[l] = {mg->L[l].u, mg->L[l].v};
+  void* save_b = F.b;
+  const bool graph = mg->P.use_graph && x_zero;
+  const bool direct_u = (xprec == mg->wprec) && !graph;
+  const bool direct_b = (yprec == mg->wprec) && !graph;
+  if (direct_b) {
+    F.b = const_cast<void*>(ydev);
+  } else {
+    GH_TRY(vec_convert(ctx, F.size, mg->wprec, mg->fine_b, yprec, ydev));
+    F.b = mg->fine_b;
+  }
+  if (direct_u)
+    F.u = xdev;
+  else if (!x_zero)
+    GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.u, xprec, xdev));
+  std::vector<double> res(mg->nlevels, 0.0), rw(mg->nlevels, 0.0);
+  const char p = mg->P.mgtype;
+  char p0 = (p == 'F') ? 'W' : p;
+  void* result = nullptr;
+  int rc = GH_OK;
+  if (graph) {
+    if (!mg->graph_ok) {
+      hipGraph_t g;
+      const int64_t before = mg->mgsteps;
+      GH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = mgstep(mg, fl, fl, clevel, p0, p, true, false, res, rw);
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(GH_EHIP, "V-cycle capture: %s", hipGetErrorString(e));
+      GH_HIP(hipGraphInstantiate(&mg->graph, g, nullptr, nullptr, 0));
+      GH_HIP(hipGraphDestroy(g));
+      mg->graph_steps = mg->mgsteps - before;
+      mg->mgsteps = before;
+      mg->graph_result = F.u;
+      mg->graph_ok = true;
+    }
+    GH_HIP(hipGraphLaunch(mg->graph, ctx->stream));
+    mg->mgsteps += mg->graph_steps;
+    result = mg->graph_result;
+  } else {
+    rc = mgstep(mg, fl, fl, clevel, p0, p, x_zero, false, res, rw);
+    result = F.u;
+  }
+  if (!rc && result != xdev) rc = vec_convert(ctx, F.size, xprec, xdev, mg->wprec, result);
+  for (int l = 0; l < mg->nlevels; ++l) {
+    mg->L[l].u = S[l].first;
+    mg->L[l].v = S[l].second;
+  }
+  F.b = save_b;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
+                 gh_hn* const* H, gh_mask* const* M, const gh_mg_params* params,
+                 gh_mg** out) {
+  GH_ARG(ctx && out && nlevels >= 1 && A && params, "gh_mg_create: bad arguments");
+  GH_ARG(params->prec == GH_F64 || params->prec == GH_F32 || params->prec == GH_F16,
+         "gh_mg_create: bad precision");
+  GH_ARG(params->mgtype == 'V' || params->mgtype == 'W' || params->mgtype == 'F',
+         "gh_mg_create: mgtype must be V, W or F");
+  GH_ARG(params->smoother == 0 || params->smoother == 1, "gh_mg_create: bad smoother");
+  *out = nullptr;
+  gh_mg* mg = new gh_mg();
+  mg->ctx = ctx;
+  mg->nlevels = nlevels;
+  mg->P = *params;
+  mg->wprec = params->prec == GH_F64 ? GH_F64 : GH_F32;
+  mg->L.resize(nlevels);
+  int rc = GH_OK;
+  for (int l = 0; l < nlevels && !rc; ++l) {
+    MgLevel& Lv = mg->L[l];
+    if (!A[l] || A[l]->prec != GH_F64) {
+      rc = fail(GH_EINVAL, "gh_mg_create: level %d needs an FP64 matrix", l);
+      break;
+    }
+    Lv.A64 = A[l];
+    Lv.n = A[l]->n;
+    Lv.ncomp = A[l]->ncomp;
+    Lv.size = Lv.n * Lv.ncomp;
+    Lv.H = H ? H[l] : nullptr;
+    Lv.M = M ? M[l] : nullptr;
+    if (Lv.H && Lv.H->n != Lv.n) rc = fail(GH_EINVAL, "gh_mg_create: HN size mismatch on level %d", l);
+    if (Lv.M && (Lv.M->n != Lv.n || Lv.M->ncomp != Lv.ncomp))
+      rc = fail(GH_EINVAL, "gh_mg_create: mask size mismatch on level %d", l);
+    if (l > 0) {
+      if (!T || !T[l]) {
+        rc = fail(GH_EINVAL, "gh_mg_create: level %d has no transfer", l);
+        break;
+      }
+      Lv.T = T[l];
+      if (Lv.T->nfine != Lv.n || Lv.T->ncoarse != A[l - 1]->n || A[l - 1]->ncomp != Lv.ncomp)
+        rc = fail(GH_EINVAL, "gh_mg_create: transfer %d does not match the level sizes", l);
+    }
+    if (rc) break;
+    if (params->prec == GH_F64) {
+      Lv.A = A[l];
+    } else {
+      rc = gh_csr_create(ctx, A[l]->n, A[l]->ncomp, A[l]->h_rowptr.data(), A[l]->h_col.data(),
+                         A[l]->h_val.data(), params->prec, &Lv.A);
+      Lv.own_A = true;
+    }
+    if (!rc) rc = dalloc(&Lv.u, Lv.size * pbytes(mg->wprec));
+    if (!rc) rc = dalloc(&Lv.b, Lv.size * pbytes(mg->wprec));
+    if (!rc) rc = dalloc(&Lv.v, Lv.size * pbytes(mg->wprec));
+    if (!rc) rc = vec_fill_zero(ctx, mg->wprec, Lv.size, Lv.u);
+    if (!rc) rc = vec_fill_zero(ctx, mg->wprec, Lv.size, Lv.b);
+    if (!rc) rc = vec_fill_zero(ctx, mg->wprec, Lv.size, Lv.v);
+    Lv.direct = Lv.n < params->ndirect;
+    if (!rc && Lv.direct) {
+      std::vector<double> inv;
+      rc = dense_inverse(A[l], inv);
+      if (!rc) rc = dupload(ctx, &Lv.Ainv, inv);
+    }
+  }
+  // transfer operators with the constraints folded in
+  for (int l = 1; l < nlevels && !rc; ++l) {
+    MgLevel& F = mg->L[l];
+    MgLevel& C = mg->L[l - 1];
+    const gh_transfer* Tr = F.T;
+    const int B = F.ncomp;
+    // restriction rows of coarse Dirichlet nodes (all components) emptied
+    std::vector<uint8_t> cdir(C.n, 0);
+    F.mask_in_restrict = (C.M == nullptr) || (C.M->ncomps == B);
+    if (C.M && F.mask_in_restrict)
+      for (int32_t v : C.M->h_nodes) cdir[v] = 1;
+    std::vector<int32_t> rrow(1, 0);
+    std::vector<uint32_t> rcol;
+    for (int64_t i = 0; i < C.n; ++i) {
+      if (!cdir[i])
+        for (int32_t q = Tr->h_rrow[i]; q < Tr->h_rrow[i + 1]; ++q) rcol.push_back(Tr->h_rcol[q]);
+      rrow.push_back((int32_t)rcol.size());
+    }
+    // prolongation rows of fine hanging / Dirichlet nodes emptied
+    F.fused_correction = (F.M == nullptr) || (F.M->ncomps == B);
+    std::vector<uint8_t> fz(F.n, 0);
+    if (F.M && F.fused_correction)
+      for (int32_t v : F.M->h_nodes) fz[v] = 1;
+    if (F.H) {
+      for (int64_t k = 0; k < F.H->ne; ++k) fz[F.H->h_edges[4 * k]] = 1;
+      for (int64_t k = 0; k < F.H->nf; ++k) fz[F.H->h_faces[10 * k]] = 1;
+    }
+    std::vector<int32_t> prow(1, 0);
+    std::vector<uint32_t> pcol;
+    for (int64_t i = 0; i < F.n; ++i) {
+      if (!fz[i])
+        for (int32_t q = Tr->h_prow[i]; q < Tr->h_prow[i + 1]; ++q) pcol.push_back(Tr->h_pcol[q]);
+      prow.push_back((int32_t)pcol.size());
+    }
+    rc = dupload(ctx, &F.r_row, rrow);
+    if (!rc) rc = dupload(ctx, &F.r_col, rcol);
+    if (!rc) rc = dupload(ctx, &F.p_row, prow);
+    if (!rc) rc = dupload(ctx, &F.p_col, pcol);
+  }
+  if (!rc) rc = dalloc(&mg->fine_b, mg->L[nlevels - 1].size * pbytes(mg->wprec));
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
+  if (rc) {
+    gh_mg_destroy(mg);
+    return rc;
+  }
+  *out = mg;
+  return GH_OK;
+}
+
+int gh_mg_destroy(gh_mg* mg) {
+  if (!mg) return GH_OK;
+  if (mg->ctx && mg->ctx->stream) (void)hipStreamSynchronize(mg->ctx->stream);
+  for (MgLevel& L : mg->L) {
+    if (L.own_A && L.A) gh_csr_destroy(L.A);
+    for (void* p : {(void*)L.r_row, (void*)L.r_col, (void*)L.p_row, (void*)L.p_col, L.u, L.b,
+                    L.v, (void*)L.Ainv})
+      if (p) (void)hipFree(p);
+  }
+  for (double* m : mg->mem)
+    if (m) (void)hipFree(m);
+  for (void* p : {(void*)mg->p, (void*)mg->r, (void*)mg->w, mg->fine_b})
+    if (p) (void)hipFree(p);
+  if (mg->graph) (void)hipGraphExecDestroy(mg->graph);
+  delete mg;
+  return GH_OK;
+}
+
+int gh_mg_nlevels(const gh_mg* mg) { return mg ? mg->nlevels : -1; }
+int64_t gh_mg_mgsteps(const gh_mg* mg) { return mg ? mg->mgsteps : -1; }
+
+int gh_mg_linear(gh_ctx* ctx, gh_mg* mg, int finelevel, int coarselevel, gh_vec* u,
+                 const gh_vec* f, gh_cginfo* info) {
+  GH_ARG(ctx && mg && u && f && info, "gh_mg_linear: NULL argument");
+  GH_ARG(finelevel >= 0 && finelevel < mg->nlevels && coarselevel >= 0,
+         "gh_mg_linear: bad levels");
+  MgLevel& F = mg->L[finelevel];
+  GH_ARG(u->n == F.n && f->n == F.n && u->ncomp == F.ncomp && f->ncomp == F.ncomp,
+         "gh_mg_linear: vector shape does not match the level");
+  GH_ARG(u->d != f->d, "gh_mg_linear: u and f must not alias");
+  const int clevel = coarsest_level(mg, finelevel, coarselevel);
+  // level vectors: u, b = _mg0_ (= f), v = _mg1_ in working precision
+  GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.u, u->prec, u->d));
+  GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.b, f->prec, f->d));
+  std::vector<double> res(mg->nlevels, 0.0), rw(mg->nlevels, 0.0);
+  GH_TRY(csr_residual(ctx, F.A, mg->wprec, F.v, F.u, F.b));
+  GH_TRY(norm_now(mg, F.v, F.size, &res[finelevel]));
+  rw[finelevel] = 0.;
+  gh_cginfo_check(info, res[finelevel], rw[finelevel]);
+  int reached = 0;
+  while (!reached) {
+    char p = mg->P.mgtype;
+    char p0 = (p == 'F') ? 'W' : p;
+    GH_TRY(mgstep(mg, finelevel, finelevel, clevel, p0, p, false, true, res, rw));
+    reached = gh_cginfo_check(info, res[finelevel], rw[finelevel]);
+  }
+  GH_TRY(vec_convert(ctx, F.size, u->prec, u->d, mg->wprec, F.u));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  // the level's buffers may have swapped roles; all three are scratch
+  return GH_OK;
+}
+
+int gh_mg_precondition(gh_ctx* ctx, gh_mg* mg, int coarselevel, gh_vec* x, const gh_vec* y,
+                       int x_is_zero) {
+  GH_ARG(ctx && mg && x && y, "gh_mg_precondition: NULL argument");
+  const int fl = mg->nlevels - 1;
+  MgLevel& F = mg->L[fl];
+  GH_ARG(x->n == F.n && y->n == F.n && x->ncomp == F.ncomp && y->ncomp == F.ncomp,
+         "gh_mg_precondition: vector shape does not match the finest level");
+  GH_ARG(x->d != y->d, "gh_mg_precondition: x and y must not alias");
+  const int cl = coarselevel < 0 ? fl : coarselevel;
+  if (mg->P.prec_maxiter != 1) {
+    // CGInfo with prec_maxiter > 1 needs the residual norms: general path
+    gh_cginfo prec;
+    gh_cginfo_init(&prec, mg->P.prec_tol, mg->P.prec_globaltol, mg->P.prec_maxiter);
+    gh_cginfo_reset(&prec);
+    gh_cginfo_check(&prec, 0., 0.);
+    return gh_mg_linear(ctx, mg, fl, cl, x, y, &prec);
+  }
+  // prec_maxiter == 1: precondition() always runs exactly one mgstep
+  // (CGInfo::check re-initialises and stops with "too many iterations")
+  const int clevel = coarsest_level(mg, fl, cl);
+  return fast_vcycle(mg, clevel, x->d, x->prec, y->d, y->prec, x_is_zero != 0);
+}
+
+}  // extern "C"
+
+// ==================================================================== GMRES
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_multi_axpy(int64_t n, double* __restrict__ x, int m,
+                                                       const double* const* __restrict__ vs,
+                                                       const double* __restrict__ h) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double acc = x[i];
+    for (int k = 0; k < m; ++k) acc += h[k] * vs[k][i];
+    x[i] = acc;
+  }
+}
+
+int ensure_gmres_memory(gh_mg* mg, int vmax) {
+  const int64_t n = mg->L[mg->nlevels - 1].size;
+  if ((int)mg->mem.size() < vmax + 1) mg->mem.resize(vmax + 1, nullptr);
+  for (int i = 0; i <= vmax; ++i)
+    if (!mg->mem[i]) GH_TRY(dalloc((void**)&mg->mem[i], n * 8));
+  if (!mg->p) GH_TRY(dalloc((void**)&mg->p, n * 8));
+  if (!mg->r) GH_TRY(dalloc((void**)&mg->r, n * 8));
+  if (!mg->w) GH_TRY(dalloc((void**)&mg->w, n * 8));
+  return GH_OK;
+}
+
+int precond_raw(gh_mg* mg, int coarselevel, double* x, const double* y) {
+  const int fl = mg->nlevels - 1;
+  const int cl = coarselevel < 0 ? fl : coarselevel;
+  if (mg->P.prec_maxiter != 1) {
+    gh_vec xv, yv;
+    xv.ctx = yv.ctx = mg->ctx;
+    xv.n = yv.n = mg->L[fl].n;
+    xv.ncomp = yv.ncomp = mg->L[fl].ncomp;
+    xv.size = yv.size = mg->L[fl].size;
+    xv.prec = yv.prec = GH_F64;
+    xv.d = x;
+    yv.d = const_cast<double*>(y);
+    GH_TRY(vec_fill_zero(mg->ctx, GH_F64, xv.size, x));
+    return gh_mg_precondition(mg->ctx, mg, coarselevel, &xv, &yv, 1);
+  }
+  const int clevel = coarsest_level(mg, fl, cl);
+  return fast_vcycle(mg, clevel, x, GH_F64, y, GH_F64, true);
+}
+
+// GMRES::solve with x_zero: the caller guarantees x == 0 on entry (Newton's
+// correction), so residualgmres is p = b with Dirichlet rows zeroed.
+int gmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
+               int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
+               int* reached_out, int* steps_out) {
+  const int64_t n = mg->L[mg->nlevels - 1].size;
+  GH_TRY(ensure_gmres_memory(mg, vmax));
+  std::vector<std::vector<double>> H(vmax);
+  for (int i = 0; i < vmax; ++i) H[i].assign(i + 2, 0.0);
+  std::vector<double> gamma(vmax + 1, 0.0), ci(vmax, 0.0), si(vmax, 0.0);
+  const int k0 = info->iteration;
+  int reached = 0;
+  double* v = mg->mem[0];
+  double* p = mg->p;
+  // residualgmres (stdsolver.cc:743-754): p = b - A x; Dirichlet rows zero
+  if (x_zero)
+    GH_TRY(vec_convert(ctx, n, GH_F64, p, GH_F64, b));
+  else
+    GH_TRY(csr_residual(ctx, A, GH_F64, p, x, b));
+  if (mask) GH_TRY(mask_apply(ctx, mask, GH_F64, p));
+  GH_TRY(precond_raw(mg, coarselevel, v, p));
+  GH_TRY(reduce_dot(ctx, GH_F64, n, v, v, 2));
+  double rr;
+  GH_TRY(fetch_scalars(ctx, 2, 1, &rr));
+  double rho = std::sqrt(rr);
+  int dim = 0;
+  if (rho == 0.) {
+    *reached_out = 1;
+    *steps_out = 0;
+    return GH_OK;
+  }
+  gamma[0] = rho;
+  GH_TRY(vec_scale_to(ctx, GH_F64, n, v, 1. / rho, v));
+  std::vector<double> d(vmax + 2);
+  while (dim < vmax - 1 && !reached) {
+    dim++;
+    const int last = dim - 1;
+    double* vv = mg->mem[dim];
+    GH_TRY(csr_spmv(ctx, A, GH_F64, p, GH_F64, mg->mem[last], 1.0, 0.0));  // vmulteq
+    GH_TRY(precond_raw(mg, coarselevel, vv, p));
+    std::vector<double>& h = H[last];
+    // orthogonalization (gmres.cc:109-120), slots 8.. hold the dots
+    for (int pass = 0; pass < 2; ++pass) {
+      GH_TRY(mgs_step(ctx, n, vv, nullptr, 0, mg->mem[0], 8));
+      for (int i = 1; i < dim; ++i) GH_TRY(mgs_step(ctx, n, vv, mg->mem[i - 1], 8 + i - 1, mg->mem[i], 8 + i));
+      GH_TRY(mgs_step(ctx, n, vv, mg->mem[dim - 1], 8 + dim - 1, nullptr, 8 + dim));
+      GH_TRY(fetch_scalars(ctx, 8, dim + 1, d.data()));
+      for (int i = 0; i < dim; ++i) h[i] += d[i];
+      h[dim] = std::sqrt(d[dim]);
+      // reortho_test (gmres.cc:124-137): sqrt(vv.vv) recomputed by the same
+      // deterministic reduction equals h[dim], so the test is h + 1e-3 h == h
+      const double oldnorm = h[dim];
+      const double newnorm = oldnorm + 1.e-3 * h[dim];
+      if (oldnorm != newnorm) break;
+    }
+    const double s = h[dim];
+    GH_TRY(vec_scale_to(ctx, GH_F64, n, vv, 1. / s, vv));
+    // givens_rotation (gmres.cc:91-105)
+    for (int i = 0; i < last; i++) {
+      const double dummy = h[i];
+      h[i] = ci[i] * dummy + si[i] * h[i + 1];
+      h[i + 1] = -si[i] * dummy + ci[i] * h[i + 1];
+    }
+    const double r = 1. / std::sqrt(h[last] * h[last] + h[last + 1] * h[last + 1]);
+    si[last] = h[last + 1] * r;
+    ci[last] = h[last] * r;
+    h[last] = ci[last] * h[last] + si[last] * h[last + 1];
+    gamma[last + 1] = -si[last] * gamma[last];
+    gamma[last] *= ci[last];
+    rho = std::fabs(gamma[dim]);
+    reached = gh_cginfo_check(info, rho, (double)(k0 + last));
+  }
+  // solution (gmres.cc:235-260): backward substitution on H1(i,j) = H[j][i]
+  std::vector<double> hs(std::max(dim, 1), 0.0);
+  for (int i = dim - 1; i >= 0; i--) {
+    double s = gamma[i];
+    for (int j = i + 1; j < dim; j++) s -= hs[j] * H[j][i];
+    hs[i] = s / H[i][i];
+  }
+  if (dim > 0) {
+    // x += sum_i h_i mem[i], added in order i = 0..dim-1 per entry
+    double** dptrs = nullptr;
+    double* dh = nullptr;
+    std::vector<double*> ptrs(mg->mem.begin(), mg->mem.begin() + dim);
+    GH_TRY(dalloc((void**)&dptrs, sizeof(double*) * dim));
+    GH_TRY(dalloc((void**)&dh, sizeof(double) * dim));
+    GH_HIP(hipMemcpyAsync(dptrs, ptrs.data(), sizeof(double*) * dim, hipMemcpyHostToDevice, ctx->stream));
+    GH_HIP(hipMemcpyAsync(dh, hs.data(), sizeof(double) * dim, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_multi_axpy, dim3(std::min<int64_t>((n + kBlock - 1) / kBlock, 8192)),
+                       dim3(kBlock), 0, ctx->stream, n, x, dim, (const double* const*)dptrs, dh);
+    GH_HIP(hipGetLastError());
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dptrs);
+    (void)hipFree(dh);
+  }
+  *reached_out = reached;
+  *steps_out = dim;
+  return GH_OK;
+}
+
+double vec_norm_kind(gh_ctx* ctx, const double* x, int64_t n, int kind, int* rc) {
+  double out = 0.0;
+  if (kind == 0) {
+    *rc = reduce_norm_l8(ctx, GH_F64, n, x, 3);
+    if (!*rc) *rc = fetch_scalars(ctx, 3, 1, &out);
+    return out;
+  }
+  *rc = reduce_dot(ctx, GH_F64, n, x, x, 3);
+  if (!*rc) *rc = fetch_scalars(ctx, 3, 1, &out);
+  return std::sqrt(out);
+}
+
+// NewtonResidual (stdmultilevelsolver.cc:608-621): y = b - A x, Dirichlet rows 0
+int newton_residual(gh_ctx* ctx, const gh_csr* A, const gh_mask* mask, double* y,
+                    const double* x, const double* b, int64_t n, int kind, double* nrm) {
+  GH_TRY(csr_residual(ctx, A, GH_F64, y, x, b));
+  if (mask) GH_TRY(mask_apply(ctx, mask, GH_F64, y));
+  int rc = 0;
+  *nrm = vec_norm_kind(ctx, y, n, kind, &rc);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
+                   int coarselevel, int vmax, gh_vec* x, const gh_vec* b, gh_cginfo* info,
+                   int* reached) {
+  GH_ARG(ctx && mg && A && x && b && info && vmax >= 2 && vmax <= 256,
+         "gh_gmres_solve: bad arguments (2 <= vmax <= 256)");
+  const MgLevel& F = mg->L[mg->nlevels - 1];
+  GH_ARG(A->prec == GH_F64 && A->n == F.n && A->ncomp == F.ncomp,
+         "gh_gmres_solve: A must be the FP64 finest-level matrix");
+  GH_ARG(x->prec == GH_F64 && b->prec == GH_F64 && x->n == F.n && b->n == F.n,
+         "gh_gmres_solve: x and b must be FP64 finest-level vectors");
+  int rdum = 0, steps = 0;
+  GH_TRY(gmres_impl(ctx, mg, A, mask, coarselevel, vmax, (double*)x->d, (const double*)b->d,
+                    false, info, reached ? reached : &rdum, &steps));
+  return GH_OK;
+}
+
+int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
+                    const gh_hn* hn, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
+                    gh_cginfo* linfo, gh_nlinfo* nl, int norm_kind, gh_solve_report* rep) {
+  GH_ARG(ctx && mg && A && u && f && linfo && nl && vmax >= 2 && vmax <= 256,
+         "gh_newton_solve: bad arguments (2 <= vmax <= 256)");
+  const MgLevel& F = mg->L[mg->nlevels - 1];
+  GH_ARG(A->prec == GH_F64 && A->n == F.n && A->ncomp == F.ncomp,
+         "gh_newton_solve: A must be the FP64 finest-level matrix");
+  GH_ARG(u->prec == GH_F64 && f->prec == GH_F64 && u->n == F.n && f->n == F.n,
+         "gh_newton_solve: u and f must be FP64 finest-level vectors");
+  const int64_t n = F.size;
+  GH_TRY(ensure_gmres_memory(mg, vmax));
+  double* r = mg->r;
+  double* w = mg->w;
+  double* ud = (double*)u->d;
+  const double* fd = (const double*)f->d;
+  const int64_t mg0 = mg->mgsteps;
+  int gm = 0;
+  if (hn) GH_TRY(hn_apply(ctx, hn, u->ncomp, GH_F64, ud, 0));  // Solve: HNAverage(u)
+  gh_nlinfo_reset(nl);
+  gh_cginfo_reset(linfo);
+  double rr;
+  GH_TRY(newton_residual(ctx, A, mask, r, ud, fd, n, norm_kind, &rr));
+  if (rep) rep->first_residual = rr;
+  int reached = gh_nlinfo_check(nl, 0, rr, 0.);
+  for (int it = 1; !reached; it++) {
+    // NewtonLinearSolve: info.reset(); Zero(x); Gmres
+    gh_cginfo_reset(linfo);
+    int greached = 0, steps = 0;
+    GH_TRY(vec_fill_zero(ctx, GH_F64, n, w));
+    GH_TRY(gmres_impl(ctx, mg, A, mask, coarselevel, vmax, w, r, true, linfo, &greached, &steps));
+    gm += steps;
+    // NewtonUpdate
+    int lex = linfo->status == GH_EXPLODED;
+    int rc = 0;
+    const double nn = vec_norm_kind(ctx, w, n, norm_kind, &rc);
+    GH_TRY(rc);
+    const double nr = vec_norm_kind(ctx, r, n, 1, &rc);
+    GH_TRY(rc);
+    if (nn > 1.e30 || !(nn >= 0.) || nr > 1.e30 || !(nr >= 0.)) lex = 1;
+    if (lex) {
+      nl->status = GH_NL_DIVERGED;
+    } else {
+      const double omega = 0.7;
+      double relax = 1.;
+      GH_TRY(vec_axpby(ctx, GH_F64, n, ud, 1.0, relax, w));
+      GH_TRY(newton_residual(ctx, A, mask, r, ud, fd, n, norm_kind, &rr));
+      for (int iter = 0; iter < nl->maxrelax; iter++) {
+        const int msg = nl_check_damping(nl, iter, rr);
+        if (msg == 0) break;
+        if (msg == 1) {
+          GH_TRY(vec_axpby(ctx, GH_F64, n, ud, 1.0, relax * (omega - 1.), w));
+          GH_TRY(newton_residual(ctx, A, mask, r, ud, fd, n, norm_kind, &rr));
+          relax *= omega;
+          continue;
+        }
+        GH_TRY(vec_axpby(ctx, GH_F64, n, ud, 1.0, -relax, w));
+        relax = 0.;
+        nl->status = GH_NL_DIVERGED;
+        break;
+      }
+    }
+    reached = gh_nlinfo_check(nl, it, rr, nn);
+  }
+  if (hn) GH_TRY(hn_apply(ctx, hn, u->ncomp, GH_F64, ud, 2));  // Solve: HNZero(u)
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  if (rep) {
+    rep->newton_iterations = nl->iteration;
+    rep->gmres_iterations = gm;
+    rep->mgsteps = mg->mgsteps - mg0;
+    rep->status = nl->status;
+    rep->last_residual = rr;
+  }
+  return GH_OK;
+}
+
+// ------------------------------------------------------------------ A^T x
+__global__ __launch_bounds__(kBlock) void k_csr_t(int64_t n, const int32_t* __restrict__ trow,
+                                                  const int32_t* __restrict__ tcol,
+                                                  const int32_t* __restrict__ tpos,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ y, double d, double b,
+                                                  int read_y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = trow[i]; k < trow[i + 1]; ++k) s += val[tpos[k]] * x[tcol[k]];
+  double v = d * s;
+  if (read_y) v += b * y[i];
+  y[i] = v;
+}
+
+int gh_spmv_t(gh_ctx* ctx, const gh_csr* Ac, gh_vec* y, const gh_vec* x, double d, double b) {
+  GH_ARG(ctx && Ac && x && y && x->n == Ac->n && y->n == Ac->n && x->ncomp == 1 &&
+             y->ncomp == 1 && Ac->ncomp == 1,
+         "gh_spmv_t: shape mismatch (scalar matrices)");
+  GH_ARG(Ac->prec == GH_F64 && x->prec == GH_F64 && y->prec == GH_F64,
+         "gh_spmv_t: FP64 only");
+  GH_ARG(x->d != y->d, "gh_spmv_t: x and y must not alias");
+  gh_csr* A = const_cast<gh_csr*>(Ac);
+  if (!A->t_rowptr) {
+    const int64_t n = A->n, nnz = A->nnz;
+    std::vector<int32_t> trow(n + 1, 0), tcol(nnz), tpos(nnz);
+    for (int64_t k = 0; k < nnz; ++k) trow[A->h_col[k] + 1]++;
+    for (int64_t i = 0; i < n; ++i) trow[i + 1] += trow[i];
+    std::vector<int32_t> fill(trow.begin(), trow.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+      for (int32_t k = A->h_rowptr[i]; k < A->h_rowptr[i + 1]; ++k) {
+        const int32_t j = A->h_col[k];
+        tcol[fill[j]] = (int32_t)i;
+        tpos[fill[j]] = k;
+        fill[j]++;
+      }
+    GH_TRY(dupload(ctx, &A->t_rowptr, trow));
+    GH_TRY(dupload(ctx, &A->t_col, tcol));
+    GH_TRY(dupload(ctx, &A->t_pos, tpos));
+  }
+  hipLaunchKernelGGL(k_csr_t, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
+                     A->t_rowptr, A->t_col, A->t_pos, (const double*)A->val,
+                     (const double*)x->d, (double*)y->d, d, b, (int)(b != 0.0));
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+}  // extern "C"

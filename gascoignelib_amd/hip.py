@@ -1,0 +1,389 @@
+"""Python mirror of the reference GPU path's API surface on top of the C ABI.
+
+Class and method names follow the reference's cuda/ backend so that code and
+tests read like it:
+  CudaVectorInterface      cuda/cudavectorinterface.h:26-77
+  CudaCSRMatrixInterface   cuda/cudacsrmatrixinterface.h:40-159
+  CudaMgInterpolatorNested cuda/cudamgintepolatornested.h:17-43
+  CudaMultiLevelSolver     cuda/cudamultilevelsolver.h:15-45 (with the
+                           StdMultiLevelSolver LinearMg / precondition / Gmres /
+                           Solve entry points it inherits)
+Every call goes to libgascoigne_hip.so; nothing here computes on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+from .native import check, ptr
+
+
+class Context:
+    """Device, stream and reduction scratch (one per solver)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(N.hip().gh_ctx_create(device, C.byref(self.h)))
+
+    def sync(self):
+        check(N.hip().gh_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        return N.hip().gh_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            N.hip().gh_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CudaVectorInterface:
+    def __init__(self, ctx, n, n_comp=1, prec="f64"):
+        self.ctx = ctx
+        self.n = int(n)
+        self.n_comp = int(n_comp)
+        self.prec = prec
+        self.h = C.c_void_p()
+        check(N.hip().gh_vec_create(ctx.h, self.n, self.n_comp, N.PREC[prec], C.byref(self.h)))
+
+    @classmethod
+    def from_global(cls, ctx, gv, n_comp=1, prec="f64"):
+        gv = np.ascontiguousarray(gv, dtype=np.float64)
+        v = cls(ctx, len(gv) // n_comp, n_comp, prec)
+        v.assign(gv)
+        return v
+
+    def assign(self, gv):
+        gv = np.ascontiguousarray(gv, dtype=np.float64)
+        if gv.size != self.n * self.n_comp:
+            raise ValueError("size mismatch")
+        check(N.hip().gh_vec_upload(self.ctx.h, self.h, ptr(gv)))
+
+    def copy_back(self):
+        out = np.empty(self.n * self.n_comp)
+        check(N.hip().gh_vec_download(self.ctx.h, self.h, ptr(out)))
+        return out
+
+    def zero(self):
+        check(N.hip().gh_vec_zero(self.ctx.h, self.h))
+
+    def copy_from(self, other):
+        check(N.hip().gh_vec_copy(self.ctx.h, self.h, other.h))
+
+    def equ(self, s, src):
+        check(N.hip().gh_vec_equ(self.ctx.h, self.h, s, src.h))
+
+    def add(self, d, y):
+        check(N.hip().gh_vec_add(self.ctx.h, self.h, d, y.h))
+
+    def sadd(self, a, d, y):
+        check(N.hip().gh_vec_sadd(self.ctx.h, self.h, a, d, y.h))
+
+    def scal(self, d):
+        check(N.hip().gh_vec_scal(self.ctx.h, self.h, d))
+
+    def norm(self):
+        out = C.c_double()
+        check(N.hip().gh_vec_norm(self.ctx.h, self.h, C.byref(out)))
+        return out.value
+
+    def norm_l8(self):
+        out = C.c_double()
+        check(N.hip().gh_vec_norm_l8(self.ctx.h, self.h, C.byref(out)))
+        return out.value
+
+    def dot(self, y):
+        out = C.c_double()
+        check(N.hip().gh_vec_dot(self.ctx.h, self.h, y.h, C.byref(out)))
+        return out.value
+
+    def free(self):
+        if self.h:
+            N.hip().gh_vec_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CudaCSRMatrixInterface:
+    def __init__(self, ctx, rowptr, col, val, n_comp=1, prec="f64"):
+        self.ctx = ctx
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        self._n = len(rowptr) - 1
+        self.n_comp = n_comp
+        self.prec = prec
+        self.h = C.c_void_p()
+        check(N.hip().gh_csr_create(ctx.h, self._n, n_comp, ptr(rowptr, N.i32p),
+                                    ptr(col, N.i32p), ptr(val), N.PREC[prec], C.byref(self.h)))
+
+    def n(self):
+        return self._n
+
+    def n_entries(self):
+        return N.hip().gh_csr_nnz(self.h)
+
+    def vmult(self, y, x, d=1.0, b=0.0):
+        """y = d*A*x + b*y"""
+        check(N.hip().gh_spmv(self.ctx.h, self.h, y.h, x.h, d, b))
+
+    def vmult_transpose(self, y, x, d=1.0, b=0.0):
+        check(N.hip().gh_spmv_t(self.ctx.h, self.h, y.h, x.h, d, b))
+
+    def residual(self, r, x, b):
+        check(N.hip().gh_residual(self.ctx.h, self.h, r.h, x.h, b.h))
+
+    def smooth(self, x, b, h, niter, omega, smoother="jacobi"):
+        check(N.hip().gh_smooth(self.ctx.h, self.h, x.h, b.h, h.h, niter, omega,
+                                {"jacobi": 0, "richardson": 1}[smoother]))
+
+    def jacobi(self, h):
+        check(N.hip().gh_jacobi(self.ctx.h, self.h, h.h))
+
+    def free(self):
+        if self.h:
+            N.hip().gh_csr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CudaMgInterpolatorNested:
+    def __init__(self, ctx, ncoarse, c2f, zweier, vierer, achter):
+        self.ctx = ctx
+        arrs = [np.ascontiguousarray(a, dtype=np.int32).reshape(-1) for a in
+                (c2f, zweier, vierer, achter)]
+        c2f, z, v, a = arrs
+        self.h = C.c_void_p()
+        check(N.hip().gh_transfer_create(ctx.h, int(ncoarse), ptr(c2f, N.i32p), len(z) // 3,
+                                         ptr(z, N.i32p), len(v) // 5, ptr(v, N.i32p),
+                                         len(a) // 9, ptr(a, N.i32p), C.byref(self.h)))
+
+    def restrict_zero(self, uc, uf):
+        check(N.hip().gh_restrict_zero(self.ctx.h, self.h, uc.h, uf.h))
+
+    def prolongate_add(self, uf, uc):
+        check(N.hip().gh_prolongate_add(self.ctx.h, self.h, uf.h, uc.h))
+
+    def free(self):
+        if self.h:
+            N.hip().gh_transfer_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HNStructure:
+    """Hanging-node constraints (HNStructureQ12d/Q13d/Q22d/Q23d semantics)."""
+
+    def __init__(self, ctx, kind, n, edges, faces):
+        self.ctx = ctx
+        e = np.ascontiguousarray(edges, dtype=np.int32).reshape(-1, 4)
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 10)
+        self.h = C.c_void_p()
+        check(N.hip().gh_hn_create(ctx.h, int(kind), int(n), len(e), ptr(e, N.i32p), len(f),
+                                   ptr(f, N.i32p), C.byref(self.h)))
+
+    def average(self, x):
+        check(N.hip().gh_hn_average(self.ctx.h, self.h, x.h))
+
+    def distribute(self, x):
+        check(N.hip().gh_hn_distribute(self.ctx.h, self.h, x.h))
+
+    def zero(self, x):
+        check(N.hip().gh_hn_zero(self.ctx.h, self.h, x.h))
+
+    def free(self):
+        if self.h:
+            N.hip().gh_hn_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DirichletMask:
+    def __init__(self, ctx, n, ncomp, nodes, comps=(0,)):
+        self.ctx = ctx
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        comps = np.ascontiguousarray(comps, dtype=np.int32)
+        self.h = C.c_void_p()
+        check(N.hip().gh_mask_create(ctx.h, int(n), int(ncomp), len(nodes), ptr(nodes, N.i32p),
+                                     len(comps), ptr(comps, N.i32p), C.byref(self.h)))
+
+    def zero(self, x):
+        check(N.hip().gh_mask_zero(self.ctx.h, self.h, x.h))
+
+    def free(self):
+        if self.h:
+            N.hip().gh_mask_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CGInfo:
+    def __init__(self, tol=1e-6, globaltol=1e-14, maxiter=100):
+        self.s = N.GhCGInfo()
+        N.hip().gh_cginfo_init(C.byref(self.s), tol, globaltol, maxiter)
+
+    def reset(self):
+        N.hip().gh_cginfo_reset(C.byref(self.s))
+
+    def check(self, res, cor=0.0):
+        return N.hip().gh_cginfo_check(C.byref(self.s), res, cor)
+
+    @property
+    def status(self):
+        return N.CG_STATUS[self.s.status]
+
+    def __getattr__(self, k):
+        return getattr(self.__dict__["s"], k)
+
+
+class NLInfo:
+    def __init__(self, tol=1e-4, globaltol=1e-12, maxiter=10):
+        self.s = N.GhNLInfo()
+        N.hip().gh_nlinfo_init(C.byref(self.s), tol, globaltol, maxiter)
+
+    @property
+    def status(self):
+        return N.NL_STATUS[self.s.status]
+
+    def __getattr__(self, k):
+        return getattr(self.__dict__["s"], k)
+
+
+class CudaMultiLevelSolver:
+    """Device multigrid hierarchy built from a gascoignelib_amd.problem.Hierarchy.
+
+    Parameters follow the reference's MultiLevelSolver / Solver blocks:
+    mgtype, iterpre, iterpost, iterexact, omega, mgomega, linear_smooth,
+    ndirect, prec_maxiter; `prec` selects the V-cycle storage ("f64", "f32"
+    or "f16" matrix values with FP32 arithmetic)."""
+
+    def __init__(self, ctx, hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
+                 mgomega=1.0, linear_smooth="jacobi", ndirect=1, prec_maxiter=1, prec="f64",
+                 use_graph=False):
+        self.ctx = ctx
+        self.hier = hier
+        nl = hier.nlevels
+        self.A, self.T, self.H, self.M = [], [], [], []
+        for l in range(nl):
+            lev = hier.level(l)
+            self.A.append(CudaCSRMatrixInterface(ctx, lev.rowptr, lev.col, lev.val, lev.ncomp, "f64"))
+            if l > 0:
+                self.T.append(CudaMgInterpolatorNested(ctx, hier.level(l - 1).n, *lev.transfer))
+            else:
+                self.T.append(None)
+            if len(lev.hn_edges) or len(lev.hn_faces):
+                self.H.append(HNStructure(ctx, lev.hn_kind, lev.n, lev.hn_edges, lev.hn_faces))
+            else:
+                self.H.append(None)
+            if len(lev.dirichlet):
+                self.M.append(DirichletMask(ctx, lev.n, lev.ncomp, lev.dirichlet, lev.dirichlet_comps))
+            else:
+                self.M.append(None)
+        p = N.GhMgParams()
+        N.hip().gh_mg_params_default(C.byref(p))
+        p.mgtype = mgtype.encode()
+        p.iterpre, p.iterpost, p.iterexact = iterpre, iterpost, iterexact
+        p.omega, p.mgomega = omega, mgomega
+        p.smoother = {"jacobi": 0, "richardson": 1}[linear_smooth]
+        p.ndirect = ndirect
+        p.prec_maxiter = prec_maxiter
+        p.prec = N.PREC[prec]
+        p.use_graph = int(use_graph)
+        self.params = p
+
+        def arr(objs):
+            a = (C.c_void_p * nl)()
+            for i, o in enumerate(objs):
+                a[i] = o.h if o is not None else None
+            return a
+
+        self._arrays = [arr(self.A), arr(self.T), arr(self.H), arr(self.M)]
+        self.h = C.c_void_p()
+        check(N.hip().gh_mg_create(ctx.h, nl, self._arrays[0], self._arrays[1], self._arrays[2],
+                                   self._arrays[3], C.byref(p), C.byref(self.h)))
+
+    @property
+    def fine_matrix(self):
+        return self.A[-1]
+
+    @property
+    def fine_mask(self):
+        return self.M[-1]
+
+    @property
+    def fine_hn(self):
+        return self.H[-1]
+
+    @property
+    def mgsteps(self):
+        return N.hip().gh_mg_mgsteps(self.h)
+
+    def new_vector(self, level=None, prec="f64"):
+        lev = self.hier.level(self.hier.nlevels - 1 if level is None else level)
+        return CudaVectorInterface(self.ctx, lev.n, lev.ncomp, prec)
+
+    def LinearMg(self, u, f, info, finelevel=None, coarselevel=0):
+        fl = self.hier.nlevels - 1 if finelevel is None else finelevel
+        check(N.hip().gh_mg_linear(self.ctx.h, self.h, fl, coarselevel, u.h, f.h, C.byref(info.s)))
+
+    def precondition(self, x, y, x_is_zero=False, coarselevel=0):
+        check(N.hip().gh_mg_precondition(self.ctx.h, self.h, coarselevel, x.h, y.h, int(x_is_zero)))
+
+    def Gmres(self, x, b, info, vmax=10, coarselevel=0):
+        reached = C.c_int()
+        m = self.fine_mask
+        check(N.hip().gh_gmres_solve(self.ctx.h, self.h, self.fine_matrix.h, m.h if m else None,
+                                     coarselevel, vmax, x.h, b.h, C.byref(info.s), C.byref(reached)))
+        return reached.value
+
+    def Solve(self, u, f, linfo, nlinfo, vmax=10, coarselevel=0, norm_kind=0):
+        rep = N.GhSolveReport()
+        m, hn = self.fine_mask, self.fine_hn
+        check(N.hip().gh_newton_solve(self.ctx.h, self.h, self.fine_matrix.h, m.h if m else None,
+                                      hn.h if hn else None, coarselevel, vmax, u.h, f.h,
+                                      C.byref(linfo.s), C.byref(nlinfo.s), norm_kind,
+                                      C.byref(rep)))
+        return rep
+
+    def free(self):
+        if self.h:
+            N.hip().gh_mg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,128 @@
+"""Multigrid problems from the native structure producer (libgh_structure.so).
+
+`uniform_hierarchy` refines a coarse .inp mesh uniformly and returns a
+`problem.Hierarchy` with the Q1 matrices of -div(k grad u) (k = 1 unless a
+per-cell coefficient is given for the finest level), the mesh-transfer maps
+and the strong Dirichlet node lists of the given colours, in the form
+Gascoigne's CPU side hands them to the GPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import native as N
+from .native import check_s, ptr
+from .problem import Hierarchy, Level
+
+CONFIGS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs")
+
+
+class Mesh:
+    """Handle on a refined mesh hierarchy inside libgh_structure.so."""
+
+    def __init__(self, inp, dim, prerefine, drop_coarse=0):
+        path = inp if os.path.isabs(inp) else os.path.join(CONFIGS, inp)
+        self.dim = dim
+        self.h = C.c_void_p()
+        check_s(N.structure().ghs_build(path.encode(), dim, prerefine, drop_coarse, C.byref(self.h)))
+        self.nlevels = N.structure().ghs_nlevels(self.h)
+
+    def sizes(self, l, with_nnz=False):
+        n, c, z = C.c_int64(), C.c_int64(), C.c_int64()
+        check_s(N.structure().ghs_level_sizes(self.h, l, C.byref(n), C.byref(c),
+                                              C.byref(z) if with_nnz else None))
+        return n.value, c.value, (z.value if with_nnz else None)
+
+    def coords(self, l):
+        n, _, _ = self.sizes(l)
+        x = np.empty((n, 3))
+        check_s(N.structure().ghs_level_coords(self.h, l, ptr(x)))
+        return x[:, : self.dim].copy()
+
+    def cells(self, l):
+        _, nc, _ = self.sizes(l)
+        npc = 8 if self.dim == 3 else 4
+        c = np.empty((nc, npc), np.int32)
+        check_s(N.structure().ghs_level_cells(self.h, l, ptr(c, N.i32p)))
+        return c
+
+    def structure(self, l):
+        n, _, nnz = self.sizes(l, with_nnz=True)
+        rp = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        check_s(N.structure().ghs_level_structure(self.h, l, ptr(rp, N.i32p), ptr(col, N.i32p)))
+        return rp, col
+
+    def dirichlet(self, l, colours=(1,)):
+        cols = np.ascontiguousarray(colours, dtype=np.int32)
+        cnt = C.c_int64()
+        check_s(N.structure().ghs_level_dirichlet_count(self.h, l, len(cols), ptr(cols, N.i32p),
+                                                        C.byref(cnt)))
+        out = np.empty(cnt.value, np.int32)
+        check_s(N.structure().ghs_level_dirichlet(self.h, l, len(cols), ptr(cols, N.i32p),
+                                                  ptr(out, N.i32p)))
+        return out
+
+    def assemble(self, l, nnz, kcell=None, colours=(1,)):
+        cols = np.ascontiguousarray(colours, dtype=np.int32)
+        val = np.empty(nnz)
+        kp = None
+        if kcell is not None:
+            kcell = np.ascontiguousarray(kcell, dtype=np.float64)
+            kp = ptr(kcell)
+        check_s(N.structure().ghs_level_assemble(self.h, l, kp, len(cols), ptr(cols, N.i32p),
+                                                 ptr(val)))
+        return val
+
+    def transfer(self, l):
+        nc, n2, n4, n8 = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check_s(N.structure().ghs_level_transfer_sizes(self.h, l, C.byref(nc), C.byref(n2),
+                                                       C.byref(n4), C.byref(n8)))
+        c2f = np.empty(nc.value, np.int32)
+        z = np.empty((n2.value, 3), np.int32)
+        v = np.empty((n4.value, 5), np.int32)
+        a = np.empty((n8.value, 9), np.int32)
+        check_s(N.structure().ghs_level_transfer(self.h, l, ptr(c2f, N.i32p), ptr(z, N.i32p),
+                                                 ptr(v, N.i32p), ptr(a, N.i32p)))
+        return c2f, z, v, a
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.structure().ghs_destroy(self.h)
+        except Exception:
+            pass
+
+
+def coarse_coefficients(k_fine, cells_per_coarse=8):
+    """Per-cell coefficients of the next coarser level: the mean over the
+    children (children of coarse cell c are fine cells 8c..8c+7 in 3D)."""
+    return np.asarray(k_fine).reshape(-1, cells_per_coarse).mean(axis=1)
+
+
+def uniform_hierarchy(inp="cube.inp", dim=3, prerefine=7, drop_coarse=1, colours=(1,),
+                      kcell=None, name=None, keep_coords=True):
+    """Q1 hierarchy of a uniformly refined mesh.  drop_coarse removes the
+    coarsest meshes from the hierarchy (Gascoigne keeps nlevels - patchdepth
+    levels: the 1-cell mesh of a one-hex .inp is not a multigrid level)."""
+    M = Mesh(inp, dim, prerefine, drop_coarse)
+    nl = M.nlevels
+    ks = [None] * nl
+    if kcell is not None:
+        ks[-1] = np.asarray(kcell, dtype=np.float64)
+        for l in range(nl - 1, 0, -1):
+            ks[l - 1] = coarse_coefficients(ks[l], 8 if dim == 3 else 4)
+    levels = []
+    for l in range(nl):
+        rp, col = M.structure(l)
+        val = M.assemble(l, len(col), ks[l], colours)
+        lev = Level(n=len(rp) - 1, ncomp=1, rowptr=rp, col=col, val=val,
+                    transfer=M.transfer(l) if l > 0 else None, hn_kind=13 if dim == 3 else 12,
+                    dirichlet=M.dirichlet(l, colours), dirichlet_comps=np.zeros(1, np.int32),
+                    coords=M.coords(l) if keep_coords else None)
+        levels.append(lev)
+    h = Hierarchy(levels, dim, name or f"{inp}:{prerefine}")
+    h.mesh = M
+    h.kcell = ks
+    return h

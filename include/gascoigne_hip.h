@@ -1,0 +1,261 @@
+/*
+ * gascoigne_hip: C ABI of the MI355X (gfx950) multigrid solve path.
+ *
+ * Drop-in for the device side of Gascoigne's cuda/ backend.  Every entry point
+ * names the reference interface it replaces (paths relative to the reference
+ * repository).  Conventions:
+ *   - every call returns GH_OK (0) or a negative GH_E* code; gh_last_error()
+ *     gives the message of the last failure on the calling thread.  Solver
+ *     outcomes are reported through gh_cginfo / gh_nlinfo status fields, as
+ *     CGInfo / NLInfo strings are in the reference (src/Common/cginfo.cc:202-256,
+ *     src/Common/nlinfo.cc:247-356).  Nothing calls exit().
+ *   - host arrays are borrowed for the duration of the call (copied on create);
+ *     device memory belongs to the handle that allocated it.
+ *   - vectors are Gascoigne GlobalVectors: node-major, interleaved components,
+ *     entry (i,c) at i*ncomp+c (src/Common/compvector.h:98-113).
+ *   - indices are int32 (the reference's device CSR is int32 too,
+ *     cuda/cudacsrmatrixinterface.h:40-159).
+ *   - all work is enqueued on the context's HIP stream; only calls that return
+ *     host scalars (norms, dots, solver drivers) synchronise.
+ */
+#ifndef GASCOIGNE_HIP_H
+#define GASCOIGNE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GH_OK = 0,
+  GH_EINVAL = -1,   /* invalid argument / shape mismatch */
+  GH_EHIP = -2,     /* HIP runtime error */
+  GH_ENOMEM = -3,   /* device allocation failed */
+  GH_ESINGULAR = -4 /* singular diagonal block or coarse matrix */
+};
+
+/* storage precision of vectors and matrix values */
+enum { GH_F64 = 0, GH_F32 = 1, GH_F16 = 2 };
+
+typedef struct gh_ctx gh_ctx;
+typedef struct gh_vec gh_vec;
+typedef struct gh_csr gh_csr;
+typedef struct gh_transfer gh_transfer;
+typedef struct gh_hn gh_hn;
+typedef struct gh_mask gh_mask;
+typedef struct gh_mg gh_mg;
+
+const char* gh_last_error(void);
+const char* gh_version(void);
+
+/* ---------------------------------------------------------------- context */
+/* Replaces the per-solver cuSPARSE/cuBLAS handles and the legacy default
+ * stream of cuda/cudasolver.h:31-60.  device < 0 keeps the current device. */
+int gh_ctx_create(int device, gh_ctx** out);
+int gh_ctx_destroy(gh_ctx* ctx);
+int gh_ctx_sync(gh_ctx* ctx);
+/* the hipStream_t all work of this context is enqueued on */
+void* gh_ctx_stream(gh_ctx* ctx);
+
+/* ---------------------------------------------------------------- vectors */
+/* CudaVectorInterface (cuda/cudavectorinterface.h:26-77, .cc:16-218):
+ * reserve (:141-157) zeroes, like the (n, n_comp) constructor. */
+int gh_vec_create(gh_ctx* ctx, int64_t n, int ncomp, int prec, gh_vec** out);
+int gh_vec_destroy(gh_vec* v);
+int64_t gh_vec_n(const gh_vec* v);
+int gh_vec_ncomp(const gh_vec* v);
+int gh_vec_prec(const gh_vec* v);
+void* gh_vec_data(gh_vec* v);
+/* H2D from a GlobalVector (operator=(GlobalVector), :61-88) */
+int gh_vec_upload(gh_ctx* ctx, gh_vec* v, const double* host);
+/* D2H (copy_back, :110-117); synchronises */
+int gh_vec_download(gh_ctx* ctx, const gh_vec* v, double* host);
+/* zero (:119-126) */
+int gh_vec_zero(gh_ctx* ctx, gh_vec* v);
+/* D2D copy (operator=(CudaVectorInterface), :91-103); converts precision */
+int gh_vec_copy(gh_ctx* ctx, gh_vec* dst, const gh_vec* src);
+/* dst = s*src: CudaSolver::Equ (cuda/cudasolver.cc:403-415) */
+int gh_vec_equ(gh_ctx* ctx, gh_vec* dst, double s, const gh_vec* src);
+/* dst += s*src: add = cublasDaxpy (:162-174), CudaSolver::Add (:417-429) */
+int gh_vec_add(gh_ctx* ctx, gh_vec* dst, double s, const gh_vec* src);
+/* y = a*y + d*v: nvector::sadd (src/Common/nvector.h:507-517) */
+int gh_vec_sadd(gh_ctx* ctx, gh_vec* y, double a, double d, const gh_vec* v);
+/* x *= s: scal = cublasDscal (:185-190) */
+int gh_vec_scal(gh_ctx* ctx, gh_vec* x, double s);
+/* Euclidean norm: norm = cublasDnrm2 (:176-183), CudaSolver::Norm (:431-447) */
+int gh_vec_norm(gh_ctx* ctx, const gh_vec* x, double* out);
+/* scalar product: StdSolver::ScalarProduct (src/Solver/stdsolver.cc:1807-1810) */
+int gh_vec_dot(gh_ctx* ctx, const gh_vec* x, const gh_vec* y, double* out);
+/* max norm: StdSolver::NewtonNorm = GlobalVector::norm_l8 (stdsolver.cc:693-697) */
+int gh_vec_norm_l8(gh_ctx* ctx, const gh_vec* x, double* out);
+
+/* ---------------------------------------------------------------- matrices */
+/* CudaCSRMatrixInterface(handle, const MatrixInterface&)
+ * (cuda/cudacsrmatrixinterface.cc:143-286) + get_inverse_diagonal (:119-141,
+ * 291-352, cuda/cusparsehelper.cu:24-118).  Node-block CSR of n block rows
+ * with B = ncomp: rowptr[n+1], col[nnz] (sorted, ColumnStencil layout),
+ * val[nnz*B*B] row-major blocks.  The matrix is stored in `prec` (GH_F64,
+ * GH_F32, or GH_F16 values with FP32 arithmetic); the inverse block diagonal
+ * (Gauss-Jordan with partial pivoting, src/LinAlg/nodematrix.h:159-213) is
+ * computed in FP64 and kept in FP64 (prec GH_F64) or FP32. */
+int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr,
+                  const int32_t* col, const double* val, int prec, gh_csr** out);
+int gh_csr_destroy(gh_csr* A);
+int64_t gh_csr_n(const gh_csr* A);
+int64_t gh_csr_nnz(const gh_csr* A);
+int gh_csr_prec(const gh_csr* A);
+/* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
+int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
+/* y = d*A^T*x + b*y (adjoint path; no reference counterpart on the device:
+ * SparseBlockMatrix::transpose / AssembleDualMatrix, src/Solver/stdsolver.cc:1815-1834) */
+int gh_spmv_t(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
+/* r = b - A*x: CudaSolver::MatrixResidual (cuda/cudasolver.cc:379-398) */
+int gh_residual(gh_ctx* ctx, const gh_csr* A, gh_vec* r, const gh_vec* x, const gh_vec* b);
+/* CudaSolver::smooth (cuda/cudasolver.cc:344-374) / StdSolver::smooth
+ * (src/Solver/stdsolver.cc:951-986): niter sweeps of
+ *   smoother 0 (jacobi):     x += omega * D^{-1} (b - A x)
+ *   smoother 1 (richardson): x += omega * (b - A x)
+ * h is scratch of x's shape; on return it holds no defined value. */
+int gh_smooth(gh_ctx* ctx, const gh_csr* A, gh_vec* x, const gh_vec* b, gh_vec* h,
+              int niter, double omega, int smoother);
+/* h = D^{-1} h: SparseBlockMatrix::Jacobi (src/LinAlg/sparseblockmatrix.xx:47-56) */
+int gh_jacobi(gh_ctx* ctx, const gh_csr* A, gh_vec* h);
+
+/* ---------------------------------------------------------------- transfer */
+/* CudaMgInterpolatorNested::BasicInit / check_matrices
+ * (cuda/cudamgintepolatornested.cc:27-110) and the CPU MgInterpolatorMatrix
+ * (src/Discretization/Q1/mginterpolatormatrix.cc:34-121): restriction and
+ * prolongation from the mesh-transfer lists c2f[ncoarse], zweier[n2][3],
+ * vierer[n4][5], achter[n8][9] (fine node, then parents). */
+int gh_transfer_create(gh_ctx* ctx, int64_t ncoarse, const int32_t* c2f, int64_t n2,
+                       const int32_t* zweier, int64_t n4, const int32_t* vierer,
+                       int64_t n8, const int32_t* achter, gh_transfer** out);
+int gh_transfer_destroy(gh_transfer* T);
+int64_t gh_transfer_nfine(const gh_transfer* T);
+/* uc = R uf: restrict_zero (cudamgintepolatornested.cc:131-139) */
+int gh_restrict_zero(gh_ctx* ctx, const gh_transfer* T, gh_vec* uc, const gh_vec* uf);
+/* uf += P uc: prolongate_add (cudamgintepolatornested.cc:160-167) */
+int gh_prolongate_add(gh_ctx* ctx, const gh_transfer* T, gh_vec* uf, const gh_vec* uc);
+
+/* ---------------------------------------------------------- hanging nodes */
+/* CudaSolver::HNZero/HNDistribute/HNAverage (cuda/cudasolver.cc:449-489) with
+ * the CPU weights (the device path of the reference selects weights by a
+ * discretisation name that never matches, cudasolver.cc:188-237):
+ * kind 12 HNStructureQ12d, 13 HNStructureQ13d, 22 HNStructureQ22d,
+ * 23 HNStructureQ23d.  edges[ne][4] = (node, p0, p1, p2),
+ * faces[nf][10] = (node, f0..f8), each sorted by node (std::map order). */
+int gh_hn_create(gh_ctx* ctx, int kind, int64_t n, int64_t ne, const int32_t* edges,
+                 int64_t nf, const int32_t* faces, gh_hn** out);
+int gh_hn_destroy(gh_hn* H);
+int gh_hn_average(gh_ctx* ctx, const gh_hn* H, gh_vec* x);
+int gh_hn_distribute(gh_ctx* ctx, const gh_hn* H, gh_vec* x);
+int gh_hn_zero(gh_ctx* ctx, const gh_hn* H, gh_vec* x);
+
+/* -------------------------------------------------------------- Dirichlet */
+/* CudaSolver::SetProblem Dirichlet mask (cuda/cudasolver.cc:272-297) and
+ * SetBoundaryVectorZero (:491-501); CPU CGDisc::StrongDirichletVectorZero
+ * (src/DofHandler/cgdisc.h:1162-1179). */
+int gh_mask_create(gh_ctx* ctx, int64_t n, int ncomp, int64_t nidx, const int32_t* idx,
+                   int ncomps, const int32_t* comps, gh_mask** out);
+int gh_mask_destroy(gh_mask* M);
+int gh_mask_zero(gh_ctx* ctx, const gh_mask* M, gh_vec* x);
+
+/* ------------------------------------------------------------ solver info */
+/* CGInfo (src/Common/cginfo.h:118-146) */
+enum { GH_WAITING = 0, GH_RUNNING, GH_CONVERGED, GH_TOO_MANY, GH_EXPLODED, GH_DIVERGING };
+typedef struct {
+  double tol, globaltol, breaktol;
+  int miniter, maxiter;
+  int status, iteration, totaliter;
+  double firstresidual, residual, aimedresidual, previousresidual;
+  double correction, previouscorrection, rate, lastrate;
+} gh_cginfo;
+void gh_cginfo_init(gh_cginfo* I, double tol, double globaltol, int maxiter);
+void gh_cginfo_reset(gh_cginfo* I);
+int gh_cginfo_check(gh_cginfo* I, double res, double cor);
+
+/* NLInfo (src/Common/nlinfo.h) */
+enum { GH_NL_WAITING = 0, GH_NL_RUNNING, GH_NL_CONVERGED, GH_NL_DIVERGED, GH_NL_SLOW,
+       GH_NL_STAGNATION, GH_NL_EXPLODED };
+typedef struct {
+  double tol, globaltol, breaktol, rho, linrho, maxresincrease;
+  int miniter, maxiter, maxrelax;
+  int status, iteration, laststepbad, relax, totaliter;
+  double firstresidual, residual, aimedresidual, previousresidual;
+  double correction, previouscorrection, rate, lastrate;
+} gh_nlinfo;
+void gh_nlinfo_init(gh_nlinfo* I, double tol, double globaltol, int maxiter);
+void gh_nlinfo_reset(gh_nlinfo* I);
+int gh_nlinfo_check(gh_nlinfo* I, int iter, double res, double cor);
+
+/* ------------------------------------------------------------- multigrid */
+/* Parameters of src/Solver/stdmultilevelsolverdata.cc:41-60 and
+ * src/Solver/solverdata.cc:41-62. */
+typedef struct {
+  char mgtype;          /* 'V', 'W', 'F' */
+  int iterpre, iterpost, iterexact;
+  double omega, mgomega;
+  int smoother;         /* 0 jacobi, 1 richardson */
+  int ndirect;          /* levels with fewer nodes get a dense FP64 direct solve */
+  int prec_maxiter;
+  double prec_tol, prec_globaltol;
+  int prec;             /* GH_F64: plain FP64 V-cycle; GH_F32 / GH_F16: FP32
+                           arithmetic with FP32 / FP16-stored matrix values */
+  int use_graph;        /* 1: replay the V-cycle from a captured HIP graph */
+} gh_mg_params;
+void gh_mg_params_default(gh_mg_params* p);
+
+/* CudaMultiLevelSolver (cuda/cudamultilevelsolver.h:15-45): NewSolver
+ * (:70-74, one level per CudaSolver) + NewMgInterpolator (:76-104).  Levels in
+ * solver numbering (0 = coarsest).  A[l] FP64 matrices (the V-cycle makes its
+ * own copies in params->prec), T[l] the transfer from level l to l-1 (T[0]
+ * unused), H[l] hanging nodes or NULL, M[l] Dirichlet masks or NULL.
+ * The handles must outlive the gh_mg. */
+int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
+                 gh_hn* const* H, gh_mask* const* M, const gh_mg_params* params,
+                 gh_mg** out);
+int gh_mg_destroy(gh_mg* mg);
+int gh_mg_nlevels(const gh_mg* mg);
+int64_t gh_mg_mgsteps(const gh_mg* mg);  /* the "--> mgstep" counter */
+
+/* LinearMg (cuda/cudamultilevelsolver.cc:106-165,
+ * src/Solver/stdmultilevelsolver.cc:411-458): MG iteration on finelevel from
+ * the content of u, until info stops it. */
+int gh_mg_linear(gh_ctx* ctx, gh_mg* mg, int finelevel, int coarselevel, gh_vec* u,
+                 const gh_vec* f, gh_cginfo* info);
+/* StdMultiLevelSolver::precondition (stdmultilevelsolver.cc:940-952):
+ * LinearMg on the finest level with the prec_* CGInfo.  x holds the initial
+ * guess (x_is_zero = 1 lets the first smoothing sweep skip A*x). */
+int gh_mg_precondition(gh_ctx* ctx, gh_mg* mg, int coarselevel, gh_vec* x,
+                       const gh_vec* y, int x_is_zero);
+
+/* GMRES::solve (src/LinAlg/gmres.cc:160-260), left preconditioned by
+ * gh_mg_precondition, FP64 Arnoldi (modified Gram-Schmidt with the reference's
+ * re-orthogonalisation test), Givens rotations and back-substitution.
+ * A is the FP64 finest-level matrix, mask its Dirichlet mask (or NULL).
+ * Returns the solve's `reached` flag in *reached. */
+int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
+                   int coarselevel, int vmax, gh_vec* x, const gh_vec* b, gh_cginfo* info,
+                   int* reached);
+
+/* StdMultiLevelSolver::Solve / newton / NewtonLinearSolve / NewtonUpdate
+ * (stdmultilevelsolver.cc:548-570, 646-673, 693-758, 832-856): FP64 iterative
+ * refinement whose correction is one GMRES(vmax) cycle.  The element residual
+ * `Form` is the assembled matrix residual f - A u with Dirichlet rows zeroed.
+ * norm_kind 0: NewtonNorm is the max norm (reference); 1: Euclidean norm. */
+typedef struct {
+  int newton_iterations, gmres_iterations;
+  int64_t mgsteps;
+  int status;
+  double first_residual, last_residual;
+} gh_solve_report;
+int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
+                    const gh_hn* hn, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
+                    gh_cginfo* linfo, gh_nlinfo* nlinfo, int norm_kind,
+                    gh_solve_report* rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

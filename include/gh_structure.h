@@ -1,0 +1,77 @@
+/*
+ * gh_structure: host-side producer of multigrid problem structure for the
+ * uniformly refined quad/hex meshes of the benchmark workloads.
+ *
+ * It stands in for Gascoigne's CPU side (MeshAgent + CGDisc), which hands the
+ * GPU path its inputs: level meshes, CSR structure, assembled matrices, mesh
+ * transfer maps and Dirichlet node lists.  Counterparts in the reference:
+ *   - coarse mesh reader: HierarchicalMesh3d::read_inp
+ *     (src/Mesh/hierarchicalmesh3d.cc:1403-1525), same .inp format
+ *   - uniform refinement: HierarchicalMesh3d::global_refine / basic_refine3d
+ *     (hierarchicalmesh3d.cc:373-472); numbering nested as in the reference
+ *     (coarse nodes keep their numbers, c2f = identity), new nodes ordered
+ *     cell midpoints, then face midpoints, then edge midpoints.  The order
+ *     inside each group follows first encounter over the cells, not the
+ *     reference's hash-map iteration order.
+ *   - mesh transfer: GascoigneMeshTransferConstructor3d
+ *     (src/Mesh/gascoignemeshtransferconstructor.cc:115-186)
+ *   - CSR structure: CGDisc::Structure + ColumnStencil
+ *     (src/DofHandler/cgdisc.h:299-315, src/LinAlg/columnstencil.cc:52-70)
+ *   - Q1 assembly: CGDisc::Matrix with Laplace<DIM> (cgdisc.h:599-635,
+ *     src/Problems/laplace.h) on axis-aligned boxes, coefficient per cell,
+ *     then SparseBlockMatrix::dirichlet (src/LinAlg/sparseblockmatrix.xx:205-218)
+ */
+#ifndef GH_STRUCTURE_H
+#define GH_STRUCTURE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ghs_hier ghs_hier;
+
+const char* ghs_last_error(void);
+
+/* Reads a 2D (quad) or 3D (hex) .inp mesh and refines it `prerefine` times.
+ * Level numbering of the result: 0 = coarsest (the .inp mesh ... after
+ * refinements), nlevels-1 = finest; nlevels = prerefine + 1 - drop_coarse. */
+int ghs_build(const char* inp_path, int dim, int prerefine, int drop_coarse, ghs_hier** out);
+int ghs_destroy(ghs_hier* h);
+int ghs_nlevels(const ghs_hier* h);
+/* sizes of level l: nodes, cells, CSR entries */
+int ghs_level_sizes(const ghs_hier* h, int l, int64_t* nnodes, int64_t* ncells, int64_t* nnz);
+int ghs_level_coords(const ghs_hier* h, int l, double* xyz /* nnodes*3 */);
+int ghs_level_cells(const ghs_hier* h, int l, int32_t* cells /* ncells*(4|8) */);
+/* CSR structure: rowptr[nnodes+1], col[nnz], sorted columns */
+int ghs_level_structure(const ghs_hier* h, int l, int32_t* rowptr, int32_t* col);
+/* Dirichlet nodes of the given colours, sorted ascending */
+int ghs_level_dirichlet_count(const ghs_hier* h, int l, int ncolours, const int32_t* colours,
+                              int64_t* count);
+int ghs_level_dirichlet(const ghs_hier* h, int l, int ncolours, const int32_t* colours,
+                        int32_t* nodes);
+/* -div(k grad u) stiffness with one coefficient per cell (kcell == NULL: k = 1),
+ * strong Dirichlet rows and columns of the given colours (identity diagonal) */
+int ghs_level_assemble(const ghs_hier* h, int l, const double* kcell, int ncolours,
+                       const int32_t* colours, double* val);
+/* transfer maps from level l to level l-1 (l >= 1) */
+int ghs_level_transfer_sizes(const ghs_hier* h, int l, int64_t* ncoarse, int64_t* n2,
+                             int64_t* n4, int64_t* n8);
+int ghs_level_transfer(const ghs_hier* h, int l, int32_t* c2f, int32_t* zweier,
+                       int32_t* vierer, int32_t* achter);
+
+/* CSR structure from a cell list (CGDisc::Structure without hanging nodes):
+ * rowptr[n+1] is filled; col may be NULL to query nnz = rowptr[n]. */
+int ghs_structure_from_cells(int64_t n, int64_t ncells, int npc, const int32_t* cells,
+                             int32_t* rowptr, int32_t* col);
+/* Q1 element-matrix assembly on a given mesh (boxes), same rules as
+ * ghs_level_assemble; rowptr/col from ghs_structure_from_cells. */
+int ghs_assemble_on(int dim, int64_t n, const double* xyz, int64_t ncells, const int32_t* cells,
+                    const double* kcell, const int32_t* rowptr, const int32_t* col,
+                    int64_t ndir, const int32_t* dir, double* val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

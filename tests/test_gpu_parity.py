@@ -1,0 +1,305 @@
+"""HIP path vs the oracle and the compiled reference's golden vectors (GPU).
+
+Tolerances (the bar of the FP64 path):
+  * integer structure, grid transfer and hanging-node operators: bit-exact
+    (their weights are powers of two or evaluated without contraction, and each
+    output is summed in the reference's order);
+  * SpMV / residual / Jacobi: per row within 4 ulp of sum_j |a_ij x_j|
+    (parallel summation order differs from the sequential CPU loop);
+  * V-cycle, GMRES and Newton iterates: 1e-12 relative to the oracle,
+    iteration counts within one.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_ctypes as O
+from conftest import fixture_path
+from gascoignelib_amd import hip as H
+from gascoignelib_amd.problem import Hierarchy, splitmix64_uniform
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada", "q1_3d_cube3_12"]
+EPS = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = H.Context(0)
+    yield c
+    c.close()
+
+
+def vec(ctx, a, prec="f64"):
+    return H.CudaVectorInterface.from_global(ctx, a, 1, prec)
+
+
+def row_scale(z, p, x):
+    A = sp.csr_matrix((np.abs(z[p + "val"]), z[p + "col"], z[p + "rowptr"]))
+    return A @ np.abs(x)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_spmv_residual_jacobi_vs_reference(ctx, golden, name):
+    z = golden(name)
+    for l in range(int(z["nlevels"])):
+        p = f"L{l}_"
+        print(f"[{name} level {l}]", flush=True)
+        A = H.CudaCSRMatrixInterface(ctx, z[p + "rowptr"], z[p + "col"], z[p + "val"])
+        x, b = z[p + "x"], z[p + "b"]
+        scale = row_scale(z, p, x) + np.abs(b)
+        X, B = vec(ctx, x), vec(ctx, b)
+        Y = H.CudaVectorInterface(ctx, len(x))
+        A.vmult(Y, X, 1.0, 0.0)
+        assert np.all(np.abs(Y.copy_back() - z[p + "Ax"]) <= 4 * EPS * scale)
+        R = H.CudaVectorInterface(ctx, len(x))
+        A.residual(R, X, B)
+        r = R.copy_back()
+        assert np.all(np.abs(r - z[p + "res"]) <= 4 * EPS * scale)
+        # Jacobi on the reference's own residual is elementwise: exact
+        R.assign(z[p + "res"])
+        A.jacobi(R)
+        assert np.array_equal(R.copy_back(), z[p + "jac"])
+        # y = d*A*x + b*y
+        Y.assign(b)
+        A.vmult(Y, X, -2.0, 0.5)
+        assert np.allclose(Y.copy_back(), -2.0 * z[p + "Ax"] + 0.5 * b, rtol=0,
+                           atol=16 * EPS * scale.max())
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_transfer_bitexact_vs_reference(ctx, golden, name):
+    z = golden(name)
+    for l in range(int(z["nlevels"]) - 1):
+        p, q = f"L{l}_", f"L{l + 1}_"
+        nc = len(z[q + "x"])
+        T = H.CudaMgInterpolatorNested(ctx, nc, z[p + "c2f"], z[p + "zweier"], z[p + "vierer"],
+                                       z[p + "achter"])
+        UF = vec(ctx, z[p + "t_in"])
+        UC = H.CudaVectorInterface(ctx, nc)
+        T.restrict_zero(UC, UF)
+        assert np.array_equal(UC.copy_back(), z[p + "restrict"])
+        UC.assign(z[q + "t_in"])
+        UF.zero()
+        T.prolongate_add(UF, UC)
+        assert np.array_equal(UF.copy_back(), z[p + "prolongate"])
+
+
+@pytest.mark.parametrize("name,kind", [("q1_3d_8_ada", 13), ("q2_3d_8_ada", 23)])
+def test_hanging_nodes_bitexact_vs_reference(ctx, golden, name, kind):
+    z = golden(name)
+    for l in range(int(z["nlevels"])):
+        p = f"L{l}_"
+        if p + "hn_average" not in z.files:
+            continue
+        n = len(z[p + "hn_in"])
+        Hs = H.HNStructure(ctx, kind, n, z[p + "hn_edge"], z[p + "hn_face"])
+        for op in ("average", "distribute", "zero"):
+            U = vec(ctx, z[p + "hn_in"])
+            getattr(Hs, op)(U)
+            assert np.array_equal(U.copy_back(), z[p + "hn_" + op]), op
+
+
+def test_vector_ops(ctx):
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(100003), rng.standard_normal(100003)
+    X, Y = vec(ctx, x), vec(ctx, y)
+    assert X.norm() == pytest.approx(np.linalg.norm(x), rel=1e-14)
+    assert X.dot(Y) == pytest.approx(x @ y, rel=1e-12)
+    assert X.norm_l8() == np.abs(x).max()
+    X.add(0.5, Y)
+    assert np.allclose(X.copy_back(), x + 0.5 * y, rtol=1e-15)
+    X.sadd(-1.0, 1.0, Y)
+    assert np.allclose(X.copy_back(), -(x + 0.5 * y) + y, rtol=1e-15, atol=1e-15)
+    X.scal(3.0)
+    X.equ(2.0, Y)
+    assert np.array_equal(X.copy_back(), 2.0 * y)
+    Z = H.CudaVectorInterface(ctx, len(x), 1, "f32")
+    Z.copy_from(Y)
+    assert np.array_equal(Z.copy_back(), y.astype(np.float32).astype(np.float64))
+    # the reductions are reproducible bit for bit
+    assert X.dot(Y) == X.dot(Y)
+
+
+def test_dirichlet_mask(ctx):
+    x = np.arange(20, dtype=np.float64) + 1.0
+    M = H.DirichletMask(ctx, 10, 2, [1, 4, 7], [1])
+    X = vec(ctx, x)
+    X2 = H.CudaVectorInterface.from_global(ctx, x, 2)
+    M.zero(X2)
+    e = x.copy()
+    e[[3, 9, 15]] = 0.0
+    assert np.array_equal(X2.copy_back(), e)
+    del X
+
+
+def test_spmv_transpose(ctx, golden):
+    z = golden("q1_3d_8_ada")
+    p = "L0_"
+    A = sp.csr_matrix((z[p + "val"], z[p + "col"], z[p + "rowptr"]))
+    M = H.CudaCSRMatrixInterface(ctx, z[p + "rowptr"], z[p + "col"], z[p + "val"])
+    x = z[p + "x"]
+    X = vec(ctx, x)
+    Y = H.CudaVectorInterface(ctx, len(x))
+    M.vmult_transpose(Y, X, 1.0, 0.0)
+    assert np.allclose(Y.copy_back(), A.T @ x, rtol=1e-13, atol=1e-14)
+
+
+def test_invalid_arguments_are_errors(ctx):
+    from gascoignelib_amd.native import GhError
+    with pytest.raises(GhError):
+        H.CudaCSRMatrixInterface(ctx, [0, 1, 2], [0, 5], [1.0, 1.0])  # column out of range
+    with pytest.raises(GhError):
+        H.CudaCSRMatrixInterface(ctx, [0, 1, 2], [1, 0], [1.0, 1.0])  # no diagonal
+    with pytest.raises(GhError):
+        H.HNStructure(ctx, 13, 4, [[3, 0, 9, 0]], np.zeros((0, 10)))  # parent out of range
+    with pytest.raises(GhError):
+        H.DirichletMask(ctx, 4, 1, [7], [0])
+    A = H.CudaCSRMatrixInterface(ctx, [0, 1, 2], [0, 1], [2.0, 2.0])
+    X, Y = H.CudaVectorInterface(ctx, 2), H.CudaVectorInterface(ctx, 3)
+    with pytest.raises(GhError):
+        A.vmult(Y, X)
+
+
+# --------------------------------------------------------------- solver parity
+def rhs(h, seed=7):
+    lev = h.fine
+    f = splitmix64_uniform(seed, lev.n * lev.ncomp)
+    O.HN(lev.hn_kind, lev.hn_edges, lev.hn_faces).distribute(f)
+    f[lev.dirichlet] = 0.0
+    return f
+
+
+MGP = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0)
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("ndirect", [1, 200])
+@pytest.mark.parametrize("mgtype", ["V", "W", "F"])
+def test_precondition_vs_oracle(ctx, name, ndirect, mgtype):
+    print(f"[{name} ndirect={ndirect} {mgtype}]", flush=True)
+    h = Hierarchy.from_fixture(fixture_path(name))
+    params = dict(MGP, mgtype=mgtype)
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=ndirect, **params)
+    om = O.MG(h, ndirect=ndirect, **params)
+    f = rhs(h)
+    xo = np.zeros_like(f)
+    om.precondition(xo, f)
+    X = mg.new_vector()
+    mg.precondition(X, vec(ctx, f), x_is_zero=True)
+    assert relerr(X.copy_back(), xo) <= 1e-12
+    assert mg.mgsteps == om.mgsteps
+    # from a non-zero initial guess
+    x0 = splitmix64_uniform(3, len(f))
+    xo = x0.copy()
+    om.precondition(xo, f)
+    X.assign(x0)
+    mg.precondition(X, vec(ctx, f), x_is_zero=False)
+    assert relerr(X.copy_back(), xo) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_8_ada"])
+def test_linear_mg_iterations_vs_oracle(ctx, name):
+    h = Hierarchy.from_fixture(fixture_path(name))
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=1, **MGP)
+    om = O.MG(h, ndirect=1, **MGP)
+    f = rhs(h)
+    io = O.cginfo(1e-6, 1e-30, 50)
+    uo = np.zeros_like(f)
+    om.linear_mg(uo, f, io)
+    ig = H.CGInfo(1e-6, 1e-30, 50)
+    U = mg.new_vector()
+    mg.LinearMg(U, vec(ctx, f), ig)
+    assert abs(ig.iteration - io.iteration) <= 1
+    assert ig.status == O.CG_STATUS[io.status]
+    if ig.iteration == io.iteration:
+        assert relerr(U.copy_back(), uo) <= 1e-12
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_gmres_vs_oracle(ctx, name):
+    h = Hierarchy.from_fixture(fixture_path(name))
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=1, **MGP)
+    om = O.MG(h, ndirect=1, **MGP)
+    f = rhs(h)
+    io = O.cginfo(1e-8, 1e-30, 100)
+    xo = np.zeros_like(f)
+    om.gmres(xo, f, io, vmax=10)
+    ig = H.CGInfo(1e-8, 1e-30, 100)
+    X = mg.new_vector()
+    mg.Gmres(X, vec(ctx, f), ig, vmax=10)
+    assert abs(ig.iteration - io.iteration) <= 1
+    if ig.iteration == io.iteration:
+        assert relerr(X.copy_back(), xo) <= 1e-12
+        assert ig.residual == pytest.approx(io.residual, rel=1e-9)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_newton_solve_vs_oracle(ctx, name):
+    """StdMultiLevelSolver::Solve (Newton + GMRES(10) + V(2,2) Jacobi) to 1e-10."""
+    print(f"[{name}]", flush=True)
+    h = Hierarchy.from_fixture(fixture_path(name))
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=1, **MGP)
+    om = O.MG(h, ndirect=1, **MGP)
+    f = rhs(h)
+    uo = np.zeros_like(f)
+    rc, rep_o = om.newton(uo, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30))
+    assert rc == 0
+    U = mg.new_vector()
+    rep = mg.Solve(U, vec(ctx, f), H.CGInfo(1e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30))
+    assert H.N.NL_STATUS[rep.status] == "converged"
+    assert abs(rep.newton_iterations - rep_o.newton_iterations) <= 1
+    assert abs(rep.gmres_iterations - rep_o.gmres_iterations) <= 1
+    assert relerr(U.copy_back(), uo) <= 1e-12
+    A = sp.csr_matrix((h.fine.val, h.fine.col, h.fine.rowptr))
+    r = f - A @ U.copy_back()
+    r[h.fine.dirichlet] = 0.0
+    assert np.abs(r).max() <= 1e-10 * rep.first_residual * 1.0001
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada"])
+def test_mixed_precision_solve(ctx, name, prec):
+    """FP32 V-cycle (FP32 or FP16-stored values) inside FP64 GMRES + iterative
+    refinement: same 1e-10 residual target, error vs the FP64 oracle stated."""
+    h = Hierarchy.from_fixture(fixture_path(name))
+    om = O.MG(h, ndirect=1, **MGP)
+    f = rhs(h)
+    uo = np.zeros_like(f)
+    om.newton(uo, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30), norm_kind=1)
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=1, prec=prec, **MGP)
+    U = mg.new_vector()
+    rep = mg.Solve(U, vec(ctx, f), H.CGInfo(1e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30),
+                   norm_kind=1)
+    assert H.N.NL_STATUS[rep.status] == "converged"
+    u = U.copy_back()
+    A = sp.csr_matrix((h.fine.val, h.fine.col, h.fine.rowptr))
+    r = f - A @ u
+    r[h.fine.dirichlet] = 0.0
+    assert np.linalg.norm(r) <= 1e-10 * rep.first_residual * 1.0001
+    err = relerr(u, uo)
+    print(f"{name} {prec}: newton {rep.newton_iterations} gmres {rep.gmres_iterations} "
+          f"rel. error vs FP64 oracle {err:.2e}")
+    assert err <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_graph_replay_matches_eager(ctx, prec):
+    h = Hierarchy.from_fixture(fixture_path("q1_3d_16"))
+    eager = H.CudaMultiLevelSolver(ctx, h, ndirect=100, prec=prec, **MGP)
+    graph = H.CudaMultiLevelSolver(ctx, h, ndirect=100, prec=prec, use_graph=True, **MGP)
+    X1, X2 = eager.new_vector(), graph.new_vector()
+    for seed in range(4):
+        f = rhs(h, seed)
+        F = vec(ctx, f)
+        eager.precondition(X1, F, x_is_zero=True)
+        graph.precondition(X2, F, x_is_zero=True)
+        # eager device work and a host round trip between replays
+        X1.add(0.0, X2)
+        assert np.array_equal(X1.copy_back(), X2.copy_back())
+    assert graph.mgsteps == eager.mgsteps
