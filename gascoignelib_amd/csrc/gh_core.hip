@@ -420,7 +420,7 @@ int gh_ctx_create(int device, gh_ctx** out) {
       hipMalloc((void**)&c->scal, sizeof(double) * kScalars) != hipSuccess ||
       hipHostMalloc((void**)&c->host_scal, sizeof(double) * kScalars, hipHostMallocDefault) !=
           hipSuccess) {
-    gh_ctx_destroy(c);
+    ctx_release(c);
     return fail(GH_ENOMEM, "gh_ctx_create: allocation failed");
   }
   GH_HIP(hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 16, c->stream));
@@ -430,8 +430,10 @@ int gh_ctx_create(int device, gh_ctx** out) {
   return GH_OK;
 }
 
-int gh_ctx_destroy(gh_ctx* ctx) {
-  if (!ctx) return GH_OK;
+}  // extern "C"
+
+namespace gh {
+static void ctx_free(gh_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->counter) (void)hipFree(ctx->counter);
@@ -439,6 +441,19 @@ int gh_ctx_destroy(gh_ctx* ctx) {
   if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+void ctx_retain(gh_ctx* ctx) {
+  if (ctx) ctx->refs.fetch_add(1);
+}
+void ctx_release(gh_ctx* ctx) {
+  if (ctx && ctx->refs.fetch_sub(1) == 1) ctx_free(ctx);
+}
+}  // namespace gh
+
+extern "C" {
+
+int gh_ctx_destroy(gh_ctx* ctx) {
+  ctx_release(ctx);
   return GH_OK;
 }
 
@@ -459,15 +474,18 @@ int gh_vec_create(gh_ctx* ctx, int64_t n, int ncomp, int prec, gh_vec** out) {
     delete v;
     return rc;
   }
+  ctx_retain(ctx);
   *out = v;
   return GH_OK;
 }
 
 int gh_vec_destroy(gh_vec* v) {
   if (!v) return GH_OK;
-  if (v->ctx && v->ctx->stream) (void)hipStreamSynchronize(v->ctx->stream);
+  gh_ctx* ctx = v->ctx;
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   vec_free(v);
   delete v;
+  ctx_release(ctx);
   return GH_OK;
 }
 
@@ -667,20 +685,24 @@ int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, cons
   if (dsing) (void)hipFree(dsing);
   if (!rc && sing) rc = fail(GH_ESINGULAR, "gh_csr_create: singular diagonal block");
   if (rc) {
+    ctx_retain(ctx);  // balanced by gh_csr_destroy
     gh_csr_destroy(A);
     return rc;
   }
+  ctx_retain(ctx);
   *out = A;
   return GH_OK;
 }
 
 int gh_csr_destroy(gh_csr* A) {
   if (!A) return GH_OK;
-  if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+  gh_ctx* ctx = A->ctx;
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void* p : {(void*)A->rowptr, (void*)A->col, A->val, A->dinv, (void*)A->t_rowptr,
                   (void*)A->t_col, (void*)A->t_pos})
     if (p) (void)hipFree(p);
   delete A;
+  ctx_release(ctx);
   return GH_OK;
 }
 
@@ -802,19 +824,23 @@ int gh_transfer_create(gh_ctx* ctx, int64_t ncoarse, const int32_t* c2f, int64_t
   if (!rc) rc = upload(ctx, &T->p_col, T->h_pcol.data(), T->h_pcol.size());
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
   if (rc) {
+    ctx_retain(ctx);  // balanced by gh_transfer_destroy
     gh_transfer_destroy(T);
     return rc;
   }
+  ctx_retain(ctx);
   *out = T;
   return GH_OK;
 }
 
 int gh_transfer_destroy(gh_transfer* T) {
   if (!T) return GH_OK;
-  if (T->ctx && T->ctx->stream) (void)hipStreamSynchronize(T->ctx->stream);
+  gh_ctx* ctx = T->ctx;
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void* p : {(void*)T->r_row, (void*)T->r_col, (void*)T->p_row, (void*)T->p_col})
     if (p) (void)hipFree(p);
   delete T;
+  ctx_release(ctx);
   return GH_OK;
 }
 
@@ -937,20 +963,24 @@ int gh_hn_create(gh_ctx* ctx, int kind, int64_t n, int64_t ne, const int32_t* ed
   if (!rc) rc = upload(ctx, &H->hanging, hanging.data(), hanging.size());
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
   if (rc) {
+    ctx_retain(ctx);  // balanced by gh_hn_destroy
     gh_hn_destroy(H);
     return rc;
   }
+  ctx_retain(ctx);
   *out = H;
   return GH_OK;
 }
 
 int gh_hn_destroy(gh_hn* H) {
   if (!H) return GH_OK;
-  if (H->ctx && H->ctx->stream) (void)hipStreamSynchronize(H->ctx->stream);
+  gh_ctx* ctx = H->ctx;
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void* p : {(void*)H->edges, (void*)H->faces, (void*)H->d_parent, (void*)H->d_row,
                   (void*)H->d_src, (void*)H->d_w, (void*)H->hanging})
     if (p) (void)hipFree(p);
   delete H;
+  ctx_release(ctx);
   return GH_OK;
 }
 
@@ -996,18 +1026,22 @@ int gh_mask_create(gh_ctx* ctx, int64_t n, int ncomp, int64_t nidx, const int32_
   int rc = upload(ctx, &M->idx, flat.data(), flat.size());
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
   if (rc) {
+    ctx_retain(ctx);  // balanced by gh_mask_destroy
     gh_mask_destroy(M);
     return rc;
   }
+  ctx_retain(ctx);
   *out = M;
   return GH_OK;
 }
 
 int gh_mask_destroy(gh_mask* M) {
   if (!M) return GH_OK;
-  if (M->ctx && M->ctx->stream) (void)hipStreamSynchronize(M->ctx->stream);
+  gh_ctx* ctx = M->ctx;
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (M->idx) (void)hipFree(M->idx);
   delete M;
+  ctx_release(ctx);
   return GH_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -53,6 +54,9 @@ constexpr int kScalars = 512;
 }  // namespace gh
 
 struct gh_ctx {
+  // the context lives until gh_ctx_destroy was called and the last object
+  // created on it is destroyed
+  std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;
   double* partials = nullptr;      // kRedBlocks * 4 doubles of block partials
@@ -134,6 +138,8 @@ struct gh_mask {
 };
 
 namespace gh {
+void ctx_retain(gh_ctx* ctx);
+void ctx_release(gh_ctx* ctx);
 // vector helpers shared by the solver translation unit
 int vec_alloc(gh_ctx* ctx, int64_t n, int ncomp, int prec, gh_vec* v);
 void vec_free(gh_vec* v);

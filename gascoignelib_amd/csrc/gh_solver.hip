@@ -583,6 +583,7 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
   }
   if (!rc) rc = dalloc(&mg->fine_b, mg->L[nlevels - 1].size * pbytes(mg->wprec));
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
+  ctx_retain(ctx);
   if (rc) {
     gh_mg_destroy(mg);
     return rc;
@@ -605,7 +606,9 @@ int gh_mg_destroy(gh_mg* mg) {
   for (void* p : {(void*)mg->p, (void*)mg->r, (void*)mg->w, mg->fine_b})
     if (p) (void)hipFree(p);
   if (mg->graph) (void)hipGraphExecDestroy(mg->graph);
+  gh_ctx* ctx = mg->ctx;
   delete mg;
+  ctx_release(ctx);
   return GH_OK;
 }
 

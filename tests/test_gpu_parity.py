@@ -4,8 +4,9 @@ Tolerances (the bar of the FP64 path):
   * integer structure, grid transfer and hanging-node operators: bit-exact
     (their weights are powers of two or evaluated without contraction, and each
     output is summed in the reference's order);
-  * SpMV / residual / Jacobi: per row within 4 ulp of sum_j |a_ij x_j|
-    (parallel summation order differs from the sequential CPU loop);
+  * SpMV / residual: per row within (m+2) eps sum_j |a_ij x_j| for a row of m
+    entries, the forward error bound of a sum in any order (the device sums a
+    row in a different order than the sequential CPU loop); Jacobi: exact;
   * V-cycle, GMRES and Newton iterates: 1e-12 relative to the oracle,
     iteration counts within one.
 """
@@ -35,9 +36,10 @@ def vec(ctx, a, prec="f64"):
     return H.CudaVectorInterface.from_global(ctx, a, 1, prec)
 
 
-def row_scale(z, p, x):
+def row_bound(z, p, x, b):
     A = sp.csr_matrix((np.abs(z[p + "val"]), z[p + "col"], z[p + "rowptr"]))
-    return A @ np.abs(x)
+    m = np.diff(z[p + "rowptr"])
+    return (m + 2) * EPS * (A @ np.abs(x) + np.abs(b))
 
 
 @pytest.mark.parametrize("name", FIXTURES)
@@ -48,15 +50,15 @@ def test_spmv_residual_jacobi_vs_reference(ctx, golden, name):
         print(f"[{name} level {l}]", flush=True)
         A = H.CudaCSRMatrixInterface(ctx, z[p + "rowptr"], z[p + "col"], z[p + "val"])
         x, b = z[p + "x"], z[p + "b"]
-        scale = row_scale(z, p, x) + np.abs(b)
+        bound = row_bound(z, p, x, b)
         X, B = vec(ctx, x), vec(ctx, b)
         Y = H.CudaVectorInterface(ctx, len(x))
         A.vmult(Y, X, 1.0, 0.0)
-        assert np.all(np.abs(Y.copy_back() - z[p + "Ax"]) <= 4 * EPS * scale)
+        assert np.all(np.abs(Y.copy_back() - z[p + "Ax"]) <= bound)
         R = H.CudaVectorInterface(ctx, len(x))
         A.residual(R, X, B)
         r = R.copy_back()
-        assert np.all(np.abs(r - z[p + "res"]) <= 4 * EPS * scale)
+        assert np.all(np.abs(r - z[p + "res"]) <= bound)
         # Jacobi on the reference's own residual is elementwise: exact
         R.assign(z[p + "res"])
         A.jacobi(R)
@@ -64,8 +66,7 @@ def test_spmv_residual_jacobi_vs_reference(ctx, golden, name):
         # y = d*A*x + b*y
         Y.assign(b)
         A.vmult(Y, X, -2.0, 0.5)
-        assert np.allclose(Y.copy_back(), -2.0 * z[p + "Ax"] + 0.5 * b, rtol=0,
-                           atol=16 * EPS * scale.max())
+        assert np.all(np.abs(Y.copy_back() - (-2.0 * z[p + "Ax"] + 0.5 * b)) <= 4 * bound)
 
 
 @pytest.mark.parametrize("name", FIXTURES)
