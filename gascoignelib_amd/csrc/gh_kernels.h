@@ -15,12 +15,17 @@ __device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); 
 __device__ __forceinline__ float ld(const float* p) { return *p; }
 __device__ __forceinline__ double ld(const double* p) { return *p; }
 
-template <typename T> __device__ __forceinline__ T mul_rn(T a, T b);
-template <> __device__ __forceinline__ double mul_rn<double>(double a, double b) { return __dmul_rn(a, b); }
-template <> __device__ __forceinline__ float mul_rn<float>(float a, float b) { return __fmul_rn(a, b); }
-template <typename T> __device__ __forceinline__ T add_rn(T a, T b);
-template <> __device__ __forceinline__ double add_rn<double>(double a, double b) { return __dadd_rn(a, b); }
-template <> __device__ __forceinline__ float add_rn<float>(float a, float b) { return __fadd_rn(a, b); }
+// Separately rounded multiply and add: the operations carry no contraction
+// flag, so they are never fused into an FMA, even after inlining.  Used where
+// the reference's sequential CPU expression must be reproduced bit for bit.
+template <typename T> __device__ __forceinline__ T mul_rn(T a, T b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+template <typename T> __device__ __forceinline__ T add_rn(T a, T b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
 
 // weight code of packed transfer columns: w = 2^-code (1, 1/2, 1/4, 1/8)
 constexpr uint32_t kIdxMask = 0x1FFFFFFFu;
