@@ -2,13 +2,14 @@
 // transfer, hanging nodes, Dirichlet masks and the launch wrappers around the
 // kernels of gh_kernels.h.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
 
-#include "gh_kernels.h"
+#include "gh_sell.h"
 
 namespace gh {
 
@@ -106,18 +107,35 @@ __global__ __launch_bounds__(kBlock) void k_mgs(int64_t n, double* __restrict__ 
                                                 double* partials, unsigned* counter,
                                                 double* out) {
   const double md = prev ? -(*dslot) : 0.0;
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double v[4], w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = vv[i + u * stride];
+    if (prev) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[u] += md * prev[i + u * stride];
+        vv[i + u * stride] = v[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = next ? next[i + u * stride] : v[u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] += v[u] * w[u];
+  }
+  for (; i < n; i += stride) {
     double v = vv[i];
     if (prev) {
       v += md * prev[i];
       vv[i] = v;
     }
-    acc += v * (next ? next[i] : v);
+    acc[0] += v * (next ? next[i] : v);
   }
-  acc = block_reduce<false>(acc);
-  finish_reduction<false>(acc, partials, counter, out);
+  double a = block_reduce<false>((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  finish_reduction<false>(a, partials, counter, out);
 }
 
 int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
@@ -216,7 +234,7 @@ static int launch_csr(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
 
 template <typename XT, typename Acc, class Epi>
 static int dispatch_val(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
-  if (A->ncomp > 1) {
+  {
     unsigned grid = grid_for(A->n * A->ncomp);
     if (A->prec == GH_F64)
       hipLaunchKernelGGL((k_bsr<double, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
@@ -238,6 +256,21 @@ static int dispatch_val(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
 int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const void* x,
              double d, double b) {
   const bool ry = (b != 0.0);
+  if (A->ncomp == 1) {
+    // FP64 arithmetic in the stored row order, uncontracted: the reference's sums
+    if (xprec == GH_F64 && yprec == GH_F64)
+      return sell_dispatch<double, double>(ctx, A->sell, (const double*)x,
+                                           SpmvOp<double, double, true>{(double*)y, d, b, ry});
+    if (xprec == GH_F64 && yprec == GH_F32)
+      return sell_dispatch<double, double>(ctx, A->sell, (const double*)x,
+                                           SpmvOp<float, double, true>{(float*)y, d, b, ry});
+    if (xprec == GH_F32 && yprec == GH_F32)
+      return sell_dispatch<float, float>(ctx, A->sell, (const float*)x,
+                                         SpmvOp<float, float, false>{(float*)y, (float)d, (float)b, ry});
+    if (xprec == GH_F32 && yprec == GH_F64)
+      return sell_dispatch<float, float>(ctx, A->sell, (const float*)x,
+                                         SpmvOp<double, float, false>{(double*)y, (float)d, (float)b, ry});
+  }
   if (xprec == GH_F64 && yprec == GH_F64)
     return dispatch_val<double, double>(ctx, A, (const double*)x,
                                         EpiSpmv<double, double>{(double*)y, d, b, ry});
@@ -254,6 +287,13 @@ int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const 
 }
 
 int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b) {
+  if (A->ncomp == 1) {
+    if (prec == GH_F64)
+      return sell_dispatch<double, double>(ctx, A->sell, (const double*)x,
+                                           ResidualOp<double, true>{(double*)r, (const double*)b});
+    return sell_dispatch<float, float>(ctx, A->sell, (const float*)x,
+                                       ResidualOp<float, false>{(float*)r, (const float*)b});
+  }
   if (prec == GH_F64)
     return dispatch_val<double, double>(ctx, A, (const double*)x,
                                         EpiResidual<double>{(double*)r, (const double*)b});
@@ -289,8 +329,9 @@ static int jacobi_impl(gh_ctx* ctx, const gh_csr* A, T* xn, const T* x, const T*
     GH_HIP(hipGetLastError());
     return GH_OK;
   }
-  if (smoother == 0) return dispatch_val<T, T>(ctx, A, x, EpiJacobi<T, DT>{xn, x, b, dinv, omega});
-  return dispatch_val<T, T>(ctx, A, x, EpiRichardson<T>{xn, x, b, omega});
+  constexpr bool exact = std::is_same<T, double>::value;
+  return sell_dispatch<T, T>(ctx, A->sell, x,
+                             JacobiOp<T, DT, exact>{xn, x, b, dinv, omega, smoother});
 }
 
 int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const void* x,
@@ -307,6 +348,11 @@ int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const v
 // ------------------------------------------------------------- transfers
 int transfer_restrict(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, void* uc,
                       const void* uf, const int32_t* /*unused*/) {
+  if (ncomp == 1) {
+    if (prec == GH_F64)
+      return sell_w_launch<double>(ctx, T->r_sell, (const double*)uf, RestrictOp<double>{(double*)uc});
+    return sell_w_launch<float>(ctx, T->r_sell, (const float*)uf, RestrictOp<float>{(float*)uc});
+  }
   unsigned grid = grid_for(T->ncoarse * ncomp);
   if (prec == GH_F64)
     hipLaunchKernelGGL(k_restrict<double>, dim3(grid), dim3(kBlock), 0, ctx->stream, T->ncoarse,
@@ -320,6 +366,12 @@ int transfer_restrict(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, vo
 
 int transfer_prolongate_add(gh_ctx* ctx, const gh_transfer* T, int ncomp, int prec, void* uf,
                             const void* uc) {
+  if (ncomp == 1) {
+    if (prec == GH_F64)
+      return sell_w_launch<double>(ctx, T->p_sell, (const double*)uc,
+                                   ProlongAddOp<double>{(double*)uf});
+    return sell_w_launch<float>(ctx, T->p_sell, (const float*)uc, ProlongAddOp<float>{(float*)uf});
+  }
   unsigned grid = grid_for(T->nfine * ncomp);
   if (prec == GH_F64)
     hipLaunchKernelGGL(k_prolongate_add<double>, dim3(grid), dim3(kBlock), 0, ctx->stream,
@@ -386,6 +438,85 @@ static int upload(gh_ctx* ctx, T** dptr, const T* host, size_t count) {
     return fail(GH_ENOMEM, "hipMalloc(%zu) failed", bytes);
   if (count) GH_HIP(hipMemcpyAsync(*dptr, host, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
   return GH_OK;
+}
+
+int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
+               const double* vals, int prec, Sell* S) {
+  S->n = n;
+  S->prec = prec;
+  S->nslices = (n + kSlice - 1) / kSlice;
+  std::vector<uint32_t> sptr(S->nslices + 1, 0);
+  std::vector<uint16_t> rlen(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t len = rowptr[i + 1] - rowptr[i];
+    if (len > 65535) return fail(GH_EINVAL, "row %lld has more than 65535 entries", (long long)i);
+    rlen[i] = (uint16_t)len;
+  }
+  uint64_t total = 0;
+  for (int64_t s = 0; s < S->nslices; ++s) {
+    int w = 0;
+    for (int64_t i = s * kSlice; i < std::min<int64_t>(n, (s + 1) * kSlice); ++i)
+      w = std::max<int>(w, rlen[i]);
+    sptr[s] = (uint32_t)total;
+    total += (uint64_t)w * kSlice;
+    if (total > 0xFFFFFFFFull) return fail(GH_EINVAL, "operator too large for 32-bit slot offsets");
+  }
+  sptr[S->nslices] = (uint32_t)total;
+  S->slots = (int64_t)total;
+  std::vector<uint32_t> scol(total);
+  std::vector<double> sval(vals ? total : 0, 0.0);
+  for (int64_t s = 0; s < S->nslices; ++s) {
+    const int w = (int)((sptr[s + 1] - sptr[s]) / kSlice);
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int64_t i = s * kSlice + lane;
+      for (int k = 0; k < w; ++k) {
+        const uint64_t slot = sptr[s] + (uint64_t)k * kSlice + lane;
+        if (i < n && k < rlen[i]) {
+          scol[slot] = col[rowptr[i] + k];
+          if (vals) sval[slot] = vals[rowptr[i] + k];
+        } else {
+          scol[slot] = (uint32_t)(i < n ? i : 0);  // padding, never read
+        }
+      }
+    }
+  }
+  GH_TRY(upload(ctx, &S->sptr, sptr.data(), sptr.size()));
+  GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
+  GH_TRY(upload(ctx, &S->col, scol.data(), scol.size()));
+  if (vals) {
+    if (prec == GH_F64) {
+      double* d = nullptr;
+      GH_TRY(upload(ctx, &d, sval.data(), sval.size()));
+      S->val = d;
+    } else {
+      double* tmp = nullptr;
+      GH_TRY(upload(ctx, &tmp, sval.data(), sval.size()));
+      const size_t m = std::max<size_t>(sval.size(), 1);
+      if (hipMalloc(&S->val, m * (prec == GH_F32 ? 4 : 2)) != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(GH_ENOMEM, "sell_build: allocation failed");
+      }
+      if (prec == GH_F32)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(total)), dim3(kBlock), 0,
+                           ctx->stream, (int64_t)total, (float*)S->val, (const double*)tmp);
+      else
+        hipLaunchKernelGGL((k_convert<__half, double>), dim3(grid_stride(total)), dim3(kBlock), 0,
+                           ctx->stream, (int64_t)total, (__half*)S->val, (const double*)tmp);
+      GH_HIP(hipStreamSynchronize(ctx->stream));
+      (void)hipFree(tmp);
+    }
+  }
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+void sell_free(Sell* S) {
+  for (void* p : {(void*)S->sptr, (void*)S->rlen, (void*)S->col, S->val})
+    if (p) (void)hipFree(p);
+  S->sptr = nullptr;
+  S->rlen = nullptr;
+  S->col = nullptr;
+  S->val = nullptr;
 }
 
 }  // namespace gh
@@ -605,6 +736,7 @@ int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, cons
   GH_ARG(ctx && out && rowptr && n >= 0 && ncomp >= 1 && ncomp <= 8,
          "gh_csr_create: bad arguments (ncomp must be 1..8)");
   GH_ARG(prec == GH_F64 || prec == GH_F32 || prec == GH_F16, "gh_csr_create: bad precision");
+  GH_ARG(n < (int64_t)kIdxMask, "gh_csr_create: too many rows");
   *out = nullptr;
   const int64_t nnz = rowptr[n];
   GH_ARG(rowptr[0] == 0 && nnz >= 0 && (nnz == 0 || (col && val)), "gh_csr_create: bad rowptr");
@@ -676,6 +808,9 @@ int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, cons
                            (const double*)dval64);
     }
   }
+  if (!rc && ncomp == 1) {
+    rc = sell_build(ctx, n, rowptr, reinterpret_cast<const uint32_t*>(col), val, prec, &A->sell);
+  }
   int sing = 0;
   if (!rc && hipMemcpyAsync(&sing, dsing, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
     rc = GH_EHIP;
@@ -701,6 +836,7 @@ int gh_csr_destroy(gh_csr* A) {
   for (void* p : {(void*)A->rowptr, (void*)A->col, A->val, A->dinv, (void*)A->t_rowptr,
                   (void*)A->t_col, (void*)A->t_pos})
     if (p) (void)hipFree(p);
+  sell_free(&A->sell);
   delete A;
   ctx_release(ctx);
   return GH_OK;
@@ -822,6 +958,8 @@ int gh_transfer_create(gh_ctx* ctx, int64_t ncoarse, const int32_t* c2f, int64_t
   if (!rc) rc = upload(ctx, &T->r_col, T->h_rcol.data(), T->h_rcol.size());
   if (!rc) rc = upload(ctx, &T->p_row, T->h_prow.data(), T->h_prow.size());
   if (!rc) rc = upload(ctx, &T->p_col, T->h_pcol.data(), T->h_pcol.size());
+  if (!rc) rc = sell_build(ctx, ncoarse, T->h_rrow.data(), T->h_rcol.data(), nullptr, GH_F64, &T->r_sell);
+  if (!rc) rc = sell_build(ctx, nfine, T->h_prow.data(), T->h_pcol.data(), nullptr, GH_F64, &T->p_sell);
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
   if (rc) {
     ctx_retain(ctx);  // balanced by gh_transfer_destroy
@@ -839,6 +977,8 @@ int gh_transfer_destroy(gh_transfer* T) {
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void* p : {(void*)T->r_row, (void*)T->r_col, (void*)T->p_row, (void*)T->p_col})
     if (p) (void)hipFree(p);
+  sell_free(&T->r_sell);
+  sell_free(&T->p_sell);
   delete T;
   ctx_release(ctx);
   return GH_OK;

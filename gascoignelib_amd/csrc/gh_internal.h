@@ -40,7 +40,7 @@ int fail(int code, const char* fmt, ...);
 
 // launch geometry
 constexpr int kBlock = 256;       // 4 wavefronts of 64
-constexpr int kRedBlocks = 1024;  // fixed grid of every reduction: deterministic sums
+constexpr int kRedBlocks = 2048;  // fixed grid of every reduction: deterministic sums
 
 inline unsigned grid_for(int64_t work, int per_block = kBlock) {
   int64_t g = (work + per_block - 1) / per_block;
@@ -50,6 +50,24 @@ inline unsigned grid_for(int64_t work, int per_block = kBlock) {
 
 // number of device scalar slots for reduction results
 constexpr int kScalars = 512;
+
+constexpr int kSlice = 64;
+
+struct Sell {
+  int64_t n = 0;          // rows
+  int64_t nslices = 0;
+  int64_t slots = 0;      // padded entries
+  uint32_t* sptr = nullptr;  // nslices+1 slot offsets
+  uint16_t* rlen = nullptr;  // entries per row
+  uint32_t* col = nullptr;   // column (or column | weight code << 29)
+  void* val = nullptr;       // values in `prec`, nullptr for packed-weight operators
+  int prec = GH_F64;
+};
+
+// Host-side layout from CSR.  vals (double, may be empty) are stored as `prec`.
+int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
+               const double* vals, int prec, Sell* S);
+void sell_free(Sell* S);
 
 }  // namespace gh
 
@@ -86,7 +104,8 @@ struct gh_csr {
   int32_t* col = nullptr;
   void* val = nullptr;   // nnz*B*B in prec (double / float / half)
   void* dinv = nullptr;  // n*B*B inverse diagonal blocks: double (F64) or float
-  int seg = 8;           // lanes per row in the scalar kernels
+  int seg = 8;           // lanes per row in the scalar CSR kernels
+  gh::Sell sell;         // SELL-64 copy used by the scalar (ncomp == 1) kernels
   // host copies kept for derived structures (transpose, coarse direct solve)
   std::vector<int32_t> h_rowptr, h_col;
   std::vector<double> h_val;
@@ -108,6 +127,7 @@ struct gh_transfer {
   uint32_t* p_col = nullptr;
   std::vector<int32_t> h_rrow, h_prow;
   std::vector<uint32_t> h_rcol, h_pcol;
+  gh::Sell r_sell, p_sell;  // SELL-64 copies for scalar vectors
 };
 
 struct gh_hn {

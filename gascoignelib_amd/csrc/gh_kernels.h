@@ -91,15 +91,23 @@ __device__ __forceinline__ void finish_reduction(double part, double* partials,
   }
 }
 
+// Grid-stride loops with four independent accumulators keep four loads per
+// operand in flight per lane (the reductions are bandwidth kernels).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T* __restrict__ x,
                                                 const T* __restrict__ y, double* partials,
                                                 unsigned* counter, double* out) {
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock)
-    acc += (double)x[i] * (double)y[i];
-  acc = block_reduce<false>(acc);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    a0 += (double)x[i] * (double)y[i];
+    a1 += (double)x[i + stride] * (double)y[i + stride];
+    a2 += (double)x[i + 2 * stride] * (double)y[i + 2 * stride];
+    a3 += (double)x[i + 3 * stride] * (double)y[i + 3 * stride];
+  }
+  for (; i < n; i += stride) a0 += (double)x[i] * (double)y[i];
+  double acc = block_reduce<false>((a0 + a1) + (a2 + a3));
   finish_reduction<false>(acc, partials, counter, out);
 }
 
@@ -107,11 +115,17 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_amax(int64_t n, const T* __restrict__ x,
                                                  double* partials, unsigned* counter,
                                                  double* out) {
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock)
-    acc = fmax(acc, fabs((double)x[i]));
-  acc = block_reduce<true>(acc);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    a0 = fmax(a0, fabs((double)x[i]));
+    a1 = fmax(a1, fabs((double)x[i + stride]));
+    a2 = fmax(a2, fabs((double)x[i + 2 * stride]));
+    a3 = fmax(a3, fabs((double)x[i + 3 * stride]));
+  }
+  for (; i < n; i += stride) a0 = fmax(a0, fabs((double)x[i]));
+  double acc = block_reduce<true>(fmax(fmax(a0, a1), fmax(a2, a3)));
   finish_reduction<true>(acc, partials, counter, out);
 }
 

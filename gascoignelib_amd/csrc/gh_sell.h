@@ -1,0 +1,184 @@
+// SELL-64 storage and kernels for scalar sparse operators.
+//
+// Rows are grouped in slices of 64 (one wavefront).  Inside a slice the k-th
+// entry of every row is stored contiguously (column-major), so each lane
+// streams its own row while the wavefront's loads stay fully coalesced: 64
+// consecutive columns (256 B) and values (512 B in FP64) per step.  One thread
+// owns one row and sums it in the stored (ColumnStencil) order, so with
+// EXACT = true the row sums reproduce the reference's sequential loops bit for
+// bit (SparseBlockMatrix::vmult, src/LinAlg/sparseblockmatrix.xx:378-392;
+// MgInterpolatorMatrix::restrict_zero / prolongate_add,
+// src/Discretization/Q1/mginterpolatormatrix.cc:123-147).
+#pragma once
+
+#include "gh_kernels.h"
+
+namespace gh {
+
+// ------------------------------------------------------------------ ops
+// init(row) gives the accumulator start, step folds one entry, finish writes.
+template <typename T, bool EXACT>
+__device__ __forceinline__ T fma_step(T acc, T a, T x) {
+  if constexpr (EXACT) return add_rn(acc, mul_rn(a, x));
+  else return acc + a * x;
+}
+template <typename T, bool EXACT>
+__device__ __forceinline__ T fms_step(T acc, T a, T x) {
+  if constexpr (EXACT) return add_rn(acc, -mul_rn(a, x));
+  else return acc - a * x;
+}
+
+template <typename Y, typename T, bool EXACT>
+struct SpmvOp {  // y = d * (A x) + b * y
+  Y* y;
+  T d, b;
+  bool read_y;
+  __device__ __forceinline__ T init(int64_t) const { return (T)0; }
+  __device__ __forceinline__ T step(T acc, T a, T x) const { return fma_step<T, EXACT>(acc, a, x); }
+  __device__ __forceinline__ void finish(int64_t r, T acc) const {
+    T v = EXACT ? mul_rn(d, acc) : d * acc;
+    if (read_y) v = EXACT ? add_rn(v, mul_rn(b, (T)y[r])) : v + b * (T)y[r];
+    y[r] = (Y)v;
+  }
+};
+
+template <typename T, bool EXACT>
+struct ResidualOp {  // r = b - A x: r starts at b and subtracts each a_ij x_j in order
+  T* r;
+  const T* b;
+  __device__ __forceinline__ T init(int64_t i) const { return b[i]; }
+  __device__ __forceinline__ T step(T acc, T a, T x) const { return fms_step<T, EXACT>(acc, a, x); }
+  __device__ __forceinline__ void finish(int64_t i, T acc) const { r[i] = acc; }
+};
+
+template <typename T, typename DT, bool EXACT>
+struct JacobiOp {  // xn = x + omega * (dinv * (b - A x))
+  T* xn;
+  const T* x;
+  const T* b;
+  const DT* dinv;
+  T omega;
+  int richardson;
+  __device__ __forceinline__ T init(int64_t i) const { return b[i]; }
+  __device__ __forceinline__ T step(T acc, T a, T xx) const { return fms_step<T, EXACT>(acc, a, xx); }
+  __device__ __forceinline__ void finish(int64_t i, T h) const {
+    if (!richardson) h = EXACT ? mul_rn((T)dinv[i], h) : (T)dinv[i] * h;
+    xn[i] = EXACT ? add_rn(x[i], mul_rn(omega, h)) : x[i] + omega * h;
+  }
+};
+
+template <typename T>
+struct RestrictOp {  // uc = R uf (packed weights)
+  T* uc;
+  __device__ __forceinline__ T init(int64_t) const { return (T)0; }
+  __device__ __forceinline__ T step(T acc, T w, T x) const { return add_rn(acc, mul_rn(w, x)); }
+  __device__ __forceinline__ void finish(int64_t i, T acc) const { uc[i] = acc; }
+};
+
+template <typename T>
+struct ProlongAddOp {  // uf += P uc, term by term
+  T* uf;
+  __device__ __forceinline__ T init(int64_t i) const { return uf[i]; }
+  __device__ __forceinline__ T step(T acc, T w, T x) const { return add_rn(acc, mul_rn(w, x)); }
+  __device__ __forceinline__ void finish(int64_t i, T acc) const { uf[i] = acc; }
+};
+
+template <typename T, typename UT>
+struct CorrectOp {  // v = P' uc ; u += mgomega * v  (empty rows leave u as it is)
+  UT* u;
+  T mgomega;
+  __device__ __forceinline__ T init(int64_t) const { return (T)0; }
+  __device__ __forceinline__ T step(T acc, T w, T x) const { return add_rn(acc, mul_rn(w, x)); }
+  __device__ __forceinline__ void finish(int64_t i, T v) const {
+    u[i] = (UT)add_rn((T)u[i], mul_rn(mgomega, v));
+  }
+};
+
+// ---------------------------------------------------------------- kernels
+// valued operator (matrix); VT storage type, T arithmetic type
+template <typename VT, typename XT, typename T, class Op>
+__global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __restrict__ sptr,
+                                                 const uint16_t* __restrict__ rlen,
+                                                 const uint32_t* __restrict__ col,
+                                                 const VT* __restrict__ val,
+                                                 const XT* __restrict__ x, Op op) {
+  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const uint32_t base = sptr[row >> 6] + lane;
+  const int len = rlen[row];
+  const uint32_t* cp = col + base;
+  const VT* vp = val + base;
+  T acc = op.init(row);
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const uint32_t c0 = cp[(k + 0) * kSlice], c1 = cp[(k + 1) * kSlice];
+    const uint32_t c2 = cp[(k + 2) * kSlice], c3 = cp[(k + 3) * kSlice];
+    const T a0 = (T)ld(vp + (k + 0) * kSlice), a1 = (T)ld(vp + (k + 1) * kSlice);
+    const T a2 = (T)ld(vp + (k + 2) * kSlice), a3 = (T)ld(vp + (k + 3) * kSlice);
+    const T x0 = (T)x[c0], x1 = (T)x[c1], x2 = (T)x[c2], x3 = (T)x[c3];
+    acc = op.step(acc, a0, x0);
+    acc = op.step(acc, a1, x1);
+    acc = op.step(acc, a2, x2);
+    acc = op.step(acc, a3, x3);
+  }
+  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * kSlice), (T)x[cp[k * kSlice]]);
+  op.finish(row, acc);
+}
+
+// packed-weight operator (grid transfer): w = 2^-code from the column's top bits
+template <typename T, class Op>
+__global__ __launch_bounds__(kBlock) void k_sell_w(int64_t n, const uint32_t* __restrict__ sptr,
+                                                   const uint16_t* __restrict__ rlen,
+                                                   const uint32_t* __restrict__ col,
+                                                   const T* __restrict__ x, Op op) {
+  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const uint32_t* cp = col + sptr[row >> 6] + lane;
+  const int len = rlen[row];
+  T acc = op.init(row);
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const uint32_t e0 = cp[(k + 0) * kSlice], e1 = cp[(k + 1) * kSlice];
+    const uint32_t e2 = cp[(k + 2) * kSlice], e3 = cp[(k + 3) * kSlice];
+    const T x0 = x[e0 & kIdxMask], x1 = x[e1 & kIdxMask], x2 = x[e2 & kIdxMask],
+            x3 = x[e3 & kIdxMask];
+    acc = op.step(acc, code_weight<T>(e0 >> 29), x0);
+    acc = op.step(acc, code_weight<T>(e1 >> 29), x1);
+    acc = op.step(acc, code_weight<T>(e2 >> 29), x2);
+    acc = op.step(acc, code_weight<T>(e3 >> 29), x3);
+  }
+  for (; k < len; ++k) {
+    const uint32_t e = cp[k * kSlice];
+    acc = op.step(acc, code_weight<T>(e >> 29), x[e & kIdxMask]);
+  }
+  op.finish(row, acc);
+}
+
+template <typename VT, typename XT, typename T, class Op>
+inline int sell_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
+  if (S.n == 0) return GH_OK;
+  hipLaunchKernelGGL((k_sell<VT, XT, T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream,
+                     S.n, S.sptr, S.rlen, S.col, (const VT*)S.val, x, op);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+template <typename XT, typename T, class Op>
+inline int sell_dispatch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
+  if (S.prec == GH_F64) return sell_launch<double, XT, T, Op>(ctx, S, x, op);
+  if (S.prec == GH_F32) return sell_launch<float, XT, T, Op>(ctx, S, x, op);
+  return sell_launch<__half, XT, T, Op>(ctx, S, x, op);
+}
+
+template <typename T, class Op>
+inline int sell_w_launch(gh_ctx* ctx, const Sell& S, const T* x, Op op) {
+  if (S.n == 0) return GH_OK;
+  hipLaunchKernelGGL((k_sell_w<T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream, S.n,
+                     S.sptr, S.rlen, S.col, x, op);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+}  // namespace gh

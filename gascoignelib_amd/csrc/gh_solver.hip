@@ -10,7 +10,7 @@
 #include <cstring>
 #include <vector>
 
-#include "gh_kernels.h"
+#include "gh_sell.h"
 
 using namespace gh;
 
@@ -190,6 +190,7 @@ struct MgLevel {
   // prolongation from level l-1 with hanging / Dirichlet rows of this level emptied
   int32_t* p_row = nullptr;
   uint32_t* p_col = nullptr;
+  Sell r_sell, p_sell;           // SELL-64 copies of the folded operators (ncomp == 1)
   bool fused_correction = true;  // mask covers every component
   bool mask_in_restrict = true;  // coarse Dirichlet rows emptied in r_row (set on level l)
   void* u = nullptr;
@@ -326,6 +327,13 @@ int correction(gh_mg* mg, int l) {
   MgLevel& Cc = mg->L[l - 1];
   gh_ctx* ctx = mg->ctx;
   if (Cc.H) GH_TRY(hn_apply(ctx, Cc.H, Cc.ncomp, mg->wprec, Cc.u, 0));  // HNAverage(u_{l-1})
+  if (F.fused_correction && F.ncomp == 1) {
+    if (mg->wprec == GH_F64)
+      return sell_w_launch<double>(ctx, F.p_sell, (const double*)Cc.u,
+                                   CorrectOp<double, double>{(double*)F.u, mg->P.mgomega});
+    return sell_w_launch<float>(ctx, F.p_sell, (const float*)Cc.u,
+                                CorrectOp<float, float>{(float*)F.u, (float)mg->P.mgomega});
+  }
   if (F.fused_correction) {
     unsigned grid = grid_for(F.size);
     if (mg->wprec == GH_F64)
@@ -374,7 +382,14 @@ int mgstep(gh_mg* mg, int l, int finelevel, int clevel, char& p0, char p, bool u
     GH_TRY(csr_residual(ctx, L.A, mg->wprec, L.v, L.u, L.b));
   }
   MgLevel& Cc = mg->L[l - 1];
-  {
+  if (Cc.ncomp == 1) {
+    if (mg->wprec == GH_F64)
+      GH_TRY(sell_w_launch<double>(ctx, L.r_sell, (const double*)L.v,
+                                   RestrictOp<double>{(double*)Cc.b}));
+    else
+      GH_TRY(sell_w_launch<float>(ctx, L.r_sell, (const float*)L.v,
+                                  RestrictOp<float>{(float*)Cc.b}));
+  } else {
     unsigned grid = grid_for(Cc.size);
     if (mg->wprec == GH_F64)
       hipLaunchKernelGGL(k_restrict<double>, dim3(grid), dim3(kBlock), 0, ctx->stream, Cc.n,
@@ -580,6 +595,8 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
     if (!rc) rc = dupload(ctx, &F.r_col, rcol);
     if (!rc) rc = dupload(ctx, &F.p_row, prow);
     if (!rc) rc = dupload(ctx, &F.p_col, pcol);
+    if (!rc) rc = sell_build(ctx, C.n, rrow.data(), rcol.data(), nullptr, GH_F64, &F.r_sell);
+    if (!rc) rc = sell_build(ctx, F.n, prow.data(), pcol.data(), nullptr, GH_F64, &F.p_sell);
   }
   if (!rc) rc = dalloc(&mg->fine_b, mg->L[nlevels - 1].size * pbytes(mg->wprec));
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
@@ -600,6 +617,8 @@ int gh_mg_destroy(gh_mg* mg) {
     for (void* p : {(void*)L.r_row, (void*)L.r_col, (void*)L.p_row, (void*)L.p_col, L.u, L.b,
                     L.v, (void*)L.Ainv})
       if (p) (void)hipFree(p);
+    sell_free(&L.r_sell);
+    sell_free(&L.p_sell);
   }
   for (double* m : mg->mem)
     if (m) (void)hipFree(m);
