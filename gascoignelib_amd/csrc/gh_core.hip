@@ -73,40 +73,58 @@ int fetch_scalars(gh_ctx* ctx, int first, int count, double* out) {
   return GH_OK;
 }
 
-int reduce_dot(gh_ctx* ctx, int prec, int64_t size, const void* x, const void* y, int slot) {
-  if (prec == GH_F64)
-    hipLaunchKernelGGL(k_dot<double>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
-                       (const double*)x, (const double*)y, ctx->partials, ctx->counter,
-                       ctx->scal + slot);
+int finalize_slots(gh_ctx* ctx, int first, int count, bool max) {
+  if (max)
+    hipLaunchKernelGGL(k_finalize<true>, dim3(count), dim3(kBlock), 0, ctx->stream,
+                       ctx->partials + (int64_t)first * kRedBlocks, ctx->scal + first);
   else
-    hipLaunchKernelGGL(k_dot<float>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
-                       (const float*)x, (const float*)y, ctx->partials, ctx->counter,
-                       ctx->scal + slot);
+    hipLaunchKernelGGL(k_finalize<false>, dim3(count), dim3(kBlock), 0, ctx->stream,
+                       ctx->partials + (int64_t)first * kRedBlocks, ctx->scal + first);
   GH_HIP(hipGetLastError());
   return GH_OK;
+}
+
+int reduce_dot(gh_ctx* ctx, int prec, int64_t size, const void* x, const void* y, int slot) {
+  double* part = ctx->partials + (int64_t)slot * kRedBlocks;
+  if (prec == GH_F64)
+    hipLaunchKernelGGL(k_dot_part<double>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const double*)x, (const double*)y, part);
+  else
+    hipLaunchKernelGGL(k_dot_part<float>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const float*)x, (const float*)y, part);
+  GH_HIP(hipGetLastError());
+  return finalize_slots(ctx, slot, 1, false);
 }
 
 int reduce_norm_l8(gh_ctx* ctx, int prec, int64_t size, const void* x, int slot) {
+  double* part = ctx->partials + (int64_t)slot * kRedBlocks;
   if (prec == GH_F64)
-    hipLaunchKernelGGL(k_amax<double>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
-                       (const double*)x, ctx->partials, ctx->counter, ctx->scal + slot);
+    hipLaunchKernelGGL(k_amax_part<double>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const double*)x, part);
   else
-    hipLaunchKernelGGL(k_amax<float>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
-                       (const float*)x, ctx->partials, ctx->counter, ctx->scal + slot);
+    hipLaunchKernelGGL(k_amax_part<float>, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size,
+                       (const float*)x, part);
   GH_HIP(hipGetLastError());
-  return GH_OK;
+  return finalize_slots(ctx, slot, 1, true);
 }
 
 // Modified Gram-Schmidt sub-step of GMRES::orthogonalization (gmres.cc:109-120):
-//   vv += (-d) * prev  (d read from a device slot), then partial of vv.next
-//   (or vv.vv when next == nullptr).
-__global__ __launch_bounds__(kBlock) void k_mgs(int64_t n, double* __restrict__ vv,
-                                                const double* __restrict__ prev,
-                                                const double* __restrict__ dslot,
-                                                const double* __restrict__ next,
-                                                double* partials, unsigned* counter,
-                                                double* out) {
-  const double md = prev ? -(*dslot) : 0.0;
+//   d = fold of the previous step's partials (the same fold k_finalize does, so
+//   the host's Hessenberg entry and the device's d are the same number),
+//   vv += (-d) * prev, then the partials of vv.next (or vv.vv when next == nullptr).
+__global__ __launch_bounds__(kBlock) void k_mgs_part(int64_t n, double* __restrict__ vv,
+                                                     const double* __restrict__ prev,
+                                                     const double* __restrict__ prev_part,
+                                                     const double* __restrict__ next,
+                                                     double* __restrict__ part) {
+  __shared__ double dsh;
+  double md = 0.0;
+  if (prev) {
+    const double d = sum_partials<false>(prev_part);
+    if (threadIdx.x == 0) dsh = d;
+    __syncthreads();
+    md = -dsh;
+  }
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -134,14 +152,15 @@ __global__ __launch_bounds__(kBlock) void k_mgs(int64_t n, double* __restrict__ 
     }
     acc[0] += v * (next ? next[i] : v);
   }
-  double a = block_reduce<false>((acc[0] + acc[1]) + (acc[2] + acc[3]));
-  finish_reduction<false>(a, partials, counter, out);
+  const double a = block_reduce<false>((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
 int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
              const double* next, int oslot) {
-  hipLaunchKernelGGL(k_mgs, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size, vv, prev,
-                     ctx->scal + dslot, next, ctx->partials, ctx->counter, ctx->scal + oslot);
+  hipLaunchKernelGGL(k_mgs_part, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size, vv, prev,
+                     prev ? ctx->partials + (int64_t)dslot * kRedBlocks : nullptr, next,
+                     ctx->partials + (int64_t)oslot * kRedBlocks);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
@@ -546,15 +565,13 @@ int gh_ctx_create(int device, gh_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void**)&c->partials, sizeof(double) * kRedBlocks * 4) != hipSuccess ||
-      hipMalloc((void**)&c->counter, sizeof(unsigned) * 16) != hipSuccess ||
+      hipMalloc((void**)&c->partials, sizeof(double) * kRedBlocks * kScalars) != hipSuccess ||
       hipMalloc((void**)&c->scal, sizeof(double) * kScalars) != hipSuccess ||
       hipHostMalloc((void**)&c->host_scal, sizeof(double) * kScalars, hipHostMallocDefault) !=
           hipSuccess) {
     ctx_release(c);
     return fail(GH_ENOMEM, "gh_ctx_create: allocation failed");
   }
-  GH_HIP(hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 16, c->stream));
   GH_HIP(hipMemsetAsync(c->scal, 0, sizeof(double) * kScalars, c->stream));
   GH_HIP(hipStreamSynchronize(c->stream));
   *out = c;
@@ -567,7 +584,6 @@ namespace gh {
 static void ctx_free(gh_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->partials) (void)hipFree(ctx->partials);
-  if (ctx->counter) (void)hipFree(ctx->counter);
   if (ctx->scal) (void)hipFree(ctx->scal);
   if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -40,7 +40,7 @@ int fail(int code, const char* fmt, ...);
 
 // launch geometry
 constexpr int kBlock = 256;       // 4 wavefronts of 64
-constexpr int kRedBlocks = 2048;  // fixed grid of every reduction: deterministic sums
+constexpr int kRedBlocks = 1024;  // fixed grid of every reduction: deterministic sums
 
 inline unsigned grid_for(int64_t work, int per_block = kBlock) {
   int64_t g = (work + per_block - 1) / per_block;
@@ -77,8 +77,7 @@ struct gh_ctx {
   std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;
-  double* partials = nullptr;      // kRedBlocks * 4 doubles of block partials
-  unsigned* counter = nullptr;     // last-block tickets (one per slot group)
+  double* partials = nullptr;      // kScalars * kRedBlocks block partials
   double* scal = nullptr;          // device result slots
   double* host_scal = nullptr;     // pinned mirror of scal
   int num_cu = 256;
@@ -183,4 +182,5 @@ int hn_apply(gh_ctx* ctx, const gh_hn* H, int ncomp, int prec, void* x, int op);
 int mask_apply(gh_ctx* ctx, const gh_mask* M, int prec, void* x);
 int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
              const double* next, int oslot);
+int finalize_slots(gh_ctx* ctx, int first, int count, bool max);
 }  // namespace gh

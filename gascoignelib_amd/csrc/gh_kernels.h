@@ -63,40 +63,21 @@ __device__ __forceinline__ double block_reduce(double v) {
   return r;
 }
 
-// Each block stores its partial; the last block to finish folds the partials
-// in a fixed order and writes the result slot.  Same grid every call, so the
-// result is bitwise reproducible.
+// Reductions run in two phases without fences or atomics: every block of the
+// producing kernel stores one partial (kRedBlocks of them per slot), and the
+// consumer folds the partials in a fixed order (sum_partials).  The grid is
+// the same for every call, so results are bitwise reproducible.
 template <bool MAX>
-__device__ __forceinline__ void finish_reduction(double part, double* partials,
-                                                 unsigned* counter, double* out) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = part;
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    double a = 0.0;
-    const volatile double* vp = partials;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-      a = MAX ? fmax(a, vp[i]) : a + vp[i];
-    a = block_reduce<MAX>(a);
-    if (threadIdx.x == 0) {
-      *out = a;
-      atomicExch(counter, 0u);
-    }
-  }
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < kRedBlocks; i += kBlock) a = MAX ? fmax(a, part[i]) : a + part[i];
+  return block_reduce<MAX>(a);
 }
 
-// Grid-stride loops with four independent accumulators keep four loads per
-// operand in flight per lane (the reductions are bandwidth kernels).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T* __restrict__ x,
-                                                const T* __restrict__ y, double* partials,
-                                                unsigned* counter, double* out) {
+__global__ __launch_bounds__(kBlock) void k_dot_part(int64_t n, const T* __restrict__ x,
+                                                     const T* __restrict__ y,
+                                                     double* __restrict__ part) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -107,14 +88,13 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T* __restrict__
     a3 += (double)x[i + 3 * stride] * (double)y[i + 3 * stride];
   }
   for (; i < n; i += stride) a0 += (double)x[i] * (double)y[i];
-  double acc = block_reduce<false>((a0 + a1) + (a2 + a3));
-  finish_reduction<false>(acc, partials, counter, out);
+  const double acc = block_reduce<false>((a0 + a1) + (a2 + a3));
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_amax(int64_t n, const T* __restrict__ x,
-                                                 double* partials, unsigned* counter,
-                                                 double* out) {
+__global__ __launch_bounds__(kBlock) void k_amax_part(int64_t n, const T* __restrict__ x,
+                                                      double* __restrict__ part) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -125,8 +105,16 @@ __global__ __launch_bounds__(kBlock) void k_amax(int64_t n, const T* __restrict_
     a3 = fmax(a3, fabs((double)x[i + 3 * stride]));
   }
   for (; i < n; i += stride) a0 = fmax(a0, fabs((double)x[i]));
-  double acc = block_reduce<true>(fmax(fmax(a0, a1), fmax(a2, a3)));
-  finish_reduction<true>(acc, partials, counter, out);
+  const double acc = block_reduce<true>(fmax(fmax(a0, a1), fmax(a2, a3)));
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// one block per slot: out[s] = fold of the slot's partials
+template <bool MAX>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ part,
+                                                     double* __restrict__ out) {
+  const double a = sum_partials<MAX>(part + (int64_t)blockIdx.x * kRedBlocks);
+  if (threadIdx.x == 0) out[blockIdx.x] = a;
 }
 
 // ----------------------------------------------------------- vector kernels

@@ -780,6 +780,7 @@ int gmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int
       GH_TRY(mgs_step(ctx, n, vv, nullptr, 0, mg->mem[0], 8));
       for (int i = 1; i < dim; ++i) GH_TRY(mgs_step(ctx, n, vv, mg->mem[i - 1], 8 + i - 1, mg->mem[i], 8 + i));
       GH_TRY(mgs_step(ctx, n, vv, mg->mem[dim - 1], 8 + dim - 1, nullptr, 8 + dim));
+      GH_TRY(finalize_slots(ctx, 8, dim + 1, false));
       GH_TRY(fetch_scalars(ctx, 8, dim + 1, d.data()));
       for (int i = 0; i < dim; ++i) h[i] += d[i];
       h[dim] = std::sqrt(d[dim]);
