@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mixed-linear-tol", type=float, default=1e-5,
                     help="GMRES tolerance of each refinement step of the mixed solve")
+    ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
+                    help="node order inside the solver (the caller's vectors keep theirs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -96,7 +98,7 @@ def main():
 
     ctx = H.Context(0)
     common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
-                  ndirect=1000)
+                  ndirect=1000, order=None if args.order == "none" else args.order)
     t0 = time.perf_counter()
     mg64 = H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common)
     mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, **common)
@@ -143,20 +145,25 @@ def main():
     rr64, rrmx = true_rel_res(U64), true_rel_res(UMX)
     err_mx = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
 
-    # roofline of the dominant kernel: the FP64 residual SpMV on the finest level
+    # roofline of the dominant kernel: the FP64 residual SpMV on the finest level,
+    # on the matrix in the solver's internal numbering
+    from gascoignelib_amd.problem import lexicographic_order, permuted_csr
+    if args.order == "none":
+        rp, cl, vl = fine.rowptr, fine.col, fine.val
+    else:
+        rp, cl, vl = permuted_csr(fine, lexicographic_order(fine.coords))
+    Ak = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
     X = H.CudaVectorInterface.from_global(ctx, u64)
-    t_spmv = time_kernel(ctx, lambda: A64.residual(R, X, F), args.spmv_reps)
+    t_spmv = time_kernel(ctx, lambda: Ak.residual(R, X, F), args.spmv_reps)
     b64 = spmv_bytes(ndof, fine.nnz, 8, 8)
     gbs64 = b64 / t_spmv / 1e9
     # the mixed V-cycle's finest-level residual (FP32 vectors, stored values)
-    Amx = mgmx.A[-1]  # FP64 handle; the V-cycle's own copy is internal
-    Af = H.CudaCSRMatrixInterface(ctx, fine.rowptr, fine.col, fine.val, 1, args.mixed)
+    Af = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, args.mixed)
     Xf = H.CudaVectorInterface.from_global(ctx, u64, 1, "f32")
     Ff = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
     Rf = H.CudaVectorInterface(ctx, ndof, 1, "f32")
     t_spmv_mx = time_kernel(ctx, lambda: Af.residual(Rf, Xf, Ff), args.spmv_reps)
     bmx = spmv_bytes(ndof, fine.nnz, 2 if args.mixed == "f16" else 4, 4)
-    del Amx
 
     t64, tmx = float(np.median(ts64)), float(np.median(tsmx))
     dofs_mx, dofs_64 = ndof / tmx, ndof / t64
@@ -180,7 +187,8 @@ def main():
         "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, Dirichlet 0",
         "config": {"workload": f"3D Q1 Poisson, unit cube, {args.cells}^3 cells",
                    "dof": ndof, "nnz": fine.nnz, "levels": hier.nlevels,
-                   "solver": "Newton/IR + GMRES(10) + V(2,2) Jacobi omega=0.8, ndirect=1000"},
+                   "solver": "Newton/IR + GMRES(10) + V(2,2) Jacobi omega=0.8, ndirect=1000",
+                   "order": args.order},
         "fp64": {"dof_per_s": dofs_64, "ms_per_solve": t64 * 1e3, "linear_tol": 1e-10,
                  "newton": rep64.newton_iterations, "gmres": rep64.gmres_iterations,
                  "true_rel_residual": rr64},

@@ -538,6 +538,69 @@ void sell_free(Sell* S) {
   S->val = nullptr;
 }
 
+template <typename D, typename S>
+__global__ __launch_bounds__(kBlock) void k_permute(int64_t n, int ncomp,
+                                                    const int32_t* __restrict__ perm,
+                                                    D* __restrict__ dst, const S* __restrict__ src,
+                                                    int gather) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * ncomp) return;
+  const int64_t i = t / ncomp;
+  const int c = (int)(t - i * ncomp);
+  const int64_t o = (int64_t)perm[i] * ncomp + c;
+  if (gather)
+    dst[t] = (D)src[o];
+  else
+    dst[o] = (D)src[t];
+}
+
+int vec_permute(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* perm, int dprec, void* dst,
+                int sprec, const void* src, bool gather) {
+  const unsigned g = grid_for(n * ncomp);
+  const int gi = gather ? 1 : 0;
+  if (dprec == GH_F64 && sprec == GH_F64)
+    hipLaunchKernelGGL((k_permute<double, double>), dim3(g), dim3(kBlock), 0, ctx->stream, n, ncomp,
+                       perm, (double*)dst, (const double*)src, gi);
+  else if (dprec == GH_F32 && sprec == GH_F64)
+    hipLaunchKernelGGL((k_permute<float, double>), dim3(g), dim3(kBlock), 0, ctx->stream, n, ncomp,
+                       perm, (float*)dst, (const double*)src, gi);
+  else if (dprec == GH_F64 && sprec == GH_F32)
+    hipLaunchKernelGGL((k_permute<double, float>), dim3(g), dim3(kBlock), 0, ctx->stream, n, ncomp,
+                       perm, (double*)dst, (const float*)src, gi);
+  else
+    hipLaunchKernelGGL((k_permute<float, float>), dim3(g), dim3(kBlock), 0, ctx->stream, n, ncomp,
+                       perm, (float*)dst, (const float*)src, gi);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int transfer_from_csr(gh_ctx* ctx, int64_t ncoarse, int64_t nfine, std::vector<int32_t> rrow,
+                      std::vector<uint32_t> rcol, std::vector<int32_t> prow,
+                      std::vector<uint32_t> pcol, gh_transfer** out) {
+  *out = nullptr;
+  gh_transfer* T = new gh_transfer();
+  T->ctx = ctx;
+  T->ncoarse = ncoarse;
+  T->nfine = nfine;
+  T->h_rrow = std::move(rrow);
+  T->h_rcol = std::move(rcol);
+  T->h_prow = std::move(prow);
+  T->h_pcol = std::move(pcol);
+  int rc = upload(ctx, &T->r_row, T->h_rrow.data(), T->h_rrow.size());
+  if (!rc) rc = upload(ctx, &T->r_col, T->h_rcol.data(), T->h_rcol.size());
+  if (!rc) rc = upload(ctx, &T->p_row, T->h_prow.data(), T->h_prow.size());
+  if (!rc) rc = upload(ctx, &T->p_col, T->h_pcol.data(), T->h_pcol.size());
+  if (!rc) rc = sell_build(ctx, ncoarse, T->h_rrow.data(), T->h_rcol.data(), nullptr, GH_F64, &T->r_sell);
+  if (!rc) rc = sell_build(ctx, nfine, T->h_prow.data(), T->h_pcol.data(), nullptr, GH_F64, &T->p_sell);
+  ctx_retain(ctx);  // balanced by gh_transfer_destroy
+  if (rc) {
+    gh_transfer_destroy(T);
+    return rc;
+  }
+  *out = T;
+  return GH_OK;
+}
+
 }  // namespace gh
 
 using namespace gh;
@@ -747,8 +810,14 @@ static int choose_seg(int64_t n, int64_t nnz) {
   return s;
 }
 
-int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
-                  const double* val, int prec, gh_csr** out) {
+}  // extern "C"
+
+namespace gh {
+// check_sorted = false admits rows whose entries are kept in a given order that
+// is not ascending (operators renumbered for locality keep each row's
+// original summation order)
+int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
+                    const double* val, int prec, bool check_sorted, gh_csr** out) {
   GH_ARG(ctx && out && rowptr && n >= 0 && ncomp >= 1 && ncomp <= 8,
          "gh_csr_create: bad arguments (ncomp must be 1..8)");
   GH_ARG(prec == GH_F64 || prec == GH_F32 || prec == GH_F16, "gh_csr_create: bad precision");
@@ -765,7 +834,7 @@ int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, cons
       if (col[p] < 0 || col[p] >= n)
         return fail(GH_EINVAL, "gh_csr_create: column %d out of range in row %lld", col[p],
                     (long long)i);
-      if (p > rowptr[i] && col[p] <= col[p - 1])
+      if (check_sorted && p > rowptr[i] && col[p] <= col[p - 1])
         return fail(GH_EINVAL, "gh_csr_create: columns of row %lld not strictly ascending",
                     (long long)i);
       if (col[p] == i && diag[i] < 0) diag[i] = p;
@@ -843,6 +912,15 @@ int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, cons
   ctx_retain(ctx);
   *out = A;
   return GH_OK;
+}
+
+}  // namespace gh
+
+extern "C" {
+
+int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
+                  const double* val, int prec, gh_csr** out) {
+  return csr_create_impl(ctx, n, ncomp, rowptr, col, val, prec, true, out);
 }
 
 int gh_csr_destroy(gh_csr* A) {

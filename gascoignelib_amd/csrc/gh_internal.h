@@ -183,4 +183,12 @@ int mask_apply(gh_ctx* ctx, const gh_mask* M, int prec, void* x);
 int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
              const double* next, int oslot);
 int finalize_slots(gh_ctx* ctx, int first, int count, bool max);
+int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
+                    const double* val, int prec, bool check_sorted, gh_csr** out);
+int transfer_from_csr(gh_ctx* ctx, int64_t ncoarse, int64_t nfine, std::vector<int32_t> rrow,
+                      std::vector<uint32_t> rcol, std::vector<int32_t> prow,
+                      std::vector<uint32_t> pcol, gh_transfer** out);
+// gather: dst[i] = src[perm[i]];  scatter: dst[perm[i]] = src[i]  (node-wise)
+int vec_permute(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* perm, int dprec, void* dst,
+                int sprec, const void* src, bool gather);
 }  // namespace gh

@@ -176,9 +176,11 @@ void gh_mg_params_default(gh_mg_params* p) {
 namespace {
 
 struct MgLevel {
-  gh_csr* A64 = nullptr;   // caller's FP64 matrix
+  gh_csr* A64 = nullptr;   // FP64 matrix (the caller's, or an owned renumbered copy)
   gh_csr* A = nullptr;     // matrix of the V-cycle (A64 on the FP64 path)
   bool own_A = false;
+  bool own_A64 = false, own_T = false, own_H = false, own_M = false;
+  int32_t* d_perm = nullptr;  // internal position -> caller's node (nullptr: identity)
   gh_transfer* T = nullptr;
   gh_hn* H = nullptr;
   gh_mask* M = nullptr;
@@ -216,6 +218,12 @@ struct gh_mg {
   double* w = nullptr;
   // fine-level staging for graph replay / precision change
   void* fine_b = nullptr;
+  // renumbered solver: the caller's finest-level matrix handle and FP64
+  // buffers holding caller vectors in the internal numbering
+  bool ordered = false;
+  const gh_csr* user_fine = nullptr;
+  double* ext_x = nullptr;
+  double* ext_y = nullptr;
   // captured V-cycle (fast path from zero initial guess)
   hipGraphExec_t graph = nullptr;
   bool graph_ok = false;
@@ -495,15 +503,122 @@ int fast_vcycle(gh_mg* mg, int clevel, void* xdev, int xprec, const void* ydev, 
 
 extern "C" {
 
-int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
-                 gh_hn* const* H, gh_mask* const* M, const gh_mg_params* params,
-                 gh_mg** out) {
+}  // extern "C"
+
+namespace {
+
+// Renumbered copies of the caller's level operators: internal position k holds
+// the caller's node perm[k].  Every row keeps its entries in the caller's order,
+// so sums are formed in the same order as without the renumbering.
+int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* const* A,
+                        gh_transfer* const* T, gh_hn* const* H, gh_mask* const* M,
+                        const int32_t* const* perm, MgLevel& Lv) {
+  const gh_csr* Au = A[l];
+  const int64_t n = Au->n;
+  const int B = Au->ncomp, BB = B * B;
+  const int32_t* P = perm[l];
+  std::vector<int32_t> Q(n, -1);
+  for (int64_t k = 0; k < n; ++k) {
+    if (P[k] < 0 || P[k] >= n || Q[P[k]] >= 0)
+      return fail(GH_EINVAL, "gh_mg_create_ordered: order of level %d is not a permutation", l);
+    Q[P[k]] = (int32_t)k;
+  }
+  // matrix
+  std::vector<int32_t> rp(n + 1, 0), cl;
+  std::vector<double> vl;
+  cl.reserve(Au->nnz);
+  vl.reserve(Au->nnz * BB);
+  for (int64_t k = 0; k < n; ++k) {
+    const int32_t o = P[k];
+    for (int32_t q = Au->h_rowptr[o]; q < Au->h_rowptr[o + 1]; ++q) {
+      cl.push_back(Q[Au->h_col[q]]);
+      vl.insert(vl.end(), Au->h_val.begin() + (int64_t)q * BB, Au->h_val.begin() + (int64_t)(q + 1) * BB);
+    }
+    rp[k + 1] = (int32_t)cl.size();
+  }
+  GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cl.data(), vl.data(), GH_F64, false, &Lv.A64));
+  Lv.own_A64 = true;
+  if (params->prec != GH_F64) {
+    GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cl.data(), vl.data(), params->prec, false, &Lv.A));
+    Lv.own_A = true;
+  } else {
+    Lv.A = Lv.A64;
+  }
+  std::vector<int32_t> hperm(P, P + n);
+  GH_TRY(dupload(ctx, &Lv.d_perm, hperm));
+  // transfer to level l-1
+  if (l > 0) {
+    const gh_transfer* Tu = T[l];
+    const int64_t nc = Tu->ncoarse;
+    const int32_t* Pc = perm[l - 1];
+    std::vector<int32_t> Qc(nc);
+    for (int64_t k = 0; k < nc; ++k) Qc[Pc[k]] = (int32_t)k;
+    std::vector<int32_t> rrow(nc + 1, 0), prow(n + 1, 0);
+    std::vector<uint32_t> rcol, pcol;
+    rcol.reserve(Tu->h_rcol.size());
+    pcol.reserve(Tu->h_pcol.size());
+    for (int64_t k = 0; k < nc; ++k) {
+      const int32_t o = Pc[k];
+      for (int32_t q = Tu->h_rrow[o]; q < Tu->h_rrow[o + 1]; ++q) {
+        const uint32_t e = Tu->h_rcol[q];
+        rcol.push_back((uint32_t)Q[e & kIdxMask] | (e & ~kIdxMask));
+      }
+      rrow[k + 1] = (int32_t)rcol.size();
+    }
+    for (int64_t k = 0; k < n; ++k) {
+      const int32_t o = P[k];
+      for (int32_t q = Tu->h_prow[o]; q < Tu->h_prow[o + 1]; ++q) {
+        const uint32_t e = Tu->h_pcol[q];
+        pcol.push_back((uint32_t)Qc[e & kIdxMask] | (e & ~kIdxMask));
+      }
+      prow[k + 1] = (int32_t)pcol.size();
+    }
+    GH_TRY(transfer_from_csr(ctx, nc, n, std::move(rrow), std::move(rcol), std::move(prow),
+                             std::move(pcol), &Lv.T));
+    Lv.own_T = true;
+  }
+  auto mapv = [&](int32_t v) { return (v >= 0 && v < n) ? Q[v] : v; };
+  if (H && H[l]) {
+    std::vector<int32_t> e(H[l]->h_edges), f(H[l]->h_faces);
+    for (int32_t& v : e) v = mapv(v);
+    for (int32_t& v : f) v = mapv(v);
+    GH_TRY(gh_hn_create(ctx, H[l]->kind, n, H[l]->ne, e.data(), H[l]->nf, f.data(), &Lv.H));
+    Lv.own_H = true;
+  }
+  if (M && M[l]) {
+    std::vector<int32_t> nodes(M[l]->h_nodes);
+    for (int32_t& v : nodes) v = Q[v];
+    GH_TRY(gh_mask_create(ctx, n, B, (int64_t)nodes.size(), nodes.data(), M[l]->ncomps,
+                          M[l]->h_comps.data(), &Lv.M));
+    Lv.own_M = true;
+  }
+  return GH_OK;
+}
+
+int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, gh_hn* const* H,
+             gh_mask* const* M, const int32_t* const* perm, const gh_mg_params* params,
+             gh_mg** out) {
   GH_ARG(ctx && out && nlevels >= 1 && A && params, "gh_mg_create: bad arguments");
   GH_ARG(params->prec == GH_F64 || params->prec == GH_F32 || params->prec == GH_F16,
          "gh_mg_create: bad precision");
   GH_ARG(params->mgtype == 'V' || params->mgtype == 'W' || params->mgtype == 'F',
          "gh_mg_create: mgtype must be V, W or F");
   GH_ARG(params->smoother == 0 || params->smoother == 1, "gh_mg_create: bad smoother");
+  // shapes of the caller's objects
+  for (int l = 0; l < nlevels; ++l) {
+    if (!A[l] || A[l]->prec != GH_F64)
+      return fail(GH_EINVAL, "gh_mg_create: level %d needs an FP64 matrix", l);
+    if (H && H[l] && H[l]->n != A[l]->n)
+      return fail(GH_EINVAL, "gh_mg_create: HN size mismatch on level %d", l);
+    if (M && M[l] && (M[l]->n != A[l]->n || M[l]->ncomp != A[l]->ncomp))
+      return fail(GH_EINVAL, "gh_mg_create: mask size mismatch on level %d", l);
+    if (l > 0) {
+      if (!T || !T[l]) return fail(GH_EINVAL, "gh_mg_create: level %d has no transfer", l);
+      if (T[l]->nfine != A[l]->n || T[l]->ncoarse != A[l - 1]->n || A[l - 1]->ncomp != A[l]->ncomp)
+        return fail(GH_EINVAL, "gh_mg_create: transfer %d does not match the level sizes", l);
+    }
+    if (perm && !perm[l]) return fail(GH_EINVAL, "gh_mg_create_ordered: level %d has no order", l);
+  }
   *out = nullptr;
   gh_mg* mg = new gh_mg();
   mg->ctx = ctx;
@@ -511,38 +626,29 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
   mg->P = *params;
   mg->wprec = params->prec == GH_F64 ? GH_F64 : GH_F32;
   mg->L.resize(nlevels);
+  mg->ordered = perm != nullptr;
+  mg->user_fine = A[nlevels - 1];
+  ctx_retain(ctx);  // balanced by gh_mg_destroy
   int rc = GH_OK;
   for (int l = 0; l < nlevels && !rc; ++l) {
     MgLevel& Lv = mg->L[l];
-    if (!A[l] || A[l]->prec != GH_F64) {
-      rc = fail(GH_EINVAL, "gh_mg_create: level %d needs an FP64 matrix", l);
-      break;
-    }
-    Lv.A64 = A[l];
     Lv.n = A[l]->n;
     Lv.ncomp = A[l]->ncomp;
     Lv.size = Lv.n * Lv.ncomp;
-    Lv.H = H ? H[l] : nullptr;
-    Lv.M = M ? M[l] : nullptr;
-    if (Lv.H && Lv.H->n != Lv.n) rc = fail(GH_EINVAL, "gh_mg_create: HN size mismatch on level %d", l);
-    if (Lv.M && (Lv.M->n != Lv.n || Lv.M->ncomp != Lv.ncomp))
-      rc = fail(GH_EINVAL, "gh_mg_create: mask size mismatch on level %d", l);
-    if (l > 0) {
-      if (!T || !T[l]) {
-        rc = fail(GH_EINVAL, "gh_mg_create: level %d has no transfer", l);
-        break;
-      }
-      Lv.T = T[l];
-      if (Lv.T->nfine != Lv.n || Lv.T->ncoarse != A[l - 1]->n || A[l - 1]->ncomp != Lv.ncomp)
-        rc = fail(GH_EINVAL, "gh_mg_create: transfer %d does not match the level sizes", l);
-    }
-    if (rc) break;
-    if (params->prec == GH_F64) {
-      Lv.A = A[l];
+    if (perm) {
+      rc = build_ordered_level(ctx, params, l, A, T, H, M, perm, Lv);
     } else {
-      rc = gh_csr_create(ctx, A[l]->n, A[l]->ncomp, A[l]->h_rowptr.data(), A[l]->h_col.data(),
-                         A[l]->h_val.data(), params->prec, &Lv.A);
-      Lv.own_A = true;
+      Lv.A64 = A[l];
+      Lv.T = l > 0 ? T[l] : nullptr;
+      Lv.H = H ? H[l] : nullptr;
+      Lv.M = M ? M[l] : nullptr;
+      if (params->prec == GH_F64) {
+        Lv.A = A[l];
+      } else {
+        rc = gh_csr_create(ctx, A[l]->n, A[l]->ncomp, A[l]->h_rowptr.data(), A[l]->h_col.data(),
+                           A[l]->h_val.data(), params->prec, &Lv.A);
+        Lv.own_A = true;
+      }
     }
     if (!rc) rc = dalloc(&Lv.u, Lv.size * pbytes(mg->wprec));
     if (!rc) rc = dalloc(&Lv.b, Lv.size * pbytes(mg->wprec));
@@ -553,7 +659,7 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
     Lv.direct = Lv.n < params->ndirect;
     if (!rc && Lv.direct) {
       std::vector<double> inv;
-      rc = dense_inverse(A[l], inv);
+      rc = dense_inverse(Lv.A64, inv);
       if (!rc) rc = dupload(ctx, &Lv.Ainv, inv);
     }
   }
@@ -598,9 +704,11 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
     if (!rc) rc = sell_build(ctx, C.n, rrow.data(), rcol.data(), nullptr, GH_F64, &F.r_sell);
     if (!rc) rc = sell_build(ctx, F.n, prow.data(), pcol.data(), nullptr, GH_F64, &F.p_sell);
   }
-  if (!rc) rc = dalloc(&mg->fine_b, mg->L[nlevels - 1].size * pbytes(mg->wprec));
+  const int64_t nf = mg->L[nlevels - 1].size;
+  if (!rc) rc = dalloc(&mg->fine_b, nf * pbytes(mg->wprec));
+  if (!rc && mg->ordered) rc = dalloc((void**)&mg->ext_x, nf * 8);
+  if (!rc && mg->ordered) rc = dalloc((void**)&mg->ext_y, nf * 8);
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(GH_EHIP, "sync");
-  ctx_retain(ctx);
   if (rc) {
     gh_mg_destroy(mg);
     return rc;
@@ -609,11 +717,33 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
   return GH_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
+                 gh_hn* const* H, gh_mask* const* M, const gh_mg_params* params,
+                 gh_mg** out) {
+  return mg_build(ctx, nlevels, A, T, H, M, nullptr, params, out);
+}
+
+int gh_mg_create_ordered(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
+                         gh_hn* const* H, gh_mask* const* M, const int32_t* const* order,
+                         const gh_mg_params* params, gh_mg** out) {
+  GH_ARG(order, "gh_mg_create_ordered: order is NULL");
+  return mg_build(ctx, nlevels, A, T, H, M, order, params, out);
+}
+
 int gh_mg_destroy(gh_mg* mg) {
   if (!mg) return GH_OK;
   if (mg->ctx && mg->ctx->stream) (void)hipStreamSynchronize(mg->ctx->stream);
   for (MgLevel& L : mg->L) {
     if (L.own_A && L.A) gh_csr_destroy(L.A);
+    if (L.own_A64 && L.A64) gh_csr_destroy(L.A64);
+    if (L.own_T && L.T) gh_transfer_destroy(L.T);
+    if (L.own_H && L.H) gh_hn_destroy(L.H);
+    if (L.own_M && L.M) gh_mask_destroy(L.M);
+    if (L.d_perm) (void)hipFree(L.d_perm);
     for (void* p : {(void*)L.r_row, (void*)L.r_col, (void*)L.p_row, (void*)L.p_col, L.u, L.b,
                     L.v, (void*)L.Ainv})
       if (p) (void)hipFree(p);
@@ -622,7 +752,8 @@ int gh_mg_destroy(gh_mg* mg) {
   }
   for (double* m : mg->mem)
     if (m) (void)hipFree(m);
-  for (void* p : {(void*)mg->p, (void*)mg->r, (void*)mg->w, mg->fine_b})
+  for (void* p : {(void*)mg->p, (void*)mg->r, (void*)mg->w, mg->fine_b, (void*)mg->ext_x,
+                  (void*)mg->ext_y})
     if (p) (void)hipFree(p);
   if (mg->graph) (void)hipGraphExecDestroy(mg->graph);
   gh_ctx* ctx = mg->ctx;
@@ -645,8 +776,13 @@ int gh_mg_linear(gh_ctx* ctx, gh_mg* mg, int finelevel, int coarselevel, gh_vec*
   GH_ARG(u->d != f->d, "gh_mg_linear: u and f must not alias");
   const int clevel = coarsest_level(mg, finelevel, coarselevel);
   // level vectors: u, b = _mg0_ (= f), v = _mg1_ in working precision
-  GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.u, u->prec, u->d));
-  GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.b, f->prec, f->d));
+  if (F.d_perm) {
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, mg->wprec, F.u, u->prec, u->d, true));
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, mg->wprec, F.b, f->prec, f->d, true));
+  } else {
+    GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.u, u->prec, u->d));
+    GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.b, f->prec, f->d));
+  }
   std::vector<double> res(mg->nlevels, 0.0), rw(mg->nlevels, 0.0);
   GH_TRY(csr_residual(ctx, F.A, mg->wprec, F.v, F.u, F.b));
   GH_TRY(norm_now(mg, F.v, F.size, &res[finelevel]));
@@ -659,7 +795,10 @@ int gh_mg_linear(gh_ctx* ctx, gh_mg* mg, int finelevel, int coarselevel, gh_vec*
     GH_TRY(mgstep(mg, finelevel, finelevel, clevel, p0, p, false, true, res, rw));
     reached = gh_cginfo_check(info, res[finelevel], rw[finelevel]);
   }
-  GH_TRY(vec_convert(ctx, F.size, u->prec, u->d, mg->wprec, F.u));
+  if (F.d_perm)
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, u->prec, u->d, mg->wprec, F.u, false));
+  else
+    GH_TRY(vec_convert(ctx, F.size, u->prec, u->d, mg->wprec, F.u));
   GH_HIP(hipStreamSynchronize(ctx->stream));
   // the level's buffers may have swapped roles; all three are scratch
   return GH_OK;
@@ -685,7 +824,12 @@ int gh_mg_precondition(gh_ctx* ctx, gh_mg* mg, int coarselevel, gh_vec* x, const
   // prec_maxiter == 1: precondition() always runs exactly one mgstep
   // (CGInfo::check re-initialises and stops with "too many iterations")
   const int clevel = coarsest_level(mg, fl, cl);
-  return fast_vcycle(mg, clevel, x->d, x->prec, y->d, y->prec, x_is_zero != 0);
+  if (!F.d_perm) return fast_vcycle(mg, clevel, x->d, x->prec, y->d, y->prec, x_is_zero != 0);
+  GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, mg->ext_y, y->prec, y->d, true));
+  if (!x_is_zero)
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, mg->ext_x, x->prec, x->d, true));
+  GH_TRY(fast_vcycle(mg, clevel, mg->ext_x, GH_F64, mg->ext_y, GH_F64, x_is_zero != 0));
+  return vec_permute(ctx, F.n, F.ncomp, F.d_perm, x->prec, x->d, GH_F64, mg->ext_x, false);
 }
 
 }  // extern "C"
@@ -719,6 +863,7 @@ int precond_raw(gh_mg* mg, int coarselevel, double* x, const double* y) {
   const int fl = mg->nlevels - 1;
   const int cl = coarselevel < 0 ? fl : coarselevel;
   if (mg->P.prec_maxiter != 1) {
+    // internal numbering: run LinearMg without the boundary permutation
     gh_vec xv, yv;
     xv.ctx = yv.ctx = mg->ctx;
     xv.n = yv.n = mg->L[fl].n;
@@ -728,7 +873,15 @@ int precond_raw(gh_mg* mg, int coarselevel, double* x, const double* y) {
     xv.d = x;
     yv.d = const_cast<double*>(y);
     GH_TRY(vec_fill_zero(mg->ctx, GH_F64, xv.size, x));
-    return gh_mg_precondition(mg->ctx, mg, coarselevel, &xv, &yv, 1);
+    gh_cginfo prec;
+    gh_cginfo_init(&prec, mg->P.prec_tol, mg->P.prec_globaltol, mg->P.prec_maxiter);
+    gh_cginfo_reset(&prec);
+    gh_cginfo_check(&prec, 0., 0.);
+    int32_t* saved = mg->L[fl].d_perm;
+    mg->L[fl].d_perm = nullptr;
+    int rc = gh_mg_linear(mg->ctx, mg, fl, cl, &xv, &yv, &prec);
+    mg->L[fl].d_perm = saved;
+    return rc;
   }
   const int clevel = coarsest_level(mg, fl, cl);
   return fast_vcycle(mg, clevel, x, GH_F64, y, GH_F64, true);
@@ -861,7 +1014,7 @@ int newton_residual(gh_ctx* ctx, const gh_csr* A, const gh_mask* mask, double* y
 
 extern "C" {
 
-int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
+int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mask*/,
                    int coarselevel, int vmax, gh_vec* x, const gh_vec* b, gh_cginfo* info,
                    int* reached) {
   GH_ARG(ctx && mg && A && x && b && info && vmax >= 2 && vmax <= 256,
@@ -871,14 +1024,26 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
          "gh_gmres_solve: A must be the FP64 finest-level matrix");
   GH_ARG(x->prec == GH_F64 && b->prec == GH_F64 && x->n == F.n && b->n == F.n,
          "gh_gmres_solve: x and b must be FP64 finest-level vectors");
+  GH_ARG(A == mg->user_fine, "gh_gmres_solve: A must be the finest-level matrix of the solver");
   int rdum = 0, steps = 0;
-  GH_TRY(gmres_impl(ctx, mg, A, mask, coarselevel, vmax, (double*)x->d, (const double*)b->d,
-                    false, info, reached ? reached : &rdum, &steps));
+  double* xd = (double*)x->d;
+  const double* bd = (const double*)b->d;
+  if (F.d_perm) {
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, mg->ext_x, GH_F64, xd, true));
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, mg->ext_y, GH_F64, bd, true));
+    xd = mg->ext_x;
+    bd = mg->ext_y;
+  }
+  GH_TRY(gmres_impl(ctx, mg, F.A64, F.M, coarselevel, vmax, xd, bd, false, info,
+                    reached ? reached : &rdum, &steps));
+  if (F.d_perm)
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, x->d, GH_F64, mg->ext_x, false));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
   return GH_OK;
 }
 
-int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
-                    const gh_hn* hn, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
+int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mask*/,
+                    const gh_hn* /*hn*/, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
                     gh_cginfo* linfo, gh_nlinfo* nl, int norm_kind, gh_solve_report* rep) {
   GH_ARG(ctx && mg && A && u && f && linfo && nl && vmax >= 2 && vmax <= 256,
          "gh_newton_solve: bad arguments (2 <= vmax <= 256)");
@@ -887,12 +1052,22 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask
          "gh_newton_solve: A must be the FP64 finest-level matrix");
   GH_ARG(u->prec == GH_F64 && f->prec == GH_F64 && u->n == F.n && f->n == F.n,
          "gh_newton_solve: u and f must be FP64 finest-level vectors");
+  GH_ARG(A == mg->user_fine, "gh_newton_solve: A must be the finest-level matrix of the solver");
   const int64_t n = F.size;
   GH_TRY(ensure_gmres_memory(mg, vmax));
   double* r = mg->r;
   double* w = mg->w;
   double* ud = (double*)u->d;
   const double* fd = (const double*)f->d;
+  if (F.d_perm) {
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, mg->ext_x, GH_F64, ud, true));
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, mg->ext_y, GH_F64, fd, true));
+    ud = mg->ext_x;
+    fd = mg->ext_y;
+  }
+  A = F.A64;
+  const gh_mask* mask = F.M;
+  const gh_hn* hn = F.H;
   const int64_t mg0 = mg->mgsteps;
   int gm = 0;
   if (hn) GH_TRY(hn_apply(ctx, hn, u->ncomp, GH_F64, ud, 0));  // Solve: HNAverage(u)
@@ -942,6 +1117,8 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask
     reached = gh_nlinfo_check(nl, it, rr, nn);
   }
   if (hn) GH_TRY(hn_apply(ctx, hn, u->ncomp, GH_F64, ud, 2));  // Solve: HNZero(u)
+  if (F.d_perm)
+    GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, u->d, GH_F64, mg->ext_x, false));
   GH_HIP(hipStreamSynchronize(ctx->stream));
   if (rep) {
     rep->newton_iterations = nl->iteration;

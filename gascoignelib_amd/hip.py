@@ -291,7 +291,10 @@ class CudaMultiLevelSolver:
 
     def __init__(self, ctx, hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
                  mgomega=1.0, linear_smooth="jacobi", ndirect=1, prec_maxiter=1, prec="f64",
-                 use_graph=False):
+                 use_graph=False, order=None):
+        """order: None (keep the hierarchy's numbering), "lexicographic" (sort
+        each level's nodes by their coordinates) or a list of per-level
+        permutations (internal position -> node)."""
         self.ctx = ctx
         self.hier = hier
         nl = hier.nlevels
@@ -331,8 +334,21 @@ class CudaMultiLevelSolver:
 
         self._arrays = [arr(self.A), arr(self.T), arr(self.H), arr(self.M)]
         self.h = C.c_void_p()
-        check(N.hip().gh_mg_create(ctx.h, nl, self._arrays[0], self._arrays[1], self._arrays[2],
-                                   self._arrays[3], C.byref(p), C.byref(self.h)))
+        if order is None:
+            check(N.hip().gh_mg_create(ctx.h, nl, self._arrays[0], self._arrays[1],
+                                       self._arrays[2], self._arrays[3], C.byref(p),
+                                       C.byref(self.h)))
+        else:
+            if isinstance(order, str):
+                from .problem import lexicographic_order
+                if order != "lexicographic":
+                    raise ValueError(f"unknown order {order!r}")
+                order = [lexicographic_order(hier.level(l).coords) for l in range(nl)]
+            self._order = [np.ascontiguousarray(o, dtype=np.int32) for o in order]
+            oarr = (N.i32p * nl)(*[ptr(o, N.i32p) for o in self._order])
+            check(N.hip().gh_mg_create_ordered(ctx.h, nl, self._arrays[0], self._arrays[1],
+                                               self._arrays[2], self._arrays[3], oarr,
+                                               C.byref(p), C.byref(self.h)))
 
     @property
     def fine_matrix(self):

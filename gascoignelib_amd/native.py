@@ -121,6 +121,8 @@ HIP_SIGNATURES = {
     "gh_nlinfo_check": (C.c_int, [P(GhNLInfo), C.c_int, C.c_double, C.c_double]),
     "gh_mg_params_default": (None, [P(GhMgParams)]),
     "gh_mg_create": (C.c_int, [vp, C.c_int, P(vp), P(vp), P(vp), P(vp), P(GhMgParams), P(vp)]),
+    "gh_mg_create_ordered": (C.c_int, [vp, C.c_int, P(vp), P(vp), P(vp), P(vp), P(i32p),
+                                       P(GhMgParams), P(vp)]),
     "gh_mg_destroy": (C.c_int, [vp]),
     "gh_mg_nlevels": (C.c_int, [vp]),
     "gh_mg_mgsteps": (C.c_int64, [vp]),

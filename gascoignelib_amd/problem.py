@@ -95,6 +95,33 @@ class Hierarchy:
         return h
 
 
+def lexicographic_order(coords):
+    """Nodes sorted by (z, y, x): internal position k -> node.  On lattice
+    meshes the neighbours of a node then sit at fixed offsets (+-1, +-row,
+    +-plane), which keeps the sparse gathers of every level cache-resident."""
+    c = np.asarray(coords)
+    keys = [np.round(c[:, d], 12) for d in range(c.shape[1])]
+    return np.lexsort(keys).astype(np.int32)
+
+
+def permuted_csr(level, order):
+    """The level's matrix with rows and columns renumbered (internal position
+    k = node order[k]), columns sorted, explicit zeros of the pattern kept:
+    the layout the ordered solver works in."""
+    n = level.n
+    inv = np.empty(n, np.int64)
+    inv[np.asarray(order)] = np.arange(n)
+    rows = np.repeat(np.arange(n), np.diff(level.rowptr))
+    r_new = inv[rows]
+    c_new = inv[level.col]
+    idx = np.lexsort((c_new, r_new))
+    rowptr = np.zeros(n + 1, np.int64)
+    np.add.at(rowptr, r_new + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    return (rowptr.astype(np.int32), c_new[idx].astype(np.int32),
+            np.ascontiguousarray(level.val[idx], dtype=np.float64))
+
+
 def splitmix64_uniform(seed, n, offset=0):
     """Uniform doubles in [-1,1) from SplitMix64 at positions offset..offset+n-1:
     x_k = mix(seed + (k+1)*0x9E3779B97F4A7C15), v = 2*((x>>11)*2^-53) - 1.

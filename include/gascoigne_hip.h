@@ -215,6 +215,18 @@ void gh_mg_params_default(gh_mg_params* p);
 int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
                  gh_hn* const* H, gh_mask* const* M, const gh_mg_params* params,
                  gh_mg** out);
+/* As gh_mg_create, with the solver renumbering every level for locality:
+ * order[l][k] is the caller's node placed at internal position k (a
+ * permutation of 0..n_l-1, e.g. the nodes sorted lexicographically by their
+ * coordinates).  The caller's handles are read once and copied in the new
+ * numbering; each matrix / transfer row keeps its entries in the caller's
+ * order, so every sum is formed as without renumbering.  Vectors passed to
+ * the solver stay in the caller's numbering: they are permuted on entry and
+ * exit.  No counterpart in the reference, whose device path runs in
+ * Gascoigne's hash-ordered numbering (src/Mesh/levelmesh3d.cc:83-94). */
+int gh_mg_create_ordered(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
+                         gh_hn* const* H, gh_mask* const* M, const int32_t* const* order,
+                         const gh_mg_params* params, gh_mg** out);
 int gh_mg_destroy(gh_mg* mg);
 int gh_mg_nlevels(const gh_mg* mg);
 int64_t gh_mg_mgsteps(const gh_mg* mg);  /* the "--> mgstep" counter */
@@ -233,7 +245,9 @@ int gh_mg_precondition(gh_ctx* ctx, gh_mg* mg, int coarselevel, gh_vec* x,
 /* GMRES::solve (src/LinAlg/gmres.cc:160-260), left preconditioned by
  * gh_mg_precondition, FP64 Arnoldi (modified Gram-Schmidt with the reference's
  * re-orthogonalisation test), Givens rotations and back-substitution.
- * A is the FP64 finest-level matrix, mask its Dirichlet mask (or NULL).
+ * A must be the finest-level matrix the solver was created with; the solve
+ * uses that level's Dirichlet mask and hanging nodes given at creation (mask
+ * is accepted for the reference's call shape and not read).
  * Returns the solve's `reached` flag in *reached. */
 int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
                    int coarselevel, int vmax, gh_vec* x, const gh_vec* b, gh_cginfo* info,
@@ -243,6 +257,8 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
  * (stdmultilevelsolver.cc:548-570, 646-673, 693-758, 832-856): FP64 iterative
  * refinement whose correction is one GMRES(vmax) cycle.  The element residual
  * `Form` is the assembled matrix residual f - A u with Dirichlet rows zeroed.
+ * A must be the solver's finest-level matrix; mask and hn are taken from the
+ * solver's finest level (the arguments are accepted and not read).
  * norm_kind 0: NewtonNorm is the max norm (reference); 1: Euclidean norm. */
 typedef struct {
   int newton_iterations, gmres_iterations;
