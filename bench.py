@@ -75,6 +75,9 @@ def main():
                     help="GMRES tolerance of each refinement step of the mixed solve")
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
+                         "and a plain copy of known size, for counter runs under rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -97,6 +100,8 @@ def main():
           f"built in {t_struct:.1f}s", file=sys.stderr, flush=True)
 
     ctx = H.Context(0)
+    if args.kernel_only:
+        return kernel_only(ctx, fine, f, args)
     common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
                   ndirect=1000, order=None if args.order == "none" else args.order)
     t0 = time.perf_counter()
@@ -207,6 +212,37 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def kernel_only(ctx, fine, f, args):
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd.problem import lexicographic_order, permuted_csr
+    if args.order == "none":
+        rp, cl, vl = fine.rowptr, fine.col, fine.val
+    else:
+        rp, cl, vl = permuted_csr(fine, lexicographic_order(fine.coords))
+    n = fine.n
+    A = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
+    X = H.CudaVectorInterface.from_global(ctx, f)
+    F = H.CudaVectorInterface.from_global(ctx, f)
+    R = H.CudaVectorInterface(ctx, n)
+    for _ in range(args.spmv_reps):
+        A.residual(R, X, F)
+    Af = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, args.mixed)
+    Xf = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
+    Ff = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
+    Rf = H.CudaVectorInterface(ctx, n, 1, "f32")
+    for _ in range(args.spmv_reps):
+        Af.residual(Rf, Xf, Ff)
+    # plain copy of 2^25 doubles (256 MiB read + 256 MiB written): counter scale
+    m = 1 << 25
+    C1, C2 = H.CudaVectorInterface(ctx, m), H.CudaVectorInterface(ctx, m)
+    for _ in range(args.spmv_reps):
+        C2.copy_from(C1)
+    ctx.sync()
+    print(json.dumps({"kernel_only": True, "reps": args.spmv_reps,
+                      "residual_bytes_f64": spmv_bytes(n, fine.nnz, 8, 8),
+                      "copy_bytes_each_way": m * 8}), flush=True)
 
 
 def cpu_baseline(hier, f):
