@@ -2,11 +2,16 @@
 (2,146,689 DOF), mixed-precision vs plain FP64 MG-GMRES on one MI355X, plus
 the residual SpMV's achieved HBM bandwidth against the 8 TB/s peak.
 
-One "step" is one full solve from u = 0: Newton / iterative refinement whose
-correction is one GMRES(10) cycle left-preconditioned by one V(2,2) cycle of
-damped Jacobi (omega = 0.8), dense FP64 direct solve on levels below 1000
-nodes, stopping at ||f - A u||_2 <= 1e-10 ||f - A u0||_2.  Inputs (matrix
-hierarchy, right-hand side) are resident in HBM before the timed region.
+One "step" is one full solve from u = 0, stopping at
+||f - A u||_2 <= 1e-10 ||f - A u0||_2 (true FP64 residual):
+  * FP64: Newton / iterative refinement whose correction is one GMRES(10) cycle
+    left-preconditioned by one V(2,2) cycle of damped Jacobi (omega = 0.8), dense
+    FP64 direct solve on levels below 1000 nodes -- the reference's algorithm;
+  * mixed: the same V(2,2) cycle in FP32 arithmetic with FP16-stored matrix
+    values, as the preconditioner of a flexible GMRES(10) whose Arnoldi vectors,
+    products with A, Givens rotations and update are FP64.
+Inputs (matrix hierarchy, right-hand side) are resident in HBM before the timed
+region.
 
 usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cells 128]
 """
@@ -57,9 +62,18 @@ def time_kernel(ctx, fn, reps):
 
 
 def spmv_bytes(n, nnz, sv, sx):
-    """algorithmic bytes of r = b - A x: values + column indices + row pointers,
-    x once, b read, r written"""
+    """bytes of r = b - A x in CSR with int32 columns: values + column indices +
+    row pointers, x once, b read, r written"""
     return nnz * (sv + 4) + (n + 1) * 4 + n * sx * 3
+
+
+def sell_bytes(n, layout, sv, sx):
+    """bytes of r = b - A x in the device layout: stored entries (value + 16-bit
+    column offset, or 32-bit column in slices that need it), per-slice metadata
+    (slot, column start, column base: 12 B), row lengths (2 B), x once, b, r"""
+    narrow = layout["narrow"] / max(layout["slices"], 1)
+    col = 2.0 * narrow + 4.0 * (1.0 - narrow)
+    return layout["stored"] * (sv + col) + layout["slices"] * 12 + n * 2 + n * sx * 3
 
 
 def main():
@@ -71,14 +85,20 @@ def main():
     ap.add_argument("--mixed", default="f16", choices=["f32", "f16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--mixed-linear-tol", type=float, default=1e-5,
-                    help="GMRES tolerance of each refinement step of the mixed solve")
+    ap.add_argument("--mixed-krylov", default="fgmres", choices=["fgmres", "gmres"],
+                    help="outer solver of the mixed solve: flexible GMRES, or left-preconditioned "
+                         "GMRES inside iterative refinement")
+    ap.add_argument("--mixed-linear-tol", type=float, default=None,
+                    help="GMRES tolerance of each outer step of the mixed solve "
+                         "(default 0.5e-10 with fgmres, 1e-5 with gmres)")
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
     args = ap.parse_args()
+    if args.mixed_linear_tol is None:
+        args.mixed_linear_tol = 0.5e-10 if args.mixed_krylov == "fgmres" else 1e-5
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described, not built)", file=sys.stderr)
@@ -106,7 +126,7 @@ def main():
                   ndirect=1000, order=None if args.order == "none" else args.order)
     t0 = time.perf_counter()
     mg64 = H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common)
-    mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, **common)
+    mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, krylov=args.mixed_krylov, **common)
     t_setup = time.perf_counter() - t0
     F = H.CudaVectorInterface.from_global(ctx, f)
     U64, UMX = mg64.new_vector(), mgmx.new_vector()
@@ -160,15 +180,18 @@ def main():
     Ak = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
     X = H.CudaVectorInterface.from_global(ctx, u64)
     t_spmv = time_kernel(ctx, lambda: Ak.residual(R, X, F), args.spmv_reps)
-    b64 = spmv_bytes(ndof, fine.nnz, 8, 8)
+    lay = Ak.layout()
+    b64 = sell_bytes(ndof, lay, 8, 8)
     gbs64 = b64 / t_spmv / 1e9
+    csr64 = spmv_bytes(ndof, fine.nnz, 8, 8)
     # the mixed V-cycle's finest-level residual (FP32 vectors, stored values)
     Af = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, args.mixed)
     Xf = H.CudaVectorInterface.from_global(ctx, u64, 1, "f32")
     Ff = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
     Rf = H.CudaVectorInterface(ctx, ndof, 1, "f32")
     t_spmv_mx = time_kernel(ctx, lambda: Af.residual(Rf, Xf, Ff), args.spmv_reps)
-    bmx = spmv_bytes(ndof, fine.nnz, 2 if args.mixed == "f16" else 4, 4)
+    bmx = sell_bytes(ndof, Af.layout(), 2 if args.mixed == "f16" else 4, 4)
+    traffic = measured_traffic(lay)
 
     t64, tmx = float(np.median(ts64)), float(np.median(tsmx))
     dofs_mx, dofs_64 = ndof / tmx, ndof / t64
@@ -192,7 +215,10 @@ def main():
         "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, Dirichlet 0",
         "config": {"workload": f"3D Q1 Poisson, unit cube, {args.cells}^3 cells",
                    "dof": ndof, "nnz": fine.nnz, "levels": hier.nlevels,
-                   "solver": "Newton/IR + GMRES(10) + V(2,2) Jacobi omega=0.8, ndirect=1000",
+                   "solver": "FP64: Newton/IR + GMRES(10) left-preconditioned; mixed: "
+                             + ("flexible GMRES(10)" if args.mixed_krylov == "fgmres"
+                                else "Newton/IR + GMRES(10)")
+                             + "; V(2,2) Jacobi omega=0.8, ndirect=1000",
                    "order": args.order},
         "fp64": {"dof_per_s": dofs_64, "ms_per_solve": t64 * 1e3, "linear_tol": 1e-10,
                  "newton": rep64.newton_iterations, "gmres": rep64.gmres_iterations,
@@ -200,18 +226,35 @@ def main():
         "mixed": {"dof_per_s": dofs_mx, "ms_per_solve": tmx * 1e3,
                   "newton": repmx.newton_iterations, "gmres": repmx.gmres_iterations,
                   "true_rel_residual": rrmx, "rel_error_vs_fp64": err_mx,
-                  "linear_tol": args.mixed_linear_tol},
+                  "linear_tol": args.mixed_linear_tol, "krylov": args.mixed_krylov},
         "speedup_mixed_vs_fp64": t64 / tmx,
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "FP64 residual SpMV r = b - A x, finest level",
-                     "bytes_per_launch": b64, "us_per_launch": t_spmv * 1e6},
+                     "bytes_per_launch": b64, "us_per_launch": t_spmv * 1e6,
+                     "stored_entries": lay["stored"], "pattern_entries": fine.nnz,
+                     "csr_equivalent_gbs": csr64 / t_spmv / 1e9},
         "spmv_mixed": {"gbs": bmx / t_spmv_mx / 1e9, "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS,
                        "bytes_per_launch": bmx, "us_per_launch": t_spmv_mx * 1e6},
         "setup_s": {"structure": t_struct, "device_upload": t_setup},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(layout):
+    """HBM bytes per launch of the FP64 residual from the committed counter
+    summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs of `bench.py --kernel-only`,
+    each scaled by the factor a plain copy of known size gives), when it was
+    taken on the same layout; else None."""
+    path = os.path.join(ROOT, "profiles", "residual_spmv_128.traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    if t.get("stored_entries") != layout["stored"]:
+        return None
+    return t["bytes_per_launch"]
 
 
 def kernel_only(ctx, fine, f, args):
@@ -241,7 +284,8 @@ def kernel_only(ctx, fine, f, args):
         C2.copy_from(C1)
     ctx.sync()
     print(json.dumps({"kernel_only": True, "reps": args.spmv_reps,
-                      "residual_bytes_f64": spmv_bytes(n, fine.nnz, 8, 8),
+                      "residual_bytes_f64": sell_bytes(n, A.layout(), 8, 8),
+                      "stored_entries": A.layout()["stored"],
                       "copy_bytes_each_way": m * 8}), flush=True)
 
 

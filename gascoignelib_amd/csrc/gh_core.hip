@@ -305,6 +305,15 @@ int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const 
   return fail(GH_EINVAL, "csr_spmv: bad precisions");
 }
 
+// y = A x with FP32 x and FP64 values and arithmetic (flexible GMRES products)
+int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x) {
+  if (A->prec != GH_F64) return fail(GH_EINVAL, "csr_spmv_x32: needs an FP64 matrix");
+  if (A->ncomp == 1)
+    return sell_dispatch<float, double>(ctx, A->sell, x,
+                                        SpmvOp<double, double, true>{y, 1.0, 0.0, false});
+  return dispatch_val<float, double>(ctx, A, x, EpiSpmv<double, double>{y, 1.0, 0.0, false});
+}
+
 int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b) {
   if (A->ncomp == 1) {
     if (prec == GH_F64)
@@ -464,13 +473,20 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
   S->n = n;
   S->prec = prec;
   S->nslices = (n + kSlice - 1) / kSlice;
+  const bool valued = vals != nullptr;
+  // entries kept per row: a valued operator drops its exact zeros
+  auto keep = [&](int64_t q) { return !valued || vals[q] != 0.0; };
   std::vector<uint32_t> sptr(S->nslices + 1, 0);
   std::vector<uint16_t> rlen(n);
+  int64_t stored = 0;
   for (int64_t i = 0; i < n; ++i) {
-    const int64_t len = rowptr[i + 1] - rowptr[i];
+    int64_t len = 0;
+    for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) len += keep(q);
     if (len > 65535) return fail(GH_EINVAL, "row %lld has more than 65535 entries", (long long)i);
     rlen[i] = (uint16_t)len;
+    stored += len;
   }
+  S->stored = stored;
   uint64_t total = 0;
   for (int64_t s = 0; s < S->nslices; ++s) {
     int w = 0;
@@ -482,59 +498,115 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
   }
   sptr[S->nslices] = (uint32_t)total;
   S->slots = (int64_t)total;
-  std::vector<uint32_t> scol(total);
-  std::vector<double> sval(vals ? total : 0, 0.0);
-  for (int64_t s = 0; s < S->nslices; ++s) {
-    const int w = (int)((sptr[s + 1] - sptr[s]) / kSlice);
+  // per slot: column (global) and value, rows' kept entries in order
+  std::vector<uint32_t> scol(total, 0);
+  std::vector<double> sval(valued ? total : 0, 0.0);
+  for (int64_t s = 0; s < S->nslices; ++s)
     for (int lane = 0; lane < kSlice; ++lane) {
       const int64_t i = s * kSlice + lane;
-      for (int k = 0; k < w; ++k) {
-        const uint64_t slot = sptr[s] + (uint64_t)k * kSlice + lane;
-        if (i < n && k < rlen[i]) {
-          scol[slot] = col[rowptr[i] + k];
-          if (vals) sval[slot] = vals[rowptr[i] + k];
-        } else {
-          scol[slot] = (uint32_t)(i < n ? i : 0);  // padding, never read
-        }
+      if (i >= n) break;
+      uint64_t slot = sptr[s] + lane;
+      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        if (!keep(q)) continue;
+        scol[slot] = col[q];
+        if (valued) sval[slot] = vals[q];
+        slot += kSlice;
+      }
+    }
+  GH_TRY(upload(ctx, &S->sptr, sptr.data(), sptr.size()));
+  GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
+  if (!valued) {
+    GH_TRY(upload(ctx, &S->col, scol.data(), scol.size()));
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    return GH_OK;
+  }
+  // column encoding per slice: 16-bit offsets when the slice's columns span
+  // fewer than 2^16 indices, else 32-bit (two 16-bit units, 4-byte aligned)
+  std::vector<uint32_t> cptr(S->nslices), cbase(S->nslices, 0);
+  uint64_t units = 0;
+  std::vector<uint8_t> wide(S->nslices, 0);
+  S->narrow = 0;
+  for (int64_t s = 0; s < S->nslices; ++s) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int64_t i = s * kSlice + lane;
+      if (i >= n) break;
+      for (int k = 0; k < rlen[i]; ++k) {
+        const uint32_t c = scol[sptr[s] + (uint64_t)k * kSlice + lane];
+        lo = std::min(lo, c);
+        hi = std::max(hi, c);
+      }
+    }
+    const uint64_t w = (sptr[s + 1] - sptr[s]);
+    if (lo > hi) lo = hi = 0;  // no entries
+    wide[s] = (hi - lo) > 0xFFFFu;
+    if (wide[s]) {
+      units += units & 1;  // 4-byte alignment
+      cptr[s] = (uint32_t)units | 0x80000000u;
+      units += 2 * w;
+    } else {
+      cptr[s] = (uint32_t)units;
+      cbase[s] = lo;
+      units += w;
+      S->narrow++;
+    }
+    if (units >= 0x80000000ull) return fail(GH_EINVAL, "operator too large for its column index");
+  }
+  std::vector<uint16_t> cidx(std::max<uint64_t>(units, 1), 0);
+  for (int64_t s = 0; s < S->nslices; ++s) {
+    const uint64_t w = (sptr[s + 1] - sptr[s]);
+    const uint64_t at = cptr[s] & 0x7FFFFFFFu;
+    for (uint64_t j = 0; j < w; ++j) {
+      const int lane = (int)(j % kSlice);
+      const int64_t i = s * kSlice + lane;
+      const bool used = i < n && (int64_t)(j / kSlice) < rlen[i];
+      const uint32_t c = used ? scol[sptr[s] + j] : cbase[s];  // padding: never read
+      if (wide[s]) {
+        cidx[at + 2 * j] = (uint16_t)(c & 0xFFFFu);
+        cidx[at + 2 * j + 1] = (uint16_t)(c >> 16);
+      } else {
+        cidx[at + j] = (uint16_t)(c - cbase[s]);
       }
     }
   }
-  GH_TRY(upload(ctx, &S->sptr, sptr.data(), sptr.size()));
-  GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
-  GH_TRY(upload(ctx, &S->col, scol.data(), scol.size()));
-  if (vals) {
-    if (prec == GH_F64) {
-      double* d = nullptr;
-      GH_TRY(upload(ctx, &d, sval.data(), sval.size()));
-      S->val = d;
-    } else {
-      double* tmp = nullptr;
-      GH_TRY(upload(ctx, &tmp, sval.data(), sval.size()));
-      const size_t m = std::max<size_t>(sval.size(), 1);
-      if (hipMalloc(&S->val, m * (prec == GH_F32 ? 4 : 2)) != hipSuccess) {
-        (void)hipFree(tmp);
-        return fail(GH_ENOMEM, "sell_build: allocation failed");
-      }
-      if (prec == GH_F32)
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(total)), dim3(kBlock), 0,
-                           ctx->stream, (int64_t)total, (float*)S->val, (const double*)tmp);
-      else
-        hipLaunchKernelGGL((k_convert<__half, double>), dim3(grid_stride(total)), dim3(kBlock), 0,
-                           ctx->stream, (int64_t)total, (__half*)S->val, (const double*)tmp);
-      GH_HIP(hipStreamSynchronize(ctx->stream));
+  GH_TRY(upload(ctx, &S->cidx, cidx.data(), cidx.size()));
+  GH_TRY(upload(ctx, &S->cptr, cptr.data(), cptr.size()));
+  GH_TRY(upload(ctx, &S->cbase, cbase.data(), cbase.size()));
+  if (prec == GH_F64) {
+    double* d = nullptr;
+    GH_TRY(upload(ctx, &d, sval.data(), sval.size()));
+    S->val = d;
+  } else {
+    double* tmp = nullptr;
+    GH_TRY(upload(ctx, &tmp, sval.data(), sval.size()));
+    const size_t m = std::max<size_t>(sval.size(), 1);
+    if (hipMalloc(&S->val, m * (prec == GH_F32 ? 4 : 2)) != hipSuccess) {
       (void)hipFree(tmp);
+      return fail(GH_ENOMEM, "sell_build: allocation failed");
     }
+    if (prec == GH_F32)
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(total)), dim3(kBlock), 0,
+                         ctx->stream, (int64_t)total, (float*)S->val, (const double*)tmp);
+    else
+      hipLaunchKernelGGL((k_convert<__half, double>), dim3(grid_stride(total)), dim3(kBlock), 0,
+                         ctx->stream, (int64_t)total, (__half*)S->val, (const double*)tmp);
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(tmp);
   }
   GH_HIP(hipStreamSynchronize(ctx->stream));
   return GH_OK;
 }
 
 void sell_free(Sell* S) {
-  for (void* p : {(void*)S->sptr, (void*)S->rlen, (void*)S->col, S->val})
+  for (void* p : {(void*)S->sptr, (void*)S->rlen, (void*)S->col, S->val, (void*)S->cidx,
+                  (void*)S->cptr, (void*)S->cbase})
     if (p) (void)hipFree(p);
   S->sptr = nullptr;
   S->rlen = nullptr;
   S->col = nullptr;
+  S->cidx = nullptr;
+  S->cptr = nullptr;
+  S->cbase = nullptr;
   S->val = nullptr;
 }
 
@@ -939,6 +1011,19 @@ int gh_csr_destroy(gh_csr* A) {
 int64_t gh_csr_n(const gh_csr* A) { return A ? A->n : -1; }
 int64_t gh_csr_nnz(const gh_csr* A) { return A ? A->nnz : -1; }
 int gh_csr_prec(const gh_csr* A) { return A ? A->prec : -1; }
+int gh_csr_layout(const gh_csr* A, int64_t* out) {
+  GH_ARG(A && out, "gh_csr_layout: NULL argument");
+  if (A->ncomp == 1) {
+    out[0] = A->sell.stored;
+    out[1] = A->sell.slots;
+    out[2] = A->sell.nslices;
+    out[3] = A->sell.narrow;
+  } else {
+    out[0] = out[1] = A->nnz;
+    out[2] = out[3] = 0;
+  }
+  return GH_OK;
+}
 
 static int vec_matches(const gh_csr* A, const gh_vec* v) {
   return v && v->n == A->n && v->ncomp == A->ncomp;

@@ -57,14 +57,25 @@ struct Sell {
   int64_t n = 0;          // rows
   int64_t nslices = 0;
   int64_t slots = 0;      // padded entries
+  int64_t stored = 0;     // entries kept (valued operators drop exact zeros)
+  int64_t narrow = 0;     // slices with 16-bit column offsets
   uint32_t* sptr = nullptr;  // nslices+1 slot offsets
   uint16_t* rlen = nullptr;  // entries per row
-  uint32_t* col = nullptr;   // column (or column | weight code << 29)
+  // valued operators: per slice either 16-bit offsets from cbase[s] or plain
+  // 32-bit indices; cptr[s] = start in 16-bit units | (wide << 31)
+  uint16_t* cidx = nullptr;
+  uint32_t* cptr = nullptr;
+  uint32_t* cbase = nullptr;
+  // packed-weight operators: column | weight code << 29, one per slot
+  uint32_t* col = nullptr;
   void* val = nullptr;       // values in `prec`, nullptr for packed-weight operators
   int prec = GH_F64;
 };
 
-// Host-side layout from CSR.  vals (double, may be empty) are stored as `prec`.
+// Host-side layout from CSR.  With vals (double) the operator is valued: exact
+// zeros are dropped (each row keeps its other entries in order, so every row
+// sum takes the same values in the same order) and values are stored as
+// `prec`.  Without vals, col holds packed-weight entries.
 int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
                const double* vals, int prec, Sell* S);
 void sell_free(Sell* S);
@@ -168,6 +179,7 @@ int fetch_scalars(gh_ctx* ctx, int first, int count, double* out);
 int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const void* x,
              double d, double b);
 int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b);
+int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x);
 int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const void* x,
                      const void* b, double omega, int smoother, bool x_zero);
 int vec_axpby(gh_ctx* ctx, int prec, int64_t size, void* y, double a, double d, const void* x);

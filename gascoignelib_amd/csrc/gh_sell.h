@@ -95,25 +95,27 @@ struct CorrectOp {  // v = P' uc ; u += mgomega * v  (empty rows leave u as it i
 };
 
 // ---------------------------------------------------------------- kernels
-// valued operator (matrix); VT storage type, T arithmetic type
-template <typename VT, typename XT, typename T, class Op>
-__global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __restrict__ sptr,
-                                                 const uint16_t* __restrict__ rlen,
-                                                 const uint32_t* __restrict__ col,
-                                                 const VT* __restrict__ val,
-                                                 const XT* __restrict__ x, Op op) {
-  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (row >= n) return;
-  const int lane = threadIdx.x & (kSlice - 1);
-  const uint32_t base = sptr[row >> 6] + lane;
-  const int len = rlen[row];
-  const uint32_t* cp = col + base;
-  const VT* vp = val + base;
-  T acc = op.init(row);
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i mod 8).  Remapping the workgroup index gives every XCD one contiguous
+// range of rows, so the x entries a row range gathers are fetched into that
+// XCD's own L2 instead of into all eight.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+  const unsigned q = G / kXcds, r = G % kXcds;
+  const unsigned xcd = b % kXcds, k = b / kXcds;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// One row of a valued operator: entries k = 0..len-1 at stride kSlice, column
+// = cb + cp[k * kSlice] (16-bit offsets) or cp[k * kSlice] (32-bit, cb = 0).
+template <typename CT, typename VT, typename XT, typename T, class Op>
+__device__ __forceinline__ T sell_row(const CT* __restrict__ cp, uint32_t cb,
+                                      const VT* __restrict__ vp, int len,
+                                      const XT* __restrict__ x, T acc, const Op& op) {
   int k = 0;
   for (; k + 4 <= len; k += 4) {
-    const uint32_t c0 = cp[(k + 0) * kSlice], c1 = cp[(k + 1) * kSlice];
-    const uint32_t c2 = cp[(k + 2) * kSlice], c3 = cp[(k + 3) * kSlice];
+    const uint32_t c0 = cb + cp[(k + 0) * kSlice], c1 = cb + cp[(k + 1) * kSlice];
+    const uint32_t c2 = cb + cp[(k + 2) * kSlice], c3 = cb + cp[(k + 3) * kSlice];
     const T a0 = (T)ld(vp + (k + 0) * kSlice), a1 = (T)ld(vp + (k + 1) * kSlice);
     const T a2 = (T)ld(vp + (k + 2) * kSlice), a3 = (T)ld(vp + (k + 3) * kSlice);
     const T x0 = (T)x[c0], x1 = (T)x[c1], x2 = (T)x[c2], x3 = (T)x[c3];
@@ -122,7 +124,32 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __re
     acc = op.step(acc, a2, x2);
     acc = op.step(acc, a3, x3);
   }
-  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * kSlice), (T)x[cp[k * kSlice]]);
+  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * kSlice), (T)x[cb + cp[k * kSlice]]);
+  return acc;
+}
+
+// valued operator (matrix); VT storage type, T arithmetic type.  A slice is one
+// wavefront, so the column-encoding branch is uniform across the wave.
+template <typename VT, typename XT, typename T, class Op>
+__global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __restrict__ sptr,
+                                                 const uint16_t* __restrict__ rlen,
+                                                 const uint16_t* __restrict__ cidx,
+                                                 const uint32_t* __restrict__ cptr,
+                                                 const uint32_t* __restrict__ cbase,
+                                                 const VT* __restrict__ val,
+                                                 const XT* __restrict__ x, Op op) {
+  const int64_t row = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int64_t s = row >> 6;
+  const int len = rlen[row];
+  const VT* vp = val + sptr[s] + lane;
+  const uint32_t meta = cptr[s];
+  T acc = op.init(row);
+  if (meta & 0x80000000u)
+    acc = sell_row((const uint32_t*)(cidx + (meta & 0x7FFFFFFFu)) + lane, 0u, vp, len, x, acc, op);
+  else
+    acc = sell_row(cidx + meta + lane, cbase[s], vp, len, x, acc, op);
   op.finish(row, acc);
 }
 
@@ -132,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_w(int64_t n, const uint32_t* __
                                                    const uint16_t* __restrict__ rlen,
                                                    const uint32_t* __restrict__ col,
                                                    const T* __restrict__ x, Op op) {
-  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (row >= n) return;
   const int lane = threadIdx.x & (kSlice - 1);
   const uint32_t* cp = col + sptr[row >> 6] + lane;
@@ -160,7 +187,7 @@ template <typename VT, typename XT, typename T, class Op>
 inline int sell_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
   if (S.n == 0) return GH_OK;
   hipLaunchKernelGGL((k_sell<VT, XT, T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream,
-                     S.n, S.sptr, S.rlen, S.col, (const VT*)S.val, x, op);
+                     S.n, S.sptr, S.rlen, S.cidx, S.cptr, S.cbase, (const VT*)S.val, x, op);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }

@@ -168,6 +168,7 @@ void gh_mg_params_default(gh_mg_params* p) {
   p->prec_globaltol = 1.e-12;
   p->prec = GH_F64;
   p->use_graph = 0;
+  p->krylov = 0;
 }
 
 }  // extern "C"
@@ -216,6 +217,14 @@ struct gh_mg {
   double* p = nullptr;
   double* r = nullptr;
   double* w = nullptr;
+  // flexible GMRES: preconditioned directions z_j in the V-cycle's precision
+  std::vector<void*> zmem;
+  // solution update x += sum_j c_j basis_j: device pointer tables and the
+  // coefficients, staged through pinned host memory
+  const void** d_basis = nullptr;  // mem[0..vmax] then zmem[0..vmax]
+  double* d_coef = nullptr;
+  double* h_coef = nullptr;
+  int basis_cap = 0;
   // fine-level staging for graph replay / precision change
   void* fine_b = nullptr;
   // renumbered solver: the caller's finest-level matrix handle and FP64
@@ -604,6 +613,7 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
   GH_ARG(params->mgtype == 'V' || params->mgtype == 'W' || params->mgtype == 'F',
          "gh_mg_create: mgtype must be V, W or F");
   GH_ARG(params->smoother == 0 || params->smoother == 1, "gh_mg_create: bad smoother");
+  GH_ARG(params->krylov == 0 || params->krylov == 1, "gh_mg_create: krylov must be 0 or 1");
   // shapes of the caller's objects
   for (int l = 0; l < nlevels; ++l) {
     if (!A[l] || A[l]->prec != GH_F64)
@@ -752,6 +762,11 @@ int gh_mg_destroy(gh_mg* mg) {
   }
   for (double* m : mg->mem)
     if (m) (void)hipFree(m);
+  for (void* z : mg->zmem)
+    if (z) (void)hipFree(z);
+  if (mg->d_basis) (void)hipFree((void*)mg->d_basis);
+  if (mg->d_coef) (void)hipFree(mg->d_coef);
+  if (mg->h_coef) (void)hipHostFree(mg->h_coef);
   for (void* p : {(void*)mg->p, (void*)mg->r, (void*)mg->w, mg->fine_b, (void*)mg->ext_x,
                   (void*)mg->ext_y})
     if (p) (void)hipFree(p);
@@ -837,25 +852,79 @@ int gh_mg_precondition(gh_ctx* ctx, gh_mg* mg, int coarselevel, gh_vec* x, const
 // ==================================================================== GMRES
 namespace {
 
+// x += sum_k h_k vs[k], the terms added in order k = 0..m-1 for every entry
+// (GMRES::solution, gmres.cc:235-260, is one Add per basis vector).
+template <typename Z>
 __global__ __launch_bounds__(kBlock) void k_multi_axpy(int64_t n, double* __restrict__ x, int m,
-                                                       const double* const* __restrict__ vs,
+                                                       const Z* const* __restrict__ vs,
                                                        const double* __restrict__ h) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     double acc = x[i];
-    for (int k = 0; k < m; ++k) acc += h[k] * vs[k][i];
+    for (int k = 0; k < m; ++k) acc = add_rn(acc, mul_rn(h[k], (double)vs[k][i]));
     x[i] = acc;
   }
 }
 
-int ensure_gmres_memory(gh_mg* mg, int vmax) {
+constexpr int kMaxBasis = 256;  // vmax <= 256
+
+// GMRES memory of the finest level: mem[0..vmax] (FP64 Arnoldi basis), with
+// `flexible` also zmem[0..vmax] (preconditioned directions in the V-cycle's
+// precision), and the pointer table the solution update reads.
+int ensure_gmres_memory(gh_mg* mg, int vmax, bool flexible) {
   const int64_t n = mg->L[mg->nlevels - 1].size;
+  bool grown = false;
   if ((int)mg->mem.size() < vmax + 1) mg->mem.resize(vmax + 1, nullptr);
   for (int i = 0; i <= vmax; ++i)
-    if (!mg->mem[i]) GH_TRY(dalloc((void**)&mg->mem[i], n * 8));
+    if (!mg->mem[i]) {
+      GH_TRY(dalloc((void**)&mg->mem[i], n * 8));
+      grown = true;
+    }
+  if (flexible) {
+    if ((int)mg->zmem.size() < vmax + 1) mg->zmem.resize(vmax + 1, nullptr);
+    for (int i = 0; i <= vmax; ++i)
+      if (!mg->zmem[i]) {
+        GH_TRY(dalloc(&mg->zmem[i], n * pbytes(mg->wprec)));
+        grown = true;
+      }
+  }
   if (!mg->p) GH_TRY(dalloc((void**)&mg->p, n * 8));
   if (!mg->r) GH_TRY(dalloc((void**)&mg->r, n * 8));
   if (!mg->w) GH_TRY(dalloc((void**)&mg->w, n * 8));
+  if (!mg->d_coef) GH_TRY(dalloc((void**)&mg->d_coef, kMaxBasis * 8));
+  if (!mg->h_coef) GH_HIP(hipHostMalloc((void**)&mg->h_coef, kMaxBasis * 8, hipHostMallocDefault));
+  const int cap = (int)(mg->mem.size() + mg->zmem.size());
+  if (grown || !mg->d_basis || mg->basis_cap != cap) {
+    if (mg->d_basis) (void)hipFree((void*)mg->d_basis);
+    mg->d_basis = nullptr;
+    GH_TRY(dalloc((void**)&mg->d_basis, sizeof(void*) * cap));
+    std::vector<const void*> t;
+    for (double* m : mg->mem) t.push_back(m);
+    for (void* z : mg->zmem) t.push_back(z);
+    GH_HIP(hipMemcpy((void*)mg->d_basis, t.data(), sizeof(void*) * cap, hipMemcpyHostToDevice));
+    mg->basis_cap = cap;
+  }
+  return GH_OK;
+}
+
+// x += sum_{i<dim} c_i basis_i with basis mem (FP64) or zmem (V-cycle precision).
+// The coefficients go through pinned memory; every caller synchronises the
+// stream (a fetched scalar) before the next update rewrites them.
+int basis_update(gh_mg* mg, double* x, bool zbasis, int dim, const double* c) {
+  gh_ctx* ctx = mg->ctx;
+  const int64_t n = mg->L[mg->nlevels - 1].size;
+  for (int i = 0; i < dim; ++i) mg->h_coef[i] = c[i];
+  GH_HIP(hipMemcpyAsync(mg->d_coef, mg->h_coef, sizeof(double) * dim, hipMemcpyHostToDevice,
+                        ctx->stream));
+  const dim3 grid((unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 8192));
+  const void* const* tab = mg->d_basis + (zbasis ? mg->mem.size() : 0);
+  if (!zbasis || mg->wprec == GH_F64)
+    hipLaunchKernelGGL(k_multi_axpy<double>, grid, dim3(kBlock), 0, ctx->stream, n, x, dim,
+                       (const double* const*)tab, (const double*)mg->d_coef);
+  else
+    hipLaunchKernelGGL(k_multi_axpy<float>, grid, dim3(kBlock), 0, ctx->stream, n, x, dim,
+                       (const float* const*)tab, (const double*)mg->d_coef);
+  GH_HIP(hipGetLastError());
   return GH_OK;
 }
 
@@ -887,13 +956,67 @@ int precond_raw(gh_mg* mg, int coarselevel, double* x, const double* y) {
   return fast_vcycle(mg, clevel, x, GH_F64, y, GH_F64, true);
 }
 
+// Arnoldi step shared by both GMRES forms (gmres.cc:109-137, 91-105): modified
+// Gram-Schmidt of vv against mem[0..dim-1] into h (dots in slots 8..), the
+// re-orthogonalisation test, normalisation, and the Givens rotation of column
+// `dim-1`.  Returns |gamma[dim]|.
+int arnoldi_step(gh_mg* mg, int dim, double* vv, std::vector<double>& h,
+                 std::vector<double>& gamma, std::vector<double>& ci, std::vector<double>& si,
+                 std::vector<double>& d, double* rho) {
+  gh_ctx* ctx = mg->ctx;
+  const int64_t n = mg->L[mg->nlevels - 1].size;
+  const int last = dim - 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    GH_TRY(mgs_step(ctx, n, vv, nullptr, 0, mg->mem[0], 8));
+    for (int i = 1; i < dim; ++i) GH_TRY(mgs_step(ctx, n, vv, mg->mem[i - 1], 8 + i - 1, mg->mem[i], 8 + i));
+    GH_TRY(mgs_step(ctx, n, vv, mg->mem[dim - 1], 8 + dim - 1, nullptr, 8 + dim));
+    GH_TRY(finalize_slots(ctx, 8, dim + 1, false));
+    GH_TRY(fetch_scalars(ctx, 8, dim + 1, d.data()));
+    for (int i = 0; i < dim; ++i) h[i] += d[i];
+    h[dim] = std::sqrt(d[dim]);
+    // reortho_test (gmres.cc:124-137): sqrt(vv.vv) recomputed by the same
+    // deterministic reduction equals h[dim], so the test is h + 1e-3 h == h
+    const double oldnorm = h[dim];
+    const double newnorm = oldnorm + 1.e-3 * h[dim];
+    if (oldnorm != newnorm) break;
+  }
+  const double s = h[dim];
+  GH_TRY(vec_scale_to(ctx, GH_F64, n, vv, 1. / s, vv));
+  // givens_rotation (gmres.cc:91-105)
+  for (int i = 0; i < last; i++) {
+    const double dummy = h[i];
+    h[i] = ci[i] * dummy + si[i] * h[i + 1];
+    h[i + 1] = -si[i] * dummy + ci[i] * h[i + 1];
+  }
+  const double r = 1. / std::sqrt(h[last] * h[last] + h[last + 1] * h[last + 1]);
+  si[last] = h[last + 1] * r;
+  ci[last] = h[last] * r;
+  h[last] = ci[last] * h[last] + si[last] * h[last + 1];
+  gamma[last + 1] = -si[last] * gamma[last];
+  gamma[last] *= ci[last];
+  *rho = std::fabs(gamma[dim]);
+  return GH_OK;
+}
+
+// backward (gmres.cc:235-260) on H1(i,j) = H[j][i]
+std::vector<double> gmres_backward(const std::vector<std::vector<double>>& H,
+                                   const std::vector<double>& gamma, int dim) {
+  std::vector<double> hs(std::max(dim, 1), 0.0);
+  for (int i = dim - 1; i >= 0; i--) {
+    double s = gamma[i];
+    for (int j = i + 1; j < dim; j++) s -= hs[j] * H[j][i];
+    hs[i] = s / H[i][i];
+  }
+  return hs;
+}
+
 // GMRES::solve with x_zero: the caller guarantees x == 0 on entry (Newton's
 // correction), so residualgmres is p = b with Dirichlet rows zeroed.
 int gmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
                int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
                int* reached_out, int* steps_out) {
   const int64_t n = mg->L[mg->nlevels - 1].size;
-  GH_TRY(ensure_gmres_memory(mg, vmax));
+  GH_TRY(ensure_gmres_memory(mg, vmax, false));
   std::vector<std::vector<double>> H(vmax);
   for (int i = 0; i < vmax; ++i) H[i].assign(i + 2, 0.0);
   std::vector<double> gamma(vmax + 1, 0.0), ci(vmax, 0.0), si(vmax, 0.0);
@@ -927,65 +1050,97 @@ int gmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int
     double* vv = mg->mem[dim];
     GH_TRY(csr_spmv(ctx, A, GH_F64, p, GH_F64, mg->mem[last], 1.0, 0.0));  // vmulteq
     GH_TRY(precond_raw(mg, coarselevel, vv, p));
-    std::vector<double>& h = H[last];
-    // orthogonalization (gmres.cc:109-120), slots 8.. hold the dots
-    for (int pass = 0; pass < 2; ++pass) {
-      GH_TRY(mgs_step(ctx, n, vv, nullptr, 0, mg->mem[0], 8));
-      for (int i = 1; i < dim; ++i) GH_TRY(mgs_step(ctx, n, vv, mg->mem[i - 1], 8 + i - 1, mg->mem[i], 8 + i));
-      GH_TRY(mgs_step(ctx, n, vv, mg->mem[dim - 1], 8 + dim - 1, nullptr, 8 + dim));
-      GH_TRY(finalize_slots(ctx, 8, dim + 1, false));
-      GH_TRY(fetch_scalars(ctx, 8, dim + 1, d.data()));
-      for (int i = 0; i < dim; ++i) h[i] += d[i];
-      h[dim] = std::sqrt(d[dim]);
-      // reortho_test (gmres.cc:124-137): sqrt(vv.vv) recomputed by the same
-      // deterministic reduction equals h[dim], so the test is h + 1e-3 h == h
-      const double oldnorm = h[dim];
-      const double newnorm = oldnorm + 1.e-3 * h[dim];
-      if (oldnorm != newnorm) break;
-    }
-    const double s = h[dim];
-    GH_TRY(vec_scale_to(ctx, GH_F64, n, vv, 1. / s, vv));
-    // givens_rotation (gmres.cc:91-105)
-    for (int i = 0; i < last; i++) {
-      const double dummy = h[i];
-      h[i] = ci[i] * dummy + si[i] * h[i + 1];
-      h[i + 1] = -si[i] * dummy + ci[i] * h[i + 1];
-    }
-    const double r = 1. / std::sqrt(h[last] * h[last] + h[last + 1] * h[last + 1]);
-    si[last] = h[last + 1] * r;
-    ci[last] = h[last] * r;
-    h[last] = ci[last] * h[last] + si[last] * h[last + 1];
-    gamma[last + 1] = -si[last] * gamma[last];
-    gamma[last] *= ci[last];
-    rho = std::fabs(gamma[dim]);
+    GH_TRY(arnoldi_step(mg, dim, vv, H[last], gamma, ci, si, d, &rho));
     reached = gh_cginfo_check(info, rho, (double)(k0 + last));
   }
-  // solution (gmres.cc:235-260): backward substitution on H1(i,j) = H[j][i]
-  std::vector<double> hs(std::max(dim, 1), 0.0);
-  for (int i = dim - 1; i >= 0; i--) {
-    double s = gamma[i];
-    for (int j = i + 1; j < dim; j++) s -= hs[j] * H[j][i];
-    hs[i] = s / H[i][i];
-  }
   if (dim > 0) {
-    // x += sum_i h_i mem[i], added in order i = 0..dim-1 per entry
-    double** dptrs = nullptr;
-    double* dh = nullptr;
-    std::vector<double*> ptrs(mg->mem.begin(), mg->mem.begin() + dim);
-    GH_TRY(dalloc((void**)&dptrs, sizeof(double*) * dim));
-    GH_TRY(dalloc((void**)&dh, sizeof(double) * dim));
-    GH_HIP(hipMemcpyAsync(dptrs, ptrs.data(), sizeof(double*) * dim, hipMemcpyHostToDevice, ctx->stream));
-    GH_HIP(hipMemcpyAsync(dh, hs.data(), sizeof(double) * dim, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_multi_axpy, dim3(std::min<int64_t>((n + kBlock - 1) / kBlock, 8192)),
-                       dim3(kBlock), 0, ctx->stream, n, x, dim, (const double* const*)dptrs, dh);
-    GH_HIP(hipGetLastError());
-    GH_HIP(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(dptrs);
-    (void)hipFree(dh);
+    const std::vector<double> hs = gmres_backward(H, gamma, dim);
+    GH_TRY(basis_update(mg, x, false, dim, hs.data()));
   }
+  GH_HIP(hipStreamSynchronize(ctx->stream));
   *reached_out = reached;
   *steps_out = dim;
   return GH_OK;
+}
+
+// Flexible GMRES (right preconditioned; Saad, SIAM J. Sci. Comput. 14 (1993)):
+// z_j = M^-1 v_j is kept, the Arnoldi relation is A Z = V H with FP64 products
+// A z_j, and x = x0 + Z y.  The preconditioner may then differ from step to
+// step -- the lower-precision V-cycle is linear only up to its own rounding --
+// and the minimised quantity is the true FP64 residual.  Orthogonalisation,
+// re-orthogonalisation test, Givens rotations and back-substitution are those
+// of GMRES::solve; its right-preconditioned form (gmres.cc:174-178, 201-205,
+// 252-258) applies M once more to V y at the end, which needs a fixed M.
+int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
+                int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
+                int* reached_out, int* steps_out) {
+  const int64_t n = mg->L[mg->nlevels - 1].size;
+  GH_TRY(ensure_gmres_memory(mg, vmax, true));
+  std::vector<std::vector<double>> H(vmax);
+  for (int i = 0; i < vmax; ++i) H[i].assign(i + 2, 0.0);
+  std::vector<double> gamma(vmax + 1, 0.0), ci(vmax, 0.0), si(vmax, 0.0);
+  const int k0 = info->iteration;
+  int reached = 0;
+  double* v = mg->mem[0];
+  // residualgmres: v = b - A x, Dirichlet rows zero
+  if (x_zero)
+    GH_TRY(vec_convert(ctx, n, GH_F64, v, GH_F64, b));
+  else
+    GH_TRY(csr_residual(ctx, A, GH_F64, v, x, b));
+  if (mask) GH_TRY(mask_apply(ctx, mask, GH_F64, v));
+  GH_TRY(reduce_dot(ctx, GH_F64, n, v, v, 2));
+  double rr;
+  GH_TRY(fetch_scalars(ctx, 2, 1, &rr));
+  double rho = std::sqrt(rr);
+  int dim = 0;
+  if (rho == 0.) {
+    *reached_out = 1;
+    *steps_out = 0;
+    return GH_OK;
+  }
+  gamma[0] = rho;
+  GH_TRY(vec_scale_to(ctx, GH_F64, n, v, 1. / rho, v));
+  const int fl = mg->nlevels - 1;
+  const int clevel = coarsest_level(mg, fl, coarselevel < 0 ? fl : coarselevel);
+  std::vector<double> d(vmax + 2);
+  while (dim < vmax - 1 && !reached) {
+    dim++;
+    const int last = dim - 1;
+    double* vv = mg->mem[dim];
+    void* z = mg->zmem[last];
+    // z = M^-1 v_last from a zero start, kept in the V-cycle's precision
+    if (mg->P.prec_maxiter != 1) {
+      if (mg->wprec != GH_F64)
+        return fail(GH_EINVAL, "flexible GMRES with a mixed-precision V-cycle needs prec_maxiter = 1");
+      GH_TRY(precond_raw(mg, coarselevel, (double*)z, mg->mem[last]));
+    } else {
+      GH_TRY(fast_vcycle(mg, clevel, z, mg->wprec, mg->mem[last], GH_F64, true));
+    }
+    if (mg->wprec == GH_F64)
+      GH_TRY(csr_spmv(ctx, A, GH_F64, vv, GH_F64, z, 1.0, 0.0));
+    else
+      GH_TRY(csr_spmv_x32(ctx, A, vv, (const float*)z));
+    GH_TRY(arnoldi_step(mg, dim, vv, H[last], gamma, ci, si, d, &rho));
+    reached = gh_cginfo_check(info, rho, (double)(k0 + last));
+  }
+  if (dim > 0) {
+    const std::vector<double> hs = gmres_backward(H, gamma, dim);
+    GH_TRY(basis_update(mg, x, true, dim, hs.data()));
+  }
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  *reached_out = reached;
+  *steps_out = dim;
+  return GH_OK;
+}
+
+int krylov_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
+                int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
+                int* reached_out, int* steps_out) {
+  if (mg->P.krylov == 1)
+    return fgmres_impl(ctx, mg, A, mask, coarselevel, vmax, x, b, x_zero, info, reached_out,
+                       steps_out);
+  return gmres_impl(ctx, mg, A, mask, coarselevel, vmax, x, b, x_zero, info, reached_out,
+                    steps_out);
 }
 
 double vec_norm_kind(gh_ctx* ctx, const double* x, int64_t n, int kind, int* rc) {
@@ -1034,8 +1189,8 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mas
     xd = mg->ext_x;
     bd = mg->ext_y;
   }
-  GH_TRY(gmres_impl(ctx, mg, F.A64, F.M, coarselevel, vmax, xd, bd, false, info,
-                    reached ? reached : &rdum, &steps));
+  GH_TRY(krylov_impl(ctx, mg, F.A64, F.M, coarselevel, vmax, xd, bd, false, info,
+                     reached ? reached : &rdum, &steps));
   if (F.d_perm)
     GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, x->d, GH_F64, mg->ext_x, false));
   GH_HIP(hipStreamSynchronize(ctx->stream));
@@ -1054,7 +1209,7 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
          "gh_newton_solve: u and f must be FP64 finest-level vectors");
   GH_ARG(A == mg->user_fine, "gh_newton_solve: A must be the finest-level matrix of the solver");
   const int64_t n = F.size;
-  GH_TRY(ensure_gmres_memory(mg, vmax));
+  GH_TRY(ensure_gmres_memory(mg, vmax, mg->P.krylov == 1));
   double* r = mg->r;
   double* w = mg->w;
   double* ud = (double*)u->d;
@@ -1082,7 +1237,7 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
     gh_cginfo_reset(linfo);
     int greached = 0, steps = 0;
     GH_TRY(vec_fill_zero(ctx, GH_F64, n, w));
-    GH_TRY(gmres_impl(ctx, mg, A, mask, coarselevel, vmax, w, r, true, linfo, &greached, &steps));
+    GH_TRY(krylov_impl(ctx, mg, A, mask, coarselevel, vmax, w, r, true, linfo, &greached, &steps));
     gm += steps;
     // NewtonUpdate
     int lex = linfo->status == GH_EXPLODED;

@@ -135,6 +135,14 @@ class CudaCSRMatrixInterface:
     def n_entries(self):
         return N.hip().gh_csr_nnz(self.h)
 
+    def layout(self):
+        """Device layout: entries stored, padded slots, slices, slices with
+        16-bit column offsets."""
+        out = np.zeros(4, np.int64)
+        N.check(N.hip().gh_csr_layout(self.h, N.ptr(out, N.i64p)))
+        return dict(stored=int(out[0]), slots=int(out[1]), slices=int(out[2]),
+                    narrow=int(out[3]))
+
     def vmult(self, y, x, d=1.0, b=0.0):
         """y = d*A*x + b*y"""
         check(N.hip().gh_spmv(self.ctx.h, self.h, y.h, x.h, d, b))
@@ -287,11 +295,13 @@ class CudaMultiLevelSolver:
     Parameters follow the reference's MultiLevelSolver / Solver blocks:
     mgtype, iterpre, iterpost, iterexact, omega, mgomega, linear_smooth,
     ndirect, prec_maxiter; `prec` selects the V-cycle storage ("f64", "f32"
-    or "f16" matrix values with FP32 arithmetic)."""
+    or "f16" matrix values with FP32 arithmetic); `krylov` the outer solver of
+    Gmres/Solve: "gmres" (left-preconditioned, as GMRES::solve) or "fgmres"
+    (flexible, right-preconditioned)."""
 
     def __init__(self, ctx, hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
                  mgomega=1.0, linear_smooth="jacobi", ndirect=1, prec_maxiter=1, prec="f64",
-                 use_graph=False, order=None):
+                 use_graph=False, order=None, krylov="gmres"):
         """order: None (keep the hierarchy's numbering), "lexicographic" (sort
         each level's nodes by their coordinates) or a list of per-level
         permutations (internal position -> node)."""
@@ -324,6 +334,7 @@ class CudaMultiLevelSolver:
         p.prec_maxiter = prec_maxiter
         p.prec = N.PREC[prec]
         p.use_graph = int(use_graph)
+        p.krylov = {"gmres": 0, "fgmres": 1}[krylov]
         self.params = p
 
         def arr(objs):

@@ -50,7 +50,7 @@ class GhMgParams(C.Structure):
                 ("iterexact", C.c_int), ("omega", C.c_double), ("mgomega", C.c_double),
                 ("smoother", C.c_int), ("ndirect", C.c_int), ("prec_maxiter", C.c_int),
                 ("prec_tol", C.c_double), ("prec_globaltol", C.c_double), ("prec", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("krylov", C.c_int)]
 
 
 class GhSolveReport(C.Structure):
@@ -94,6 +94,7 @@ HIP_SIGNATURES = {
     "gh_csr_n": (C.c_int64, [vp]),
     "gh_csr_nnz": (C.c_int64, [vp]),
     "gh_csr_prec": (C.c_int, [vp]),
+    "gh_csr_layout": (C.c_int, [vp, i64p]),
     "gh_spmv": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_double]),
     "gh_spmv_t": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_double]),
     "gh_residual": (C.c_int, [vp, vp, vp, vp, vp]),

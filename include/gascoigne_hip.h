@@ -105,6 +105,13 @@ int gh_csr_destroy(gh_csr* A);
 int64_t gh_csr_n(const gh_csr* A);
 int64_t gh_csr_nnz(const gh_csr* A);
 int gh_csr_prec(const gh_csr* A);
+/* Device layout (no reference counterpart; for byte accounting).  Scalar
+ * matrices: out[0] entries stored (exact zeros of the pattern are not stored;
+ * every row sum takes its other terms in the same order, so the values of
+ * A x are unchanged), out[1] padded slots, out[2] slices of 64 rows, out[3]
+ * slices whose columns are 16-bit offsets.  Block matrices: out[0] = out[1] =
+ * nnz blocks, out[2] = out[3] = 0. */
+int gh_csr_layout(const gh_csr* A, int64_t* out);
 /* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
 /* y = d*A^T*x + b*y (adjoint path; no reference counterpart on the device:
@@ -203,6 +210,13 @@ typedef struct {
   int prec;             /* GH_F64: plain FP64 V-cycle; GH_F32 / GH_F16: FP32
                            arithmetic with FP32 / FP16-stored matrix values */
   int use_graph;        /* 1: replay the V-cycle from a captured HIP graph */
+  int krylov;           /* outer Krylov method of gh_gmres_solve / gh_newton_solve:
+                           0: GMRES left-preconditioned (gmres.cc:160-233);
+                           1: flexible GMRES, right-preconditioned with the
+                           preconditioned directions kept, so that a V-cycle
+                           in lower precision may serve as the preconditioner
+                           while the Arnoldi process minimises the FP64
+                           residual (no reference counterpart) */
 } gh_mg_params;
 void gh_mg_params_default(gh_mg_params* p);
 

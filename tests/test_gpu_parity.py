@@ -356,3 +356,70 @@ def test_ordered_graph_mixed(ctx, prec):
     assert H.N.NL_STATUS[rb.status] == "converged"
     assert abs(ra.gmres_iterations - rb.gmres_iterations) <= 1
     assert relerr(Ub.copy_back(), Ua.copy_back()) <= 1e-9
+
+
+def test_sell_column_encodings_and_dropped_zeros(ctx):
+    """A matrix whose slices mix 16-bit and 32-bit column encodings and whose
+    pattern holds exact zeros: SpMV and residual equal the oracle's (same
+    values, same per-row order)."""
+    rng = np.random.default_rng(5)
+    n = 150_000
+    rows = []
+    for i in range(n):
+        if (i // 64) % 3 == 2:  # far columns: this slice needs 32-bit indices
+            cols = np.unique(np.concatenate([[i], rng.integers(0, n, 6)]))
+        else:                   # banded: 16-bit offsets from the slice base
+            cols = np.unique(np.clip(i + np.array([-300, -17, -1, 0, 1, 17, 300]), 0, n - 1))
+        rows.append(cols)
+    rowptr = np.zeros(n + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(c) for c in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    val = rng.uniform(-1, 1, len(col))
+    val[rng.random(len(col)) < 0.2] = 0.0                     # explicit zeros
+    val[rowptr[:-1][np.arange(n) % 5 == 0]] = 0.0             # incl. first entries
+    diag = np.concatenate([np.nonzero(c == i)[0] + rowptr[i] for i, c in enumerate(rows)])
+    val[diag] = 4.0
+    A = H.CudaCSRMatrixInterface(ctx, rowptr, col, val)
+    lay = A.layout()
+    assert lay["stored"] == int(np.count_nonzero(val))
+    assert 0 < lay["narrow"] < lay["slices"]
+    Ao = O.Matrix(rowptr, col, val)
+    x, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    y = np.zeros(n)
+    Ao.vmult(y, x)
+    Y = H.CudaVectorInterface(ctx, n)
+    A.vmult(Y, vec(ctx, x), 1.0, 0.0)
+    assert np.array_equal(Y.copy_back(), y)
+    r = np.zeros(n)
+    Ao.residual(r, x, b)
+    R = H.CudaVectorInterface(ctx, n)
+    A.residual(R, vec(ctx, x), vec(ctx, b))
+    assert np.array_equal(R.copy_back(), r)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada"])
+def test_flexible_gmres_solve(ctx, name, prec):
+    """Flexible GMRES around the V-cycle (FP64 or lower precision): the true FP64
+    residual reaches the 1e-10 target in one outer step, and the solution agrees
+    with the FP64 oracle solve."""
+    print(f"[{name} {prec}]", flush=True)
+    h = Hierarchy.from_fixture(fixture_path(name))
+    om = O.MG(h, ndirect=1, **MGP)
+    f = rhs(h)
+    uo = np.zeros_like(f)
+    om.newton(uo, f, O.cginfo(1e-12, 1e-30, 100), O.nlinfo(1e-12, 1e-30, 30), norm_kind=1)
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=1, prec=prec, krylov="fgmres", **MGP)
+    U = mg.new_vector()
+    rep = mg.Solve(U, vec(ctx, f), H.CGInfo(0.5e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30),
+                   norm_kind=1)
+    assert H.N.NL_STATUS[rep.status] == "converged"
+    u = U.copy_back()
+    A = sp.csr_matrix((h.fine.val, h.fine.col, h.fine.rowptr))
+    r = f - A @ u
+    r[h.fine.dirichlet] = 0.0
+    assert np.linalg.norm(r) <= 1e-10 * rep.first_residual * 1.0001
+    err = relerr(u, uo)
+    print(f"{name} {prec}: newton {rep.newton_iterations} fgmres {rep.gmres_iterations} "
+          f"rel. error vs FP64 oracle {err:.2e}")
+    assert err <= 1e-8
