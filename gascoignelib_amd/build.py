@@ -17,13 +17,14 @@ ARCH = "gfx950"
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # packed FP32 instructions stay off ("Build")
+    # no packed FP32 instructions: some of their forms have given wrong results
+    # on this hardware without any error being raised
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
     f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["gh_core.hip", "gh_solver.hip"]
+HIP_SOURCES = ["gh_core.hip", "gh_solver.hip", "gh_adjoint.hip"]
 HIP_LIB = os.path.join(PKG, "libgascoigne_hip.so")
 
 CXX = os.environ.get("CXX", "g++")

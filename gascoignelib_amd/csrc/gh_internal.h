@@ -13,6 +13,16 @@
 
 #include "gascoigne_hip.h"
 
+// Q1 box cells for coefficient gradients
+struct gh_cells {
+  gh_ctx* ctx = nullptr;
+  int dim = 3;
+  int64_t n = 0;        // nodes
+  int64_t ncells = 0;
+  int32_t* cells = nullptr;  // ncells * (4 | 8) node indices, local corner order
+  double* h = nullptr;       // ncells * 3 box edge lengths (hz = 1 in 2D)
+};
+
 namespace gh {
 
 // ---------------------------------------------------------------- errors
@@ -180,6 +190,11 @@ int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const 
              double d, double b);
 int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b);
 int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x);
+// host transpose of a node-block CSR (blocks transposed too); rows of the
+// result list their columns ascending
+void csr_transpose_host(int64_t n, int B, const int32_t* rowptr, const int32_t* col,
+                        const double* val, std::vector<int32_t>& rpT, std::vector<int32_t>& clT,
+                        std::vector<double>& vlT);
 int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const void* x,
                      const void* b, double omega, int smoother, bool x_zero);
 int vec_axpby(gh_ctx* ctx, int prec, int64_t size, void* y, double a, double d, const void* x);

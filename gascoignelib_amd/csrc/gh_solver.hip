@@ -169,6 +169,7 @@ void gh_mg_params_default(gh_mg_params* p) {
   p->prec = GH_F64;
   p->use_graph = 0;
   p->krylov = 0;
+  p->adjoint = 0;
 }
 
 }  // extern "C"
@@ -532,16 +533,28 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
       return fail(GH_EINVAL, "gh_mg_create_ordered: order of level %d is not a permutation", l);
     Q[P[k]] = (int32_t)k;
   }
-  // matrix
+  // matrix (the adjoint solver: its transpose, formed in the caller's
+  // numbering first so that rows keep the unordered solver's entry order)
+  std::vector<int32_t> srp, scl;
+  std::vector<double> svl;
+  const int32_t* s_rp = Au->h_rowptr.data();
+  const int32_t* s_col = Au->h_col.data();
+  const double* s_val = Au->h_val.data();
+  if (params->adjoint) {
+    csr_transpose_host(n, B, s_rp, s_col, s_val, srp, scl, svl);
+    s_rp = srp.data();
+    s_col = scl.data();
+    s_val = svl.data();
+  }
   std::vector<int32_t> rp(n + 1, 0), cl;
   std::vector<double> vl;
   cl.reserve(Au->nnz);
   vl.reserve(Au->nnz * BB);
   for (int64_t k = 0; k < n; ++k) {
     const int32_t o = P[k];
-    for (int32_t q = Au->h_rowptr[o]; q < Au->h_rowptr[o + 1]; ++q) {
-      cl.push_back(Q[Au->h_col[q]]);
-      vl.insert(vl.end(), Au->h_val.begin() + (int64_t)q * BB, Au->h_val.begin() + (int64_t)(q + 1) * BB);
+    for (int32_t q = s_rp[o]; q < s_rp[o + 1]; ++q) {
+      cl.push_back(Q[s_col[q]]);
+      vl.insert(vl.end(), s_val + (int64_t)q * BB, s_val + (int64_t)(q + 1) * BB);
     }
     rp[k + 1] = (int32_t)cl.size();
   }
@@ -614,6 +627,10 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
          "gh_mg_create: mgtype must be V, W or F");
   GH_ARG(params->smoother == 0 || params->smoother == 1, "gh_mg_create: bad smoother");
   GH_ARG(params->krylov == 0 || params->krylov == 1, "gh_mg_create: krylov must be 0 or 1");
+  GH_ARG(params->adjoint == 0 || params->adjoint == 1, "gh_mg_create: adjoint must be 0 or 1");
+  if (params->adjoint && H)
+    for (int l = 0; l < nlevels; ++l)
+      GH_ARG(!H[l], "gh_mg_create: the adjoint solver does not support hanging nodes");
   // shapes of the caller's objects
   for (int l = 0; l < nlevels; ++l) {
     if (!A[l] || A[l]->prec != GH_F64)
@@ -647,6 +664,25 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
     Lv.size = Lv.n * Lv.ncomp;
     if (perm) {
       rc = build_ordered_level(ctx, params, l, A, T, H, M, perm, Lv);
+    } else if (params->adjoint) {
+      // the transposed level operator, once; everything downstream runs on it
+      std::vector<int32_t> rpT, clT;
+      std::vector<double> vlT;
+      csr_transpose_host(A[l]->n, A[l]->ncomp, A[l]->h_rowptr.data(), A[l]->h_col.data(),
+                         A[l]->h_val.data(), rpT, clT, vlT);
+      rc = csr_create_impl(ctx, Lv.n, Lv.ncomp, rpT.data(), clT.data(), vlT.data(), GH_F64, false,
+                           &Lv.A64);
+      Lv.own_A64 = !rc;
+      if (!rc && params->prec == GH_F64) {
+        Lv.A = Lv.A64;
+      } else if (!rc) {
+        rc = csr_create_impl(ctx, Lv.n, Lv.ncomp, rpT.data(), clT.data(), vlT.data(), params->prec,
+                             false, &Lv.A);
+        Lv.own_A = !rc;
+      }
+      Lv.T = l > 0 ? T[l] : nullptr;
+      Lv.H = nullptr;
+      Lv.M = M ? M[l] : nullptr;
     } else {
       Lv.A64 = A[l];
       Lv.T = l > 0 ? T[l] : nullptr;

@@ -297,11 +297,12 @@ class CudaMultiLevelSolver:
     ndirect, prec_maxiter; `prec` selects the V-cycle storage ("f64", "f32"
     or "f16" matrix values with FP32 arithmetic); `krylov` the outer solver of
     Gmres/Solve: "gmres" (left-preconditioned, as GMRES::solve) or "fgmres"
-    (flexible, right-preconditioned)."""
+    (flexible, right-preconditioned); `adjoint` makes Gmres/Solve/LinearMg
+    solve with the transposed matrices (A^T z = g)."""
 
     def __init__(self, ctx, hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
                  mgomega=1.0, linear_smooth="jacobi", ndirect=1, prec_maxiter=1, prec="f64",
-                 use_graph=False, order=None, krylov="gmres"):
+                 use_graph=False, order=None, krylov="gmres", adjoint=False):
         """order: None (keep the hierarchy's numbering), "lexicographic" (sort
         each level's nodes by their coordinates) or a list of per-level
         permutations (internal position -> node)."""
@@ -335,6 +336,7 @@ class CudaMultiLevelSolver:
         p.prec = N.PREC[prec]
         p.use_graph = int(use_graph)
         p.krylov = {"gmres": 0, "fgmres": 1}[krylov]
+        p.adjoint = int(adjoint)
         self.params = p
 
         def arr(objs):
@@ -412,5 +414,33 @@ class CudaMultiLevelSolver:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class CoefficientCells:
+    """Q1 box cells of a mesh level for per-cell coefficient gradients
+    (gh_cells_create / gh_dJdk_cells): dJdk(z, u, g) writes
+    g_c = -z^T (dA/dk_c) u for the state u and the adjoint z."""
+
+    def __init__(self, ctx, dim, coords, cells):
+        self.ctx = ctx
+        xyz = np.zeros((len(coords), 3))
+        xyz[:, : np.asarray(coords).shape[1]] = coords
+        self._xyz = np.ascontiguousarray(xyz)
+        self._cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.ncells = len(self._cells)
+        self.n = len(xyz)
+        self.h = C.c_void_p()
+        check(N.hip().gh_cells_create(ctx.h, dim, self.n, ptr(self._xyz), self.ncells,
+                                      ptr(self._cells, N.i32p), C.byref(self.h)))
+
+    def dJdk(self, z, u, g):
+        check(N.hip().gh_dJdk_cells(self.ctx.h, self.h, z.h, u.h, g.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.hip().gh_cells_destroy(self.h)
         except Exception:
             pass

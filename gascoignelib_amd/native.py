@@ -50,7 +50,7 @@ class GhMgParams(C.Structure):
                 ("iterexact", C.c_int), ("omega", C.c_double), ("mgomega", C.c_double),
                 ("smoother", C.c_int), ("ndirect", C.c_int), ("prec_maxiter", C.c_int),
                 ("prec_tol", C.c_double), ("prec_globaltol", C.c_double), ("prec", C.c_int),
-                ("use_graph", C.c_int), ("krylov", C.c_int)]
+                ("use_graph", C.c_int), ("krylov", C.c_int), ("adjoint", C.c_int)]
 
 
 class GhSolveReport(C.Structure):
@@ -133,6 +133,9 @@ HIP_SIGNATURES = {
                                  P(C.c_int)]),
     "gh_newton_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, P(GhCGInfo),
                                   P(GhNLInfo), C.c_int, P(GhSolveReport)]),
+    "gh_cells_create": (C.c_int, [vp, C.c_int, C.c_int64, f64p, C.c_int64, i32p, P(vp)]),
+    "gh_cells_destroy": (C.c_int, [vp]),
+    "gh_dJdk_cells": (C.c_int, [vp, vp, vp, vp, vp]),
 }
 
 STRUCT_SIGNATURES = {

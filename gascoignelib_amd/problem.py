@@ -155,3 +155,54 @@ def lattice_rhs(coords, seed, ncomp=1):
         v = splitmix64_uniform(seed + comp, int(key.max()) + 1)
         vals[comp::ncomp] = v[key]
     return vals
+
+
+def splitmix64_normal(seed, n):
+    """Standard normal samples by Box-Muller from two SplitMix64 streams."""
+    u1 = 0.5 * (splitmix64_uniform(seed, n) + 1.0)          # [0, 1)
+    u2 = 0.5 * (splitmix64_uniform(seed + 0x5151, n) + 1.0)
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def gaussian_random_field(points, seed, sigma=1.0, length=0.1, modes=1024):
+    """g(x) = sigma sqrt(2/M) sum_m cos(w_m . x + b_m), w_m ~ N(0, I / length^2),
+    b_m ~ U[0, 2 pi): random Fourier features of the squared-exponential
+    covariance sigma^2 exp(-|x-y|^2 / (2 length^2)) (Rahimi & Recht, 2007)."""
+    x = np.asarray(points, dtype=np.float64)
+    d = x.shape[1]
+    w = splitmix64_normal(seed, modes * d).reshape(modes, d) / length
+    b = np.pi * (splitmix64_uniform(seed + 0xB0B, modes) + 1.0)
+    out = np.zeros(len(x))
+    for lo in range(0, modes, 128):  # chunks keep the temporary small
+        out += np.cos(x @ w[lo:lo + 128].T + b[lo:lo + 128]).sum(axis=1)
+    return sigma * np.sqrt(2.0 / modes) * out
+
+
+def smooth_weight(points, seed, modes=2):
+    """w(x) = sum_{p,q,r <= modes} a_pqr prod_d sin(p_d pi x_d), a ~ U[-1, 1)."""
+    x = np.asarray(points, dtype=np.float64)
+    d = x.shape[1]
+    ks = np.array(np.meshgrid(*[np.arange(1, modes + 1)] * d, indexing="ij")).reshape(d, -1).T
+    a = splitmix64_uniform(seed, len(ks))
+    out = np.zeros(len(x))
+    for coef, k in zip(a, ks):
+        out += coef * np.prod(np.sin(np.pi * k * x), axis=1)
+    return out
+
+
+def cell_centres(coords, cells):
+    return np.asarray(coords)[np.asarray(cells)].mean(axis=1)
+
+
+def lumped_mass(coords, cells):
+    """int phi_i over axis-aligned Q1 box cells: each corner gets vol / corners"""
+    X = np.asarray(coords)
+    cells = np.asarray(cells)
+    npc = cells.shape[1]
+    d = X.shape[1]
+    lo = X[cells].min(axis=1)
+    hi = X[cells].max(axis=1)
+    vol = np.prod(hi - lo, axis=1)
+    m = np.zeros(len(X))
+    np.add.at(m, cells.ravel(), np.repeat(vol / npc, npc))
+    return m

@@ -107,7 +107,14 @@ def uniform_hierarchy(inp="cube.inp", dim=3, prerefine=7, drop_coarse=1, colours
     coarsest meshes from the hierarchy (Gascoigne keeps nlevels - patchdepth
     levels: the 1-cell mesh of a one-hex .inp is not a multigrid level)."""
     M = Mesh(inp, dim, prerefine, drop_coarse)
+    return assemble_hierarchy(M, kcell, colours, name or f"{inp}:{prerefine}", keep_coords)
+
+
+def assemble_hierarchy(M, kcell=None, colours=(1,), name="", keep_coords=True):
+    """Matrices, transfers and Dirichlet lists of every level of the mesh M;
+    kcell (one coefficient per finest cell) is averaged onto coarser cells."""
     nl = M.nlevels
+    dim = M.dim
     ks = [None] * nl
     if kcell is not None:
         ks[-1] = np.asarray(kcell, dtype=np.float64)
@@ -122,7 +129,7 @@ def uniform_hierarchy(inp="cube.inp", dim=3, prerefine=7, drop_coarse=1, colours
                     dirichlet=M.dirichlet(l, colours), dirichlet_comps=np.zeros(1, np.int32),
                     coords=M.coords(l) if keep_coords else None)
         levels.append(lev)
-    h = Hierarchy(levels, dim, name or f"{inp}:{prerefine}")
+    h = Hierarchy(levels, dim, name)
     h.mesh = M
     h.kcell = ks
     return h

@@ -45,6 +45,7 @@ typedef struct gh_transfer gh_transfer;
 typedef struct gh_hn gh_hn;
 typedef struct gh_mask gh_mask;
 typedef struct gh_mg gh_mg;
+typedef struct gh_cells gh_cells;
 
 const char* gh_last_error(void);
 const char* gh_version(void);
@@ -217,6 +218,14 @@ typedef struct {
                            in lower precision may serve as the preconditioner
                            while the Arnoldi process minimises the FP64
                            residual (no reference counterpart) */
+  int adjoint;          /* 1: the solver works on the transposed level matrices:
+                           gh_gmres_solve / gh_newton_solve / gh_mg_linear then
+                           solve A^T z = g for the finest-level A given at
+                           creation, with smoothing, residuals and products by
+                           A_l^T and restriction = transpose of prolongation (no
+                           reference counterpart: the reference has no adjoint
+                           solve, src/Solver/stdsolver.cc:1068-1090).  Hanging
+                           nodes are not supported in this mode. */
 } gh_mg_params;
 void gh_mg_params_default(gh_mg_params* p);
 
@@ -284,6 +293,26 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask
                     const gh_hn* hn, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
                     gh_cginfo* linfo, gh_nlinfo* nlinfo, int norm_kind,
                     gh_solve_report* rep);
+
+/* --------------------------------------------------------------- adjoint */
+/* Coefficient gradients of J(u(k)) for -div(k grad u) = f assembled with one
+ * coefficient per cell (A = sum_c k_c K^c, Dirichlet rows and columns
+ * eliminated).  No reference counterpart: the reference assembles dual
+ * matrices (StdSolver::AssembleDualMatrix, src/Solver/stdsolver.cc:1815-1834)
+ * but has no adjoint solve or coefficient gradient.
+ *
+ * gh_cells_create: the Q1 cells of an axis-aligned box mesh, ncells x (4 | 8)
+ * node indices in Gascoigne's corner order (hex: (0,0,0) (1,0,0) (1,1,0)
+ * (0,1,0) then the same at z = 1), node coordinates xyz[n*3]. */
+int gh_cells_create(gh_ctx* ctx, int dim, int64_t n, const double* xyz, int64_t ncells,
+                    const int32_t* cells, gh_cells** out);
+int gh_cells_destroy(gh_cells* C);
+/* g[c] = dJ/dk_c = -z^T (dA/dk_c) u = -sum_{a,b} z(v_a) K^c_ab u(v_b), with K^c
+ * the unit-coefficient Q1 Laplace element matrix of cell c (tensor products of
+ * the 1D stiffness and mass matrices, as the assembly forms it), u the state
+ * and z the adjoint (A^T z = dJ/du).  u and z must vanish on Dirichlet nodes
+ * (then only free-node couplings contribute, as in the eliminated matrix). */
+int gh_dJdk_cells(gh_ctx* ctx, const gh_cells* C, const gh_vec* z, const gh_vec* u, gh_vec* g);
 
 #ifdef __cplusplus
 }

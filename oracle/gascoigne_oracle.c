@@ -17,6 +17,40 @@
 #include <omp.h>
 #endif
 
+/* ------------------------------------------------------------ adjoint */
+static double or_s1(int p, int q, double h) { return (p == q ? 1.0 : -1.0) / h; }
+static double or_m1(int p, int q, double h) { return h * (p == q ? 2.0 : 1.0) / 6.0; }
+
+void or_dJdk_cells(int dim, const double* xyz, int64_t ncells, const int32_t* cells,
+                   const double* z, const double* u, double* g) {
+  static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                   {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  const int npc = dim == 3 ? 8 : 4;
+  for (int64_t c = 0; c < ncells; ++c) {
+    const int32_t* v = cells + c * npc;
+    const double* x0 = xyz + 3 * (int64_t)v[0];
+    const double hx = xyz[3 * (int64_t)v[1]] - x0[0];
+    const double hy = xyz[3 * (int64_t)v[3] + 1] - x0[1];
+    const double hz = dim == 3 ? xyz[3 * (int64_t)v[4] + 2] - x0[2] : 1.0;
+    double s = 0.0;
+    for (int a = 0; a < npc; ++a)
+      for (int b = 0; b < npc; ++b) {
+        const int* pa = corner[a];
+        const int* pb = corner[b];
+        double e;
+        if (dim == 3)
+          e = or_s1(pa[0], pb[0], hx) * or_m1(pa[1], pb[1], hy) * or_m1(pa[2], pb[2], hz) +
+              or_m1(pa[0], pb[0], hx) * or_s1(pa[1], pb[1], hy) * or_m1(pa[2], pb[2], hz) +
+              or_m1(pa[0], pb[0], hx) * or_m1(pa[1], pb[1], hy) * or_s1(pa[2], pb[2], hz);
+        else
+          e = or_s1(pa[0], pb[0], hx) * or_m1(pa[1], pb[1], hy) +
+              or_m1(pa[0], pb[0], hx) * or_s1(pa[1], pb[1], hy);
+        s += z[v[a]] * e * u[v[b]];
+      }
+    g[c] = -s;
+  }
+}
+
 int or_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -25,6 +25,10 @@
  *       explicit matrix, GMRES minimal-residual property, convergence to the
  *       dense direct solution (the reference's StdMultiLevelSolver/GMRES
  *       compile but cannot link here, see oracle/ref/Makefile)
+ *   or_dJdk_cells                                              mathematics
+ *       (no reference counterpart): Euler's identity sum_c k_c dJ/dk_c = -J,
+ *       the element matrices summing to the assembled matrix, and central
+ *       finite differences of J on direct solves
  *   or_block_gauss_jordan                                      inverse times
  *       block = identity, permutation cases
  *   or_direct_*                                                residual of the
@@ -182,6 +186,15 @@ typedef struct {
 int or_newton_solve(or_mg* M, int coarselevel, int vmax, double* u, const double* f,
                     or_cginfo* linfo, or_nlinfo* nlinfo, int norm_kind,
                     or_solve_report* rep);
+
+/* Coefficient gradient of J(u(k)) for -div(k grad u) = f with one coefficient
+ * per cell of an axis-aligned Q1 box mesh (no reference counterpart: the
+ * reference has no adjoint solve).  g[c] = -sum_{a,b} z(v_a) K^c_ab u(v_b),
+ * K^c the unit-coefficient element matrix (tensor products of the 1D Q1
+ * stiffness and mass on the cell's edge lengths), cells in Gascoigne's corner
+ * order, xyz[n*3] node coordinates. */
+void or_dJdk_cells(int dim, const double* xyz, int64_t ncells, const int32_t* cells,
+                   const double* z, const double* u, double* g);
 
 /* OpenMP threads used by the loops the reference parallelises */
 int or_num_threads(void);

@@ -122,6 +122,7 @@ def _load():
                                       C.POINTER(OrCGInfo), C.POINTER(OrNLInfo), C.c_int,
                                       C.POINTER(OrSolveReport)]),
         "or_num_threads": (C.c_int, []),
+        "or_dJdk_cells": (None, [C.c_int, f64p, C.c_int64, i32p, f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -305,3 +306,15 @@ class MG:
         rc = lib().or_newton_solve(self.p, coarselevel, vmax, ptr(u), ptr(f), C.byref(linfo),
                                    C.byref(nl), norm_kind, C.byref(rep))
         return rc, rep
+
+
+def dJdk_cells(dim, coords, cells, z, u):
+    """oracle per-cell coefficient gradient -z^T K^c u"""
+    xyz = np.zeros((len(coords), 3))
+    xyz[:, : np.asarray(coords).shape[1]] = coords
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    g = np.zeros(len(cells))
+    lib().or_dJdk_cells(dim, ptr(np.ascontiguousarray(xyz)), len(cells), ptr(cells, i32p),
+                        ptr(np.ascontiguousarray(z, dtype=np.float64)),
+                        ptr(np.ascontiguousarray(u, dtype=np.float64)), ptr(g))
+    return g

@@ -1,0 +1,84 @@
+"""The oracle's coefficient gradient (or_dJdk_cells) pinned by mathematics.
+
+The reference has no adjoint solve, so nothing of it pins this function; the
+checks below fix it instead (direct sparse solves, FP64):
+  * Euler's identity: J(k) = g^T A(k)^-1 f is homogeneous of degree -1 in k,
+    so sum_c k_c dJ/dk_c = -J;
+  * the element matrices sum to the assembled matrix: -sum_c k_c (dJ/dk_c with
+    z = e_i, u = e_j) = A_ij on free nodes;
+  * central finite differences of J (h = 1e-6).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import oracle_ctypes as O
+from gascoignelib_amd.problem import (cell_centres, gaussian_random_field, lattice_rhs,
+                                      lumped_mass, smooth_weight, splitmix64_uniform)
+from gascoignelib_amd.structure import Mesh, assemble_hierarchy
+
+
+def problem(inp="cube3.inp", dim=3, pre=1):
+    M = Mesh(inp, dim, pre, 0)
+    fl = M.nlevels - 1
+    X, cells, dirichlet = M.coords(fl), M.cells(fl), M.dirichlet(fl)
+    k = np.exp(gaussian_random_field(cell_centres(X, cells), seed=5, length=0.3))
+    f = lattice_rhs(X, 2024)
+    f[dirichlet] = 0.0
+    g = lumped_mass(X, cells) * smooth_weight(X, 77)
+    g[dirichlet] = 0.0
+    return M, X, cells, dirichlet, k, f, g
+
+
+def fine_matrix(M, k):
+    lev = assemble_hierarchy(M, k).fine
+    return sp.csr_matrix((lev.val, lev.col, lev.rowptr), shape=(lev.n, lev.n))
+
+
+def J_of(M, k, f, g):
+    return float(g @ spl.spsolve(fine_matrix(M, k).tocsc(), f))
+
+
+@pytest.mark.parametrize("inp,dim,pre", [("cube3.inp", 3, 1), ("square.inp", 2, 4)])
+def test_euler_identity(inp, dim, pre):
+    M, X, cells, _, k, f, g = problem(inp, dim, pre)
+    A = fine_matrix(M, k).tocsc()
+    u = spl.spsolve(A, f)
+    z = spl.spsolve(A.T.tocsc(), g)
+    dk = O.dJdk_cells(dim, X, cells, z, u)
+    J = g @ u
+    assert abs(k @ dk + J) <= 1e-12 * abs(J)
+
+
+def test_element_matrices_sum_to_assembly():
+    M, X, cells, dirichlet, k, _, _ = problem()
+    A = fine_matrix(M, k)
+    free = np.setdiff1d(np.arange(len(X)), dirichlet)
+    rng = np.random.default_rng(3)
+    n = len(X)
+    for i in rng.choice(free, 6, replace=False):
+        cols = A.indices[A.indptr[i]:A.indptr[i + 1]]
+        for j in cols:
+            if j in dirichlet:
+                continue
+            ei, ej = np.zeros(n), np.zeros(n)
+            ei[i], ej[j] = 1.0, 1.0
+            aij = -(k @ O.dJdk_cells(3, X, cells, ei, ej))
+            assert abs(aij - A[i, j]) <= 1e-14 * abs(A[i, i]), (i, j)
+
+
+def test_central_differences():
+    M, X, cells, _, k, f, g = problem()
+    A = fine_matrix(M, k).tocsc()
+    u = spl.spsolve(A, f)
+    z = spl.spsolve(A.T.tocsc(), g)
+    dk = O.dJdk_cells(3, X, cells, z, u)
+    h = 1e-6
+    picks = (0.5 * (splitmix64_uniform(99, 8) + 1.0) * len(cells)).astype(int)
+    for c in picks:
+        kp, km = k.copy(), k.copy()
+        kp[c] += h
+        km[c] -= h
+        fd = (J_of(M, kp, f, g) - J_of(M, km, f, g)) / (2 * h)
+        assert abs(fd - dk[c]) <= 1e-6 * np.abs(dk).max(), c

@@ -1,0 +1,153 @@
+"""Adjoint path on the GPU: the transposed multigrid solve, the per-cell
+coefficient gradient kernel, and the gradient check of J(k) = g . u(k) for
+-div(k grad u) = f with k = exp(Gaussian random field) per cell.
+
+Tolerances: the adjoint solver runs the forward algorithm on the transposed
+levels, so it matches the oracle run on explicitly transposed matrices to
+1e-12 (iterates) with equal iteration counts; the gradient kernel matches the
+oracle's restatement to 1e-13 (sum order differs); the gradient matches the
+FP64 CPU adjoint to 1e-9 and central differences (h = 1e-6, solves to 1e-13)
+to 1e-5 of its largest entry; the mixed-precision gradient matches the FP64
+one to 1e-8.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle_ctypes as O
+from conftest import fixture_path
+from gascoignelib_amd import hip as H
+from gascoignelib_amd.autograd import CoefficientFunctional, DiffusionProblem
+from gascoignelib_amd.problem import (Hierarchy, Level, cell_centres, gaussian_random_field,
+                                      splitmix64_uniform)
+
+pytestmark = pytest.mark.gpu
+
+MGP = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = H.Context(0)
+    yield c
+    c.close()
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def skewed(h):
+    """The hierarchy with every free-free off-diagonal entry scaled by 1.1 above
+    the diagonal and 0.9 below: a non-symmetric operator of the same pattern."""
+    levels = []
+    for lev in h.levels:
+        rows = np.repeat(np.arange(lev.n), np.diff(lev.rowptr))
+        d = np.zeros(lev.n, bool)
+        d[lev.dirichlet] = True
+        free = ~d[rows] & ~d[lev.col]
+        val = lev.val.copy()
+        val[free & (lev.col > rows)] *= 1.1
+        val[free & (lev.col < rows)] *= 0.9
+        levels.append(Level(lev.n, 1, lev.rowptr, lev.col, val, lev.transfer, lev.hn_kind,
+                            dirichlet=lev.dirichlet, dirichlet_comps=lev.dirichlet_comps,
+                            coords=lev.coords))
+    return Hierarchy(levels, h.dim, h.name + ":skewed")
+
+
+def transposed(h):
+    levels = []
+    for lev in h.levels:
+        T = sp.csr_matrix((lev.val, lev.col, lev.rowptr), shape=(lev.n, lev.n)).T.tocsr()
+        T.sort_indices()
+        levels.append(Level(lev.n, 1, T.indptr.astype(np.int32), T.indices.astype(np.int32),
+                            T.data.copy(), lev.transfer, lev.hn_kind, dirichlet=lev.dirichlet,
+                            dirichlet_comps=lev.dirichlet_comps, coords=lev.coords))
+    return Hierarchy(levels, h.dim, h.name + ":T")
+
+
+@pytest.mark.parametrize("ndirect", [1, 200])
+@pytest.mark.parametrize("order", [None, "lexicographic"])
+def test_adjoint_solver_vs_oracle_on_transpose(ctx, ndirect, order):
+    print(f"[adjoint ndirect={ndirect} order={order}]", flush=True)
+    h = skewed(Hierarchy.from_fixture(fixture_path("q1_3d_16")))
+    g = splitmix64_uniform(11, h.fine.n)
+    g[h.fine.dirichlet] = 0.0
+    om = O.MG(transposed(h), ndirect=ndirect, **MGP)
+    zo = np.zeros_like(g)
+    _, orep = om.newton(zo, g, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30),
+                        norm_kind=1)
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=ndirect, adjoint=True, order=order, **MGP)
+    Z = mg.new_vector()
+    rep = mg.Solve(Z, H.CudaVectorInterface.from_global(ctx, g), H.CGInfo(1e-10, 1e-30, 100),
+                   H.NLInfo(1e-10, 1e-30, 30), norm_kind=1)
+    z = Z.copy_back()
+    A = sp.csr_matrix((h.fine.val, h.fine.col, h.fine.rowptr))
+    assert relerr(A @ z, A.T @ z) > 1e-3  # the operator is not symmetric
+    r = g - A.T @ z
+    r[h.fine.dirichlet] = 0.0
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(g) * 1.0001
+    assert rep.newton_iterations == orep.newton_iterations
+    assert abs(rep.gmres_iterations - orep.gmres_iterations) <= 1
+    assert relerr(z, zo) <= 1e-12
+
+
+def test_dJdk_kernel_vs_oracle(ctx):
+    P = DiffusionProblem(ctx, prerefine=3)  # 24^3 cells
+    rng = np.random.default_rng(1)
+    z, u = rng.uniform(-1, 1, len(P.coords)), rng.uniform(-1, 1, len(P.coords))
+    z[P.dirichlet] = 0.0
+    u[P.dirichlet] = 0.0
+    G = H.CudaVectorInterface(ctx, P.ncells)
+    P.cellmap.dJdk(H.CudaVectorInterface.from_global(ctx, z),
+                   H.CudaVectorInterface.from_global(ctx, u), G)
+    go = O.dJdk_cells(3, P.coords, P.cells, z, u)
+    assert np.abs(G.copy_back() - go).max() <= 1e-13 * np.abs(go).max()
+
+
+def coefficient(P):
+    return np.exp(gaussian_random_field(cell_centres(P.coords, P.cells), seed=31))
+
+
+def test_gradient_check_fd_and_cpu_adjoint(ctx):
+    P = DiffusionProblem(ctx, prerefine=2)  # 12^3 cells, 2197 nodes
+    k = torch.tensor(coefficient(P), dtype=torch.float64, requires_grad=True)
+    J = CoefficientFunctional.apply(k, P)
+    J.backward()
+    grad = k.grad.numpy()
+    kk = k.detach().numpy()
+    # FP64 CPU adjoint: oracle solve of A^T z = g on the transposed hierarchy
+    hier = P.hierarchy(kk)
+    om = O.MG(transposed(hier), ndirect=400, **MGP)
+    zo = np.zeros(len(P.coords))
+    om.newton(zo, P.g, O.cginfo(1e-13, 1e-30, 200), O.nlinfo(1e-13, 1e-30, 50), norm_kind=1)
+    uo = np.zeros(len(P.coords))
+    O.MG(hier, ndirect=400, **MGP).newton(uo, P.f, O.cginfo(1e-13, 1e-30, 200),
+                                           O.nlinfo(1e-13, 1e-30, 50), norm_kind=1)
+    go = O.dJdk_cells(3, P.coords, P.cells, zo, uo)
+    assert relerr(grad, go) <= 1e-9
+    assert abs(kk @ grad + float(J)) <= 1e-10 * abs(float(J))  # Euler's identity
+    # central differences on 20 seeded cells
+    h = 1e-6
+    picks = (0.5 * (splitmix64_uniform(4242, 20) + 1.0) * P.ncells).astype(int)
+    for c in picks:
+        kp, km = kk.copy(), kk.copy()
+        kp[c] += h
+        km[c] -= h
+        fd = (P.J(kp) - P.J(km)) / (2 * h)
+        print(f"cell {c}: adjoint {grad[c]:+.10e} fd {fd:+.10e}")
+        assert abs(fd - grad[c]) <= 1e-5 * np.abs(grad).max(), c
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_precision_gradient(ctx, prec):
+    P64 = DiffusionProblem(ctx, prerefine=3, tol=1e-12)
+    Pmx = DiffusionProblem(ctx, prerefine=3, tol=1e-12, prec=prec, krylov="fgmres")
+    k = coefficient(P64)
+    h64, U64, _ = P64.state(k)
+    _, g64 = P64.gradient(h64, U64)
+    hmx, Umx, _ = Pmx.state(k)
+    _, gmx = Pmx.gradient(hmx, Umx)
+    print(f"{prec}: rel. difference of dJ/dk to FP64 {relerr(gmx, g64):.2e}")
+    assert relerr(gmx, g64) <= 1e-8
