@@ -93,6 +93,10 @@ def main():
                          "(default 0.5e-10 with fgmres, 1e-5 with gmres)")
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
+    ap.add_argument("--workload", default="poisson128", choices=["poisson128", "adjoint96"],
+                    help="poisson128: 3D Q1 Poisson on 128^3 cells (the headline); adjoint96: "
+                         "variable-coefficient diffusion on 96^3 cells, one gradient evaluation "
+                         "(forward solve + adjoint solve + dJ/dk per cell) per step")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
@@ -103,6 +107,9 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "adjoint96":
+        return bench_adjoint(args)
 
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.problem import lattice_rhs
@@ -287,6 +294,103 @@ def kernel_only(ctx, fine, f, args):
                       "residual_bytes_f64": sell_bytes(n, A.layout(), 8, 8),
                       "stored_entries": A.layout()["stored"],
                       "copy_bytes_each_way": m * 8}), flush=True)
+
+
+def bench_adjoint(args):
+    """Variable-coefficient diffusion -div(k grad u) = f on the 3x3x3-hex unit
+    cube refined to 96^3 cells (912,673 DOF, 5 levels), k = exp(g) per cell with
+    g a Gaussian random field (sigma 1, correlation length 0.1, 1024 random
+    Fourier modes), J = int u w dx for a smooth seeded w.  One step = forward
+    solve + adjoint solve (both to 1e-10 relative FP64 residual) + dJ/dk for
+    every cell; FP64 (GMRES, the reference's algorithm on A and A^T) against
+    mixed precision (f16-value V-cycle inside flexible GMRES)."""
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd.autograd import DiffusionProblem
+    from gascoignelib_amd.problem import cell_centres, gaussian_random_field
+
+    ctx = H.Context(0)
+    t0 = time.perf_counter()
+    P = DiffusionProblem(ctx, inp="cube3.inp", prerefine=5, ndirect=400)
+    k = np.exp(gaussian_random_field(cell_centres(P.coords, P.cells), seed=31))
+    hier = P.hierarchy(k)
+    t_struct = time.perf_counter() - t0
+    ndof = hier.fine.n
+    print(f"# 96^3 var-k: {ndof} DOF, {P.ncells} cells, {hier.nlevels} levels, "
+          f"setup {t_struct:.1f}s", file=sys.stderr, flush=True)
+    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
+                  ndirect=400, order="lexicographic")
+    solvers = {}
+    for tag, prec, kry, tol in (("fp64", "f64", "gmres", 1e-10),
+                                ("mixed", args.mixed, "fgmres", 0.5e-10)):
+        solvers[tag] = (H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, **common),
+                        H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, adjoint=True,
+                                               **common), tol)
+    U, Z = H.CudaVectorInterface(ctx, ndof), H.CudaVectorInterface(ctx, ndof)
+    Gk = H.CudaVectorInterface(ctx, P.ncells)
+
+    def step(tag):
+        mgF, mgA, tol = solvers[tag]
+        U.zero()
+        rf = mgF.Solve(U, P.F, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
+                       norm_kind=1)
+        Z.zero()
+        ra = mgA.Solve(Z, P.G, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
+                       norm_kind=1)
+        P.cellmap.dJdk(Z, U, Gk)
+        for r in (rf, ra):
+            if H.N.NL_STATUS[r.status] != "converged":
+                raise RuntimeError(f"{tag} solve did not converge")
+        return rf, ra
+
+    res = {}
+    for tag in ("fp64", "mixed"):
+        for _ in range(args.warmup):
+            step(tag)
+        ts = []
+        for i in range(args.steps):
+            ctx.sync()
+            t = time.perf_counter()
+            rf, ra = step(tag)
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+            print(f"# {tag} step {i}: {ts[-1] * 1e3:.1f} ms (forward gmres {rf.gmres_iterations}, "
+                  f"adjoint gmres {ra.gmres_iterations})", file=sys.stderr, flush=True)
+        res[tag] = dict(ms=float(np.median(ts)) * 1e3, forward_gmres=rf.gmres_iterations,
+                        adjoint_gmres=ra.gmres_iterations, u=U.copy_back(), z=Z.copy_back(),
+                        g=Gk.copy_back())
+    J = float(P.g @ res["fp64"]["u"])
+    g64, gmx = res["fp64"]["g"], res["mixed"]["g"]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    t_dj = time_kernel(ctx, lambda: P.cellmap.dJdk(Z, U, Gk), args.spmv_reps)
+    bytes_dj = P.ncells * (8 * 4 + 3 * 8 + 8) + ndof * 16
+    line = {
+        "metric": "gradient evaluations: DOF/s of forward + adjoint solve to 1e-10 rel residual "
+                  "+ dJ/dk per cell (mixed vs FP64)",
+        "value": ndof / (res["mixed"]["ms"] * 1e-3), "unit": "DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["mixed"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"f64 outer / f32 V-cycle ({args.mixed}-stored matrix values)",
+        "data": "synthetic: k = exp(GRF) per cell, seeded f keyed by lattice position, "
+                "smooth seeded w",
+        "config": {"workload": "3D variable-coefficient diffusion, Q1, 96^3 cells (cube3.inp)",
+                   "dof": ndof, "cells": P.ncells, "levels": hier.nlevels},
+        "fp64": {"dof_per_s": ndof / (res["fp64"]["ms"] * 1e-3), "ms_per_step": res["fp64"]["ms"],
+                 "forward_gmres": res["fp64"]["forward_gmres"],
+                 "adjoint_gmres": res["fp64"]["adjoint_gmres"]},
+        "mixed": {"dof_per_s": ndof / (res["mixed"]["ms"] * 1e-3),
+                  "ms_per_step": res["mixed"]["ms"],
+                  "forward_gmres": res["mixed"]["forward_gmres"],
+                  "adjoint_gmres": res["mixed"]["adjoint_gmres"],
+                  "rel_error_u": rel(res["mixed"]["u"], res["fp64"]["u"]),
+                  "rel_error_z": rel(res["mixed"]["z"], res["fp64"]["z"]),
+                  "rel_error_dJdk": rel(gmx, g64)},
+        "speedup_mixed_vs_fp64": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "J": J, "euler_identity_residual": float(abs(k @ g64 + J) / abs(J)),
+        "dJdk_kernel": {"us_per_launch": t_dj * 1e6, "bytes_per_launch": bytes_dj,
+                        "gbs": bytes_dj / t_dj / 1e9},
+        "setup_s": t_struct,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(hier, f):

@@ -167,15 +167,33 @@ def splitmix64_normal(seed, n):
 def gaussian_random_field(points, seed, sigma=1.0, length=0.1, modes=1024):
     """g(x) = sigma sqrt(2/M) sum_m cos(w_m . x + b_m), w_m ~ N(0, I / length^2),
     b_m ~ U[0, 2 pi): random Fourier features of the squared-exponential
-    covariance sigma^2 exp(-|x-y|^2 / (2 length^2)) (Rahimi & Recht, 2007)."""
+    covariance sigma^2 exp(-|x-y|^2 / (2 length^2)) (Rahimi & Recht, 2007).
+    On a tensor lattice each mode factors per axis, cos(w.x + b) =
+    Re(e^{ib} prod_d e^{i w_d x_d}), and the sum becomes one complex matrix
+    product; other point sets are summed directly."""
     x = np.asarray(points, dtype=np.float64)
     d = x.shape[1]
     w = splitmix64_normal(seed, modes * d).reshape(modes, d) / length
     b = np.pi * (splitmix64_uniform(seed + 0xB0B, modes) + 1.0)
+    scale = sigma * np.sqrt(2.0 / modes)
+    axes = [np.unique(np.round(x[:, a], 12)) for a in range(d)]
+    if d in (2, 3) and np.prod([len(u) for u in axes]) == len(x):
+        idx = [np.searchsorted(axes[a], np.round(x[:, a], 12)) for a in range(d)]
+        E = [np.exp(1j * np.outer(w[:, a], axes[a])) for a in range(d)]  # modes x n_a
+        c = np.exp(1j * b)
+        if d == 2:
+            G = ((c[:, None] * E[0]).T @ E[1]).real                     # n0 x n1
+            return scale * G[idx[0], idx[1]]
+        n0, n1 = len(axes[0]), len(axes[1])
+        T = (c[:, None, None] * E[0][:, :, None] * E[1][:, None, :]).reshape(modes, n0 * n1)
+        G = (T.T @ E[2]).real                                           # (n0 n1) x n2
+        return scale * G[idx[0] * n1 + idx[1], idx[2]]
     out = np.zeros(len(x))
-    for lo in range(0, modes, 128):  # chunks keep the temporary small
-        out += np.cos(x @ w[lo:lo + 128].T + b[lo:lo + 128]).sum(axis=1)
-    return sigma * np.sqrt(2.0 / modes) * out
+    for p0 in range(0, len(x), 32768):  # chunks keep the temporaries small
+        xs = x[p0:p0 + 32768]
+        for lo in range(0, modes, 256):
+            out[p0:p0 + 32768] += np.cos(xs @ w[lo:lo + 256].T + b[lo:lo + 256]).sum(axis=1)
+    return scale * out
 
 
 def smooth_weight(points, seed, modes=2):

@@ -82,3 +82,15 @@ def test_central_differences():
         km[c] -= h
         fd = (J_of(M, kp, f, g) - J_of(M, km, f, g)) / (2 * h)
         assert abs(fd - dk[c]) <= 1e-6 * np.abs(dk).max(), c
+
+
+def test_random_field_lattice_path_equals_direct_sum():
+    """The separable evaluation on a lattice is the same sum of cosines."""
+    from gascoignelib_amd.problem import gaussian_random_field
+    a = (np.arange(10) + 0.5) / 10
+    X = np.array(np.meshgrid(a, a[:7], a[:5], indexing="ij")).reshape(3, -1).T
+    X = X[np.random.default_rng(2).permutation(len(X))]
+    lattice = gaussian_random_field(X, 17)
+    direct = gaussian_random_field(np.vstack([X, [[0.31, 0.62, 0.93]]]), 17)[:-1]
+    assert np.abs(lattice - direct).max() <= 1e-10
+    assert 0.5 < np.std(gaussian_random_field(np.random.default_rng(0).random((4000, 3)), 5)) < 1.5
