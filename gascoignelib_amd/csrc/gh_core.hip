@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <unordered_map>
 
 #include "gh_sell.h"
 
@@ -224,52 +225,21 @@ int vec_fill_zero(gh_ctx* ctx, int prec, int64_t size, void* x) {
 }
 
 // ------------------------------------------------------------------ SpMV
-template <typename VT, typename XT, typename Acc, class Epi>
-static int launch_csr(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
-  const int64_t n = A->n;
-  const int seg = A->seg;
-  unsigned grid = grid_for(n * seg);
-  const VT* val = (const VT*)A->val;
-#define GH_SEG_CASE(S)                                                                 \
-  case S:                                                                              \
-    hipLaunchKernelGGL((k_csr_seg<S, VT, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0,  \
-                       ctx->stream, n, A->rowptr, A->col, val, x, epi);                \
-    break;
-  switch (seg) {
-    GH_SEG_CASE(1)
-    GH_SEG_CASE(2)
-    GH_SEG_CASE(4)
-    GH_SEG_CASE(8)
-    GH_SEG_CASE(16)
-    GH_SEG_CASE(32)
-    GH_SEG_CASE(64)
-    default:
-      return fail(GH_EINVAL, "bad segment width %d", seg);
-  }
-#undef GH_SEG_CASE
+// block matrices (ncomp > 1): one thread per block row component
+template <typename XT, typename Acc, class Op>
+static int dispatch_val(gh_ctx* ctx, const gh_csr* A, const XT* x, Op op) {
+  unsigned grid = grid_for(A->n * A->ncomp);
+  if (A->prec == GH_F64)
+    hipLaunchKernelGGL((k_bsr<double, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       A->n, A->ncomp, A->rowptr, A->col, (const double*)A->val, x, op);
+  else if (A->prec == GH_F32)
+    hipLaunchKernelGGL((k_bsr<float, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       A->n, A->ncomp, A->rowptr, A->col, (const float*)A->val, x, op);
+  else
+    hipLaunchKernelGGL((k_bsr<__half, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       A->n, A->ncomp, A->rowptr, A->col, (const __half*)A->val, x, op);
   GH_HIP(hipGetLastError());
   return GH_OK;
-}
-
-template <typename XT, typename Acc, class Epi>
-static int dispatch_val(gh_ctx* ctx, const gh_csr* A, const XT* x, Epi epi) {
-  {
-    unsigned grid = grid_for(A->n * A->ncomp);
-    if (A->prec == GH_F64)
-      hipLaunchKernelGGL((k_bsr<double, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
-                         A->n, A->ncomp, A->rowptr, A->col, (const double*)A->val, x, epi);
-    else if (A->prec == GH_F32)
-      hipLaunchKernelGGL((k_bsr<float, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
-                         A->n, A->ncomp, A->rowptr, A->col, (const float*)A->val, x, epi);
-    else
-      hipLaunchKernelGGL((k_bsr<__half, XT, Acc, Epi>), dim3(grid), dim3(kBlock), 0, ctx->stream,
-                         A->n, A->ncomp, A->rowptr, A->col, (const __half*)A->val, x, epi);
-    GH_HIP(hipGetLastError());
-    return GH_OK;
-  }
-  if (A->prec == GH_F64) return launch_csr<double, XT, Acc, Epi>(ctx, A, x, epi);
-  if (A->prec == GH_F32) return launch_csr<float, XT, Acc, Epi>(ctx, A, x, epi);
-  return launch_csr<__half, XT, Acc, Epi>(ctx, A, x, epi);
 }
 
 int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const void* x,
@@ -292,16 +262,16 @@ int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const 
   }
   if (xprec == GH_F64 && yprec == GH_F64)
     return dispatch_val<double, double>(ctx, A, (const double*)x,
-                                        EpiSpmv<double, double>{(double*)y, d, b, ry});
+                                        BsrSpmvOp<double, double>{(double*)y, d, b, ry});
   if (xprec == GH_F64 && yprec == GH_F32)
     return dispatch_val<double, double>(ctx, A, (const double*)x,
-                                        EpiSpmv<float, double>{(float*)y, d, b, ry});
+                                        BsrSpmvOp<float, double>{(float*)y, d, b, ry});
   if (xprec == GH_F32 && yprec == GH_F32)
     return dispatch_val<float, float>(ctx, A, (const float*)x,
-                                      EpiSpmv<float, float>{(float*)y, (float)d, (float)b, ry});
+                                      BsrSpmvOp<float, float>{(float*)y, (float)d, (float)b, ry});
   if (xprec == GH_F32 && yprec == GH_F64)
     return dispatch_val<float, float>(ctx, A, (const float*)x,
-                                      EpiSpmv<double, float>{(double*)y, (float)d, (float)b, ry});
+                                      BsrSpmvOp<double, float>{(double*)y, (float)d, (float)b, ry});
   return fail(GH_EINVAL, "csr_spmv: bad precisions");
 }
 
@@ -311,7 +281,7 @@ int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x) {
   if (A->ncomp == 1)
     return sell_dispatch<float, double>(ctx, A->sell, x,
                                         SpmvOp<double, double, true>{y, 1.0, 0.0, false});
-  return dispatch_val<float, double>(ctx, A, x, EpiSpmv<double, double>{y, 1.0, 0.0, false});
+  return dispatch_val<float, double>(ctx, A, x, BsrSpmvOp<double, double>{y, 1.0, 0.0, false});
 }
 
 int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b) {
@@ -324,9 +294,9 @@ int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x,
   }
   if (prec == GH_F64)
     return dispatch_val<double, double>(ctx, A, (const double*)x,
-                                        EpiResidual<double>{(double*)r, (const double*)b});
+                                        BsrResidualOp<double>{(double*)r, (const double*)b});
   return dispatch_val<float, float>(ctx, A, (const float*)x,
-                                    EpiResidual<float>{(float*)r, (const float*)b});
+                                    BsrResidualOp<float>{(float*)r, (const float*)b});
 }
 
 template <typename T, typename DT>
@@ -472,6 +442,7 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
                const double* vals, int prec, Sell* S) {
   S->n = n;
   S->prec = prec;
+  S->maxlen = 0;
   S->nslices = (n + kSlice - 1) / kSlice;
   const bool valued = vals != nullptr;
   // entries kept per row: a valued operator drops its exact zeros
@@ -485,6 +456,7 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
     if (len > 65535) return fail(GH_EINVAL, "row %lld has more than 65535 entries", (long long)i);
     rlen[i] = (uint16_t)len;
     stored += len;
+    S->maxlen = std::max<int>(S->maxlen, (int)len);
   }
   S->stored = stored;
   uint64_t total = 0;
@@ -514,19 +486,94 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
       }
     }
   GH_TRY(upload(ctx, &S->sptr, sptr.data(), sptr.size()));
-  GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
   if (!valued) {
+    GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
     GH_TRY(upload(ctx, &S->col, scol.data(), scol.size()));
     GH_HIP(hipStreamSynchronize(ctx->stream));
     return GH_OK;
   }
-  // column encoding per slice: 16-bit offsets when the slice's columns span
-  // fewer than 2^16 indices, else 32-bit (two 16-bit units, 4-byte aligned)
+  // Relative column patterns (col - row over a row's kept entries, in order)
+  // that at least one slice worth of rows share.  A slice whose rows all follow
+  // table patterns stores no columns: rlen holds the pattern ids.
+  std::vector<int32_t> rpid(n, -1), ptab, pstart(1, 0);
+  {
+    constexpr size_t kMaxCandidates = 1 << 16;
+    constexpr int64_t kMaxTable = 1 << 16;  // offsets in the table
+    std::unordered_map<uint64_t, std::vector<int32_t>> byhash;
+    std::vector<std::vector<int32_t>> cand;
+    std::vector<int64_t> count;
+    std::vector<int32_t> rcand(n, -1), offs;
+    for (int64_t i = 0; i < n; ++i) {
+      offs.clear();
+      uint64_t hsh = 1469598103934665603ull;
+      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        if (!keep(q)) continue;
+        const int32_t o = (int32_t)((int64_t)col[q] - i);
+        offs.push_back(o);
+        hsh = (hsh ^ (uint32_t)o) * 1099511628211ull;
+      }
+      std::vector<int32_t>& ids = byhash[hsh];
+      int32_t id = -1;
+      for (int32_t c : ids)
+        if (cand[c] == offs) id = c;
+      if (id < 0 && cand.size() < kMaxCandidates) {
+        id = (int32_t)cand.size();
+        cand.push_back(offs);
+        count.push_back(0);
+        ids.push_back(id);
+      }
+      if (id >= 0) {
+        count[id]++;
+        rcand[i] = id;
+      }
+    }
+    // table in order of first appearance: patterns used by >= kSlice rows
+    std::vector<int32_t> pid_of(cand.size(), -1);
+    for (size_t c = 0; c < cand.size(); ++c) {
+      if (count[c] < kSlice || (int64_t)(ptab.size() + cand[c].size()) > kMaxTable ||
+          pstart.size() > 65535)
+        continue;
+      pid_of[c] = (int32_t)pstart.size() - 1;
+      ptab.insert(ptab.end(), cand[c].begin(), cand[c].end());
+      pstart.push_back((int32_t)ptab.size());
+    }
+    for (int64_t i = 0; i < n; ++i)
+      if (rcand[i] >= 0) rpid[i] = pid_of[rcand[i]];
+  }
+  S->npat = (int64_t)pstart.size() - 1;
+  S->ntab = (int64_t)ptab.size();
+  // column encoding per slice: pattern; else 16-bit offsets when the slice's
+  // columns span fewer than 2^16 indices; else 32-bit
   std::vector<uint32_t> cptr(S->nslices), cbase(S->nslices, 0);
   uint64_t units = 0;
-  std::vector<uint8_t> wide(S->nslices, 0);
+  std::vector<uint8_t> wide(S->nslices, 0), pat(S->nslices, 0);
   S->narrow = 0;
+  S->patterned = 0;
+  S->col_bytes = 0;
   for (int64_t s = 0; s < S->nslices; ++s) {
+    bool all = true;
+    int64_t kept = 0;
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int64_t i = s * kSlice + lane;
+      if (i >= n) break;
+      all = all && rpid[i] >= 0;
+      kept += rlen[i];
+    }
+    if (all) {
+      pat[s] = 1;
+      cptr[s] = 0x40000000u;
+      S->patterned++;
+      bool same = true;
+      for (int lane = 0; lane < kSlice && s * kSlice + lane < n; ++lane) {
+        rlen[s * kSlice + lane] = (uint16_t)rpid[s * kSlice + lane];
+        same = same && rpid[s * kSlice + lane] == rpid[s * kSlice];
+      }
+      if (same) {  // one pattern for the whole wavefront: its id in cbase
+        cptr[s] |= 0x20000000u;
+        cbase[s] = (uint32_t)rpid[s * kSlice];
+      }
+      continue;
+    }
     uint32_t lo = 0xFFFFFFFFu, hi = 0;
     for (int lane = 0; lane < kSlice; ++lane) {
       const int64_t i = s * kSlice + lane;
@@ -544,18 +591,21 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
       units += units & 1;  // 4-byte alignment
       cptr[s] = (uint32_t)units | 0x80000000u;
       units += 2 * w;
+      S->col_bytes += 4 * kept;
     } else {
       cptr[s] = (uint32_t)units;
       cbase[s] = lo;
       units += w;
       S->narrow++;
+      S->col_bytes += 2 * kept;
     }
-    if (units >= 0x80000000ull) return fail(GH_EINVAL, "operator too large for its column index");
+    if (units >= 0x20000000ull) return fail(GH_EINVAL, "operator too large for its column index");
   }
   std::vector<uint16_t> cidx(std::max<uint64_t>(units, 1), 0);
   for (int64_t s = 0; s < S->nslices; ++s) {
+    if (pat[s]) continue;
     const uint64_t w = (sptr[s + 1] - sptr[s]);
-    const uint64_t at = cptr[s] & 0x7FFFFFFFu;
+    const uint64_t at = cptr[s] & 0x1FFFFFFFu;
     for (uint64_t j = 0; j < w; ++j) {
       const int lane = (int)(j % kSlice);
       const int64_t i = s * kSlice + lane;
@@ -569,9 +619,12 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
       }
     }
   }
+  GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
   GH_TRY(upload(ctx, &S->cidx, cidx.data(), cidx.size()));
   GH_TRY(upload(ctx, &S->cptr, cptr.data(), cptr.size()));
   GH_TRY(upload(ctx, &S->cbase, cbase.data(), cbase.size()));
+  GH_TRY(upload(ctx, &S->ptab, ptab.data(), ptab.size()));
+  GH_TRY(upload(ctx, &S->pstart, pstart.data(), pstart.size()));
   if (prec == GH_F64) {
     double* d = nullptr;
     GH_TRY(upload(ctx, &d, sval.data(), sval.size()));
@@ -599,8 +652,10 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
 
 void sell_free(Sell* S) {
   for (void* p : {(void*)S->sptr, (void*)S->rlen, (void*)S->col, S->val, (void*)S->cidx,
-                  (void*)S->cptr, (void*)S->cbase})
+                  (void*)S->cptr, (void*)S->cbase, (void*)S->ptab, (void*)S->pstart})
     if (p) (void)hipFree(p);
+  S->ptab = nullptr;
+  S->pstart = nullptr;
   S->sptr = nullptr;
   S->rlen = nullptr;
   S->col = nullptr;
@@ -1018,9 +1073,13 @@ int gh_csr_layout(const gh_csr* A, int64_t* out) {
     out[1] = A->sell.slots;
     out[2] = A->sell.nslices;
     out[3] = A->sell.narrow;
+    out[4] = A->sell.patterned;
+    out[5] = A->sell.ntab;
+    out[6] = A->sell.col_bytes;
   } else {
     out[0] = out[1] = A->nnz;
-    out[2] = out[3] = 0;
+    out[2] = out[3] = out[4] = out[5] = 0;
+    out[6] = A->nnz * 4;
   }
   return GH_OK;
 }

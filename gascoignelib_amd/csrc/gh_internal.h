@@ -13,6 +13,20 @@
 
 #include "gascoigne_hip.h"
 
+// Vanka patch smoother: patch LU factors and the node -> patch lists
+struct gh_vanka {
+  gh_ctx* ctx = nullptr;
+  int B = 1, psize = 0, m = 0;
+  int prec = GH_F64;          // storage of the factors and of the local solves
+  int64_t n = 0, npatches = 0;
+  int32_t* nodes = nullptr;   // npatches * psize
+  void* lu = nullptr;         // npatches * m * m, column-major, L unit lower
+  int32_t* piv = nullptr;     // npatches * m
+  int32_t* nptr = nullptr;    // n + 1
+  int32_t* nent = nullptr;    // patch * psize + local node, ascending patch per node
+  void* xp = nullptr;         // npatches * m local solutions
+};
+
 // Q1 box cells for coefficient gradients
 struct gh_cells {
   gh_ctx* ctx = nullptr;
@@ -69,13 +83,23 @@ struct Sell {
   int64_t slots = 0;      // padded entries
   int64_t stored = 0;     // entries kept (valued operators drop exact zeros)
   int64_t narrow = 0;     // slices with 16-bit column offsets
+  int maxlen = 0;         // longest row (entries kept)
   uint32_t* sptr = nullptr;  // nslices+1 slot offsets
   uint16_t* rlen = nullptr;  // entries per row
-  // valued operators: per slice either 16-bit offsets from cbase[s] or plain
-  // 32-bit indices; cptr[s] = start in 16-bit units | (wide << 31)
+  // valued operators, per slice one of three column encodings:
+  //   pattern (cptr bit 30): every row follows a shared relative pattern,
+  //     rlen[row] holds its id, columns are row + ptab[pstart[id] + k]; with
+  //     bit 29 all 64 rows share one pattern, whose id is cbase[s];
+  //   16-bit offsets from cbase[s] (cptr = start in 16-bit units);
+  //   32-bit indices (cptr bit 31, start in 16-bit units, 4-byte aligned)
   uint16_t* cidx = nullptr;
   uint32_t* cptr = nullptr;
   uint32_t* cbase = nullptr;
+  int32_t* ptab = nullptr;
+  int32_t* pstart = nullptr;
+  int64_t npat = 0, ntab = 0;
+  int64_t patterned = 0;     // slices in pattern mode
+  int64_t col_bytes = 0;     // column bytes a sweep reads (stored entries only)
   // packed-weight operators: column | weight code << 29, one per slot
   uint32_t* col = nullptr;
   void* val = nullptr;       // values in `prec`, nullptr for packed-weight operators
@@ -190,6 +214,9 @@ int csr_spmv(gh_ctx* ctx, const gh_csr* A, int yprec, void* y, int xprec, const 
              double d, double b);
 int csr_residual(gh_ctx* ctx, const gh_csr* A, int prec, void* r, const void* x, const void* b);
 int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x);
+int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, const int32_t* nodes,
+                 int prec, gh_vanka** out);
+int vanka_solve(gh_ctx* ctx, const gh_vanka* V, int prec, void* x);
 // host transpose of a node-block CSR (blocks transposed too); rows of the
 // result list their columns ascending
 void csr_transpose_host(int64_t n, int B, const int32_t* rowptr, const int32_t* col,

@@ -121,15 +121,17 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, T* __restrict__ y, T a, T d,
                                                   const T* __restrict__ x, int mode) {
-  // mode 0: y = a*y + d*x ; 1: y += d*x ; 2: y = d*x
+  // mode 0: y = a*y + d*x ; 1: y += d*x ; 2: y = d*x -- each product rounded
+  // before the sum, as nvector's loops compute them (src/Common/nvector.h:
+  // equ / add / sadd), not fused into one multiply-add
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     if (mode == 0)
-      y[i] = a * y[i] + d * x[i];
+      y[i] = add_rn(mul_rn(a, y[i]), mul_rn(d, x[i]));
     else if (mode == 1)
-      y[i] += d * x[i];
+      y[i] = add_rn(y[i], mul_rn(d, x[i]));
     else
-      y[i] = d * x[i];
+      y[i] = mul_rn(d, x[i]);
   }
 }
 
@@ -148,72 +150,58 @@ __global__ __launch_bounds__(kBlock) void k_fill0(int64_t n, T* __restrict__ x) 
     x[i] = (T)0;
 }
 
-// --------------------------------------------------------------- SpMV (B=1)
-// Rows are mapped to segments of SEG lanes of a wavefront; a segment streams
-// its row's (col,val) pairs with unit stride and folds them with xor shuffles.
+// ------------------------------------------------------ block SpMV (B > 1)
+// SparseBlockMatrix<FMatrixBlock<B>>::vmult (src/LinAlg/sparseblockmatrix.xx:
+// 378-392) adds, block by block in stored order, s * (row r of the block times
+// x_col) into y (FMatrixBlock::cadd, src/LinAlg/fmatrixblock.h); the ops below
+// take the same per-block sums in the same order, without contraction.
 template <typename Y, typename Acc>
-struct EpiSpmv {
+struct BsrSpmvOp {  // y = d * A x + b * y
   Y* y;
   Acc d, b;
   bool read_y;
-  __device__ __forceinline__ void operator()(int64_t r, Acc s) const {
-    Acc v = d * s;
-    if (read_y) v += b * (Acc)y[r];
-    y[r] = (Y)v;
+  __device__ __forceinline__ Acc init(int64_t t) const {
+    if (!read_y) return (Acc)0;
+    return b == (Acc)1 ? (Acc)y[t] : mul_rn(b, (Acc)y[t]);
   }
+  __device__ __forceinline__ Acc step(Acc acc, Acc sum) const {
+    return add_rn(acc, mul_rn(d, sum));
+  }
+  __device__ __forceinline__ void finish(int64_t t, Acc acc) const { y[t] = (Y)acc; }
 };
 
 template <typename T>
-struct EpiResidual {  // r = b - A x   (MatrixResidual)
+struct BsrResidualOp {  // r = b - A x (MatrixResidual: r = b; vmult(r, x, -1))
   T* r;
   const T* b;
-  __device__ __forceinline__ void operator()(int64_t i, T s) const { r[i] = b[i] - s; }
+  __device__ __forceinline__ T init(int64_t t) const { return b[t]; }
+  __device__ __forceinline__ T step(T acc, T sum) const { return add_rn(acc, -sum); }
+  __device__ __forceinline__ void finish(int64_t t, T acc) const { r[t] = acc; }
 };
 
-template <typename T, typename DT>
-struct EpiJacobi {  // xn = x + omega * (dinv * (b - A x))   (StdSolver::smooth jacobi)
-  T* xn;
-  const T* x;
-  const T* b;
-  const DT* dinv;
-  T omega;
-  __device__ __forceinline__ void operator()(int64_t i, T s) const {
-    T h = b[i] - s;
-    h = (T)dinv[i] * h;
-    xn[i] = x[i] + omega * h;
-  }
-};
-
-template <typename T>
-struct EpiRichardson {  // xn = x + omega * (b - A x)
-  T* xn;
-  const T* x;
-  const T* b;
-  T omega;
-  __device__ __forceinline__ void operator()(int64_t i, T s) const {
-    xn[i] = x[i] + omega * (b[i] - s);
-  }
-};
-
-template <int SEG, typename VT, typename XT, typename Acc, class Epi>
-__global__ __launch_bounds__(kBlock) void k_csr_seg(int64_t n, const int32_t* __restrict__ rowptr,
-                                                    const int32_t* __restrict__ col,
-                                                    const VT* __restrict__ val,
-                                                    const XT* __restrict__ x, Epi epi) {
+// One thread per block row component (i, r)
+template <typename VT, typename XT, typename Acc, class Op>
+__global__ __launch_bounds__(kBlock) void k_bsr(int64_t n, int B, const int32_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ col,
+                                                const VT* __restrict__ val,
+                                                const XT* __restrict__ x, Op op) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t row = t / SEG;
-  const int lane = threadIdx.x & (SEG - 1);
-  Acc sum = (Acc)0;
-  if (row < n) {
-    const int s = rowptr[row], e = rowptr[row + 1];
-    for (int k = s + lane; k < e; k += SEG) sum += (Acc)ld(val + k) * (Acc)x[col[k]];
+  if (t >= n * B) return;
+  const int64_t i = t / B;
+  const int r = (int)(t - i * B);
+  Acc acc = op.init(t);
+  for (int pos = rowptr[i]; pos < rowptr[i + 1]; ++pos) {
+    const VT* m = val + (int64_t)pos * B * B + (int64_t)r * B;
+    const XT* q = x + (int64_t)col[pos] * B;
+    Acc sum = (Acc)0;
+    for (int c = 0; c < B; ++c) sum = add_rn(sum, mul_rn((Acc)ld(m + c), (Acc)q[c]));
+    acc = op.step(acc, sum);
   }
-#pragma unroll
-  for (int off = SEG / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, SEG);
-  if (lane == 0 && row < n) epi(row, sum);
+  op.finish(t, acc);
 }
 
-// first Jacobi / Richardson sweep from x = 0: xn = omega * dinv * b
+// First sweep from x = 0 (no product with A): h = b - 0, h = dinv h,
+// xn = 0 + omega h, the operations of a sweep whose iterate is zero
 template <typename T, typename DT>
 __global__ __launch_bounds__(kBlock) void k_jacobi_zero(int64_t n, T* __restrict__ xn,
                                                         const T* __restrict__ b,
@@ -227,28 +215,6 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_zero(int64_t n, T* __restrict
   }
 }
 
-// ------------------------------------------------------------ block SpMV (B>1)
-// One thread per block row component: y(i,r) = sum_pos sum_c a(pos)(r,c) x(col,c)
-template <typename VT, typename XT, typename Acc, class Epi>
-__global__ __launch_bounds__(kBlock) void k_bsr(int64_t n, int B, const int32_t* __restrict__ rowptr,
-                                                const int32_t* __restrict__ col,
-                                                const VT* __restrict__ val,
-                                                const XT* __restrict__ x, Epi epi) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n * B) return;
-  const int64_t i = t / B;
-  const int r = (int)(t - i * B);
-  Acc total = (Acc)0;
-  for (int pos = rowptr[i]; pos < rowptr[i + 1]; ++pos) {
-    const VT* m = val + (int64_t)pos * B * B + (int64_t)r * B;
-    const XT* q = x + (int64_t)col[pos] * B;
-    Acc sum = (Acc)0;
-    for (int c = 0; c < B; ++c) sum += (Acc)ld(m + c) * (Acc)q[c];
-    total += sum;
-  }
-  epi(t, total);
-}
-
 // block Jacobi: xn(i) = x(i) + omega * Dinv_i (b(i) - (A x)(i)); one thread per row
 template <typename VT, typename T, typename DT>
 __global__ __launch_bounds__(kBlock) void k_bsr_jacobi(int64_t n, int B,
@@ -259,6 +225,8 @@ __global__ __launch_bounds__(kBlock) void k_bsr_jacobi(int64_t n, int B,
                                                        const T* __restrict__ b,
                                                        const DT* __restrict__ dinv,
                                                        T* __restrict__ xn, T omega, int x_zero) {
+  // h = b - A x (block by block), h <- Dinv_i h (NodeMatrix product, sums from
+  // zero in column order), x + omega h: each step as the CPU forms it
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   T h[8];
@@ -269,16 +237,16 @@ __global__ __launch_bounds__(kBlock) void k_bsr_jacobi(int64_t n, int B,
       const T* q = x + (int64_t)col[pos] * B;
       for (int r = 0; r < B; ++r) {
         T sum = (T)0;
-        for (int c = 0; c < B; ++c) sum += (T)ld(m + r * B + c) * q[c];
-        h[r] -= sum;
+        for (int c = 0; c < B; ++c) sum = add_rn(sum, mul_rn((T)ld(m + r * B + c), q[c]));
+        h[r] = add_rn(h[r], -sum);
       }
     }
   }
   const DT* D = dinv + i * B * B;
   for (int r = 0; r < B; ++r) {
     T s = (T)0;
-    for (int c = 0; c < B; ++c) s += (T)D[r * B + c] * h[c];
-    xn[i * B + r] = (x_zero ? (T)0 : x[i * B + r]) + omega * s;
+    for (int c = 0; c < B; ++c) s = add_rn(s, mul_rn((T)D[r * B + c], h[c]));
+    xn[i * B + r] = add_rn(x_zero ? (T)0 : x[i * B + r], mul_rn(omega, s));
   }
 }
 
@@ -291,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n, int B,
                                                           const double* __restrict__ val,
                                                           DT* __restrict__ dinv,
                                                           int* __restrict__ singular) {
+#pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   double m[64];
@@ -404,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_prolongate_correct(int64_t nf, int n
     const uint32_t q = col[pos];
     v = add_rn(v, mul_rn(code_weight<T>(q >> 29), uc[(int64_t)(q & kIdxMask) * ncomp + c]));
   }
-  u[t] = (UT)((T)u[t] + mgomega * v);
+  u[t] = (UT)add_rn((T)u[t], mul_rn(mgomega, v));
 }
 
 // -------------------------------------------------------------- hanging nodes

@@ -75,6 +75,26 @@ struct RestrictOp {  // uc = R uf (packed weights)
   __device__ __forceinline__ void finish(int64_t i, T acc) const { uc[i] = acc; }
 };
 
+// uc = R uf, and the coarse level's first smoothing sweep from a zero start
+// in the same pass: ux = 0 + omega * (dinv * (uc - 0)), the operations of a
+// Jacobi (or Richardson) sweep whose iterate is zero
+template <typename T, typename DT>
+struct RestrictSmoothOp {
+  T* uc;
+  T* ux;
+  const DT* dinv;
+  T omega;
+  int richardson;
+  __device__ __forceinline__ T init(int64_t) const { return (T)0; }
+  __device__ __forceinline__ T step(T acc, T w, T x) const { return add_rn(acc, mul_rn(w, x)); }
+  __device__ __forceinline__ void finish(int64_t i, T acc) const {
+    uc[i] = acc;
+    T h = acc - (T)0;
+    if (!richardson) h = (T)dinv[i] * h;
+    ux[i] = (T)0 + omega * h;
+  }
+};
+
 template <typename T>
 struct ProlongAddOp {  // uf += P uc, term by term
   T* uf;
@@ -128,6 +148,28 @@ __device__ __forceinline__ T sell_row(const CT* __restrict__ cp, uint32_t cb,
   return acc;
 }
 
+// One row that follows a shared relative pattern: column = row + off[k]
+// (the offsets come from a table of a few KB, served by the caches).
+template <typename VT, typename XT, typename T, class Op>
+__device__ __forceinline__ T sell_row_pattern(const int32_t* __restrict__ off, uint32_t row,
+                                              const VT* __restrict__ vp, int len,
+                                              const XT* __restrict__ x, T acc, const Op& op) {
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const uint32_t c0 = row + off[k], c1 = row + off[k + 1];
+    const uint32_t c2 = row + off[k + 2], c3 = row + off[k + 3];
+    const T a0 = (T)ld(vp + (k + 0) * kSlice), a1 = (T)ld(vp + (k + 1) * kSlice);
+    const T a2 = (T)ld(vp + (k + 2) * kSlice), a3 = (T)ld(vp + (k + 3) * kSlice);
+    const T x0 = (T)x[c0], x1 = (T)x[c1], x2 = (T)x[c2], x3 = (T)x[c3];
+    acc = op.step(acc, a0, x0);
+    acc = op.step(acc, a1, x1);
+    acc = op.step(acc, a2, x2);
+    acc = op.step(acc, a3, x3);
+  }
+  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * kSlice), (T)x[row + off[k]]);
+  return acc;
+}
+
 // valued operator (matrix); VT storage type, T arithmetic type.  A slice is one
 // wavefront, so the column-encoding branch is uniform across the wave.
 template <typename VT, typename XT, typename T, class Op>
@@ -136,21 +178,39 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __re
                                                  const uint16_t* __restrict__ cidx,
                                                  const uint32_t* __restrict__ cptr,
                                                  const uint32_t* __restrict__ cbase,
+                                                 const int32_t* __restrict__ ptab,
+                                                 const int32_t* __restrict__ pstart,
                                                  const VT* __restrict__ val,
                                                  const XT* __restrict__ x, Op op) {
-  const int64_t row = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (row >= n) return;
+  // Lanes past the last row repeat row n-1 with their own (padding) slots and
+  // only skip the store, so the control flow stays uniform across the wave and
+  // wave-uniform loads can use the scalar unit.
+  const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const bool active = row0 < n;
+  const int64_t row = active ? row0 : n - 1;
   const int lane = threadIdx.x & (kSlice - 1);
-  const int64_t s = row >> 6;
-  const int len = rlen[row];
+  // the slice index is the same for the 64 lanes of a wavefront
+  const int64_t s = __builtin_amdgcn_readfirstlane((int)(row >> 6));
   const VT* vp = val + sptr[s] + lane;
   const uint32_t meta = cptr[s];
   T acc = op.init(row);
-  if (meta & 0x80000000u)
-    acc = sell_row((const uint32_t*)(cidx + (meta & 0x7FFFFFFFu)) + lane, 0u, vp, len, x, acc, op);
-  else
-    acc = sell_row(cidx + meta + lane, cbase[s], vp, len, x, acc, op);
-  op.finish(row, acc);
+  if ((meta & 0x60000000u) == 0x60000000u) {
+    // one pattern for the whole slice: wave-uniform offsets (scalar loads)
+    const int pid = __builtin_amdgcn_readfirstlane((int)cbase[s]);
+    const int p0 = pstart[pid];
+    acc = sell_row_pattern(ptab + p0, (uint32_t)row, vp, pstart[pid + 1] - p0, x, acc, op);
+  } else {
+    const int r = rlen[row];  // entry count, or pattern id in pattern slices
+    if (meta & 0x40000000u) {
+      const int p0 = pstart[r];
+      acc = sell_row_pattern(ptab + p0, (uint32_t)row, vp, pstart[r + 1] - p0, x, acc, op);
+    } else if (meta & 0x80000000u) {
+      acc = sell_row((const uint32_t*)(cidx + (meta & 0x1FFFFFFFu)) + lane, 0u, vp, r, x, acc, op);
+    } else {
+      acc = sell_row(cidx + meta + lane, cbase[s], vp, r, x, acc, op);
+    }
+  }
+  if (active) op.finish(row, acc);
 }
 
 // packed-weight operator (grid transfer): w = 2^-code from the column's top bits
@@ -187,7 +247,8 @@ template <typename VT, typename XT, typename T, class Op>
 inline int sell_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
   if (S.n == 0) return GH_OK;
   hipLaunchKernelGGL((k_sell<VT, XT, T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream,
-                     S.n, S.sptr, S.rlen, S.cidx, S.cptr, S.cbase, (const VT*)S.val, x, op);
+                     S.n, S.sptr, S.rlen, S.cidx, S.cptr, S.cbase, S.ptab, S.pstart,
+                     (const VT*)S.val, x, op);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }

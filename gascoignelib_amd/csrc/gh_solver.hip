@@ -197,6 +197,9 @@ struct MgLevel {
   Sell r_sell, p_sell;           // SELL-64 copies of the folded operators (ncomp == 1)
   bool fused_correction = true;  // mask covers every component
   bool mask_in_restrict = true;  // coarse Dirichlet rows emptied in r_row (set on level l)
+  gh_vanka* vanka = nullptr;     // patch smoother (smoother 2), owned
+  bool pre_fused = false;        // the first pre-smoothing sweep already ran (in the
+                                 // restriction that produced b); consumed by mgstep
   void* u = nullptr;
   void* b = nullptr;
   void* v = nullptr;
@@ -303,6 +306,24 @@ int dense_inverse(const gh_csr* A, std::vector<double>& inv) {
 
 // ------------------------------------------------------------- V-cycle pieces
 int sweep(gh_mg* mg, MgLevel& L, int niter, bool& u_zero, void* keep_u) {
+  if (mg->P.smoother == 2) {
+    // StdSolver::smooth with the Vanka "ilu": h = b - A x; solve(h); x += omega h
+    if (niter > 0 && !L.vanka) return fail(GH_EINVAL, "vanka smoother: level without patches");
+    for (int it = 0; it < niter; ++it) {
+      if (u_zero)
+        GH_TRY(vec_convert(mg->ctx, L.size, mg->wprec, L.v, mg->wprec, L.b));
+      else
+        GH_TRY(csr_residual(mg->ctx, L.A, mg->wprec, L.v, L.u, L.b));
+      GH_TRY(vanka_solve(mg->ctx, L.vanka, mg->wprec, L.v));
+      if (u_zero)
+        GH_TRY(vec_scale_to(mg->ctx, mg->wprec, L.size, L.u, mg->P.omega, L.v));
+      else
+        GH_TRY(vec_axpby(mg->ctx, mg->wprec, L.size, L.u, 1.0, mg->P.omega, L.v));
+      u_zero = false;
+    }
+    (void)keep_u;
+    return GH_OK;
+  }
   for (int it = 0; it < niter; ++it) {
     GH_TRY(csr_jacobi_sweep(mg->ctx, L.A, mg->wprec, L.v, L.u, L.b, mg->P.omega,
                             mg->P.smoother, u_zero));
@@ -391,7 +412,15 @@ int mgstep(gh_mg* mg, int l, int finelevel, int clevel, char& p0, char p, bool u
     return GH_OK;
   }
   bool z = u_zero;
-  GH_TRY(sweep(mg, L, mg->P.iterpre, z, nullptr));
+  int npre = mg->P.iterpre;
+  if (z && L.pre_fused) {
+    // the first sweep ran in the restriction that produced b
+    std::swap(L.u, L.v);
+    z = false;
+    npre--;
+  }
+  L.pre_fused = false;
+  GH_TRY(sweep(mg, L, npre, z, nullptr));
   if (z) {
     // no pre-smoothing sweep ran and u is still zero: v = b
     GH_TRY(vec_convert(ctx, L.size, mg->wprec, L.v, mg->wprec, L.b));
@@ -400,13 +429,32 @@ int mgstep(gh_mg* mg, int l, int finelevel, int clevel, char& p0, char p, bool u
     GH_TRY(csr_residual(ctx, L.A, mg->wprec, L.v, L.u, L.b));
   }
   MgLevel& Cc = mg->L[l - 1];
+  // The coarse level's right-hand side is final after the restriction (no
+  // hanging nodes to distribute, Dirichlet rows emptied in R) and its first
+  // pre-smoothing sweep starts from zero: that sweep runs in the restriction's
+  // epilogue, writing the buffer the sweep would write.
+  const bool fuse = Cc.ncomp == 1 && mg->P.smoother != 2 && !Cc.H && !(Cc.M && !L.mask_in_restrict) &&
+                    l - 1 != clevel && mg->P.iterpre >= 1 &&
+                    (p0 == 'V' || p0 == 'W' || p0 == 'F');
   if (Cc.ncomp == 1) {
-    if (mg->wprec == GH_F64)
+    const int rich = mg->P.smoother;
+    if (fuse && mg->wprec == GH_F64)
+      GH_TRY(sell_w_launch<double>(
+          ctx, L.r_sell, (const double*)L.v,
+          RestrictSmoothOp<double, double>{(double*)Cc.b, (double*)Cc.v,
+                                           (const double*)Cc.A->dinv, mg->P.omega, rich}));
+    else if (fuse)
+      GH_TRY(sell_w_launch<float>(
+          ctx, L.r_sell, (const float*)L.v,
+          RestrictSmoothOp<float, float>{(float*)Cc.b, (float*)Cc.v, (const float*)Cc.A->dinv,
+                                         (float)mg->P.omega, rich}));
+    else if (mg->wprec == GH_F64)
       GH_TRY(sell_w_launch<double>(ctx, L.r_sell, (const double*)L.v,
                                    RestrictOp<double>{(double*)Cc.b}));
     else
       GH_TRY(sell_w_launch<float>(ctx, L.r_sell, (const float*)L.v,
                                   RestrictOp<float>{(float*)Cc.b}));
+    Cc.pre_fused = fuse;
   } else {
     unsigned grid = grid_for(Cc.size);
     if (mg->wprec == GH_F64)
@@ -625,7 +673,8 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
          "gh_mg_create: bad precision");
   GH_ARG(params->mgtype == 'V' || params->mgtype == 'W' || params->mgtype == 'F',
          "gh_mg_create: mgtype must be V, W or F");
-  GH_ARG(params->smoother == 0 || params->smoother == 1, "gh_mg_create: bad smoother");
+  GH_ARG(params->smoother == 0 || params->smoother == 1 || params->smoother == 2,
+         "gh_mg_create: bad smoother");
   GH_ARG(params->krylov == 0 || params->krylov == 1, "gh_mg_create: krylov must be 0 or 1");
   GH_ARG(params->adjoint == 0 || params->adjoint == 1, "gh_mg_create: adjoint must be 0 or 1");
   if (params->adjoint && H)
@@ -790,6 +839,7 @@ int gh_mg_destroy(gh_mg* mg) {
     if (L.own_H && L.H) gh_hn_destroy(L.H);
     if (L.own_M && L.M) gh_mask_destroy(L.M);
     if (L.d_perm) (void)hipFree(L.d_perm);
+    if (L.vanka) gh_vanka_destroy(L.vanka);
     for (void* p : {(void*)L.r_row, (void*)L.r_col, (void*)L.p_row, (void*)L.p_col, L.u, L.b,
                     L.v, (void*)L.Ainv})
       if (p) (void)hipFree(p);
@@ -810,6 +860,32 @@ int gh_mg_destroy(gh_mg* mg) {
   gh_ctx* ctx = mg->ctx;
   delete mg;
   ctx_release(ctx);
+  return GH_OK;
+}
+
+int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int32_t* nodes) {
+  GH_ARG(mg && level >= 0 && level < mg->nlevels && (nodes || npatches == 0) && npatches >= 0 &&
+             psize >= 1,
+         "gh_mg_set_vanka: bad arguments");
+  MgLevel& L = mg->L[level];
+  std::vector<int32_t> internal(nodes, nodes + npatches * psize);
+  if (L.d_perm) {
+    // caller's node -> internal position
+    std::vector<int32_t> P(L.n), Q(L.n);
+    GH_HIP(hipMemcpy(P.data(), L.d_perm, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < L.n; ++k) Q[P[k]] = (int32_t)k;
+    for (int32_t& v : internal) {
+      if (v < 0 || v >= L.n) return fail(GH_EINVAL, "gh_mg_set_vanka: node out of range");
+      v = Q[v];
+    }
+  }
+  gh_vanka* V = nullptr;
+  GH_TRY(vanka_create(mg->ctx, L.A64, npatches, psize, internal.data(), mg->wprec, &V));
+  if (L.vanka) gh_vanka_destroy(L.vanka);
+  L.vanka = V;
+  if (mg->graph) (void)hipGraphExecDestroy(mg->graph);  // it holds the old smoother
+  mg->graph = nullptr;
+  mg->graph_ok = false;
   return GH_OK;
 }
 
@@ -1138,7 +1214,7 @@ int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, in
   GH_TRY(vec_scale_to(ctx, GH_F64, n, v, 1. / rho, v));
   const int fl = mg->nlevels - 1;
   const int clevel = coarsest_level(mg, fl, coarselevel < 0 ? fl : coarselevel);
-  std::vector<double> d(vmax + 2);
+  std::vector<double> d(vmax + 3);
   while (dim < vmax - 1 && !reached) {
     dim++;
     const int last = dim - 1;

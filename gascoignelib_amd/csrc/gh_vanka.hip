@@ -1,0 +1,325 @@
+// Vanka patch smoother (src/LinAlg/vankasmoother.cc): the patch matrices of a
+// node-block matrix, LU with partial pivoting on the device, and the solve
+//   x <- (1/count_i) sum_p R_p^T A_p^-1 R_p x.
+// Factoring: one workgroup per patch, the m x m patch matrix in LDS (m <= 128,
+// 128 KiB at most), the LU's operations uncontracted so that the factors equal
+// the oracle's.  Solving: one wavefront per patch, the local vector spread over
+// the lanes' registers (rows lane and lane + 64), column-oriented triangular
+// solves with the pivot / solved values broadcast by readlane.  Averaging: one
+// thread per unknown, the patches containing its node added in ascending order
+// (the reference adds with OpenMP atomics, in no fixed order).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gh_kernels.h"
+
+using namespace gh;
+
+namespace {
+
+constexpr int kVankaMaxM = 128;       // local unknowns per patch
+constexpr int kVankaMaxNodes = 128;   // nodes per patch
+
+__global__ __launch_bounds__(kBlock) void k_vanka_factor(int64_t npatches, int psize, int B,
+                                                         const int32_t* __restrict__ nodes,
+                                                         const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ col,
+                                                         const double* __restrict__ val,
+                                                         double* __restrict__ lu64,
+                                                         float* __restrict__ lu32,
+                                                         int32_t* __restrict__ piv) {
+  extern __shared__ double a[];  // m * m, row-major
+  __shared__ int32_t pn[kVankaMaxNodes];
+  __shared__ int s_piv;
+  const int64_t p = blockIdx.x;
+  const int m = psize * B;
+  for (int e = threadIdx.x; e < m * m; e += kBlock) a[e] = 0.0;
+  for (int r = threadIdx.x; r < psize; r += kBlock) pn[r] = nodes[p * psize + r];
+  __syncthreads();
+  // copy_entries_sparseblockmatrix: entries whose column is a patch node
+  for (int r = threadIdx.x; r < psize; r += kBlock) {
+    const int32_t row = pn[r];
+    for (int32_t q = rowptr[row]; q < rowptr[row + 1]; ++q) {
+      const int32_t c0 = col[q];
+      int c = -1;
+      for (int t = 0; t < psize; ++t)
+        if (pn[t] == c0) c = t;
+      if (c < 0) continue;
+      for (int cr = 0; cr < B; ++cr)
+        for (int cc = 0; cc < B; ++cc)
+          a[(B * r + cr) * m + B * c + cc] = val[((int64_t)q * B + cr) * B + cc];
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    if (threadIdx.x < 64) {  // first largest |a_ik|, i >= k
+      double best = -1.0;
+      int bi = m;
+      for (int i = k + (int)threadIdx.x; i < m; i += 64) {
+        const double v = fabs(a[i * m + k]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int oi = __shfl_down(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (threadIdx.x == 0) s_piv = bi;
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    if (pr != k)
+      for (int j = threadIdx.x; j < m; j += kBlock) {
+        const double t = a[k * m + j];
+        a[k * m + j] = a[pr * m + j];
+        a[pr * m + j] = t;
+      }
+    __syncthreads();
+    const double d = a[k * m + k];
+    if (d != 0.0) {
+      const int rem = m - k - 1;
+      for (int e = threadIdx.x; e < rem * rem; e += kBlock) {
+        const int i = k + 1 + e / rem, j = k + 1 + e % rem;
+        const double l = a[i * m + k] / d;
+        a[i * m + j] = add_rn(a[i * m + j], -mul_rn(l, a[k * m + j]));
+      }
+      __syncthreads();
+      for (int i = k + 1 + (int)threadIdx.x; i < m; i += kBlock) a[i * m + k] = a[i * m + k] / d;
+    }
+    if (threadIdx.x == 0) piv[p * m + k] = pr;
+    __syncthreads();
+  }
+  // column-major factors for coalesced column reads in the solve
+  const int64_t base = p * (int64_t)m * m;
+  for (int e = threadIdx.x; e < m * m; e += kBlock) {
+    const int i = e % m, k = e / m;
+    if (lu64) lu64[base + e] = a[i * m + k];
+    else lu32[base + e] = (float)a[i * m + k];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T bcast(T v, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    const int64_t b = __builtin_bit_cast(int64_t, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFF), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __builtin_bit_cast(T, ((int64_t)hi << 32) | (uint32_t)lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  }
+}
+
+// one wavefront per patch; rows lane (slot 0) and lane + 64 (slot 1)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vanka_solve(int64_t npatches, int psize, int B,
+                                                        const int32_t* __restrict__ nodes,
+                                                        const T* __restrict__ lu,
+                                                        const int32_t* __restrict__ piv,
+                                                        const T* __restrict__ h,
+                                                        T* __restrict__ xp) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (p >= npatches) return;  // the whole wavefront
+  const int m = psize * B;
+  const int32_t* pn = nodes + p * psize;
+  const T* L = lu + p * (int64_t)m * m;
+  const int32_t* pv = piv + p * m;
+  T v0 = (T)0, v1 = (T)0;
+  if (lane < m) v0 = h[(int64_t)pn[lane / B] * B + lane % B];
+  if (lane + 64 < m) v1 = h[(int64_t)pn[(lane + 64) / B] * B + (lane + 64) % B];
+  // H = P H: the recorded row exchanges in order
+  for (int k = 0; k < m; ++k) {
+    const int q = pv[k];
+    if (q == k) continue;
+    const T hk = bcast(k < 64 ? v0 : v1, k & 63);
+    const T hq = bcast(q < 64 ? v0 : v1, q & 63);
+    if (lane == (k & 63)) {
+      if (k < 64) v0 = hq;
+      else v1 = hq;
+    }
+    if (lane == (q & 63)) {
+      if (q < 64) v0 = hk;
+      else v1 = hk;
+    }
+  }
+  // unit lower: column k updates rows i > k
+  for (int k = 0; k < m - 1; ++k) {
+    const T yk = bcast(k < 64 ? v0 : v1, k & 63);
+    const T* Lk = L + (int64_t)k * m;
+    if (lane > k && lane < m) v0 = add_rn(v0, -mul_rn(Lk[lane], yk));
+    if (lane + 64 > k && lane + 64 < m) v1 = add_rn(v1, -mul_rn(Lk[lane + 64], yk));
+  }
+  // upper: h_k /= u_kk, then column k updates rows i < k
+  for (int k = m - 1; k >= 0; --k) {
+    const T* Uk = L + (int64_t)k * m;
+    const T ukk = Uk[k];
+    if (lane == (k & 63)) {
+      if (k < 64) v0 = v0 / ukk;
+      else v1 = v1 / ukk;
+    }
+    const T yk = bcast(k < 64 ? v0 : v1, k & 63);
+    if (lane < k) v0 = add_rn(v0, -mul_rn(Uk[lane], yk));
+    if (lane + 64 < k) v1 = add_rn(v1, -mul_rn(Uk[lane + 64], yk));
+  }
+  T* out = xp + p * (int64_t)m;
+  if (lane < m) out[lane] = v0;
+  if (lane + 64 < m) out[lane + 64] = v1;
+}
+
+// x(i, c) = (sum over the patches of node i, ascending) * (1 / count_i)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vanka_average(int64_t n, int B, int psize, int m,
+                                                          const int32_t* __restrict__ nptr,
+                                                          const int32_t* __restrict__ nent,
+                                                          const T* __restrict__ xp,
+                                                          T* __restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * B) return;
+  const int64_t i = t / B;
+  const int c = (int)(t - i * B);
+  const int32_t e0 = nptr[i], e1 = nptr[i + 1];
+  T acc = (T)0;
+  for (int32_t e = e0; e < e1; ++e) {
+    const int32_t pe = nent[e];
+    const int64_t pp = pe / psize;
+    const int r = pe - (int)(pp * psize);
+    acc = add_rn(acc, xp[pp * m + B * r + c]);
+  }
+  const double s = 1.0 / (double)(e1 - e0);
+  x[t] = (T)mul_rn((double)acc, s);
+}
+
+}  // namespace
+
+namespace gh {
+
+int vanka_solve(gh_ctx* ctx, const gh_vanka* V, int prec, void* x) {
+  if (prec != V->prec) return fail(GH_EINVAL, "vanka: vector precision differs from the factors'");
+  const unsigned gs = (unsigned)((V->npatches + kBlock / 64 - 1) / (kBlock / 64));
+  const unsigned ga = (unsigned)((V->n * V->B + kBlock - 1) / kBlock);
+  if (V->npatches == 0) return GH_OK;
+  if (prec == GH_F64) {
+    hipLaunchKernelGGL(k_vanka_solve<double>, dim3(gs), dim3(kBlock), 0, ctx->stream, V->npatches,
+                       V->psize, V->B, V->nodes, (const double*)V->lu, V->piv, (const double*)x,
+                       (double*)V->xp);
+    hipLaunchKernelGGL(k_vanka_average<double>, dim3(ga), dim3(kBlock), 0, ctx->stream, V->n, V->B,
+                       V->psize, V->m, V->nptr, V->nent, (const double*)V->xp, (double*)x);
+  } else {
+    hipLaunchKernelGGL(k_vanka_solve<float>, dim3(gs), dim3(kBlock), 0, ctx->stream, V->npatches,
+                       V->psize, V->B, V->nodes, (const float*)V->lu, V->piv, (const float*)x,
+                       (float*)V->xp);
+    hipLaunchKernelGGL(k_vanka_average<float>, dim3(ga), dim3(kBlock), 0, ctx->stream, V->n, V->B,
+                       V->psize, V->m, V->nptr, V->nent, (const float*)V->xp, (float*)x);
+  }
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, const int32_t* nodes,
+                 int prec, gh_vanka** out) {
+  GH_ARG(ctx && A && out && (nodes || npatches == 0) && npatches >= 0 && psize >= 1,
+         "gh_vanka_create: bad arguments");
+  GH_ARG(A->prec == GH_F64, "gh_vanka_create: the matrix must be FP64");
+  GH_ARG(prec == GH_F64 || prec == GH_F32, "gh_vanka_create: factors are FP64 or FP32");
+  const int B = A->ncomp, m = psize * B;
+  GH_ARG(psize <= kVankaMaxNodes && m <= kVankaMaxM,
+         "gh_vanka_create: at most 128 unknowns per patch");
+  const int64_t n = A->n;
+  // node -> (patch, local) lists, ascending patch; every node must be covered
+  std::vector<int32_t> nptr(n + 1, 0), nent((size_t)npatches * psize);
+  for (int64_t k = 0; k < npatches * psize; ++k) {
+    if (nodes[k] < 0 || nodes[k] >= n) return fail(GH_EINVAL, "gh_vanka_create: node out of range");
+    nptr[nodes[k] + 1]++;
+  }
+  for (int64_t p = 0; p < npatches; ++p)
+    for (int r = 0; r < psize; ++r)
+      for (int t = 0; t < r; ++t)
+        if (nodes[p * psize + t] == nodes[p * psize + r])
+          return fail(GH_EINVAL, "gh_vanka_create: patch %lld lists a node twice", (long long)p);
+  for (int64_t i = 0; i < n; ++i) {
+    if (nptr[i + 1] == 0) return fail(GH_EINVAL, "gh_vanka_create: node %lld is in no patch", (long long)i);
+    nptr[i + 1] += nptr[i];
+  }
+  {
+    std::vector<int32_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int64_t p = 0; p < npatches; ++p)
+      for (int r = 0; r < psize; ++r) nent[fill[nodes[p * psize + r]]++] = (int32_t)(p * psize + r);
+  }
+  gh_vanka* V = new gh_vanka();
+  V->ctx = ctx;
+  ctx_retain(ctx);
+  V->B = B;
+  V->psize = psize;
+  V->m = m;
+  V->prec = prec;
+  V->n = n;
+  V->npatches = npatches;
+  auto bail = [&](int rc) {
+    gh_vanka_destroy(V);
+    return rc;
+  };
+  const size_t ts = prec == GH_F64 ? 8 : 4;
+  auto alloc = [&](void** p, size_t bytes) {
+    return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess;
+  };
+  if (!alloc((void**)&V->nodes, sizeof(int32_t) * npatches * psize) ||
+      !alloc(&V->lu, ts * npatches * m * m) || !alloc((void**)&V->piv, sizeof(int32_t) * npatches * m) ||
+      !alloc((void**)&V->nptr, sizeof(int32_t) * (n + 1)) ||
+      !alloc((void**)&V->nent, sizeof(int32_t) * npatches * psize) ||
+      !alloc(&V->xp, ts * npatches * m))
+    return bail(fail(GH_ENOMEM, "gh_vanka_create: allocation failed"));
+  if (hipMemcpy(V->nodes, nodes, sizeof(int32_t) * npatches * psize, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(V->nptr, nptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(V->nent, nent.data(), sizeof(int32_t) * npatches * psize, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(GH_EHIP, "gh_vanka_create: upload failed"));
+  if (npatches > 0) {
+    const size_t lds = sizeof(double) * (size_t)m * m;
+    if (hipFuncSetAttribute((const void*)k_vanka_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return bail(fail(GH_EHIP, "gh_vanka_create: %zu bytes of LDS per patch not available", lds));
+    hipLaunchKernelGGL(k_vanka_factor, dim3((unsigned)npatches), dim3(kBlock), lds, ctx->stream,
+                       npatches, psize, B, V->nodes, A->rowptr, A->col, (const double*)A->val,
+                       prec == GH_F64 ? (double*)V->lu : nullptr,
+                       prec == GH_F64 ? nullptr : (float*)V->lu, V->piv);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return bail(fail(GH_EHIP, "gh_vanka_create: factorisation failed"));
+  }
+  *out = V;
+  return GH_OK;
+}
+
+}  // namespace gh
+
+extern "C" {
+
+int gh_vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize,
+                    const int32_t* nodes, int prec, gh_vanka** out) {
+  return vanka_create(ctx, A, npatches, psize, nodes, prec, out);
+}
+
+int gh_vanka_destroy(gh_vanka* V) {
+  if (!V) return GH_OK;
+  if (V->ctx && V->ctx->stream) (void)hipStreamSynchronize(V->ctx->stream);
+  for (void* p : {(void*)V->nodes, V->lu, (void*)V->piv, (void*)V->nptr, (void*)V->nent, V->xp})
+    if (p) (void)hipFree(p);
+  gh_ctx* ctx = V->ctx;
+  delete V;
+  if (ctx) ctx_release(ctx);
+  return GH_OK;
+}
+
+int gh_vanka_solve(gh_ctx* ctx, const gh_vanka* V, gh_vec* x) {
+  GH_ARG(ctx && V && x, "gh_vanka_solve: NULL argument");
+  GH_ARG(x->n == V->n && x->ncomp == V->B, "gh_vanka_solve: vector shape does not match");
+  return vanka_solve(ctx, V, x->prec, x->d);
+}
+
+}  // extern "C"

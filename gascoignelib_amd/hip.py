@@ -137,11 +137,13 @@ class CudaCSRMatrixInterface:
 
     def layout(self):
         """Device layout: entries stored, padded slots, slices, slices with
-        16-bit column offsets."""
-        out = np.zeros(4, np.int64)
+        16-bit column offsets, slices in pattern mode, pattern-table entries,
+        column bytes a sweep reads."""
+        out = np.zeros(7, np.int64)
         N.check(N.hip().gh_csr_layout(self.h, N.ptr(out, N.i64p)))
         return dict(stored=int(out[0]), slots=int(out[1]), slices=int(out[2]),
-                    narrow=int(out[3]))
+                    narrow=int(out[3]), patterned=int(out[4]), table=int(out[5]),
+                    column_bytes=int(out[6]))
 
     def vmult(self, y, x, d=1.0, b=0.0):
         """y = d*A*x + b*y"""
@@ -293,7 +295,9 @@ class CudaMultiLevelSolver:
     """Device multigrid hierarchy built from a gascoignelib_amd.problem.Hierarchy.
 
     Parameters follow the reference's MultiLevelSolver / Solver blocks:
-    mgtype, iterpre, iterpost, iterexact, omega, mgomega, linear_smooth,
+    mgtype, iterpre, iterpost, iterexact, omega, mgomega, linear_smooth
+    ("jacobi", "richardson", or "vanka" with `patches`: per level an
+    (npatches, psize) node array or None),
     ndirect, prec_maxiter; `prec` selects the V-cycle storage ("f64", "f32"
     or "f16" matrix values with FP32 arithmetic); `krylov` the outer solver of
     Gmres/Solve: "gmres" (left-preconditioned, as GMRES::solve) or "fgmres"
@@ -302,7 +306,7 @@ class CudaMultiLevelSolver:
 
     def __init__(self, ctx, hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
                  mgomega=1.0, linear_smooth="jacobi", ndirect=1, prec_maxiter=1, prec="f64",
-                 use_graph=False, order=None, krylov="gmres", adjoint=False):
+                 use_graph=False, order=None, krylov="gmres", adjoint=False, patches=None):
         """order: None (keep the hierarchy's numbering), "lexicographic" (sort
         each level's nodes by their coordinates) or a list of per-level
         permutations (internal position -> node)."""
@@ -330,7 +334,7 @@ class CudaMultiLevelSolver:
         p.mgtype = mgtype.encode()
         p.iterpre, p.iterpost, p.iterexact = iterpre, iterpost, iterexact
         p.omega, p.mgomega = omega, mgomega
-        p.smoother = {"jacobi": 0, "richardson": 1}[linear_smooth]
+        p.smoother = {"jacobi": 0, "richardson": 1, "vanka": 2}[linear_smooth]
         p.ndirect = ndirect
         p.prec_maxiter = prec_maxiter
         p.prec = N.PREC[prec]
@@ -362,6 +366,13 @@ class CudaMultiLevelSolver:
             check(N.hip().gh_mg_create_ordered(ctx.h, nl, self._arrays[0], self._arrays[1],
                                                self._arrays[2], self._arrays[3], oarr,
                                                C.byref(p), C.byref(self.h)))
+        if patches is not None:
+            for l, pt in enumerate(patches):
+                if pt is None:
+                    continue
+                pt = np.ascontiguousarray(pt, dtype=np.int32)
+                check(N.hip().gh_mg_set_vanka(self.h, l, pt.shape[0], pt.shape[1],
+                                              ptr(pt, N.i32p)))
 
     @property
     def fine_matrix(self):
@@ -442,5 +453,26 @@ class CoefficientCells:
         try:
             if self.h:
                 N.hip().gh_cells_destroy(self.h)
+        except Exception:
+            pass
+
+
+class VankaSmoother:
+    """VankaSmoother: patch LU factors of an FP64 matrix; solve(x) in place."""
+
+    def __init__(self, ctx, A, patches, prec="f64"):
+        self.ctx = ctx
+        self._nodes = np.ascontiguousarray(patches, dtype=np.int32)
+        self.h = C.c_void_p()
+        check(N.hip().gh_vanka_create(ctx.h, A.h, self._nodes.shape[0], self._nodes.shape[1],
+                                      ptr(self._nodes, N.i32p), N.PREC[prec], C.byref(self.h)))
+
+    def solve(self, x):
+        check(N.hip().gh_vanka_solve(self.ctx.h, self.h, x.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.hip().gh_vanka_destroy(self.h)
         except Exception:
             pass

@@ -133,6 +133,10 @@ HIP_SIGNATURES = {
                                  P(C.c_int)]),
     "gh_newton_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, P(GhCGInfo),
                                   P(GhNLInfo), C.c_int, P(GhSolveReport)]),
+    "gh_vanka_create": (C.c_int, [vp, vp, C.c_int64, C.c_int, i32p, C.c_int, P(vp)]),
+    "gh_vanka_destroy": (C.c_int, [vp]),
+    "gh_vanka_solve": (C.c_int, [vp, vp, vp]),
+    "gh_mg_set_vanka": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int, i32p]),
     "gh_cells_create": (C.c_int, [vp, C.c_int, C.c_int64, f64p, C.c_int64, i32p, P(vp)]),
     "gh_cells_destroy": (C.c_int, [vp]),
     "gh_dJdk_cells": (C.c_int, [vp, vp, vp, vp, vp]),

@@ -224,3 +224,32 @@ def lumped_mass(coords, cells):
     m = np.zeros(len(X))
     np.add.at(m, cells.ravel(), np.repeat(vol / npc, npc))
     return m
+
+
+def patches_from_cells(cells, children=8):
+    """Vanka patches of a refined level: the children of one coarser cell
+    (consecutive groups of `children` cells) form a patch; its nodes in
+    ascending order.  Every patch of a uniformly refined hex mesh has 27 nodes
+    (9 for quads)."""
+    cells = np.asarray(cells)
+    groups = cells.reshape(-1, children * cells.shape[1])
+    out = [np.unique(g) for g in groups]
+    size = len(out[0])
+    if any(len(o) != size for o in out):
+        raise ValueError("patches of unequal size")
+    return np.ascontiguousarray(np.array(out, dtype=np.int32))
+
+
+def block_operator(level, C, S):
+    """A block matrix of `level`'s pattern with B = len(C): block(i, j) =
+    a_ij C + [i == j] S (a test operator for block kernels and the Vanka
+    smoother; Dirichlet rows stay identity blocks)."""
+    B = len(C)
+    rows = np.repeat(np.arange(level.n), np.diff(level.rowptr))
+    vals = level.val[:, None, None] * np.asarray(C)[None]
+    diag = rows == level.col
+    d = np.zeros(level.n, bool)
+    d[level.dirichlet] = True
+    vals[diag] += np.asarray(S)[None]
+    vals[diag & d[rows]] = np.eye(B)
+    return np.ascontiguousarray(vals.reshape(-1))

@@ -46,6 +46,7 @@ typedef struct gh_hn gh_hn;
 typedef struct gh_mask gh_mask;
 typedef struct gh_mg gh_mg;
 typedef struct gh_cells gh_cells;
+typedef struct gh_vanka gh_vanka;
 
 const char* gh_last_error(void);
 const char* gh_version(void);
@@ -106,12 +107,14 @@ int gh_csr_destroy(gh_csr* A);
 int64_t gh_csr_n(const gh_csr* A);
 int64_t gh_csr_nnz(const gh_csr* A);
 int gh_csr_prec(const gh_csr* A);
-/* Device layout (no reference counterpart; for byte accounting).  Scalar
- * matrices: out[0] entries stored (exact zeros of the pattern are not stored;
- * every row sum takes its other terms in the same order, so the values of
- * A x are unchanged), out[1] padded slots, out[2] slices of 64 rows, out[3]
- * slices whose columns are 16-bit offsets.  Block matrices: out[0] = out[1] =
- * nnz blocks, out[2] = out[3] = 0. */
+/* Device layout (no reference counterpart; for byte accounting), out[7].
+ * Scalar matrices: out[0] entries stored (exact zeros of the pattern are not
+ * stored; every row sum takes its other terms in the same order, so the values
+ * of A x are unchanged), out[1] padded slots, out[2] slices of 64 rows,
+ * out[3] slices with 16-bit column offsets, out[4] slices whose rows all follow
+ * shared relative column patterns (no column storage), out[5] entries of the
+ * pattern table, out[6] column bytes one sweep reads.  Block matrices:
+ * out[0] = out[1] = nnz blocks, out[2..5] = 0, out[6] = 4 nnz. */
 int gh_csr_layout(const gh_csr* A, int64_t* out);
 /* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
@@ -129,6 +132,20 @@ int gh_smooth(gh_ctx* ctx, const gh_csr* A, gh_vec* x, const gh_vec* b, gh_vec* 
               int niter, double omega, int smoother);
 /* h = D^{-1} h: SparseBlockMatrix::Jacobi (src/LinAlg/sparseblockmatrix.xx:47-56) */
 int gh_jacobi(gh_ctx* ctx, const gh_csr* A, gh_vec* h);
+
+/* ----------------------------------------------------------------- Vanka */
+/* VankaSmoother (src/LinAlg/vankasmoother.h:65-125, .cc:7-237): patches of
+ * `psize` nodes, nodes[p*psize + r] (local unknown order: node r, components
+ * interleaved; at most 128 unknowns per patch).  ConstructStructure +
+ * copy_entries (:7-111): the block of the FP64 matrix A on each patch's
+ * unknowns, LU with partial pivoting (first largest |a_ik|), factors stored in
+ * `prec` (GH_F64 or GH_F32).  Every node must lie in a patch. */
+int gh_vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize,
+                    const int32_t* nodes, int prec, gh_vanka** out);
+int gh_vanka_destroy(gh_vanka* V);
+/* solve (:193-237), in place: x <- (1/count_i) sum_p R_p^T A_p^-1 R_p x, the
+ * patches of a node summed in ascending order.  x in the factors' precision. */
+int gh_vanka_solve(gh_ctx* ctx, const gh_vanka* V, gh_vec* x);
 
 /* ---------------------------------------------------------------- transfer */
 /* CudaMgInterpolatorNested::BasicInit / check_matrices
@@ -204,7 +221,9 @@ typedef struct {
   char mgtype;          /* 'V', 'W', 'F' */
   int iterpre, iterpost, iterexact;
   double omega, mgomega;
-  int smoother;         /* 0 jacobi, 1 richardson */
+  int smoother;         /* 0 jacobi, 1 richardson, 2 vanka ("ilu" with the Vanka
+                           patch smoother: h = b - A x; Vanka solve of h;
+                           x += omega h; patches from gh_mg_set_vanka) */
   int ndirect;          /* levels with fewer nodes get a dense FP64 direct solve */
   int prec_maxiter;
   double prec_tol, prec_globaltol;
@@ -250,6 +269,10 @@ int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const*
 int gh_mg_create_ordered(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
                          gh_hn* const* H, gh_mask* const* M, const int32_t* const* order,
                          const gh_mg_params* params, gh_mg** out);
+/* The Vanka patches of one level (in the caller's node numbering), for
+ * smoother 2; built on the solver's FP64 level matrix, factors in the V-cycle's
+ * precision.  Levels solved directly need none. */
+int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int32_t* nodes);
 int gh_mg_destroy(gh_mg* mg);
 int gh_mg_nlevels(const gh_mg* mg);
 int64_t gh_mg_mgsteps(const gh_mg* mg);  /* the "--> mgstep" counter */

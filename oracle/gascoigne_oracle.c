@@ -17,6 +17,116 @@
 #include <omp.h>
 #endif
 
+/* ---------------------------------------------------------------- Vanka */
+/* copy_entries_sparseblockmatrix (vankasmoother.cc:64-111): the block of A on
+ * the patch's unknowns, then PartialPivLU::compute */
+or_vanka* or_vanka_create(const or_matrix* A, int npatches, int psize, const int32_t* nodes) {
+  const int B = A->ncomp, m = psize * B;
+  or_vanka* V = (or_vanka*)calloc(1, sizeof(or_vanka));
+  V->n = A->n;
+  V->ncomp = B;
+  V->npatches = npatches;
+  V->psize = psize;
+  V->m = m;
+  V->nodes = (int32_t*)malloc(sizeof(int32_t) * (size_t)npatches * psize);
+  memcpy(V->nodes, nodes, sizeof(int32_t) * (size_t)npatches * psize);
+  V->lu = (double*)calloc((size_t)npatches * m * m, sizeof(double));
+  V->piv = (int32_t*)calloc((size_t)npatches * m, sizeof(int32_t));
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < npatches; ++p) {
+    const int32_t* pn = nodes + (int64_t)p * psize;
+    double* a = V->lu + (int64_t)p * m * m;
+    int32_t* pv = V->piv + (int64_t)p * m;
+    for (int r = 0; r < psize; ++r) {
+      const int row = pn[r];
+      for (int pos = A->rowptr[row]; pos < A->rowptr[row + 1]; ++pos) {
+        const int col = A->col[pos];
+        int c = -1;
+        for (int q = 0; q < psize; ++q)
+          if (pn[q] == col) c = q;
+        if (c < 0) continue;
+        for (int cr = 0; cr < B; ++cr)
+          for (int cc = 0; cc < B; ++cc)
+            a[(B * r + cr) * m + B * c + cc] = A->val[((int64_t)pos * B + cr) * B + cc];
+      }
+    }
+    for (int k = 0; k < m; ++k) {
+      int pr = k;
+      double best = fabs(a[k * m + k]);
+      for (int i = k + 1; i < m; ++i)
+        if (fabs(a[i * m + k]) > best) {
+          best = fabs(a[i * m + k]);
+          pr = i;
+        }
+      pv[k] = pr;
+      if (pr != k)
+        for (int j = 0; j < m; ++j) {
+          const double t = a[k * m + j];
+          a[k * m + j] = a[pr * m + j];
+          a[pr * m + j] = t;
+        }
+      const double d = a[k * m + k];
+      if (d == 0.0) continue;
+      for (int i = k + 1; i < m; ++i) {
+        const double l = a[i * m + k] / d;
+        a[i * m + k] = l;
+        for (int j = k + 1; j < m; ++j) a[i * m + j] -= l * a[k * m + j];
+      }
+    }
+  }
+  return V;
+}
+
+void or_vanka_free(or_vanka* V) {
+  if (!V) return;
+  free(V->nodes);
+  free(V->lu);
+  free(V->piv);
+  free(V);
+}
+
+/* VankaSmoother::solve (vankasmoother.cc:193-237) */
+void or_vanka_solve(const or_vanka* V, double* x) {
+  const int B = V->ncomp, m = V->m, n = V->n;
+  double* b = (double*)malloc(sizeof(double) * (size_t)n * B);
+  int* count = (int*)calloc((size_t)n, sizeof(int));
+  double* h = (double*)malloc(sizeof(double) * m);
+  memcpy(b, x, sizeof(double) * (size_t)n * B);
+  memset(x, 0, sizeof(double) * (size_t)n * B);
+  for (int p = 0; p < V->npatches; ++p) {
+    const int32_t* pn = V->nodes + (int64_t)p * V->psize;
+    const double* a = V->lu + (int64_t)p * m * m;
+    const int32_t* pv = V->piv + (int64_t)p * m;
+    for (int r = 0; r < V->psize; ++r)
+      for (int c = 0; c < B; ++c) h[B * r + c] = b[(int64_t)pn[r] * B + c];
+    for (int k = 0; k < m; ++k)
+      if (pv[k] != k) {
+        const double t = h[k];
+        h[k] = h[pv[k]];
+        h[pv[k]] = t;
+      }
+    /* triangular solves, column by column (Eigen's blocked order is not
+       available to pin; this order is the device's) */
+    for (int k = 0; k < m; ++k)
+      for (int i = k + 1; i < m; ++i) h[i] -= a[i * m + k] * h[k];
+    for (int k = m - 1; k >= 0; --k) {
+      h[k] /= a[k * m + k];
+      for (int i = 0; i < k; ++i) h[i] -= a[i * m + k] * h[k];
+    }
+    for (int r = 0; r < V->psize; ++r) {
+      count[pn[r]]++;
+      for (int c = 0; c < B; ++c) x[(int64_t)pn[r] * B + c] += h[B * r + c];
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const double s = 1.0 / count[i];
+    for (int c = 0; c < B; ++c) x[(int64_t)i * B + c] *= s;
+  }
+  free(b);
+  free(count);
+  free(h);
+}
+
 /* ------------------------------------------------------------ adjoint */
 static double or_s1(int p, int q, double h) { return (p == q ? 1.0 : -1.0) / h; }
 static double or_m1(int p, int q, double h) { return h * (p == q ? 2.0 : 1.0) / 6.0; }
@@ -743,6 +853,10 @@ static void smooth(or_mg* M, int l, int niter, double* x, const double* y, doubl
       or_add(x, M->omega, h, n);
     } else if (M->smoother == 1) {
       or_residual(A, h, x, y);
+      or_add(x, M->omega, h, n);
+    } else if (M->smoother == 3) {
+      or_residual(A, h, x, y);
+      or_vanka_solve((const or_vanka*)M->L[l].vanka, h);
       or_add(x, M->omega, h, n);
     }
   }

@@ -148,6 +148,7 @@ typedef struct {
   int direct;       /* 1: dense direct coarse solve (nnodes < ndirect) */
   double* lu;       /* dense LU of A (direct levels) */
   int32_t* piv;
+  void* vanka;      /* or_vanka* of the level (smoother 3), not owned */
 } or_level;
 
 typedef struct {
@@ -156,7 +157,7 @@ typedef struct {
   char mgtype;      /* 'V','W','F' */
   int iterpre, iterpost, iterexact;
   double omega, mgomega;
-  int smoother;     /* 0 jacobi, 1 richardson, 2 none */
+  int smoother;     /* 0 jacobi, 1 richardson, 2 none, 3 vanka ("ilu" = VankaSmoother) */
   int prec_maxiter;
   double prec_tol, prec_globaltol;
   int64_t mgsteps;  /* counter ("--> mgstep") */
@@ -186,6 +187,24 @@ typedef struct {
 int or_newton_solve(or_mg* M, int coarselevel, int vmax, double* u, const double* f,
                     or_cginfo* linfo, or_nlinfo* nlinfo, int norm_kind,
                     or_solve_report* rep);
+
+/* Vanka smoother (src/LinAlg/vankasmoother.cc): patches of `psize` nodes,
+ * nodes[p*psize + r] (local order = local unknown order, components
+ * interleaved); each patch matrix A_p = A restricted to the patch's unknowns is
+ * LU-factored with partial pivoting (Eigen PartialPivLU in the reference,
+ * :111, absent here: an unblocked LU with the same pivot rule, first largest
+ * |a_ik| in column k).  solve: x <- (1/count_i) sum_p R_p^T A_p^-1 R_p x, the
+ * patches added in ascending order (:193-237). */
+typedef struct {
+  int n, ncomp, npatches, psize, m;
+  int32_t* nodes;
+  double* lu;   /* npatches * m * m, row-major, L unit lower below the diagonal */
+  int32_t* piv; /* npatches * m: row swapped with row k at step k */
+} or_vanka;
+
+or_vanka* or_vanka_create(const or_matrix* A, int npatches, int psize, const int32_t* nodes);
+void or_vanka_free(or_vanka* V);
+void or_vanka_solve(const or_vanka* V, double* x);
 
 /* Coefficient gradient of J(u(k)) for -div(k grad u) = f with one coefficient
  * per cell of an axis-aligned Q1 box mesh (no reference counterpart: the

@@ -56,7 +56,7 @@ class OrNLInfo(C.Structure):
 class OrLevel(C.Structure):
     _fields_ = [("A", OrMatrix), ("T", C.POINTER(OrTransfer)), ("hn", OrHN),
                 ("ndir", C.c_int), ("dir", i32p), ("ndircomp", C.c_int), ("dircomp", i32p),
-                ("direct", C.c_int), ("lu", f64p), ("piv", i32p)]
+                ("direct", C.c_int), ("lu", f64p), ("piv", i32p), ("vanka", C.c_void_p)]
 
 
 class OrMG(C.Structure):
@@ -123,6 +123,9 @@ def _load():
                                       C.POINTER(OrSolveReport)]),
         "or_num_threads": (C.c_int, []),
         "or_dJdk_cells": (None, [C.c_int, f64p, C.c_int64, i32p, f64p, f64p, f64p]),
+        "or_vanka_create": (C.c_void_p, [C.POINTER(OrMatrix), C.c_int, C.c_int, i32p]),
+        "or_vanka_free": (None, [C.c_void_p]),
+        "or_vanka_solve": (None, [C.c_void_p, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -248,7 +251,7 @@ class MG:
     object (solver numbering: level 0 = coarsest)."""
 
     def __init__(self, hier, *, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
-                 mgomega=1.0, smoother="jacobi", ndirect=1):
+                 mgomega=1.0, smoother="jacobi", ndirect=1, patches=None):
         nl = hier.nlevels
         self.hier = hier
         self.p = lib().or_mg_create(nl)
@@ -256,7 +259,8 @@ class MG:
         M.mgtype = mgtype.encode()
         M.iterpre, M.iterpost, M.iterexact = iterpre, iterpost, iterexact
         M.omega, M.mgomega = omega, mgomega
-        M.smoother = {"jacobi": 0, "richardson": 1, "none": 2}[smoother]
+        M.smoother = {"jacobi": 0, "richardson": 1, "none": 2, "vanka": 3}[smoother]
+        self.vankas = []
         self.keep = []
         self.mats, self.transfers, self.hns = [], [], []
         for l in range(nl):
@@ -278,6 +282,10 @@ class MG:
             L.ndir, L.dir = len(d), ptr(d, i32p)
             L.ndircomp, L.dircomp = len(dc), ptr(dc, i32p)
             L.direct = 1 if lev.n < ndirect else 0
+            if smoother == "vanka" and patches is not None and patches[l] is not None:
+                V = Vanka(A, patches[l])
+                self.vankas.append(V)
+                L.vanka = V.p
         if lib().or_mg_setup(self.p):
             raise RuntimeError("or_mg_setup failed")
 
@@ -318,3 +326,22 @@ def dJdk_cells(dim, coords, cells, z, u):
                         ptr(np.ascontiguousarray(z, dtype=np.float64)),
                         ptr(np.ascontiguousarray(u, dtype=np.float64)), ptr(g))
     return g
+
+
+class Vanka:
+    """oracle Vanka smoother over a Matrix and a (npatches, psize) node array"""
+
+    def __init__(self, A, patches):
+        self.A = A
+        self.nodes = np.ascontiguousarray(patches, dtype=np.int32)
+        self.p = lib().or_vanka_create(C.byref(A.s), self.nodes.shape[0], self.nodes.shape[1],
+                                       ptr(self.nodes, i32p))
+
+    def solve(self, x):
+        lib().or_vanka_solve(self.p, ptr(x))
+
+    def __del__(self):
+        try:
+            lib().or_vanka_free(self.p)
+        except Exception:
+            pass

@@ -383,6 +383,7 @@ def test_sell_column_encodings_and_dropped_zeros(ctx):
     lay = A.layout()
     assert lay["stored"] == int(np.count_nonzero(val))
     assert 0 < lay["narrow"] < lay["slices"]
+    assert 0 < lay["patterned"] < lay["slices"] - lay["narrow"]  # and some 32-bit slices
     Ao = O.Matrix(rowptr, col, val)
     x, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
     y = np.zeros(n)
