@@ -259,7 +259,8 @@ def measured_traffic(layout):
         return None
     with open(path) as fh:
         t = json.load(fh)
-    if t.get("stored_entries") != layout["stored"]:
+    if (t.get("stored_entries") != layout["stored"]
+            or t.get("column_bytes") != layout["column_bytes"]):
         return None
     return t["bytes_per_launch"]
 
@@ -293,6 +294,7 @@ def kernel_only(ctx, fine, f, args):
     print(json.dumps({"kernel_only": True, "reps": args.spmv_reps,
                       "residual_bytes_f64": sell_bytes(n, A.layout(), 8, 8),
                       "stored_entries": A.layout()["stored"],
+                      "column_bytes": A.layout()["column_bytes"],
                       "copy_bytes_each_way": m * 8}), flush=True)
 
 
