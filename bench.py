@@ -93,10 +93,12 @@ def main():
                          "(default 0.9e-10 with fgmres: its estimate is the true residual; 1e-5 with gmres)")
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
-    ap.add_argument("--workload", default="poisson128", choices=["poisson128", "adjoint96"],
+    ap.add_argument("--workload", default="poisson128",
+                    choices=["poisson128", "adjoint96", "stokes64"],
                     help="poisson128: 3D Q1 Poisson on 128^3 cells (the headline); adjoint96: "
                          "variable-coefficient diffusion on 96^3 cells, one gradient evaluation "
-                         "(forward solve + adjoint solve + dJ/dk per cell) per step")
+                         "(forward solve + adjoint solve + dJ/dk per cell) per step; stokes64: "
+                         "Stokes-LPS channel on 64^3 cells, 4x4 blocks, Vanka multigrid")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
@@ -110,6 +112,8 @@ def main():
 
     if args.workload == "adjoint96":
         return bench_adjoint(args)
+    if args.workload == "stokes64":
+        return bench_stokes(args)
 
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.problem import lattice_rhs
@@ -391,6 +395,124 @@ def bench_adjoint(args):
         "dJdk_kernel": {"us_per_launch": t_dj * 1e6, "bytes_per_launch": bytes_dj,
                         "gbs": bytes_dj / t_dj / 1e9},
         "setup_s": t_struct,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_stokes(args):
+    """Stokes-LPS channel (NavierStokesLps<3> at U = 0, CGDiscQ13dLps) on the
+    unit cube refined to 64^3 cells: 274,625 nodes x (p, u, v, w) = 1,098,500
+    unknowns, 16,974,593 4x4 blocks on the patch structure, 6 levels (2^3 ..
+    64^3 cells, the 2^3 level solved directly).  Parabolic inflow of seeded
+    amplitude at x = 0, no-slip walls, do-nothing outflow.  One step = one solve
+    of the Newton correction from zero to 1e-10 relative FP64 residual with
+    V(2,2) cycles of the Vanka smoother (omega 0.8, 2x2x2-cell patches, 108
+    unknowns each): FP64 (left-preconditioned GMRES(10) in Newton, the
+    reference's algorithm) against mixed precision (FP32 matrices, FP32 patch
+    factors and FP32 V-cycle inside FP64 flexible GMRES(10))."""
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd.structure import stokes_hierarchy
+
+    cells = 64 if args.cells == 128 else args.cells
+    pre = int(round(np.log2(cells)))
+    t0 = time.perf_counter()
+    hier = stokes_hierarchy(pre, seed=2024)
+    t_struct = time.perf_counter() - t0
+    fine = hier.fine
+    nunk = fine.n * 4
+    f = hier.rhs
+    print(f"# stokes {cells}^3: {fine.n} nodes, {nunk} unknowns, {fine.nnz} blocks, "
+          f"{hier.nlevels} levels, built in {t_struct:.1f}s", file=sys.stderr, flush=True)
+    ctx = H.Context(0)
+    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=4, omega=0.8, mgomega=1.0,
+                  ndirect=200, linear_smooth="vanka", patches=hier.patches,
+                  order=None if args.order == "none" else args.order)
+    t0 = time.perf_counter()
+    solvers = {"fp64": (H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common), 1e-10),
+               "mixed": (H.CudaMultiLevelSolver(ctx, hier, prec="f32", krylov="fgmres", **common),
+                         0.9e-10)}
+    t_setup = time.perf_counter() - t0
+    F = H.CudaVectorInterface.from_global(ctx, f, 4)
+    U = H.CudaVectorInterface(ctx, fine.n, 4)
+
+    def step(tag):
+        mg, tol = solvers[tag]
+        U.zero()
+        rep = mg.Solve(U, F, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
+                       norm_kind=1)
+        if H.N.NL_STATUS[rep.status] != "converged":
+            raise RuntimeError(f"{tag} Stokes solve did not converge")
+        return rep
+
+    res = {}
+    for tag in ("fp64", "mixed"):
+        for _ in range(args.warmup):
+            step(tag)
+        ts = []
+        for i in range(args.steps):
+            ctx.sync()
+            t = time.perf_counter()
+            rep = step(tag)
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+            print(f"# {tag} step {i}: {ts[-1] * 1e3:.1f} ms (newton {rep.newton_iterations}, "
+                  f"gmres {rep.gmres_iterations})", file=sys.stderr, flush=True)
+        u = U.copy_back()
+        res[tag] = dict(ms=float(np.median(ts)) * 1e3, gmres=rep.gmres_iterations,
+                        newton=rep.newton_iterations, u=u)
+    # true FP64 residual of the mixed result and mass conservation of both
+    A = solvers["fp64"][0].fine_matrix
+    U.assign(res["mixed"]["u"])
+    R = H.CudaVectorInterface.from_global(ctx, f, 4)
+    A.vmult(R, U, -1.0, 1.0)
+    r = R.copy_back()
+    r.reshape(-1, 4)[fine.dirichlet, 1:] = 0.0
+    fin = float(hier.mass_in @ hier.u0[:, 1])
+
+    def flux(x):
+        return float(hier.mass_out @ (x.reshape(-1, 4)[:, 1] + hier.u0[:, 1]))
+
+    # dominant kernels: FP64 block residual on the finest level, one Vanka solve
+    X = H.CudaVectorInterface.from_global(ctx, res["fp64"]["u"], 4)
+    t_res = time_kernel(ctx, lambda: A.residual(R, X, F), args.spmv_reps)
+    bytes_res = fine.nnz * (16 * 8 + 4) + (fine.n + 1) * 4 + nunk * 8 * 3
+    V = H.VankaSmoother(ctx, A, hier.patches[-1])
+    t_vk = time_kernel(ctx, lambda: V.solve(X), max(5, args.spmv_reps // 5))
+    npat = len(hier.patches[-1])
+    m = 27 * 4
+    bytes_vk = npat * (m * m * 8 + m * 4) + nunk * 8 * 3
+    line = {
+        "metric": "unknowns/s of a Stokes-LPS solve to 1e-10 rel residual (mixed vs FP64)",
+        "value": nunk / (res["mixed"]["ms"] * 1e-3), "unit": "unknowns/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["mixed"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 outer / f32 Vanka V-cycle (f32 matrices and patch factors)",
+        "data": "synthetic: parabolic inflow of seeded amplitude "
+                f"{hier.amplitude:.6f}, visc {hier.visc}, alpha {hier.alpha}",
+        "config": {"workload": f"3D Stokes Q1/Q1-LPS channel, {cells}^3 cells, 4x4 blocks, "
+                               "Vanka V(2,2) on 2x2x2-cell patches",
+                   "nodes": fine.n, "unknowns": nunk, "blocks": fine.nnz,
+                   "levels": hier.nlevels},
+        "fp64": {"unknowns_per_s": nunk / (res["fp64"]["ms"] * 1e-3),
+                 "ms_per_step": res["fp64"]["ms"], "newton": res["fp64"]["newton"],
+                 "gmres": res["fp64"]["gmres"],
+                 "flux_error": abs(flux(res["fp64"]["u"]) - fin) / fin},
+        "mixed": {"unknowns_per_s": nunk / (res["mixed"]["ms"] * 1e-3),
+                  "ms_per_step": res["mixed"]["ms"], "newton": res["mixed"]["newton"],
+                  "gmres": res["mixed"]["gmres"],
+                  "rel_error_vs_fp64": float(np.linalg.norm(res["mixed"]["u"] - res["fp64"]["u"])
+                                             / np.linalg.norm(res["fp64"]["u"])),
+                  "true_rel_residual": float(np.linalg.norm(r) / np.linalg.norm(f)),
+                  "flux_error": abs(flux(res["mixed"]["u"]) - fin) / fin},
+        "speedup_mixed_vs_fp64": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "kernels": {
+            "block_residual_f64": {"us_per_launch": t_res * 1e6, "bytes_per_launch": bytes_res,
+                                   "gbs": bytes_res / t_res / 1e9,
+                                   "frac": bytes_res / t_res / 1e9 / HBM_PEAK_GBS},
+            "vanka_solve_f64": {"us_per_launch": t_vk * 1e6, "bytes_per_launch": bytes_vk,
+                                "gbs": bytes_vk / t_vk / 1e9,
+                                "frac": bytes_vk / t_vk / 1e9 / HBM_PEAK_GBS}},
+        "setup_s": {"structure": t_struct, "device": t_setup},
     }
     print(json.dumps(line), flush=True)
 

@@ -1,6 +1,7 @@
 // Host-side structure producer (see include/gh_structure.h): .inp reader,
 // nested uniform refinement of quad/hex meshes, mesh-transfer maps, Dirichlet
-// lists, CSR structure and Q1 assembly for -div(k grad u).
+// lists, CSR structure, Q1 assembly for -div(k grad u), patches and the
+// patch-based Stokes-LPS block assembly.
 #include "gh_structure.h"
 
 #include <algorithm>
@@ -10,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -440,6 +442,142 @@ int assemble(int dim, int64_t n, const double* X, int64_t ncells, const int32_t*
   return 0;
 }
 
+// Patches of a refined level: the children of coarse cell p are cells
+// 8p..8p+7 (4p..4p+3 in 2D), child (i, j, k) = 8p + i + 2j + 4k.  Patch node
+// (a, b, c), a, b, c in {0, 1, 2}, is stored at a + 3b + 9c (x fastest), the
+// element order of Gascoigne's patch dof handler (the Q1-patch basis
+// src/Discretization/Q2/baseq13dpatch.h:58-68,104-140 places child k's corner
+// i at perm[i] + perm[k] with perm = 0,1,3,4,9,10,12,13).
+int patches(const Mesh& M, std::vector<int32_t>& P) {
+  if (M.ncoarse == 0) return fail("the coarsest mesh of a hierarchy has no patches");
+  const int dim = M.dim, npc = M.npc;
+  const int64_t nch = dim == 3 ? 8 : 4;
+  const int nloc = dim == 3 ? 27 : 9;
+  const int64_t np = M.ncells() / nch;
+  if (np * nch != M.ncells()) return fail("cell count is not a multiple of %d", (int)nch);
+  P.assign(np * nloc, -1);
+  for (int64_t p = 0; p < np; ++p) {
+    int32_t* out = &P[p * nloc];
+    for (int64_t ch = 0; ch < nch; ++ch) {
+      const int i = ch & 1, j = (ch >> 1) & 1, k = dim == 3 ? (int)(ch >> 2) : 0;
+      const int32_t* v = &M.cells[(p * nch + ch) * npc];
+      for (int m = 0; m < npc; ++m) {
+        const int a = i + (dim == 3 ? kHexCorner[m][0] : kQuadCorner[m][0]);
+        const int b = j + (dim == 3 ? kHexCorner[m][1] : kQuadCorner[m][1]);
+        const int c = k + (dim == 3 ? kHexCorner[m][2] : 0);
+        int32_t& slot = out[a + 3 * b + 9 * c];
+        if (slot >= 0 && slot != v[m]) return fail("patch %lld: children do not share nodes", (long long)p);
+        slot = v[m];
+      }
+    }
+    for (int a = 0; a < nloc; ++a)
+      if (out[a] < 0) return fail("patch %lld: node %d missing", (long long)p, a);
+  }
+  return 0;
+}
+
+// 1D matrices of the piecewise-linear basis on a patch interval [0, 2h] with
+// nodes 0, h, 2h: stiffness S, mass M, and C(a, b) = int phi_a phi_b'.
+struct Patch1d {
+  double S[3][3], M[3][3], C[3][3];
+  explicit Patch1d(double h) {
+    const double s[3][3] = {{1, -1, 0}, {-1, 2, -1}, {0, -1, 1}};
+    const double m[3][3] = {{2, 1, 0}, {1, 4, 1}, {0, 1, 2}};
+    const double c[3][3] = {{-0.5, 0.5, 0}, {-0.5, 0, 0.5}, {0, -0.5, 0.5}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        S[a][b] = s[a][b] / h;
+        M[a][b] = h * m[a][b] / 6.0;
+        C[a][b] = c[a][b];
+      }
+  }
+};
+
+// Stokes-LPS patch element of NavierStokesLps<3> at U = 0 with
+// symmetrictensor = 0 (src/Problems/navierstokes.h:144-168 Galerkin part,
+// :248-254 pressure stabilisation, :218-224 lpspoint) on an axis-aligned box
+// patch of 2x2x2 cells of size hx x hy x hz.  Components (p, u, v, w); the
+// 4x4 block of row node a (test function) and column node b (trial function):
+//   (0, d+1)   +int dphi_b/dx_d phi_a                     divergence
+//   (d+1, d+1) +visc int grad phi_b . grad phi_a           viscosity
+//   (d+1, 0)   -int phi_b dphi_a/dx_d                      pressure
+//   (0, 0)     +alpha_P int grad (I-pi)phi_b . grad (I-pi)phi_a
+// with pi the trilinear interpolation from the 8 patch corners (the
+// fluctuation operator of ElementLpsIntegrator::Projection,
+// src/DofHandler/elementlpsintegrator.h:62-105) and
+// alpha_P = alpha / (visc / h_P^2), h_P = (patch volume)^(1/3)
+// (Volume2MeshSize of the patch Jacobian, elementintegrator.h:53-56).  The
+// integrals are exact (the reference's 2x2x2 Gauss rule per cell is exact for
+// these integrands), so they are formed as tensor products of 1D matrices.
+struct StokesPatch {
+  double blk[27][27][16];
+  StokesPatch(double hx, double hy, double hz, double visc, double alpha) {
+    const Patch1d X(hx), Y(hy), Z(hz);
+    double K[27][27], D[3][27][27];
+    for (int a = 0; a < 27; ++a)
+      for (int b = 0; b < 27; ++b) {
+        const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+        const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+        K[a][b] = X.S[ax][bx] * Y.M[ay][by] * Z.M[az][bz] + X.M[ax][bx] * Y.S[ay][by] * Z.M[az][bz] +
+                  X.M[ax][bx] * Y.M[ay][by] * Z.S[az][bz];
+        D[0][a][b] = X.C[ax][bx] * Y.M[ay][by] * Z.M[az][bz];
+        D[1][a][b] = X.M[ax][bx] * Y.C[ay][by] * Z.M[az][bz];
+        D[2][a][b] = X.M[ax][bx] * Y.M[ay][by] * Z.C[az][bz];
+      }
+    // F = I - Pi: column b holds the nodal values of (I - pi) phi_b.  pi phi_b
+    // is zero unless b is a corner, where it is the coarse trilinear hat
+    // (1 at b, 1/2 on the adjacent edge midpoints, 1/4 on the adjacent face
+    // centres, 1/8 at the patch centre).
+    double F[27][27];
+    const double p1[3][3] = {{1, 0, 0}, {0.5, 0, 0.5}, {0, 0, 1}};  // p1[node][corner]
+    for (int a = 0; a < 27; ++a)
+      for (int b = 0; b < 27; ++b) {
+        const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+        const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+        F[a][b] = (a == b ? 1.0 : 0.0) - p1[ax][bx] * p1[ay][by] * p1[az][bz];
+      }
+    double KF[27][27];
+    for (int a = 0; a < 27; ++a)
+      for (int b = 0; b < 27; ++b) {
+        double s = 0;
+        for (int k = 0; k < 27; ++k) s += K[a][k] * F[k][b];
+        KF[a][b] = s;
+      }
+    const double hP = std::cbrt(8.0 * hx * hy * hz);
+    const double alphaP = alpha / (visc / hP / hP);
+    for (int a = 0; a < 27; ++a)
+      for (int b = 0; b < 27; ++b) {
+        double L = 0;
+        for (int k = 0; k < 27; ++k) L += F[k][a] * KF[k][b];
+        double* e = blk[a][b];
+        for (int t = 0; t < 16; ++t) e[t] = 0.0;
+        e[0] = alphaP * L;
+        for (int d = 0; d < 3; ++d) {
+          e[0 * 4 + d + 1] = D[d][a][b];
+          e[(d + 1) * 4 + d + 1] = visc * K[a][b];
+          e[(d + 1) * 4 + 0] = -D[d][b][a];
+        }
+      }
+  }
+};
+
+// Box sizes of a patch from its lexicographic nodes; fails unless the patch
+// is an axis-aligned box split into 8 equal cells.
+int patch_box(const Mesh& M, const int32_t* P, double h[3]) {
+  const double* x0 = &M.X[3 * (int64_t)P[0]];
+  h[0] = M.X[3 * (int64_t)P[1]] - x0[0];
+  h[1] = M.X[3 * (int64_t)P[3] + 1] - x0[1];
+  h[2] = M.X[3 * (int64_t)P[9] + 2] - x0[2];
+  for (int a = 0; a < 27; ++a) {
+    const int idx[3] = {a % 3, (a / 3) % 3, a / 9};
+    for (int d = 0; d < 3; ++d) {
+      const double e = x0[d] + idx[d] * h[d];
+      if (std::fabs(M.X[3 * (int64_t)P[a] + d] - e) > 1e-12 * (std::fabs(e) + 1.0)) return -1;
+    }
+  }
+  return (h[0] > 0 && h[1] > 0 && h[2] > 0) ? 0 : -1;
+}
+
 }  // namespace
 
 struct ghs_hier {
@@ -602,6 +740,163 @@ int ghs_assemble_on(int dim, int64_t n, const double* xyz, int64_t ncells, const
                     const double* kcell, const int32_t* rowptr, const int32_t* col, int64_t ndir,
                     const int32_t* dir, double* val) {
   return assemble(dim, n, xyz, ncells, cells, kcell, rowptr, col, ndir, dir, val);
+}
+
+int ghs_level_patch_sizes(const ghs_hier* h, int l, int64_t* npatches, int64_t* nnz) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  std::vector<int32_t> P;
+  if (patches(*M, P)) return -1;
+  const int nloc = M->dim == 3 ? 27 : 9;
+  if (npatches) *npatches = (int64_t)P.size() / nloc;
+  if (nnz) {
+    std::vector<int32_t> rp, c;
+    if (structure(M->n(), (int64_t)P.size() / nloc, nloc, P.data(), rp, c)) return -1;
+    *nnz = rp.back();
+  }
+  return 0;
+}
+
+int ghs_level_patches(const ghs_hier* h, int l, int32_t* nodes) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  std::vector<int32_t> P;
+  if (patches(*M, P)) return -1;
+  std::memcpy(nodes, P.data(), P.size() * sizeof(int32_t));
+  return 0;
+}
+
+int ghs_level_patch_structure(const ghs_hier* h, int l, int32_t* rowptr, int32_t* col) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  std::vector<int32_t> P, rp, c;
+  if (patches(*M, P)) return -1;
+  const int nloc = M->dim == 3 ? 27 : 9;
+  if (structure(M->n(), (int64_t)P.size() / nloc, nloc, P.data(), rp, c)) return -1;
+  std::memcpy(rowptr, rp.data(), rp.size() * sizeof(int32_t));
+  if (col) std::memcpy(col, c.data(), c.size() * sizeof(int32_t));
+  return 0;
+}
+
+int ghs_level_assemble_stokes(const ghs_hier* h, int l, double visc, double alpha, int ncolours,
+                              const int32_t* colours, int dircomps, const double* u0, double* rhs,
+                              double* val) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  if (M->dim != 3) return fail("Stokes-LPS assembly is three-dimensional");
+  if (!(visc > 0) || !(alpha >= 0)) return fail("Stokes-LPS: visc must be > 0 and alpha >= 0");
+  if (dircomps & ~0xF) return fail("Stokes-LPS: Dirichlet component mask has bits beyond 4 components");
+  if ((u0 == nullptr) != (rhs == nullptr)) return fail("Stokes-LPS: u0 and rhs go together");
+  std::vector<int32_t> P, rp, c, d;
+  if (patches(*M, P)) return -1;
+  const int64_t n = M->n(), np = (int64_t)P.size() / 27;
+  if (structure(n, np, 27, P.data(), rp, c)) return -1;
+  // patch shapes (one per distinct box size; a uniform level has one)
+  std::vector<int32_t> shape(np);
+  std::vector<std::array<double, 3>> sizes;
+  std::vector<std::unique_ptr<StokesPatch>> elem;
+  for (int64_t p = 0; p < np; ++p) {
+    double hb[3];
+    if (patch_box(*M, &P[p * 27], hb)) return fail("patch %lld is not an axis-aligned box of 8 equal cells", (long long)p);
+    int s = 0;
+    while (s < (int)sizes.size() && !(sizes[s][0] == hb[0] && sizes[s][1] == hb[1] && sizes[s][2] == hb[2])) ++s;
+    if (s == (int)sizes.size()) {
+      if (s >= 64) return fail("Stokes-LPS: more than 64 distinct patch sizes");
+      sizes.push_back({hb[0], hb[1], hb[2]});
+      elem.emplace_back(new StokesPatch(hb[0], hb[1], hb[2], visc, alpha));
+    }
+    shape[p] = s;
+  }
+  // node -> (patch, local index), ascending patch order
+  std::vector<int32_t> pstart(n + 1, 0), plist(np * 27);
+  std::vector<int8_t> ploc(np * 27);
+  for (int64_t k = 0; k < np * 27; ++k) pstart[P[k] + 1]++;
+  for (int64_t i = 0; i < n; ++i) pstart[i + 1] += pstart[i];
+  {
+    std::vector<int32_t> fill(pstart.begin(), pstart.end() - 1);
+    for (int64_t p = 0; p < np; ++p)
+      for (int a = 0; a < 27; ++a) {
+        const int32_t k = fill[P[p * 27 + a]]++;
+        plist[k] = (int32_t)p;
+        ploc[k] = (int8_t)a;
+      }
+  }
+  dir_nodes(M, ncolours, colours, d);
+  std::vector<uint8_t> isdir(n, 0);
+  if (dircomps)
+    for (int32_t i : d) isdir[i] = 1;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : bad)
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t rs = rp[i], re = rp[i + 1];
+    double* vr = val + (int64_t)rs * 16;
+    std::fill(vr, vr + (int64_t)(re - rs) * 16, 0.0);
+    for (int32_t k = pstart[i]; k < pstart[i + 1]; ++k) {
+      const int64_t p = plist[k];
+      const int a = ploc[k];
+      const StokesPatch& E = *elem[shape[p]];
+      for (int b = 0; b < 27; ++b) {
+        const int32_t j = P[p * 27 + b];
+        const int32_t* it = std::lower_bound(c.data() + rs, c.data() + re, j);
+        if (it == c.data() + re || *it != j) {
+          bad++;
+          continue;
+        }
+        double* e = val + (int64_t)(it - c.data()) * 16;
+        for (int t = 0; t < 16; ++t) e[t] += E.blk[a][b][t];
+      }
+    }
+    if (rhs) {
+      // residual of the lifted state: rhs = -A u0 before the Dirichlet rows go
+      double r[4] = {0, 0, 0, 0};
+      for (int32_t q = rs; q < re; ++q) {
+        const double* e = val + (int64_t)q * 16;
+        const double* x = u0 + 4 * (int64_t)c[q];
+        for (int s = 0; s < 4; ++s)
+          for (int t = 0; t < 4; ++t) r[s] -= e[4 * s + t] * x[t];
+      }
+      for (int s = 0; s < 4; ++s)
+        rhs[4 * i + s] = (isdir[i] && (dircomps >> s & 1)) ? 0.0 : r[s];
+    }
+    // SparseBlockMatrix::dirichlet (src/LinAlg/sparseblockmatrix.xx:205-218,
+    // fmatrixblock.xx:54-82): rows and columns of the constrained components
+    // zero, their diagonal entries one
+    for (int32_t q = rs; q < re; ++q) {
+      double* e = val + (int64_t)q * 16;
+      for (int s = 0; s < 4; ++s) {
+        if (!(dircomps >> s & 1)) continue;
+        if (isdir[i])
+          for (int t = 0; t < 4; ++t) e[4 * s + t] = 0.0;
+        if (isdir[c[q]])
+          for (int t = 0; t < 4; ++t) e[4 * t + s] = 0.0;
+        if (isdir[i] && c[q] == i) e[4 * s + s] = 1.0;
+      }
+    }
+  }
+  if (bad) return fail("patch structure misses %d couplings", bad);
+  return 0;
+}
+
+int ghs_level_face_mass(const ghs_hier* h, int l, int colour, double* m) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  if (M->dim != 3) return fail("face masses are three-dimensional");
+  std::fill(m, m + M->n(), 0.0);
+  for (size_t f = 0; f < M->bcol.size(); ++f) {
+    if (M->bcol[f] != colour) continue;
+    const int32_t* q = &M->bnd[4 * f];
+    double e1 = 0, e2 = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double a = M->X[3 * (int64_t)q[1] + d] - M->X[3 * (int64_t)q[0] + d];
+      const double b = M->X[3 * (int64_t)q[3] + d] - M->X[3 * (int64_t)q[0] + d];
+      e1 += a * a;
+      e2 += b * b;
+    }
+    // bilinear hats on a rectangle: int phi_i ds = area / 4
+    const double w = std::sqrt(e1) * std::sqrt(e2) / 4.0;
+    for (int t = 0; t < 4; ++t) m[q[t]] += w;
+  }
+  return 0;
 }
 
 }  // extern "C"

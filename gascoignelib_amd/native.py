@@ -159,6 +159,12 @@ STRUCT_SIGNATURES = {
     "ghs_structure_from_cells": (C.c_int, [C.c_int64, C.c_int64, C.c_int, i32p, i32p, i32p]),
     "ghs_assemble_on": (C.c_int, [C.c_int, C.c_int64, f64p, C.c_int64, i32p, f64p, i32p, i32p,
                                   C.c_int64, i32p, f64p]),
+    "ghs_level_patch_sizes": (C.c_int, [vp, C.c_int, i64p, i64p]),
+    "ghs_level_patches": (C.c_int, [vp, C.c_int, i32p]),
+    "ghs_level_patch_structure": (C.c_int, [vp, C.c_int, i32p, i32p]),
+    "ghs_level_assemble_stokes": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_int, i32p,
+                                            C.c_int, f64p, f64p, f64p]),
+    "ghs_level_face_mass": (C.c_int, [vp, C.c_int, C.c_int, f64p]),
 }
 
 _libs = {}

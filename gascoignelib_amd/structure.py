@@ -87,6 +87,50 @@ class Mesh:
                                                  ptr(v, N.i32p), ptr(a, N.i32p)))
         return c2f, z, v, a
 
+    def patch_sizes(self, l):
+        npat, nnz = C.c_int64(), C.c_int64()
+        check_s(N.structure().ghs_level_patch_sizes(self.h, l, C.byref(npat), C.byref(nnz)))
+        return npat.value, nnz.value
+
+    def patches(self, l):
+        """(npatches, 27) patch nodes, patch-local lexicographic order"""
+        npat, _ = self.patch_sizes(l)
+        out = np.empty((npat, 27 if self.dim == 3 else 9), np.int32)
+        check_s(N.structure().ghs_level_patches(self.h, l, ptr(out, N.i32p)))
+        return out
+
+    def patch_structure(self, l):
+        n, _, _ = self.sizes(l)
+        _, nnz = self.patch_sizes(l)
+        rp = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        check_s(N.structure().ghs_level_patch_structure(self.h, l, ptr(rp, N.i32p),
+                                                        ptr(col, N.i32p)))
+        return rp, col
+
+    def assemble_stokes(self, l, nnz, visc, alpha, colours, dircomps, u0=None):
+        """4x4 blocks of the Stokes-LPS operator on the patch structure; with
+        u0 (n, 4) also the lifted residual -A u0 (zero in constrained entries)."""
+        cols = np.ascontiguousarray(colours, dtype=np.int32)
+        mask = 0
+        for c in dircomps:
+            mask |= 1 << int(c)
+        val = np.empty(nnz * 16)
+        rhs = None
+        if u0 is not None:
+            u0 = np.ascontiguousarray(u0, dtype=np.float64)
+            rhs = np.empty(u0.size)
+        check_s(N.structure().ghs_level_assemble_stokes(
+            self.h, l, float(visc), float(alpha), len(cols), ptr(cols, N.i32p), mask,
+            ptr(u0) if u0 is not None else None, ptr(rhs) if rhs is not None else None, ptr(val)))
+        return val, rhs
+
+    def face_mass(self, l, colour):
+        n, _, _ = self.sizes(l)
+        m = np.empty(n)
+        check_s(N.structure().ghs_level_face_mass(self.h, l, int(colour), ptr(m)))
+        return m
+
     def __del__(self):
         try:
             if self.h:
@@ -132,4 +176,63 @@ def assemble_hierarchy(M, kcell=None, colours=(1,), name="", keep_coords=True):
     h = Hierarchy(levels, dim, name)
     h.mesh = M
     h.kcell = ks
+    return h
+
+
+def inflow_profile(coords, amplitude):
+    """Parabolic inflow u = amplitude * 16 y(1-y) z(1-z) on the unit cube's
+    x = 0 face (zero on its edges)."""
+    y, z = coords[:, 1], coords[:, 2]
+    return amplitude * 16.0 * y * (1.0 - y) * z * (1.0 - z)
+
+
+def inflow_amplitude(seed):
+    """Seeded inflow amplitude, uniform in [0.5, 1.5) (SplitMix64 position 0)."""
+    from .problem import splitmix64_uniform
+    return 1.0 + 0.5 * float(splitmix64_uniform(seed, 1)[0])
+
+
+def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", drop_coarse=1,
+                     name=None, keep_coords=True):
+    """Stokes-LPS channel on the unit cube (NavierStokesLps<3> at U = 0,
+    CGDiscQ13dLps): components (p, u, v, w), 4x4 blocks on the patch
+    structure, velocity Dirichlet on colours 1 (inflow, x = 0, parabolic
+    profile of seeded amplitude) and 2 (walls, zero), do-nothing outflow on
+    colour 3 (x = 1).  The finest level carries the Newton residual of the
+    lifted inflow data: h.rhs (n*4), h.u0 (n, 4); per level h.patches
+    (npatches, 27) for the Vanka smoother, h.mass_in / h.mass_out the
+    boundary masses of the inflow and outflow faces on the finest level."""
+    M = Mesh(inp, 3, prerefine, drop_coarse)
+    nl = M.nlevels
+    colours, comps = (1, 2), (1, 2, 3)
+    amp = inflow_amplitude(seed)
+    levels, patches = [], []
+    u0 = rhs = None
+    for l in range(nl):
+        rp, col = M.patch_structure(l)
+        lift = None
+        if l == nl - 1:
+            X = M.coords(l)
+            lift = np.zeros((len(rp) - 1, 4))
+            inflow = M.dirichlet(l, (1,))
+            lift[inflow, 1] = inflow_profile(X[inflow], amp)
+            walls = M.dirichlet(l, (2,))
+            lift[walls, 1:] = 0.0
+        val, r = M.assemble_stokes(l, len(col), visc, alpha, colours, comps, lift)
+        if lift is not None:
+            u0, rhs = lift, r
+        lev = Level(n=len(rp) - 1, ncomp=4, rowptr=rp, col=col, val=val,
+                    transfer=M.transfer(l) if l > 0 else None, hn_kind=13,
+                    dirichlet=M.dirichlet(l, colours),
+                    dirichlet_comps=np.asarray(comps, np.int32),
+                    coords=M.coords(l) if keep_coords else None)
+        levels.append(lev)
+        patches.append(M.patches(l))
+    h = Hierarchy(levels, 3, name or f"stokes:{inp}:{prerefine}")
+    h.mesh = M
+    h.patches = patches
+    h.u0, h.rhs, h.amplitude = u0, rhs, amp
+    h.visc, h.alpha = visc, alpha
+    h.mass_in = M.face_mass(nl - 1, 1)
+    h.mass_out = M.face_mass(nl - 1, 3)
     return h

@@ -71,6 +71,33 @@ int ghs_assemble_on(int dim, int64_t n, const double* xyz, int64_t ncells, const
                     const double* kcell, const int32_t* rowptr, const int32_t* col,
                     int64_t ndir, const int32_t* dir, double* val);
 
+/* Patches of level l (l must come from a refinement): the 2x2x2 (2x2) children
+ * of one coarser cell, nodes in patch-local lexicographic order
+ * a + 3b + 9c (x fastest), the element order of Gascoigne's patch dof
+ * handler (used by VankaSmoother::ConstructStructure,
+ * src/LinAlg/vankasmoother.cc:47-60).  nodes[npatches*27] (9 in 2D);
+ * nnz = entries of the patch-based CSR structure (CGDisc<DIM,2,...>::Structure
+ * of the patch discretisations, src/DofHandler/cgdisc.h:299-315,1314-1319). */
+int ghs_level_patch_sizes(const ghs_hier* h, int l, int64_t* npatches, int64_t* nnz);
+int ghs_level_patches(const ghs_hier* h, int l, int32_t* nodes);
+int ghs_level_patch_structure(const ghs_hier* h, int l, int32_t* rowptr, int32_t* col);
+/* Stokes with local projection stabilisation on patches (NavierStokesLps<3>
+ * at U = 0, symmetrictensor 0, discretisation CGDiscQ13dLps:
+ * src/Problems/navierstokes.h:92-270, src/DofHandler/elementlpsintegrator.h),
+ * components (p, u, v, w), 4x4 blocks val[nnz*16] (row-major per block) on
+ * the patch structure.  Nodes on the given colours get strong Dirichlet
+ * conditions for the components in the bit mask dircomps
+ * (SparseBlockMatrix::dirichlet).  With u0 (n*4, the Dirichlet lifting) the
+ * residual rhs = -A u0 of the unconstrained operator is returned, zero in the
+ * constrained entries (the Newton residual StdMultiLevelSolver hands to the
+ * linear solve); u0 and rhs are both NULL or both given. */
+int ghs_level_assemble_stokes(const ghs_hier* h, int l, double visc, double alpha, int ncolours,
+                              const int32_t* colours, int dircomps, const double* u0, double* rhs,
+                              double* val);
+/* m[i] = integral of the bilinear hat of node i over the rectangular boundary
+ * faces of the given colour (boundary mass row sums), m[nnodes] */
+int ghs_level_face_mass(const ghs_hier* h, int l, int colour, double* m);
+
 #ifdef __cplusplus
 }
 #endif

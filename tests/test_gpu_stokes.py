@@ -1,0 +1,120 @@
+"""Stokes-LPS channel on the GPU against the oracle.
+
+The problem: NavierStokesLps<3> at U = 0 on the unit cube (components p, u,
+v, w; 4x4 blocks on the patch structure of CGDiscQ13dLps), parabolic inflow
+of seeded amplitude at x = 0, no-slip walls, do-nothing outflow at x = 1,
+multigrid with the Vanka smoother on the 2x2x2-cell patches (108 local
+unknowns) and a direct solve on the 2x2x2-cell level.
+
+FP64: the V-cycle agrees with the oracle bit for bit when every level is
+smoothed (the block products are taken in the reference's order, the patch
+factors with the oracle's pivots) and within 1e-12 with a direct coarse level;
+Newton/GMRES iterates to 1e-12, iteration counts within one.  Mixed precision
+(FP32 Vanka V-cycle inside FP64 flexible GMRES): error against the FP64 oracle
+solve within 1e-8.  Mass conservation is checked on every solve: the discrete
+divergence rows of the constant test function sum to the boundary flux, and
+the LPS term vanishes on constants, so the outflow flux equals the inflow
+flux of the lifted data up to the solver tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+from gascoignelib_amd import hip as H
+from gascoignelib_amd.structure import stokes_hierarchy
+
+pytestmark = pytest.mark.gpu
+
+MGP = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=4, omega=0.8, mgomega=1.0, ndirect=200)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = H.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def h8():
+    return stokes_hierarchy(3, seed=5)  # 8^3 cells, 2,916 unknowns
+
+
+@pytest.fixture(scope="module")
+def h16():
+    return stokes_hierarchy(4, seed=7)  # 16^3 cells, 19,652 unknowns
+
+
+def flux_error(h, x):
+    u = x.reshape(-1, 4) + h.u0
+    fin = h.mass_in @ h.u0[:, 1]
+    return abs(h.mass_out @ u[:, 1] - fin) / fin
+
+
+@pytest.mark.parametrize("ndirect", [1, 200])
+@pytest.mark.parametrize("order", [None, "lexicographic"])
+def test_stokes_vcycle_vs_oracle(ctx, h8, order, ndirect):
+    """ndirect 1: the 2x2x2-cell level is one Vanka patch, smoothed like the
+    others, and the V-cycle is bit-exact; ndirect 200: that level is solved by
+    the dense direct solve (LU in the oracle, an explicit inverse on the
+    device), within 1e-12"""
+    print(f"[stokes V-cycle 8^3 order={order} ndirect={ndirect}]", flush=True)
+    f = h8.rhs
+    p = dict(MGP, ndirect=ndirect)
+    om = O.MG(h8, smoother="vanka", patches=h8.patches, **p)
+    xo = np.zeros_like(f)
+    om.precondition(xo, f)
+    mg = H.CudaMultiLevelSolver(ctx, h8, linear_smooth="vanka", patches=h8.patches, order=order,
+                                **p)
+    X = mg.new_vector()
+    mg.precondition(X, H.CudaVectorInterface.from_global(ctx, f, 4))
+    x = X.copy_back()
+    if ndirect == 1:
+        assert np.array_equal(x, xo)
+    assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.norm(xo)
+    assert mg.mgsteps == om.mgsteps
+
+
+def test_stokes_solve_vs_oracle(ctx, h16):
+    print("[stokes Newton/GMRES 16^3 FP64]", flush=True)
+    f = h16.rhs
+    om = O.MG(h16, smoother="vanka", patches=h16.patches, **MGP)
+    uo = np.zeros_like(f)
+    rc, orep = om.newton(uo, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30),
+                         norm_kind=1)
+    assert rc == 0
+    mg = H.CudaMultiLevelSolver(ctx, h16, linear_smooth="vanka", patches=h16.patches,
+                                order="lexicographic", **MGP)
+    U = mg.new_vector()
+    rep = mg.Solve(U, H.CudaVectorInterface.from_global(ctx, f, 4), H.CGInfo(1e-10, 1e-30, 100),
+                   H.NLInfo(1e-10, 1e-30, 30), norm_kind=1)
+    u = U.copy_back()
+    assert H.N.NL_STATUS[rep.status] == "converged"
+    assert rep.newton_iterations == orep.newton_iterations
+    assert abs(rep.gmres_iterations - orep.gmres_iterations) <= 1
+    assert np.linalg.norm(u - uo) <= 1e-12 * np.linalg.norm(uo)
+    assert flux_error(h16, u) < 1e-8
+
+
+def test_stokes_mixed_precision(ctx, h16):
+    print("[stokes FP32 Vanka V-cycle in FP64 FGMRES 16^3]", flush=True)
+    f = h16.rhs
+    om = O.MG(h16, smoother="vanka", patches=h16.patches, **MGP)
+    uo = np.zeros_like(f)
+    om.newton(uo, f, O.cginfo(1e-12, 1e-30, 100), O.nlinfo(1e-12, 1e-30, 30), norm_kind=1)
+    mg = H.CudaMultiLevelSolver(ctx, h16, linear_smooth="vanka", patches=h16.patches,
+                                order="lexicographic", prec="f32", krylov="fgmres", **MGP)
+    U = mg.new_vector()
+    rep = mg.Solve(U, H.CudaVectorInterface.from_global(ctx, f, 4),
+                   H.CGInfo(0.9e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30), norm_kind=1)
+    assert H.N.NL_STATUS[rep.status] == "converged"
+    u = U.copy_back()
+    assert np.linalg.norm(u - uo) <= 1e-8 * np.linalg.norm(uo)
+    assert flux_error(h16, u) < 1e-8
+    # the FP64 residual of the returned iterate is what the outer loop certified
+    A = H.CudaCSRMatrixInterface(ctx, h16.fine.rowptr, h16.fine.col, h16.fine.val, 4)
+    R = H.CudaVectorInterface.from_global(ctx, f, 4)
+    A.vmult(R, U, -1.0, 1.0)
+    r = R.copy_back()
+    r.reshape(-1, 4)[h16.fine.dirichlet, 1:] = 0.0
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(f)
