@@ -226,8 +226,32 @@ int vec_fill_zero(gh_ctx* ctx, int prec, int64_t size, void* x) {
 
 // ------------------------------------------------------------------ SpMV
 // block matrices (ncomp > 1): one thread per block row component
+template <int B, typename XT, typename Acc, class Op>
+static void launch_bsell(gh_ctx* ctx, const BSell& S, const XT* x, Op op) {
+  const unsigned grid = grid_for(S.nslices * 64);
+  if (S.prec == GH_F64)
+    hipLaunchKernelGGL((k_bsell<B, double, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       S.nslices, S.sptr, S.row, S.rlen, S.col, (const double*)S.val, x, op);
+  else if (S.prec == GH_F32)
+    hipLaunchKernelGGL((k_bsell<B, float, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       S.nslices, S.sptr, S.row, S.rlen, S.col, (const float*)S.val, x, op);
+  else
+    hipLaunchKernelGGL((k_bsell<B, __half, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
+                       S.nslices, S.sptr, S.row, S.rlen, S.col, (const __half*)S.val, x, op);
+}
+
 template <typename XT, typename Acc, class Op>
 static int dispatch_val(gh_ctx* ctx, const gh_csr* A, const XT* x, Op op) {
+  // component-row kernel where it is built and x is aligned for its block loads
+  const BSell& S = A->bsell;
+  if (S.B == A->ncomp && ((uintptr_t)x % (sizeof(XT) * (S.B == 4 ? 4 : 1))) == 0) {
+    if (S.nslices == 0) return GH_OK;
+    if (S.B == 4) launch_bsell<4, XT, Acc>(ctx, S, x, op);
+    else if (S.B == 3) launch_bsell<3, XT, Acc>(ctx, S, x, op);
+    else launch_bsell<2, XT, Acc>(ctx, S, x, op);
+    GH_HIP(hipGetLastError());
+    return GH_OK;
+  }
   unsigned grid = grid_for(A->n * A->ncomp);
   if (A->prec == GH_F64)
     hipLaunchKernelGGL((k_bsr<double, XT, Acc, Op>), dim3(grid), dim3(kBlock), 0, ctx->stream,
@@ -665,6 +689,113 @@ void sell_free(Sell* S) {
   S->val = nullptr;
 }
 
+int bsell_build(gh_ctx* ctx, int64_t n, int B, const int32_t* rowptr, const int32_t* col,
+                const double* vals, int prec, BSell* S) {
+  const int BB = B * B;
+  const int64_t nr = n * B;
+  S->B = B;
+  S->nrows = nr;
+  S->prec = prec;
+  auto kept = [&](int64_t q, int r) {
+    const double* m = vals + q * BB + (int64_t)r * B;
+    for (int c = 0; c < B; ++c)
+      if (m[c] != 0.0) return true;
+    return false;
+  };
+  std::vector<uint16_t> len(nr);
+  int64_t stored = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int r = 0; r < B; ++r) {
+      int64_t c = 0;
+      for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) c += kept(q, r);
+      if (c > 65535) return fail(GH_EINVAL, "block row %lld has more than 65535 entries", (long long)i);
+      len[i * B + r] = (uint16_t)c;
+      stored += c;
+    }
+  // longest rows first inside each window, so that slices hold rows of equal length
+  std::vector<int32_t> order(nr);
+  for (int64_t t = 0; t < nr; ++t) order[t] = (int32_t)t;
+  for (int64_t w0 = 0; w0 < nr; w0 += kBSellWindow) {
+    const int64_t w1 = std::min<int64_t>(nr, w0 + kBSellWindow);
+    std::stable_sort(order.begin() + w0, order.begin() + w1,
+                     [&](int32_t a, int32_t b) { return len[a] > len[b]; });
+  }
+  S->nslices = (nr + 63) / 64;
+  std::vector<uint32_t> sptr(S->nslices + 1, 0);
+  std::vector<int32_t> lrow(S->nslices * 64, -1);
+  std::vector<uint16_t> llen(S->nslices * 64, 0);
+  uint64_t units = 0;
+  for (int64_t s = 0; s < S->nslices; ++s) {
+    int w = 0;
+    for (int l = 0; l < 64 && s * 64 + l < nr; ++l) {
+      const int32_t t = order[s * 64 + l];
+      lrow[s * 64 + l] = t;
+      llen[s * 64 + l] = len[t];
+      w = std::max<int>(w, len[t]);
+    }
+    sptr[s] = (uint32_t)units;
+    units += (uint64_t)w;
+    if (units > 0xFFFFFFFFull) return fail(GH_EINVAL, "block operator too large for its slot offsets");
+  }
+  sptr[S->nslices] = (uint32_t)units;
+  S->slots = (int64_t)units * 64;
+  S->stored = stored;
+  std::vector<int32_t> scol(std::max<int64_t>(S->slots, 1), 0);
+  std::vector<double> sval(std::max<int64_t>(S->slots * B, 1), 0.0);
+  for (int64_t s = 0; s < S->nslices; ++s)
+    for (int l = 0; l < 64; ++l) {
+      const int32_t t = lrow[s * 64 + l];
+      if (t < 0) continue;
+      const int64_t i = t / B;
+      const int r = (int)(t - i * B);
+      int64_t k = 0;
+      for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        if (!kept(q, r)) continue;
+        const int64_t slot = (int64_t)sptr[s] + k;
+        scol[slot * 64 + l] = col[q];
+        for (int c = 0; c < B; ++c)
+          sval[(slot * B + c) * 64 + l] = vals[(int64_t)q * BB + (int64_t)r * B + c];
+        ++k;
+      }
+    }
+  GH_TRY(upload(ctx, &S->sptr, sptr.data(), sptr.size()));
+  GH_TRY(upload(ctx, &S->row, lrow.data(), lrow.size()));
+  GH_TRY(upload(ctx, &S->rlen, llen.data(), llen.size()));
+  GH_TRY(upload(ctx, &S->col, scol.data(), scol.size()));
+  double* tmp = nullptr;
+  GH_TRY(upload(ctx, &tmp, sval.data(), sval.size()));
+  if (prec == GH_F64) {
+    S->val = tmp;
+  } else {
+    const size_t m = sval.size();
+    if (hipMalloc(&S->val, m * (prec == GH_F32 ? 4 : 2)) != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(GH_ENOMEM, "bsell_build: allocation failed");
+    }
+    if (prec == GH_F32)
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride((int64_t)m)), dim3(kBlock), 0,
+                         ctx->stream, (int64_t)m, (float*)S->val, (const double*)tmp);
+    else
+      hipLaunchKernelGGL((k_convert<__half, double>), dim3(grid_stride((int64_t)m)), dim3(kBlock),
+                         0, ctx->stream, (int64_t)m, (__half*)S->val, (const double*)tmp);
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(tmp);
+  }
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+void bsell_free(BSell* S) {
+  for (void* p : {(void*)S->sptr, (void*)S->row, (void*)S->rlen, (void*)S->col, S->val})
+    if (p) (void)hipFree(p);
+  S->sptr = nullptr;
+  S->row = nullptr;
+  S->rlen = nullptr;
+  S->col = nullptr;
+  S->val = nullptr;
+  S->B = 0;
+}
+
 template <typename D, typename S>
 __global__ __launch_bounds__(kBlock) void k_permute(int64_t n, int ncomp,
                                                     const int32_t* __restrict__ perm,
@@ -1023,6 +1154,7 @@ int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, co
   if (!rc && ncomp == 1) {
     rc = sell_build(ctx, n, rowptr, reinterpret_cast<const uint32_t*>(col), val, prec, &A->sell);
   }
+  if (!rc && ncomp >= 2 && ncomp <= 4) rc = bsell_build(ctx, n, ncomp, rowptr, col, val, prec, &A->bsell);
   int sing = 0;
   if (!rc && hipMemcpyAsync(&sing, dsing, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
     rc = GH_EHIP;
@@ -1058,6 +1190,7 @@ int gh_csr_destroy(gh_csr* A) {
                   (void*)A->t_col, (void*)A->t_pos})
     if (p) (void)hipFree(p);
   sell_free(&A->sell);
+  bsell_free(&A->bsell);
   delete A;
   ctx_release(ctx);
   return GH_OK;
@@ -1076,6 +1209,13 @@ int gh_csr_layout(const gh_csr* A, int64_t* out) {
     out[4] = A->sell.patterned;
     out[5] = A->sell.ntab;
     out[6] = A->sell.col_bytes;
+  } else if (A->bsell.B) {
+    // component rows: entries are block rows (B values each), 32-bit columns
+    out[0] = A->bsell.stored;
+    out[1] = A->bsell.slots;
+    out[2] = A->bsell.nslices;
+    out[3] = out[4] = out[5] = 0;
+    out[6] = A->bsell.stored * 4;
   } else {
     out[0] = out[1] = A->nnz;
     out[2] = out[3] = out[4] = out[5] = 0;

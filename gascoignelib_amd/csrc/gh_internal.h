@@ -21,7 +21,8 @@ struct gh_vanka {
   int64_t n = 0, npatches = 0;
   int32_t* nodes = nullptr;   // npatches * psize
   void* lu = nullptr;         // npatches * m * m, column-major, L unit lower
-  int32_t* piv = nullptr;     // npatches * m
+  int32_t* piv = nullptr;     // npatches * m: row exchanges, then (after set-up) the
+                              // global entry gathered into each factor row
   int32_t* nptr = nullptr;    // n + 1
   int32_t* nent = nullptr;    // patch * psize + local node, ascending patch per node
   void* xp = nullptr;         // npatches * m local solutions
@@ -106,6 +107,32 @@ struct Sell {
   int prec = GH_F64;
 };
 
+// Block matrices (B = 2, 3, 4) as component rows: row t = i*B + r of block
+// row i lists, in the block row's column order, the blocks whose row r is not
+// all zero, each entry = (block column, the B values of row r).  Component
+// rows are sorted by length (descending, stable) inside windows of
+// kBSellWindow rows and cut into slices of 64; one lane runs one component
+// row.  Entry k of lane l of slice s: column at (sptr[s] + k) * 64 + l, value
+// c at ((sptr[s] + k) * B + c) * 64 + l (each value load of a wavefront reads
+// 64 consecutive elements).
+constexpr int64_t kBSellWindow = 4096;
+struct BSell {
+  int B = 0;              // 0: not built (the CSR kernels run)
+  int64_t nrows = 0;      // n * B component rows
+  int64_t nslices = 0;
+  int64_t slots = 0;      // padded entries (multiple of 64)
+  int64_t stored = 0;     // entries kept
+  uint32_t* sptr = nullptr;  // nslices+1, in units of 64 entries
+  int32_t* row = nullptr;    // nslices*64 component row of each lane (-1: none)
+  uint16_t* rlen = nullptr;  // nslices*64 entries of each lane
+  int32_t* col = nullptr;    // slots block columns
+  void* val = nullptr;       // slots*B values in `prec`
+  int prec = GH_F64;
+};
+int bsell_build(gh_ctx* ctx, int64_t n, int B, const int32_t* rowptr, const int32_t* col,
+                const double* vals, int prec, BSell* S);
+void bsell_free(BSell* S);
+
 // Host-side layout from CSR.  With vals (double) the operator is valued: exact
 // zeros are dropped (each row keeps its other entries in order, so every row
 // sum takes the same values in the same order) and values are stored as
@@ -150,6 +177,7 @@ struct gh_csr {
   void* dinv = nullptr;  // n*B*B inverse diagonal blocks: double (F64) or float
   int seg = 8;           // lanes per row in the scalar CSR kernels
   gh::Sell sell;         // SELL-64 copy used by the scalar (ncomp == 1) kernels
+  gh::BSell bsell;       // component-row copy used by the block (ncomp 2..4) products
   // host copies kept for derived structures (transpose, coarse direct solve)
   std::vector<int32_t> h_rowptr, h_col;
   std::vector<double> h_val;

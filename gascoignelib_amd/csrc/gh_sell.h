@@ -11,6 +11,8 @@
 // src/Discretization/Q1/mginterpolatormatrix.cc:123-147).
 #pragma once
 
+#include <type_traits>
+
 #include "gh_kernels.h"
 
 namespace gh {
@@ -124,6 +126,92 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
   const unsigned q = G / kXcds, r = G % kXcds;
   const unsigned xcd = b % kXcds, k = b / kXcds;
   return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// ------------------------------------------------- block matrices (BSell)
+// B consecutive values of one x block, widened to the accumulator type;
+// 16-byte vector loads where the data allow it (the host checks the alignment
+// before it picks this kernel).
+template <int B, typename T, typename Acc>
+__device__ __forceinline__ void load_block_row(const T* __restrict__ p, Acc (&o)[B]) {
+  if constexpr (B == 4 && std::is_same<T, double>::value) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 a = *reinterpret_cast<const d2*>(p), b = *reinterpret_cast<const d2*>(p + 2);
+    o[0] = (Acc)a.x; o[1] = (Acc)a.y; o[2] = (Acc)b.x; o[3] = (Acc)b.y;
+  } else if constexpr (B == 4 && std::is_same<T, float>::value) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 a = *reinterpret_cast<const f4*>(p);
+    o[0] = (Acc)a.x; o[1] = (Acc)a.y; o[2] = (Acc)a.z; o[3] = (Acc)a.w;
+  } else {
+#pragma unroll
+    for (int c = 0; c < B; ++c) o[c] = (Acc)ld(p + c);
+  }
+}
+
+// One lane per component row t = i*B + r: for each kept block of row i (in
+// the row's column order) sum = sum_c A(r, c) x_c from zero, then
+// acc = op.step(acc, sum) -- the per-block sums of k_bsr, in the same order,
+// so both kernels return the same bits.  Only all-zero block rows are left
+// out (their sum is an exact zero).
+template <int B, typename VT, typename XT, typename Acc, class Op>
+__global__ __launch_bounds__(kBlock) void k_bsell(int64_t nslices, const uint32_t* __restrict__ sptr,
+                                                  const int32_t* __restrict__ row,
+                                                  const uint16_t* __restrict__ rlen,
+                                                  const int32_t* __restrict__ col,
+                                                  const VT* __restrict__ val,
+                                                  const XT* __restrict__ x, Op op) {
+  const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int64_t s = g >> 6;
+  if (s >= nslices) return;
+  const int32_t t = row[g];
+  if (t < 0) return;
+  const int len = rlen[g];
+  const int64_t p0 = (int64_t)sptr[s] * 64 + (threadIdx.x & 63);
+  Acc acc = op.init(t);
+  const int lane = threadIdx.x & 63;
+  const int64_t v0 = (int64_t)sptr[s] * B * 64 + lane;  // value c of entry k: v0 + (k*B + c)*64
+  int k = 0;
+  // Four entries per step.  The step's value and x loads are all issued
+  // before its arithmetic (the scheduling barrier keeps the compiler from
+  // serialising the x gathers to save registers), and the next step's columns
+  // are fetched one step ahead.
+  int32_t c[4];
+  if (len >= 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = col[p0 + (int64_t)u * 64];
+  }
+  for (; k + 4 <= len; k += 4) {
+    Acc m[4][B], q[4][B];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int cc = 0; cc < B; ++cc) m[u][cc] = (Acc)ld(val + v0 + ((int64_t)(k + u) * B + cc) * 64);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) load_block_row<B>(x + (int64_t)c[u] * B, q[u]);
+    if (k + 8 <= len) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = col[p0 + (int64_t)(k + 4 + u) * 64];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      Acc sum = (Acc)0;
+#pragma unroll
+      for (int cc = 0; cc < B; ++cc) sum = add_rn(sum, mul_rn(m[u][cc], q[u][cc]));
+      acc = op.step(acc, sum);
+    }
+  }
+  for (; k < len; ++k) {
+    Acc m[B], q[B];
+#pragma unroll
+    for (int cc = 0; cc < B; ++cc) m[cc] = (Acc)ld(val + v0 + ((int64_t)k * B + cc) * 64);
+    load_block_row<B>(x + (int64_t)col[p0 + (int64_t)k * 64] * B, q);
+    Acc sum = (Acc)0;
+#pragma unroll
+    for (int cc = 0; cc < B; ++cc) sum = add_rn(sum, mul_rn(m[cc], q[cc]));
+    acc = op.step(acc, sum);
+  }
+  op.finish(t, acc);
 }
 
 // One row of a valued operator: entries k = 0..len-1 at stride kSlice, column

@@ -106,8 +106,8 @@ def lexicographic_order(coords):
 
 def permuted_csr(level, order):
     """The level's matrix with rows and columns renumbered (internal position
-    k = node order[k]), columns sorted, explicit zeros of the pattern kept:
-    the layout the ordered solver works in."""
+    k = node order[k]), columns sorted, explicit zeros of the pattern kept,
+    blocks moved whole: the layout the ordered solver works in."""
     n = level.n
     inv = np.empty(n, np.int64)
     inv[np.asarray(order)] = np.arange(n)
@@ -118,8 +118,9 @@ def permuted_csr(level, order):
     rowptr = np.zeros(n + 1, np.int64)
     np.add.at(rowptr, r_new + 1, 1)
     rowptr = np.cumsum(rowptr)
+    vals = np.asarray(level.val).reshape(len(level.col), -1)  # one row per (block) entry
     return (rowptr.astype(np.int32), c_new[idx].astype(np.int32),
-            np.ascontiguousarray(level.val[idx], dtype=np.float64))
+            np.ascontiguousarray(vals[idx].reshape(-1), dtype=np.float64))
 
 
 def splitmix64_uniform(seed, n, offset=0):

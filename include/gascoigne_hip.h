@@ -113,8 +113,11 @@ int gh_csr_prec(const gh_csr* A);
  * of A x are unchanged), out[1] padded slots, out[2] slices of 64 rows,
  * out[3] slices with 16-bit column offsets, out[4] slices whose rows all follow
  * shared relative column patterns (no column storage), out[5] entries of the
- * pattern table, out[6] column bytes one sweep reads.  Block matrices:
- * out[0] = out[1] = nnz blocks, out[2..5] = 0, out[6] = 4 nnz. */
+ * pattern table, out[6] column bytes one sweep reads.  Block matrices with
+ * 2..4 components (stored as component rows): out[0] block rows kept (row r
+ * of a block is left out when it is all zero), out[1] padded slots, out[2]
+ * slices, out[3..5] = 0, out[6] = 4 out[0]; other block sizes: out[0] =
+ * out[1] = nnz blocks, out[2..5] = 0, out[6] = 4 nnz. */
 int gh_csr_layout(const gh_csr* A, int64_t* out);
 /* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
