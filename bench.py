@@ -44,18 +44,23 @@ def time_kernel(ctx, fn, reps):
     """average duration of fn() (one kernel launch on ctx's stream), HIP events"""
     ev = hip_events()
     a, b = C.c_void_p(), C.c_void_p()
-    ev.hipEventCreate(C.byref(a))
-    ev.hipEventCreate(C.byref(b))
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): the event runtime is not the library's")
+
+    ok(ev.hipEventCreate(C.byref(a)), "hipEventCreate")
+    ok(ev.hipEventCreate(C.byref(b)), "hipEventCreate")
     for _ in range(3):
         fn()
     ctx.sync()
-    ev.hipEventRecord(a, ctx.stream)
+    ok(ev.hipEventRecord(a, ctx.stream), "hipEventRecord")
     for _ in range(reps):
         fn()
-    ev.hipEventRecord(b, ctx.stream)
-    ev.hipEventSynchronize(b)
+    ok(ev.hipEventRecord(b, ctx.stream), "hipEventRecord")
+    ok(ev.hipEventSynchronize(b), "hipEventSynchronize")
     ms = C.c_float()
-    ev.hipEventElapsedTime(C.byref(ms), a, b)
+    ok(ev.hipEventElapsedTime(C.byref(ms), a, b), "hipEventElapsedTime")
     ev.hipEventDestroy(a)
     ev.hipEventDestroy(b)
     return ms.value / reps * 1e-3
@@ -409,14 +414,17 @@ def bench_stokes(args):
     V(2,2) cycles of the Vanka smoother (omega 0.8, 2x2x2-cell patches, 108
     unknowns each): FP64 (left-preconditioned GMRES(10) in Newton, the
     reference's algorithm) against mixed precision (FP32 matrices, FP32 patch
-    factors and FP32 V-cycle inside FP64 flexible GMRES(10))."""
+    factors and FP32 V-cycle inside FP64 flexible GMRES(10)).  Beside each
+    solve, the gradient of the outflow flux with respect to the inflow data
+    (adjoint solve on the transposed levels + one transposed product) is timed
+    and checked against the inflow-face masses."""
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.structure import stokes_hierarchy
 
     cells = 64 if args.cells == 128 else args.cells
     pre = int(round(np.log2(cells)))
     t0 = time.perf_counter()
-    hier = stokes_hierarchy(pre, seed=2024)
+    hier = stokes_hierarchy(pre, seed=2024, with_full=True)
     t_struct = time.perf_counter() - t0
     fine = hier.fine
     nunk = fine.n * 4
@@ -424,44 +432,55 @@ def bench_stokes(args):
     print(f"# stokes {cells}^3: {fine.n} nodes, {nunk} unknowns, {fine.nnz} blocks, "
           f"{hier.nlevels} levels, built in {t_struct:.1f}s", file=sys.stderr, flush=True)
     ctx = H.Context(0)
-    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=4, omega=0.8, mgomega=1.0,
-                  ndirect=200, linear_smooth="vanka", patches=hier.patches,
-                  order=None if args.order == "none" else args.order)
+    from gascoignelib_amd.stokes import StokesFluxProblem
+    order = None if args.order == "none" else args.order
     t0 = time.perf_counter()
-    solvers = {"fp64": (H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common), 1e-10),
-               "mixed": (H.CudaMultiLevelSolver(ctx, hier, prec="f32", krylov="fgmres", **common),
-                         0.9e-10)}
+    # forward and adjoint solvers of each precision (the adjoint on the
+    # transposed levels, its Vanka factors those of the transposed patches)
+    probs = {"fp64": StokesFluxProblem(ctx, hier, prec="f64", krylov="gmres", tol=1e-10,
+                                       order=order),
+             "mixed": StokesFluxProblem(ctx, hier, prec="f32", krylov="fgmres", tol=1e-10,
+                                        order=order)}
     t_setup = time.perf_counter() - t0
     F = H.CudaVectorInterface.from_global(ctx, f, 4)
     U = H.CudaVectorInterface(ctx, fine.n, 4)
+    grad_want = hier.mass_in[hier.inflow_nodes]
 
     def step(tag):
-        mg, tol = solvers[tag]
+        P = probs[tag]
         U.zero()
-        rep = mg.Solve(U, F, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
-                       norm_kind=1)
+        rep = P.mg.Solve(U, F, H.CGInfo(P.lin, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
+                         norm_kind=1)
         if H.N.NL_STATUS[rep.status] != "converged":
             raise RuntimeError(f"{tag} Stokes solve did not converge")
         return rep
 
-    res = {}
-    for tag in ("fp64", "mixed"):
+    def timed(fn, label):
         for _ in range(args.warmup):
-            step(tag)
-        ts = []
+            fn()
+        ts, out = [], None
         for i in range(args.steps):
             ctx.sync()
             t = time.perf_counter()
-            rep = step(tag)
+            out = fn()
             ctx.sync()
             ts.append(time.perf_counter() - t)
-            print(f"# {tag} step {i}: {ts[-1] * 1e3:.1f} ms (newton {rep.newton_iterations}, "
-                  f"gmres {rep.gmres_iterations})", file=sys.stderr, flush=True)
+            print(f"# {label} step {i}: {ts[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        return float(np.median(ts)) * 1e3, out
+
+    res = {}
+    for tag in ("fp64", "mixed"):
+        ms, rep = timed(lambda: step(tag), f"{tag} forward")
         u = U.copy_back()
-        res[tag] = dict(ms=float(np.median(ts)) * 1e3, gmres=rep.gmres_iterations,
-                        newton=rep.newton_iterations, u=u)
+        ms_adj, (grad, _, arep) = timed(lambda: probs[tag].gradient(), f"{tag} adjoint")
+        res[tag] = dict(ms=ms, gmres=rep.gmres_iterations, newton=rep.newton_iterations, u=u,
+                        ms_adjoint=ms_adj, adjoint_gmres=arep.gmres_iterations,
+                        grad_err=float(np.abs(grad - grad_want).max() / grad_want.max()))
+        print(f"# {tag}: forward {ms:.1f} ms (gmres {rep.gmres_iterations}), flux gradient "
+              f"{ms_adj:.1f} ms (gmres {arep.gmres_iterations}, error vs inflow-face masses "
+              f"{res[tag]['grad_err']:.2e})", file=sys.stderr, flush=True)
     # true FP64 residual of the mixed result and mass conservation of both
-    A = solvers["fp64"][0].fine_matrix
+    A = probs["fp64"].mg.fine_matrix
     U.assign(res["mixed"]["u"])
     R = H.CudaVectorInterface.from_global(ctx, f, 4)
     A.vmult(R, U, -1.0, 1.0)
@@ -528,15 +547,24 @@ def bench_stokes(args):
         "fp64": {"unknowns_per_s": nunk / (res["fp64"]["ms"] * 1e-3),
                  "ms_per_step": res["fp64"]["ms"], "newton": res["fp64"]["newton"],
                  "gmres": res["fp64"]["gmres"],
-                 "flux_error": abs(flux(res["fp64"]["u"]) - fin) / fin},
+                 "flux_error": abs(flux(res["fp64"]["u"]) - fin) / fin,
+                 "flux_gradient_ms": res["fp64"]["ms_adjoint"],
+                 "adjoint_gmres": res["fp64"]["adjoint_gmres"],
+                 "flux_gradient_error": res["fp64"]["grad_err"]},
         "mixed": {"unknowns_per_s": nunk / (res["mixed"]["ms"] * 1e-3),
                   "ms_per_step": res["mixed"]["ms"], "newton": res["mixed"]["newton"],
                   "gmres": res["mixed"]["gmres"],
+                  "flux_gradient_ms": res["mixed"]["ms_adjoint"],
+                  "adjoint_gmres": res["mixed"]["adjoint_gmres"],
+                  "flux_gradient_error": res["mixed"]["grad_err"],
                   "rel_error_vs_fp64": float(np.linalg.norm(res["mixed"]["u"] - res["fp64"]["u"])
                                              / np.linalg.norm(res["fp64"]["u"])),
                   "true_rel_residual": float(np.linalg.norm(r) / np.linalg.norm(f)),
                   "flux_error": abs(flux(res["mixed"]["u"]) - fin) / fin},
         "speedup_mixed_vs_fp64": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "flux_gradient": "dJ/dg of the outflow flux w.r.t. the inflow u_x values: adjoint "
+                         "solve on the transposed levels + (j - A^T z); error = max deviation "
+                         "from the inflow-face masses that mass conservation fixes",
         "kernels": kern,
         "setup_s": {"structure": t_struct, "device": t_setup},
     }

@@ -1,4 +1,6 @@
-"""Coefficient gradients through the HIP forward and adjoint solves.
+"""Gradients through the HIP forward and adjoint solves: per-cell coefficient
+gradients of a diffusion functional, and the gradient of the Stokes channel's
+outflow flux with respect to its inflow data (StokesFluxProblem, InflowFlux).
 
 Problem: -div(k grad u) = f on a Q1 box mesh with one coefficient k_c per cell,
 homogeneous Dirichlet data, and the functional J(k) = g . u(k) (g = dJ/du, for
@@ -18,6 +20,7 @@ import numpy as np
 
 from . import hip as H
 from .problem import lattice_rhs, lumped_mass, smooth_weight
+from .stokes import StokesFluxProblem  # noqa: F401  (re-exported for the torch wrapper)
 from .structure import Mesh, assemble_hierarchy
 
 
@@ -102,5 +105,21 @@ try:
         def backward(fctx, grad_out):
             _, dk = fctx.problem.gradient(fctx.hier, fctx.U)
             return grad_out * torch.from_numpy(dk), None
+
+    class InflowFlux(torch.autograd.Function):
+        """J = InflowFlux.apply(g, problem): outflow flux of the Stokes channel
+        for a float64 tensor g of inflow values; backward runs the adjoint."""
+
+        @staticmethod
+        def forward(fctx, g, problem):
+            fctx.problem = problem
+            return torch.tensor(problem.J(g.detach().cpu().numpy().astype(np.float64)),
+                                dtype=torch.float64)
+
+        @staticmethod
+        def backward(fctx, grad_out):
+            dg, _, _ = fctx.problem.gradient()
+            return grad_out * torch.from_numpy(dg), None
 except ImportError:  # torch is only needed for the gradient check
     CoefficientFunctional = None
+    InflowFlux = None

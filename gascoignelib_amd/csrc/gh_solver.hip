@@ -1398,26 +1398,36 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
 }
 
 // ------------------------------------------------------------------ A^T x
-__global__ __launch_bounds__(kBlock) void k_csr_t(int64_t n, const int32_t* __restrict__ trow,
+// One thread per component c of block row i of A^T (block column i of A):
+// y(i, c) = d * sum over the blocks (j, i) of A of sum_r A_ji(r, c) x(j, r)
+// + b * y(i, c), blocks in ascending j.
+__global__ __launch_bounds__(kBlock) void k_csr_t(int64_t n, int B, const int32_t* __restrict__ trow,
                                                   const int32_t* __restrict__ tcol,
                                                   const int32_t* __restrict__ tpos,
                                                   const double* __restrict__ val,
                                                   const double* __restrict__ x,
                                                   double* __restrict__ y, double d, double b,
                                                   int read_y) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * B) return;
+  const int64_t i = t / B;
+  const int c = (int)(t - i * B);
+  const int BB = B * B;
   double s = 0.0;
-  for (int k = trow[i]; k < trow[i + 1]; ++k) s += val[tpos[k]] * x[tcol[k]];
+  for (int k = trow[i]; k < trow[i + 1]; ++k) {
+    const double* a = val + (int64_t)tpos[k] * BB + c;
+    const double* xj = x + (int64_t)tcol[k] * B;
+    for (int r = 0; r < B; ++r) s += a[r * B] * xj[r];
+  }
   double v = d * s;
-  if (read_y) v += b * y[i];
-  y[i] = v;
+  if (read_y) v += b * y[t];
+  y[t] = v;
 }
 
 int gh_spmv_t(gh_ctx* ctx, const gh_csr* Ac, gh_vec* y, const gh_vec* x, double d, double b) {
-  GH_ARG(ctx && Ac && x && y && x->n == Ac->n && y->n == Ac->n && x->ncomp == 1 &&
-             y->ncomp == 1 && Ac->ncomp == 1,
-         "gh_spmv_t: shape mismatch (scalar matrices)");
+  GH_ARG(ctx && Ac && x && y && x->n == Ac->n && y->n == Ac->n && x->ncomp == Ac->ncomp &&
+             y->ncomp == Ac->ncomp,
+         "gh_spmv_t: shape mismatch");
   GH_ARG(Ac->prec == GH_F64 && x->prec == GH_F64 && y->prec == GH_F64,
          "gh_spmv_t: FP64 only");
   GH_ARG(x->d != y->d, "gh_spmv_t: x and y must not alias");
@@ -1439,8 +1449,8 @@ int gh_spmv_t(gh_ctx* ctx, const gh_csr* Ac, gh_vec* y, const gh_vec* x, double 
     GH_TRY(dupload(ctx, &A->t_col, tcol));
     GH_TRY(dupload(ctx, &A->t_pos, tpos));
   }
-  hipLaunchKernelGGL(k_csr_t, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
-                     A->t_rowptr, A->t_col, A->t_pos, (const double*)A->val,
+  hipLaunchKernelGGL(k_csr_t, dim3(grid_for(A->n * A->ncomp)), dim3(kBlock), 0, ctx->stream, A->n,
+                     A->ncomp, A->t_rowptr, A->t_col, A->t_pos, (const double*)A->val,
                      (const double*)x->d, (double*)y->d, d, b, (int)(b != 0.0));
   GH_HIP(hipGetLastError());
   return GH_OK;

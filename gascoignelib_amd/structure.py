@@ -193,7 +193,7 @@ def inflow_amplitude(seed):
 
 
 def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", drop_coarse=1,
-                     name=None, keep_coords=True):
+                     name=None, keep_coords=True, with_full=False):
     """Stokes-LPS channel on the unit cube (NavierStokesLps<3> at U = 0,
     CGDiscQ13dLps): components (p, u, v, w), 4x4 blocks on the patch
     structure, velocity Dirichlet on colours 1 (inflow, x = 0, parabolic
@@ -201,7 +201,9 @@ def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", 
     colour 3 (x = 1).  The finest level carries the Newton residual of the
     lifted inflow data: h.rhs (n*4), h.u0 (n, 4); per level h.patches
     (npatches, 27) for the Vanka smoother, h.mass_in / h.mass_out the
-    boundary masses of the inflow and outflow faces on the finest level."""
+    boundary masses of the inflow and outflow faces on the finest level;
+    with_full: h.full_val, the finest operator without Dirichlet conditions
+    (the lifting and the adjoint gradient with respect to boundary data)."""
     M = Mesh(inp, 3, prerefine, drop_coarse)
     nl = M.nlevels
     colours, comps = (1, 2), (1, 2, 3)
@@ -235,4 +237,12 @@ def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", 
     h.visc, h.alpha = visc, alpha
     h.mass_in = M.face_mass(nl - 1, 1)
     h.mass_out = M.face_mass(nl - 1, 3)
+    h.full_val = None
+    if with_full:
+        h.full_val, _ = M.assemble_stokes(nl - 1, h.fine.nnz, visc, alpha, (), ())
+    # inflow data: u_x at the inflow nodes that no wall constrains to zero
+    walls = np.zeros(h.fine.n, bool)
+    walls[M.dirichlet(nl - 1, (2,))] = True
+    inflow = M.dirichlet(nl - 1, (1,))
+    h.inflow_nodes = inflow[~walls[inflow]]
     return h

@@ -121,7 +121,8 @@ int gh_csr_prec(const gh_csr* A);
 int gh_csr_layout(const gh_csr* A, int64_t* out);
 /* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
-/* y = d*A^T*x + b*y (adjoint path; no reference counterpart on the device:
+/* y = d*A^T*x + b*y, FP64; block matrices are transposed blockwise
+ * (adjoint path; no reference counterpart on the device:
  * SparseBlockMatrix::transpose / AssembleDualMatrix, src/Solver/stdsolver.cc:1815-1834) */
 int gh_spmv_t(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
 /* r = b - A*x: CudaSolver::MatrixResidual (cuda/cudasolver.cc:379-398) */

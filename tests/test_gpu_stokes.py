@@ -161,3 +161,75 @@ def test_block_products_bitexact(ctx, B):
     ye = np.zeros(m)
     Ao.vmult(ye, x)
     assert np.abs(Y32.copy_back() - ye).max() <= 1e-5 * np.abs(ye).max()
+
+
+def test_block_transpose_product(ctx, h8):
+    """y = d A^T x + b y for the 4x4-block operator against scipy"""
+    import scipy.sparse as sp
+    lev = h8.fine
+    A = H.CudaCSRMatrixInterface(ctx, lev.rowptr, lev.col, lev.val, 4)
+    S = sp.bsr_matrix((lev.val.reshape(-1, 4, 4), lev.col, lev.rowptr),
+                      shape=(4 * lev.n, 4 * lev.n))
+    rng = np.random.default_rng(4)
+    x, y0 = rng.uniform(-1, 1, 4 * lev.n), rng.uniform(-1, 1, 4 * lev.n)
+    Y = H.CudaVectorInterface.from_global(ctx, y0, 4)
+    A.vmult_transpose(Y, H.CudaVectorInterface.from_global(ctx, x, 4), -0.5, 2.0)
+    want = -0.5 * (S.T @ x) + 2.0 * y0
+    assert np.abs(Y.copy_back() - want).max() <= 1e-13 * np.abs(want).max()
+
+
+@pytest.fixture(scope="module")
+def h8full():
+    return stokes_hierarchy(3, seed=5, with_full=True)
+
+
+def test_flux_gradient_vs_oracle(ctx, h8full):
+    """adjoint solve on the transposed levels (Vanka factors of the transposed
+    patch matrices) and dJ/dg = (j - A^T z) on the inflow u_x entries: z
+    against the oracle's adjoint to 1e-12, the gradient against the
+    inflow-face masses that mass conservation fixes"""
+    from gascoignelib_amd.autograd import StokesFluxProblem
+    from test_stokes_adjoint_oracle import transposed
+    print("[stokes flux gradient FP64 8^3]", flush=True)
+    h = h8full
+    P = StokesFluxProblem(ctx, h, prec="f64", krylov="gmres", tol=1e-12)
+    grad, Z, rep = P.gradient()
+    jF = P.j.copy()
+    jF.reshape(-1, 4)[h.fine.dirichlet, 1:] = 0.0
+    om = O.MG(transposed(h), smoother="vanka", patches=h.patches, **MGP)
+    zo = np.zeros_like(jF)
+    rc, orep = om.newton(zo, jF, O.cginfo(1e-12, 1e-30, 100), O.nlinfo(1e-12, 1e-30, 30),
+                         norm_kind=1)
+    assert rc == 0
+    assert abs(rep.gmres_iterations - orep.gmres_iterations) <= 1
+    assert np.linalg.norm(Z.copy_back() - zo) <= 1e-12 * np.linalg.norm(zo)
+    want = h.mass_in[h.inflow_nodes]
+    assert np.abs(grad - want).max() <= 1e-9 * want.max()
+    # forward: outflow flux = inflow flux of the lifted data
+    g = h.u0[h.inflow_nodes, 1]
+    assert abs(P.J(g) - want @ g) <= 1e-9 * abs(want @ g)
+
+
+def test_flux_gradient_mixed_and_autograd(ctx, h8full):
+    """FP32 Vanka V-cycles in FP64 flexible GMRES for both solves; the torch
+    wrapper's backward against central differences of forward solves"""
+    import torch
+    from gascoignelib_amd.autograd import InflowFlux, StokesFluxProblem
+    print("[stokes flux gradient mixed + autograd 8^3]", flush=True)
+    h = h8full
+    P = StokesFluxProblem(ctx, h, prec="f32", krylov="fgmres", tol=1e-12)
+    want = h.mass_in[h.inflow_nodes]
+    g0 = h.u0[h.inflow_nodes, 1].copy()
+    g = torch.tensor(g0, dtype=torch.float64, requires_grad=True)
+    J = InflowFlux.apply(g, P)
+    J.backward()
+    assert np.abs(g.grad.numpy() - want).max() <= 1e-8 * want.max()
+    assert abs(float(J) - want @ g0) <= 1e-9 * abs(want @ g0)
+    rng = np.random.default_rng(8)
+    eps = 1e-3
+    for k in rng.choice(len(g0), 3, replace=False):
+        gp, gm = g0.copy(), g0.copy()
+        gp[k] += eps
+        gm[k] -= eps
+        fd = (P.J(gp) - P.J(gm)) / (2 * eps)
+        assert abs(fd - float(g.grad[k])) <= 1e-6 * want.max()
