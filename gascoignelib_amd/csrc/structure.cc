@@ -877,6 +877,103 @@ int ghs_level_assemble_stokes(const ghs_hier* h, int l, double visc, double alph
   return 0;
 }
 
+int ghs_level_assemble_q2(const ghs_hier* h, int l, int ncolours, const int32_t* colours,
+                          double* val) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  if (M->dim != 3) return fail("Q2 patch assembly is three-dimensional");
+  std::vector<int32_t> P, rp, c, d;
+  if (patches(*M, P)) return -1;
+  const int64_t n = M->n(), np = (int64_t)P.size() / 27;
+  if (structure(n, np, 27, P.data(), rp, c)) return -1;
+  // Laplace<3> on the triquadratic patch element (CGDiscQ23d: the 27 patch
+  // nodes carry the Lagrange basis of BaseQ23d; src/Problems/laplace.h,
+  // src/DofHandler/cgdisc.h:1291): per direction the quadratic element on
+  // [0, L], L = 2h, nodes 0, L/2, L, with stiffness (7,-8,1;-8,16,-8;1,-8,7)/(3L)
+  // and mass (4,2,-1;2,16,2;-1,2,4) L/30, exact (the reference's 3x3x3 Gauss
+  // rule is exact for these products)
+  auto q2 = [](double L, double S[3][3], double Mm[3][3]) {
+    const double s[3][3] = {{7, -8, 1}, {-8, 16, -8}, {1, -8, 7}};
+    const double m[3][3] = {{4, 2, -1}, {2, 16, 2}, {-1, 2, 4}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        S[a][b] = s[a][b] / (3.0 * L);
+        Mm[a][b] = m[a][b] * L / 30.0;
+      }
+  };
+  std::vector<int32_t> shape(np);
+  std::vector<std::array<double, 3>> sizes;
+  std::vector<std::vector<double>> elem;  // 27 x 27 per shape
+  for (int64_t p = 0; p < np; ++p) {
+    double hb[3];
+    if (patch_box(*M, &P[p * 27], hb)) return fail("patch %lld is not an axis-aligned box of 8 equal cells", (long long)p);
+    int sidx = 0;
+    while (sidx < (int)sizes.size() && !(sizes[sidx][0] == hb[0] && sizes[sidx][1] == hb[1] && sizes[sidx][2] == hb[2])) ++sidx;
+    if (sidx == (int)sizes.size()) {
+      if (sidx >= 64) return fail("Q2 assembly: more than 64 distinct patch sizes");
+      sizes.push_back({hb[0], hb[1], hb[2]});
+      double SX[3][3], MX[3][3], SY[3][3], MY[3][3], SZ[3][3], MZ[3][3];
+      q2(2 * hb[0], SX, MX);
+      q2(2 * hb[1], SY, MY);
+      q2(2 * hb[2], SZ, MZ);
+      std::vector<double> E(27 * 27);
+      for (int a = 0; a < 27; ++a)
+        for (int b = 0; b < 27; ++b) {
+          const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+          const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+          E[a * 27 + b] = SX[ax][bx] * MY[ay][by] * MZ[az][bz] + MX[ax][bx] * SY[ay][by] * MZ[az][bz] +
+                          MX[ax][bx] * MY[ay][by] * SZ[az][bz];
+        }
+      elem.push_back(std::move(E));
+    }
+    shape[p] = sidx;
+  }
+  std::vector<int32_t> pstart(n + 1, 0), plist(np * 27);
+  std::vector<int8_t> ploc(np * 27);
+  for (int64_t k = 0; k < np * 27; ++k) pstart[P[k] + 1]++;
+  for (int64_t i = 0; i < n; ++i) pstart[i + 1] += pstart[i];
+  {
+    std::vector<int32_t> fill(pstart.begin(), pstart.end() - 1);
+    for (int64_t p = 0; p < np; ++p)
+      for (int a = 0; a < 27; ++a) {
+        const int32_t k = fill[P[p * 27 + a]]++;
+        plist[k] = (int32_t)p;
+        ploc[k] = (int8_t)a;
+      }
+  }
+  dir_nodes(M, ncolours, colours, d);
+  std::vector<uint8_t> isdir(n, 0);
+  for (int32_t i : d) isdir[i] = 1;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : bad)
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t rs = rp[i], re = rp[i + 1];
+    for (int32_t q = rs; q < re; ++q) val[q] = 0.0;
+    for (int32_t k = pstart[i]; k < pstart[i + 1]; ++k) {
+      const int64_t p = plist[k];
+      const int a = ploc[k];
+      const double* E = elem[shape[p]].data() + a * 27;
+      for (int b = 0; b < 27; ++b) {
+        const int32_t j = P[p * 27 + b];
+        const int32_t* it = std::lower_bound(c.data() + rs, c.data() + re, j);
+        if (it == c.data() + re || *it != j) {
+          bad++;
+          continue;
+        }
+        val[it - c.data()] += E[b];
+      }
+    }
+    for (int32_t q = rs; q < re; ++q) {
+      if (isdir[i])
+        val[q] = (c[q] == i) ? 1.0 : 0.0;
+      else if (isdir[c[q]])
+        val[q] = 0.0;
+    }
+  }
+  if (bad) return fail("patch structure misses %d couplings", bad);
+  return 0;
+}
+
 int ghs_level_face_mass(const ghs_hier* h, int l, int colour, double* m) {
   const Mesh* M = lev(h, l);
   if (!M) return -1;

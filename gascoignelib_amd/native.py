@@ -165,6 +165,7 @@ STRUCT_SIGNATURES = {
     "ghs_level_assemble_stokes": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_int, i32p,
                                             C.c_int, f64p, f64p, f64p]),
     "ghs_level_face_mass": (C.c_int, [vp, C.c_int, C.c_int, f64p]),
+    "ghs_level_assemble_q2": (C.c_int, [vp, C.c_int, C.c_int, i32p, f64p]),
 }
 
 _libs = {}

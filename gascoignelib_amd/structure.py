@@ -125,6 +125,14 @@ class Mesh:
             ptr(u0) if u0 is not None else None, ptr(rhs) if rhs is not None else None, ptr(val)))
         return val, rhs
 
+    def assemble_q2(self, l, nnz, colours=(1,)):
+        """Laplace on the triquadratic patch element, values on the patch structure"""
+        cols = np.ascontiguousarray(colours, dtype=np.int32)
+        val = np.empty(nnz)
+        check_s(N.structure().ghs_level_assemble_q2(self.h, l, len(cols), ptr(cols, N.i32p),
+                                                    ptr(val)))
+        return val
+
     def face_mass(self, l, colour):
         n, _, _ = self.sizes(l)
         m = np.empty(n)
@@ -245,4 +253,26 @@ def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", 
     walls[M.dirichlet(nl - 1, (2,))] = True
     inflow = M.dirichlet(nl - 1, (1,))
     h.inflow_nodes = inflow[~walls[inflow]]
+    return h
+
+
+def q2_hierarchy(prerefine, inp="cube.inp", drop_coarse=1, colours=(1,), name=None,
+                 keep_coords=True):
+    """Poisson with the triquadratic patch element (CGDiscQ23d, Laplace<3>) on a
+    uniformly refined hex mesh: one unknown per mesh node, 2x2x2-cell patches
+    as elements (up to 125 couplings per row), nodal transfers between levels
+    (MgInterpolatorMatrix, as for every discretisation of the reference's
+    multilevel solver), Dirichlet nodes of the given colours."""
+    M = Mesh(inp, 3, prerefine, drop_coarse)
+    levels = []
+    for l in range(M.nlevels):
+        rp, col = M.patch_structure(l)
+        val = M.assemble_q2(l, len(col), colours)
+        levels.append(Level(n=len(rp) - 1, ncomp=1, rowptr=rp, col=col, val=val,
+                            transfer=M.transfer(l) if l > 0 else None, hn_kind=23,
+                            dirichlet=M.dirichlet(l, colours),
+                            dirichlet_comps=np.zeros(1, np.int32),
+                            coords=M.coords(l) if keep_coords else None))
+    h = Hierarchy(levels, 3, name or f"q2:{inp}:{prerefine}")
+    h.mesh = M
     return h

@@ -94,6 +94,11 @@ int ghs_level_patch_structure(const ghs_hier* h, int l, int32_t* rowptr, int32_t
 int ghs_level_assemble_stokes(const ghs_hier* h, int l, double visc, double alpha, int ncolours,
                               const int32_t* colours, int dircomps, const double* u0, double* rhs,
                               double* val);
+/* Laplace on the triquadratic patch element (CGDiscQ23d with Laplace<3>):
+ * scalar values val[nnz] on the patch structure, strong Dirichlet rows and
+ * columns on the nodes of the given colours (identity diagonal). */
+int ghs_level_assemble_q2(const ghs_hier* h, int l, int ncolours, const int32_t* colours,
+                          double* val);
 /* m[i] = integral of the bilinear hat of node i over the rectangular boundary
  * faces of the given colour (boundary mass row sums), m[nnodes] */
 int ghs_level_face_mass(const ghs_hier* h, int l, int colour, double* m);
