@@ -99,11 +99,13 @@ def main():
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
     ap.add_argument("--workload", default="poisson128",
-                    choices=["poisson128", "adjoint96", "stokes64"],
+                    choices=["poisson128", "adjoint96", "stokes64", "q2_64"],
                     help="poisson128: 3D Q1 Poisson on 128^3 cells (the headline); adjoint96: "
                          "variable-coefficient diffusion on 96^3 cells, one gradient evaluation "
                          "(forward solve + adjoint solve + dJ/dk per cell) per step; stokes64: "
-                         "Stokes-LPS channel on 64^3 cells, 4x4 blocks, Vanka multigrid")
+                         "Stokes-LPS channel on 64^3 cells, 4x4 blocks, Vanka multigrid; q2_64: "
+                         "Poisson with the triquadratic patch element on 64^3 cells (125-point "
+                         "rows), forward and adjoint solves")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
@@ -119,6 +121,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "stokes64":
         return bench_stokes(args)
+    if args.workload == "q2_64":
+        return bench_q2(args)
 
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.problem import lattice_rhs
@@ -566,6 +570,124 @@ def bench_stokes(args):
                          "solve on the transposed levels + (j - A^T z); error = max deviation "
                          "from the inflow-face masses that mass conservation fixes",
         "kernels": kern,
+        "setup_s": {"structure": t_struct, "device": t_setup},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_q2(args):
+    """Poisson with the triquadratic patch element (CGDiscQ23d, Laplace<3>) on
+    the unit cube refined to 64^3 cells: 274,625 DOF, 16,974,593 entries (up to
+    125 per row), 6 levels, nodal transfers, Dirichlet 0, seeded f.  One step =
+    one solve to 1e-10 relative FP64 residual (FP64 GMRES in Newton vs f16-value
+    FP32 V-cycle in FP64 flexible GMRES, V(2,2) Jacobi omega 0.8, levels below
+    1000 nodes direct), then the adjoint solve for J = mean of the nodal values
+    in the ball of radius 0.25 around a seeded centre."""
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd.problem import (lattice_rhs, lexicographic_order, permuted_csr,
+                                          splitmix64_uniform)
+    from gascoignelib_amd.structure import q2_hierarchy
+
+    cells = 64 if args.cells == 128 else args.cells
+    t0 = time.perf_counter()
+    hier = q2_hierarchy(int(round(np.log2(cells))))
+    t_struct = time.perf_counter() - t0
+    fine = hier.fine
+    n = fine.n
+    f = lattice_rhs(fine.coords, seed=2024)
+    f[fine.dirichlet] = 0.0
+    centre = 0.25 + 0.5 * (0.5 * (splitmix64_uniform(4242, 3) + 1.0))
+    inside = np.linalg.norm(fine.coords - centre, axis=1) < 0.25
+    g = inside / max(1, int(inside.sum()))
+    g[fine.dirichlet] = 0.0
+    print(f"# q2 {cells}^3: {n} DOF, nnz {fine.nnz}, {hier.nlevels} levels, ball of "
+          f"{int(inside.sum())} nodes, built in {t_struct:.1f}s", file=sys.stderr, flush=True)
+    ctx = H.Context(0)
+    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
+                  ndirect=1000, order=None if args.order == "none" else args.order)
+    t0 = time.perf_counter()
+    sol = {}
+    for tag, prec, kry, tol in (("fp64", "f64", "gmres", 1e-10),
+                                ("mixed", args.mixed, "fgmres", 0.9e-10)):
+        sol[tag] = (H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, **common),
+                    H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, adjoint=True,
+                                           **common), tol)
+    t_setup = time.perf_counter() - t0
+    F = H.CudaVectorInterface.from_global(ctx, f)
+    G = H.CudaVectorInterface.from_global(ctx, g)
+    U, Z = H.CudaVectorInterface(ctx, n), H.CudaVectorInterface(ctx, n)
+
+    def solve(mg, X, B, tol):
+        X.zero()
+        rep = mg.Solve(X, B, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50), norm_kind=1)
+        if H.N.NL_STATUS[rep.status] != "converged":
+            raise RuntimeError("Q2 solve did not converge")
+        return rep
+
+    res = {}
+    for tag in ("fp64", "mixed"):
+        mgF, mgA, tol = sol[tag]
+        out = {}
+        for what, mg, X, B in (("forward", mgF, U, F), ("adjoint", mgA, Z, G)):
+            for _ in range(args.warmup):
+                solve(mg, X, B, tol)
+            ts = []
+            for i in range(args.steps):
+                ctx.sync()
+                t = time.perf_counter()
+                rep = solve(mg, X, B, tol)
+                ctx.sync()
+                ts.append(time.perf_counter() - t)
+                print(f"# {tag} {what} step {i}: {ts[-1] * 1e3:.2f} ms (gmres "
+                      f"{rep.gmres_iterations})", file=sys.stderr, flush=True)
+            out[what] = dict(ms=float(np.median(ts)) * 1e3, gmres=rep.gmres_iterations,
+                             x=X.copy_back())
+        res[tag] = out
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    # the FP64 residual SpMV on the finest level, internal numbering
+    order = (np.arange(n, dtype=np.int32) if args.order == "none"
+             else lexicographic_order(fine.coords))
+    rp, cl, vl = permuted_csr(fine, order)
+    Ak = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
+    X = H.CudaVectorInterface.from_global(ctx, res["fp64"]["forward"]["x"][order])
+    Fk = H.CudaVectorInterface.from_global(ctx, f[order])
+    R = H.CudaVectorInterface(ctx, n)
+    t_spmv = time_kernel(ctx, lambda: Ak.residual(R, X, Fk), args.spmv_reps)
+    lay = Ak.layout()
+    b64 = sell_bytes(n, lay, 8, 8)
+    line = {
+        "metric": "DOF/s of a Q2 Poisson solve to 1e-10 rel residual (mixed vs FP64), "
+                  "plus the adjoint solve",
+        "value": n / (res["mixed"]["forward"]["ms"] * 1e-3), "unit": "DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["forward"]["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": f"f64 outer / f32 V-cycle ({args.mixed}-stored matrix values)",
+        "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, "
+                "Dirichlet 0; J = mean of u over a seeded ball of radius 0.25",
+        "config": {"workload": f"3D Poisson, triquadratic patch element (Q23d), {cells}^3 cells, "
+                               "uniform (no adaptive levels)", "dof": n, "nnz": fine.nnz,
+                   "levels": hier.nlevels},
+        "fp64": {"forward_ms": res["fp64"]["forward"]["ms"],
+                 "forward_gmres": res["fp64"]["forward"]["gmres"],
+                 "adjoint_ms": res["fp64"]["adjoint"]["ms"],
+                 "adjoint_gmres": res["fp64"]["adjoint"]["gmres"],
+                 "dof_per_s": n / (res["fp64"]["forward"]["ms"] * 1e-3)},
+        "mixed": {"forward_ms": res["mixed"]["forward"]["ms"],
+                  "forward_gmres": res["mixed"]["forward"]["gmres"],
+                  "adjoint_ms": res["mixed"]["adjoint"]["ms"],
+                  "adjoint_gmres": res["mixed"]["adjoint"]["gmres"],
+                  "rel_error_u_vs_fp64": rel(res["mixed"]["forward"]["x"],
+                                             res["fp64"]["forward"]["x"]),
+                  "rel_error_z_vs_fp64": rel(res["mixed"]["adjoint"]["x"],
+                                             res["fp64"]["adjoint"]["x"])},
+        "speedup_mixed_vs_fp64": res["fp64"]["forward"]["ms"] / res["mixed"]["forward"]["ms"],
+        "J": float(g @ res["fp64"]["forward"]["x"]),
+        "J_adjoint": float(f @ res["fp64"]["adjoint"]["x"]),
+        "residual_spmv_f64": {"us_per_launch": t_spmv * 1e6, "bytes_per_launch": b64,
+                              "gbs": b64 / t_spmv / 1e9,
+                              "frac": b64 / t_spmv / 1e9 / HBM_PEAK_GBS,
+                              "stored_entries": lay["stored"], "pattern_entries": fine.nnz},
         "setup_s": {"structure": t_struct, "device": t_setup},
     }
     print(json.dumps(line), flush=True)
