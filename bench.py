@@ -141,7 +141,7 @@ def main():
 
     ctx = H.Context(0)
     if args.kernel_only:
-        return kernel_only(ctx, fine, f, args)
+        return kernel_only(ctx, hier, f, args)
     common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
                   ndirect=1000, order=None if args.order == "none" else args.order)
     t0 = time.perf_counter()
@@ -190,24 +190,24 @@ def main():
     rr64, rrmx = true_rel_res(U64), true_rel_res(UMX)
     err_mx = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
 
-    # roofline of the dominant kernel: the FP64 residual SpMV on the finest level,
-    # on the matrix in the solver's internal numbering
-    from gascoignelib_amd.problem import lexicographic_order, permuted_csr
-    if args.order == "none":
-        rp, cl, vl = fine.rowptr, fine.col, fine.val
-    else:
-        rp, cl, vl = permuted_csr(fine, lexicographic_order(fine.coords))
-    Ak = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
-    X = H.CudaVectorInterface.from_global(ctx, u64)
-    t_spmv = time_kernel(ctx, lambda: Ak.residual(R, X, F), args.spmv_reps)
+    # roofline of the dominant kernel: the FP64 residual SpMV on the finest
+    # level, on the very matrix the FP64 solve runs (the solver's internal
+    # numbering; rows keep the caller's entry order, so sums follow the
+    # reference's), and the mixed V-cycle's finest-level residual on its own
+    # matrix (FP32 vectors, stored values, columns in internal order)
+    from gascoignelib_amd.problem import lexicographic_order
+    lex = (np.arange(ndof) if args.order == "none" else lexicographic_order(fine.coords))
+    Ak = mg64.level_matrix(fp64=True)
+    X = H.CudaVectorInterface.from_global(ctx, u64[lex])
+    Fk = H.CudaVectorInterface.from_global(ctx, f[lex])
+    t_spmv = time_kernel(ctx, lambda: Ak.residual(R, X, Fk), args.spmv_reps)
     lay = Ak.layout()
     b64 = sell_bytes(ndof, lay, 8, 8)
     gbs64 = b64 / t_spmv / 1e9
     csr64 = spmv_bytes(ndof, fine.nnz, 8, 8)
-    # the mixed V-cycle's finest-level residual (FP32 vectors, stored values)
-    Af = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, args.mixed)
-    Xf = H.CudaVectorInterface.from_global(ctx, u64, 1, "f32")
-    Ff = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
+    Af = mgmx.level_matrix(fp64=False)
+    Xf = H.CudaVectorInterface.from_global(ctx, u64[lex], 1, "f32")
+    Ff = H.CudaVectorInterface.from_global(ctx, f[lex], 1, "f32")
     Rf = H.CudaVectorInterface(ctx, ndof, 1, "f32")
     t_spmv_mx = time_kernel(ctx, lambda: Af.residual(Rf, Xf, Ff), args.spmv_reps)
     bmx = sell_bytes(ndof, Af.layout(), 2 if args.mixed == "f16" else 4, 4)
@@ -278,21 +278,23 @@ def measured_traffic(layout):
     return t["bytes_per_launch"]
 
 
-def kernel_only(ctx, fine, f, args):
+def kernel_only(ctx, hier, f, args):
+    """the roofline kernels alone (for counter runs): the FP64 solver's and the
+    mixed V-cycle's finest-level residuals on their own matrices"""
     from gascoignelib_amd import hip as H
-    from gascoignelib_amd.problem import lexicographic_order, permuted_csr
-    if args.order == "none":
-        rp, cl, vl = fine.rowptr, fine.col, fine.val
-    else:
-        rp, cl, vl = permuted_csr(fine, lexicographic_order(fine.coords))
+    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
+                  ndirect=1000, order=None if args.order == "none" else args.order)
+    mg64 = H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common)
+    mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, krylov=args.mixed_krylov, **common)
+    fine = hier.fine
     n = fine.n
-    A = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
+    A = mg64.level_matrix(fp64=True)
     X = H.CudaVectorInterface.from_global(ctx, f)
     F = H.CudaVectorInterface.from_global(ctx, f)
     R = H.CudaVectorInterface(ctx, n)
     for _ in range(args.spmv_reps):
         A.residual(R, X, F)
-    Af = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, args.mixed)
+    Af = mgmx.level_matrix(fp64=False)
     Xf = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
     Ff = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
     Rf = H.CudaVectorInterface(ctx, n, 1, "f32")
@@ -502,7 +504,6 @@ def bench_stokes(args):
              else lexicographic_order(fine.coords))
     inv = np.empty(fine.n, np.int64)
     inv[order] = np.arange(fine.n)
-    rp, cl, vl = permuted_csr(fine, order)
     pat = np.ascontiguousarray(inv[hier.patches[-1]], dtype=np.int32)
     xk = res["fp64"]["u"].reshape(-1, 4)[order].reshape(-1)
     fk = f.reshape(-1, 4)[order].reshape(-1)
@@ -510,7 +511,10 @@ def bench_stokes(args):
     m = 27 * 4
     kern = {}
     for prec, sz in (("f64", 8), ("f32", 4)):
-        Ak = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 4, prec)
+        # the matrices the solves run: FP64 (rows in the caller's entry order)
+        # and the FP32 V-cycle's
+        Ak = (probs["fp64"].mg.level_matrix(fp64=True) if prec == "f64"
+              else probs["mixed"].mg.level_matrix(fp64=False))
         Xk = H.CudaVectorInterface.from_global(ctx, xk, 4, prec)
         Fk = H.CudaVectorInterface.from_global(ctx, fk, 4, prec)
         Rk = H.CudaVectorInterface(ctx, fine.n, 4, prec)
@@ -521,8 +525,7 @@ def bench_stokes(args):
         # read, r written
         b_res = (lay["stored"] * (4 * sz + 4) + lay["slices"] * 64 * 6
                  + (lay["slices"] + 1) * 4 + nunk * sz * 3)
-        A64k = Ak if prec == "f64" else H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 4, "f64")
-        Vk = H.VankaSmoother(ctx, A64k, pat, prec)
+        Vk = H.VankaSmoother(ctx, probs["fp64"].mg.level_matrix(fp64=True), pat, prec)
         t_vk = time_kernel(ctx, lambda: Vk.solve(Xk), max(5, args.spmv_reps // 5))
         # the factors once (lower triangle forward, upper and diagonal
         # backward), per patch row a gather index and the local solution
@@ -535,7 +538,7 @@ def bench_stokes(args):
         kern[f"vanka_solve_{prec}"] = {
             "us_per_launch": t_vk * 1e6, "bytes_per_launch": b_vk,
             "gbs": b_vk / t_vk / 1e9, "frac": b_vk / t_vk / 1e9 / HBM_PEAK_GBS}
-        del Vk, Ak, A64k
+        del Vk, Ak
     line = {
         "metric": "unknowns/s of a Stokes-LPS solve to 1e-10 rel residual (mixed vs FP64)",
         "value": nunk / (res["mixed"]["ms"] * 1e-3), "unit": "unknowns/s", "n_gpus": 1,
@@ -647,8 +650,7 @@ def bench_q2(args):
     # the FP64 residual SpMV on the finest level, internal numbering
     order = (np.arange(n, dtype=np.int32) if args.order == "none"
              else lexicographic_order(fine.coords))
-    rp, cl, vl = permuted_csr(fine, order)
-    Ak = H.CudaCSRMatrixInterface(ctx, rp, cl, vl, 1, "f64")
+    Ak = sol["fp64"][0].level_matrix(fp64=True)  # the matrix the FP64 solve runs
     X = H.CudaVectorInterface.from_global(ctx, res["fp64"]["forward"]["x"][order])
     Fk = H.CudaVectorInterface.from_global(ctx, f[order])
     R = H.CudaVectorInterface(ctx, n)

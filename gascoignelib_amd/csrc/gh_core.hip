@@ -462,8 +462,10 @@ static int upload(gh_ctx* ctx, T** dptr, const T* host, size_t count) {
   return GH_OK;
 }
 
-int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
-               const double* vals, int prec, Sell* S) {
+// orig: original row of each slot (nullptr: slot = row); rowptr/col/vals are
+// already in slot order.
+static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
+                           const double* vals, int prec, const int32_t* orig, Sell* S) {
   S->n = n;
   S->prec = prec;
   S->maxlen = 0;
@@ -530,9 +532,10 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
     for (int64_t i = 0; i < n; ++i) {
       offs.clear();
       uint64_t hsh = 1469598103934665603ull;
+      const int64_t orow = orig ? orig[i] : i;
       for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
         if (!keep(q)) continue;
-        const int32_t o = (int32_t)((int64_t)col[q] - i);
+        const int32_t o = (int32_t)((int64_t)col[q] - orow);
         offs.push_back(o);
         hsh = (hsh ^ (uint32_t)o) * 1099511628211ull;
       }
@@ -674,10 +677,84 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
   return GH_OK;
 }
 
+// Valued operators: rows with the same relative column pattern are grouped
+// (stable sort by pattern inside windows of kSellWindow rows), so that the
+// slices of the common patterns hold a single pattern (offsets from scalar
+// loads, no padding) -- applied when it makes clearly more slices uniform.
+// Each row keeps its entries and its sum; only the slot it runs in moves.
+int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
+               const double* vals, int prec, Sell* S) {
+  S->perm = nullptr;
+  if (!vals || n < 2 * kSlice) return sell_build_impl(ctx, n, rowptr, col, vals, prec, nullptr, S);
+  constexpr int64_t kSellWindow = 4096;
+  std::vector<uint64_t> key(n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t h = 1469598103934665603ull;
+    for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      if (vals[q] == 0.0) continue;
+      h = (h ^ (uint32_t)(int32_t)((int64_t)col[q] - i)) * 1099511628211ull;
+    }
+    key[i] = h;
+  }
+  std::vector<int32_t> perm(n);
+  for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
+  for (int64_t w0 = 0; w0 < n; w0 += kSellWindow) {
+    const int64_t w1 = std::min<int64_t>(n, w0 + kSellWindow);
+    std::stable_sort(perm.begin() + w0, perm.begin() + w1,
+                     [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  }
+  auto uniform = [&](bool permuted) {
+    int64_t u = 0;
+    for (int64_t s0 = 0; s0 + kSlice <= n; s0 += kSlice) {
+      const uint64_t k0 = key[permuted ? perm[s0] : s0];
+      bool same = true;
+      for (int l = 1; l < kSlice && same; ++l) same = key[permuted ? perm[s0 + l] : s0 + l] == k0;
+      u += same;
+    }
+    return u;
+  };
+  // rows a slice's lanes touch, in groups of 16 (the vectors' 128-byte lines
+  // in FP64): 4 or 5 for 64 consecutive rows
+  auto groups = [&]() {
+    int64_t g = 0;
+    std::vector<int32_t> b(kSlice);
+    for (int64_t s0 = 0; s0 < n; s0 += kSlice) {
+      const int m = (int)std::min<int64_t>(kSlice, n - s0);
+      for (int l = 0; l < m; ++l) b[l] = perm[s0 + l] >> 4;
+      std::sort(b.begin(), b.begin() + m);
+      g += std::unique(b.begin(), b.begin() + m) - b.begin();
+    }
+    return (double)g / (double)((n + kSlice - 1) / kSlice);
+  };
+  const int64_t u0 = uniform(false), u1 = uniform(true);
+  // worth it only with clearly more single-pattern slices and rows that stay
+  // close together (rows of two kinds interleaved at stride 2 touch ~8-16
+  // groups per slice and gain; rows of many kinds grouped from all over the
+  // window touch 20+ and lose the vectors' coalescing)
+  if (u1 < u0 + std::max<int64_t>(1, (n / kSlice) / 20) || groups() > 20.0)
+    return sell_build_impl(ctx, n, rowptr, col, vals, prec, nullptr, S);
+  // the operator in slot order
+  std::vector<int32_t> rp2(n + 1, 0);
+  for (int64_t t = 0; t < n; ++t) rp2[t + 1] = rp2[t] + (rowptr[perm[t] + 1] - rowptr[perm[t]]);
+  std::vector<uint32_t> c2(rp2[n]);
+  std::vector<double> v2(rp2[n]);
+  for (int64_t t = 0; t < n; ++t) {
+    const int32_t i = perm[t];
+    std::copy(col + rowptr[i], col + rowptr[i + 1], c2.begin() + rp2[t]);
+    std::copy(vals + rowptr[i], vals + rowptr[i + 1], v2.begin() + rp2[t]);
+  }
+  GH_TRY(sell_build_impl(ctx, n, rp2.data(), c2.data(), v2.data(), prec, perm.data(), S));
+  GH_TRY(upload(ctx, &S->perm, perm.data(), perm.size()));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
 void sell_free(Sell* S) {
   for (void* p : {(void*)S->sptr, (void*)S->rlen, (void*)S->col, S->val, (void*)S->cidx,
-                  (void*)S->cptr, (void*)S->cbase, (void*)S->ptab, (void*)S->pstart})
+                  (void*)S->cptr, (void*)S->cbase, (void*)S->ptab, (void*)S->pstart,
+                  (void*)S->perm})
     if (p) (void)hipFree(p);
+  S->perm = nullptr;
   S->ptab = nullptr;
   S->pstart = nullptr;
   S->sptr = nullptr;

@@ -101,6 +101,9 @@ struct Sell {
   int64_t npat = 0, ntab = 0;
   int64_t patterned = 0;     // slices in pattern mode
   int64_t col_bytes = 0;     // column bytes a sweep reads (stored entries only)
+  // row run by each slot (valued operators whose rows were grouped by
+  // pattern; nullptr: slot = row); rlen and the slot arrays are in slot order
+  int32_t* perm = nullptr;
   // packed-weight operators: column | weight code << 29, one per slot
   uint32_t* col = nullptr;
   void* val = nullptr;       // values in `prec`, nullptr for packed-weight operators

@@ -269,16 +269,19 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __re
                                                  const int32_t* __restrict__ ptab,
                                                  const int32_t* __restrict__ pstart,
                                                  const VT* __restrict__ val,
+                                                 const int32_t* __restrict__ perm,
                                                  const XT* __restrict__ x, Op op) {
-  // Lanes past the last row repeat row n-1 with their own (padding) slots and
-  // only skip the store, so the control flow stays uniform across the wave and
-  // wave-uniform loads can use the scalar unit.
-  const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  const bool active = row0 < n;
-  const int64_t row = active ? row0 : n - 1;
+  // Lanes past the last slot repeat slot n-1 with their own (padding) slots
+  // and only skip the store, so the control flow stays uniform across the wave
+  // and wave-uniform loads can use the scalar unit.  Slot t runs row perm[t]
+  // (t itself without a permutation).
+  const int64_t slot0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const bool active = slot0 < n;
+  const int64_t slot = active ? slot0 : n - 1;
+  const int64_t row = perm ? (int64_t)perm[slot] : slot;
   const int lane = threadIdx.x & (kSlice - 1);
   // the slice index is the same for the 64 lanes of a wavefront
-  const int64_t s = __builtin_amdgcn_readfirstlane((int)(row >> 6));
+  const int64_t s = __builtin_amdgcn_readfirstlane((int)(slot >> 6));
   const VT* vp = val + sptr[s] + lane;
   const uint32_t meta = cptr[s];
   T acc = op.init(row);
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __re
     const int p0 = pstart[pid];
     acc = sell_row_pattern(ptab + p0, (uint32_t)row, vp, pstart[pid + 1] - p0, x, acc, op);
   } else {
-    const int r = rlen[row];  // entry count, or pattern id in pattern slices
+    const int r = rlen[slot];  // entry count, or pattern id in pattern slices
     if (meta & 0x40000000u) {
       const int p0 = pstart[r];
       acc = sell_row_pattern(ptab + p0, (uint32_t)row, vp, pstart[r + 1] - p0, x, acc, op);
@@ -336,7 +339,7 @@ inline int sell_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
   if (S.n == 0) return GH_OK;
   hipLaunchKernelGGL((k_sell<VT, XT, T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream,
                      S.n, S.sptr, S.rlen, S.cidx, S.cptr, S.cbase, S.ptab, S.pstart,
-                     (const VT*)S.val, x, op);
+                     (const VT*)S.val, S.perm, x, op);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }

@@ -609,7 +609,23 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
   GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cl.data(), vl.data(), GH_F64, false, &Lv.A64));
   Lv.own_A64 = true;
   if (params->prec != GH_F64) {
-    GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cl.data(), vl.data(), params->prec, false, &Lv.A));
+    // The reduced-precision V-cycle reproduces no reference bits, so its rows
+    // take their entries in ascending internal column order: rows of one kind
+    // then share one relative pattern, whatever order the caller's numbering
+    // gave them.
+    std::vector<int32_t> idx, cs(cl.size());
+    std::vector<double> vs(vl.size());
+    for (int64_t k = 0; k < n; ++k) {
+      idx.resize(rp[k + 1] - rp[k]);
+      for (size_t t = 0; t < idx.size(); ++t) idx[t] = rp[k] + (int32_t)t;
+      std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return cl[a] < cl[b]; });
+      for (size_t t = 0; t < idx.size(); ++t) {
+        cs[rp[k] + t] = cl[idx[t]];
+        std::copy(vl.begin() + (int64_t)idx[t] * BB, vl.begin() + (int64_t)(idx[t] + 1) * BB,
+                  vs.begin() + (int64_t)(rp[k] + t) * BB);
+      }
+    }
+    GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cs.data(), vs.data(), params->prec, false, &Lv.A));
     Lv.own_A = true;
   } else {
     Lv.A = Lv.A64;
@@ -891,6 +907,10 @@ int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int
 
 int gh_mg_nlevels(const gh_mg* mg) { return mg ? mg->nlevels : -1; }
 int64_t gh_mg_mgsteps(const gh_mg* mg) { return mg ? mg->mgsteps : -1; }
+const gh_csr* gh_mg_level_matrix(const gh_mg* mg, int level, int fp64) {
+  if (!mg || level < 0 || level >= mg->nlevels) return nullptr;
+  return fp64 ? mg->L[level].A64 : mg->L[level].A;
+}
 
 int gh_mg_linear(gh_ctx* ctx, gh_mg* mg, int finelevel, int coarselevel, gh_vec* u,
                  const gh_vec* f, gh_cginfo* info) {

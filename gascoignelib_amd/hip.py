@@ -129,6 +129,19 @@ class CudaCSRMatrixInterface:
         check(N.hip().gh_csr_create(ctx.h, self._n, n_comp, ptr(rowptr, N.i32p),
                                     ptr(col, N.i32p), ptr(val), N.PREC[prec], C.byref(self.h)))
 
+    @classmethod
+    def view(cls, ctx, handle, n_comp, owner):
+        """A matrix owned by another object (`owner` is kept alive; the handle
+        is never destroyed through the view)."""
+        v = cls.__new__(cls)
+        v.ctx = ctx
+        v.h = C.c_void_p(handle)
+        v._n = int(N.hip().gh_csr_n(v.h))
+        v.n_comp = n_comp
+        v.prec = {0: "f64", 1: "f32", 2: "f16"}.get(int(N.hip().gh_csr_prec(v.h)), "f64")
+        v._owner = owner
+        return v
+
     def n(self):
         return self._n
 
@@ -163,6 +176,10 @@ class CudaCSRMatrixInterface:
         check(N.hip().gh_jacobi(self.ctx.h, self.h, h.h))
 
     def free(self):
+        if getattr(self, "_owner", None) is not None:  # a view: the owner frees
+            self.h = C.c_void_p()
+            self._owner = None
+            return
         if self.h:
             N.hip().gh_csr_destroy(self.h)
             self.h = C.c_void_p()
@@ -377,6 +394,15 @@ class CudaMultiLevelSolver:
     @property
     def fine_matrix(self):
         return self.A[-1]
+
+    def level_matrix(self, level=None, fp64=True):
+        """The matrix the solver runs on `level` (finest by default), in its
+        internal numbering: a view owned by the solver."""
+        l = self.hier.nlevels - 1 if level is None else level
+        h = N.hip().gh_mg_level_matrix(self.h, l, int(bool(fp64)))
+        if not h:
+            raise ValueError(f"no matrix on level {l}")
+        return CudaCSRMatrixInterface.view(self.ctx, h, self.hier.level(l).ncomp, self)
 
     @property
     def fine_mask(self):

@@ -127,6 +127,7 @@ HIP_SIGNATURES = {
     "gh_mg_destroy": (C.c_int, [vp]),
     "gh_mg_nlevels": (C.c_int, [vp]),
     "gh_mg_mgsteps": (C.c_int64, [vp]),
+    "gh_mg_level_matrix": (vp, [vp, C.c_int, C.c_int]),
     "gh_mg_linear": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, P(GhCGInfo)]),
     "gh_mg_precondition": (C.c_int, [vp, vp, C.c_int, vp, vp, C.c_int]),
     "gh_gmres_solve": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, P(GhCGInfo),

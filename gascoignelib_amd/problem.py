@@ -104,17 +104,23 @@ def lexicographic_order(coords):
     return np.lexsort(keys).astype(np.int32)
 
 
-def permuted_csr(level, order):
+def permuted_csr(level, order, sort_columns=True):
     """The level's matrix with rows and columns renumbered (internal position
-    k = node order[k]), columns sorted, explicit zeros of the pattern kept,
-    blocks moved whole: the layout the ordered solver works in."""
+    k = node order[k]), explicit zeros of the pattern kept, blocks moved
+    whole.  sort_columns=False keeps each row's entries in the caller's order:
+    the matrix the ordered solver's FP64 path runs (sums in the reference's
+    order); sort_columns=True: ascending internal columns, the matrices of its
+    reduced-precision V-cycle."""
     n = level.n
     inv = np.empty(n, np.int64)
     inv[np.asarray(order)] = np.arange(n)
     rows = np.repeat(np.arange(n), np.diff(level.rowptr))
     r_new = inv[rows]
     c_new = inv[level.col]
-    idx = np.lexsort((c_new, r_new))
+    if sort_columns:
+        idx = np.lexsort((c_new, r_new))
+    else:
+        idx = np.argsort(r_new, kind="stable")
     rowptr = np.zeros(n + 1, np.int64)
     np.add.at(rowptr, r_new + 1, 1)
     rowptr = np.cumsum(rowptr)

@@ -280,6 +280,15 @@ int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int
 int gh_mg_destroy(gh_mg* mg);
 int gh_mg_nlevels(const gh_mg* mg);
 int64_t gh_mg_mgsteps(const gh_mg* mg);  /* the "--> mgstep" counter */
+/* The matrix level l of the solver runs, in its internal numbering (the
+ * caller's when it was created unordered): fp64 != 0 the FP64 operator of
+ * the outer iteration and the FP64 V-cycle, fp64 == 0 the V-cycle's operator
+ * in the solver's precision.  Owned by the solver (valid until
+ * gh_mg_destroy; never pass it to gh_csr_destroy).  Counterpart:
+ * StdMultiLevelSolver::GetSolver(l)->GetMatrix(A)
+ * (src/Solver/stdmultilevelsolver.h:150-158, src/Solver/stdsolver.h:335-342).
+ * NULL for a bad level. */
+const gh_csr* gh_mg_level_matrix(const gh_mg* mg, int level, int fp64);
 
 /* LinearMg (cuda/cudamultilevelsolver.cc:106-165,
  * src/Solver/stdmultilevelsolver.cc:411-458): MG iteration on finelevel from

@@ -97,3 +97,33 @@ def test_q2_adjoint_vs_oracle(ctx, h16):
     mg.Solve(Z, H.CudaVectorInterface.from_global(ctx, g), H.CGInfo(1e-12, 1e-30, 100),
              H.NLInfo(1e-12, 1e-30, 30), norm_kind=1)
     assert np.linalg.norm(Z.copy_back() - zo) <= 1e-10 * np.linalg.norm(zo)
+
+
+def test_level_matrix_view(ctx, h8):
+    """gh_mg_level_matrix: the ordered solver's FP64 level matrix is the
+    caller's matrix renumbered with each row's entries in the caller's order
+    (same sums, same bits); the f16 V-cycle's matrix agrees to f16 rounding"""
+    from gascoignelib_amd.problem import lexicographic_order
+    print("[level matrix view]", flush=True)
+    lev = h8.fine
+    lex = lexicographic_order(lev.coords)
+    x = np.random.default_rng(2).uniform(-1, 1, lev.n)
+    A = H.CudaCSRMatrixInterface(ctx, lev.rowptr, lev.col, lev.val)
+    Y = H.CudaVectorInterface(ctx, lev.n)
+    A.vmult(Y, H.CudaVectorInterface.from_global(ctx, x))
+    y = Y.copy_back()
+    for prec, fp64, tol in (("f64", True, 0.0), ("f16", False, 2e-3)):
+        mg = H.CudaMultiLevelSolver(ctx, h8, ndirect=200, order="lexicographic", prec=prec,
+                                    krylov="fgmres" if prec != "f64" else "gmres", **MGP)
+        Ai = mg.level_matrix(fp64=fp64)
+        assert Ai.n() == lev.n
+        vp = "f64" if fp64 else "f32"
+        Yi = H.CudaVectorInterface(ctx, lev.n, 1, vp)
+        Ai.vmult(Yi, H.CudaVectorInterface.from_global(ctx, x[lex], 1, vp))
+        yi = Yi.copy_back()
+        if tol == 0.0:
+            assert np.array_equal(yi, y[lex])
+        else:
+            assert np.abs(yi - y[lex]).max() <= tol * np.abs(y).max()
+        del Ai
+        mg.free()
