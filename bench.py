@@ -324,10 +324,15 @@ def kernel_only(ctx, hier, f, args):
     for _ in range(args.spmv_reps):
         C2.copy_from(C1)
     ctx.sync()
+    lf = Af.layout()
     print(json.dumps({"kernel_only": True, "reps": args.spmv_reps,
                       "residual_bytes_f64": sell_bytes(n, A.layout(), 8, 8),
                       "stored_entries": A.layout()["stored"],
                       "column_bytes": A.layout()["column_bytes"],
+                      "slices": A.layout()["slices"],
+                      "residual_bytes_mixed": sell_bytes(n, lf, 2 if args.mixed == "f16" else 4, 4),
+                      "mixed_stored_entries": lf["stored"], "mixed_column_bytes": lf["column_bytes"],
+                      "mixed_patterned_slices": lf["patterned"],
                       "copy_bytes_each_way": m * 8}), flush=True)
 
 
