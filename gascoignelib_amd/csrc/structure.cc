@@ -495,7 +495,7 @@ struct Patch1d {
 
 // Stokes-LPS patch element of NavierStokesLps<3> at U = 0 with
 // symmetrictensor = 0 (src/Problems/navierstokes.h:144-168 Galerkin part,
-// :248-254 pressure stabilisation, :218-224 lpspoint) on an axis-aligned box
+// :251-268 pressure stabilisation, :221-227 lpspoint) on an axis-aligned box
 // patch of 2x2x2 cells of size hx x hy x hz.  Components (p, u, v, w); the
 // 4x4 block of row node a (test function) and column node b (trial function):
 //   (0, d+1)   +int dphi_b/dx_d phi_a                     divergence

@@ -7,7 +7,7 @@ of BaseQ13dPatch::point (src/Discretization/Q2/baseq13dpatch.h:104-140) at the
 2x2x2 Gauss points of every sub-cell, the fluctuation lists of
 ElementLpsIntegrator::Projection (src/DofHandler/elementlpsintegrator.h:77-103),
 the entries of NavierStokesLps::Matrix / StabMatrix
-(src/Problems/navierstokes.h:144-168, 243-254) and lpspoint (:218-224) - on a
+(src/Problems/navierstokes.h:144-168, 251-268) and lpspoint (:221-227) - on a
 non-cubic box so that a swapped direction, a transposed block or a wrong sign
 fails.  Further closed forms: the patch structure (125 couplings for interior
 corner nodes), the divergence of linear fields, the LPS kernel (trilinear
