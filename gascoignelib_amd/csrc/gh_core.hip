@@ -1133,18 +1133,6 @@ int gh_vec_norm_l8(gh_ctx* ctx, const gh_vec* x, double* out) {
   return fetch_scalars(ctx, 0, 1, out);
 }
 
-// ----------------------------------------------------------------- matrices
-static int choose_seg(int64_t n, int64_t nnz) {
-  if (const char* e = getenv("GH_SPMV_SEG")) {
-    int s = atoi(e);
-    if (s == 1 || s == 2 || s == 4 || s == 8 || s == 16 || s == 32 || s == 64) return s;
-  }
-  double avg = n ? (double)nnz / (double)n : 1.0;
-  int s = 1;
-  while (s < 64 && s * 4 < avg) s *= 2;
-  return s;
-}
-
 }  // extern "C"
 
 namespace gh {
@@ -1182,7 +1170,6 @@ int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, co
   A->ncomp = ncomp;
   A->nnz = nnz;
   A->prec = prec;
-  A->seg = choose_seg(n, nnz);
   A->h_rowptr.assign(rowptr, rowptr + n + 1);
   A->h_col.assign(col, col + nnz);
   A->h_val.assign(val, val + nnz * BB);

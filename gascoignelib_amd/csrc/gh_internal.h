@@ -178,7 +178,6 @@ struct gh_csr {
   int32_t* col = nullptr;
   void* val = nullptr;   // nnz*B*B in prec (double / float / half)
   void* dinv = nullptr;  // n*B*B inverse diagonal blocks: double (F64) or float
-  int seg = 8;           // lanes per row in the scalar CSR kernels
   gh::Sell sell;         // SELL-64 copy used by the scalar (ncomp == 1) kernels
   gh::BSell bsell;       // component-row copy used by the block (ncomp 2..4) products
   // host copies kept for derived structures (transpose, coarse direct solve)

@@ -366,6 +366,19 @@ class CudaMultiLevelSolver {
                           vmax, u.get(), f.get(), &linfo.c, &nlinfo.c, norm_kind, &rep));
     return rep;
   }
+  // Vanka patches of a level (smoother 2): npatches x psize node indices in
+  // the caller's numbering (VankaSmoother::ConstructStructure,
+  // src/LinAlg/vankasmoother.cc:46-58)
+  void SetVankaPatches(int level, int64_t npatches, int psize, const int32_t* nodes) {
+    check(gh_mg_set_vanka(h_.get(), level, npatches, psize, nodes));
+  }
+  // The matrix the solver runs on a level (GetSolver(l)->GetMatrix(A)),
+  // owned by the solver: FP64, or the V-cycle's precision
+  const gh_csr* LevelMatrix(int level, bool fp64 = true) const {
+    const gh_csr* A = gh_mg_level_matrix(h_.get(), level, fp64 ? 1 : 0);
+    if (!A) throw Error(GH_EINVAL, "LevelMatrix: no such level");
+    return A;
+  }
   gh_mg* get() const { return h_.get(); }
 
  private:
@@ -373,6 +386,23 @@ class CudaMultiLevelSolver {
   int nlevels_;
   const CudaCSRMatrixInterface* fine_ = nullptr;
   Handle<gh_mg, gh_mg_destroy> h_;
+};
+
+// VankaSmoother (src/LinAlg/vankasmoother.h:65-125): patch LU factors of an
+// FP64 node-block matrix (FP64 or FP32 factors); Solve(x) in place
+class VankaSmoother {
+ public:
+  VankaSmoother(const Context& ctx, const CudaCSRMatrixInterface& A, int64_t npatches, int psize,
+                const int32_t* nodes, int prec = GH_F64)
+      : ctx_(&ctx) {
+    check(gh_vanka_create(ctx.get(), A.get(), npatches, psize, nodes, prec, h_.out()));
+  }
+  void Solve(CudaVectorInterface& x) const { check(gh_vanka_solve(ctx_->get(), h_.get(), x.get())); }
+  gh_vanka* get() const { return h_.get(); }
+
+ private:
+  const Context* ctx_;
+  Handle<gh_vanka, gh_vanka_destroy> h_;
 };
 
 }  // namespace gascoigne_hip
