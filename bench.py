@@ -104,13 +104,15 @@ def main():
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
     ap.add_argument("--workload", default="poisson128",
-                    choices=["poisson128", "adjoint96", "stokes64", "q2_64"],
+                    choices=["poisson128", "adjoint96", "stokes64", "q2_64", "q2ada"],
                     help="poisson128: 3D Q1 Poisson on 128^3 cells (the headline); adjoint96: "
                          "variable-coefficient diffusion on 96^3 cells, one gradient evaluation "
                          "(forward solve + adjoint solve + dJ/dk per cell) per step; stokes64: "
                          "Stokes-LPS channel on 64^3 cells, 4x4 blocks, Vanka multigrid; q2_64: "
                          "Poisson with the triquadratic patch element on 64^3 cells (125-point "
-                         "rows), forward and adjoint solves")
+                         "rows), forward and adjoint solves; q2ada: the same on 64^3 cells "
+                         "(32^3 triquadratic elements) plus 2 adaptive levels inside a seeded "
+                         "ball of radius 0.25, with hanging-node constraints (1,432,971 DOF)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
@@ -128,6 +130,8 @@ def main():
         return bench_stokes(args)
     if args.workload == "q2_64":
         return bench_q2(args)
+    if args.workload == "q2ada":
+        return bench_q2(args, adaptive=2)
 
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.problem import lattice_rhs
@@ -601,33 +605,41 @@ def bench_stokes(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_q2(args):
+def bench_q2(args, adaptive=0):
     """Poisson with the triquadratic patch element (CGDiscQ23d, Laplace<3>) on
     the unit cube refined to 64^3 cells: 274,625 DOF, 16,974,593 entries (up to
     125 per row), 6 levels, nodal transfers, Dirichlet 0, seeded f.  One step =
     one solve to 1e-10 relative FP64 residual (FP64 GMRES in Newton vs f16-value
     FP32 V-cycle in FP64 flexible GMRES, V(2,2) Jacobi omega 0.8, levels below
     1000 nodes direct), then the adjoint solve for J = mean of the nodal values
-    in the ball of radius 0.25 around a seeded centre."""
+    in the ball of radius 0.25 around a seeded centre.
+
+    With adaptive = 2 the 64^3 mesh (32^3 triquadratic elements) is followed by
+    two levels refined patch-wise inside that same ball: 1,432,971 DOF, 86.6 M
+    entries, Q2 hanging-node constraints on the refinement interfaces (49,840
+    edge and 24,920 face nodes on the finest level) with the operator condensed
+    over them; f and the functional are distributed onto the parents of the
+    hanging nodes (HNDistribute) before the solves."""
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.problem import (lattice_rhs, lexicographic_order, permuted_csr,
                                           splitmix64_uniform)
     from gascoignelib_amd.structure import q2_hierarchy
 
     cells = 64 if args.cells == 128 else args.cells
+    centre = 0.25 + 0.5 * (0.5 * (splitmix64_uniform(4242, 3) + 1.0))
     t0 = time.perf_counter()
-    hier = q2_hierarchy(int(round(np.log2(cells))))
+    hier = q2_hierarchy(int(round(np.log2(cells))), adaptive=adaptive, radius=0.25,
+                        centre=tuple(centre))
     t_struct = time.perf_counter() - t0
     fine = hier.fine
     n = fine.n
     f = lattice_rhs(fine.coords, seed=2024)
-    f[fine.dirichlet] = 0.0
-    centre = 0.25 + 0.5 * (0.5 * (splitmix64_uniform(4242, 3) + 1.0))
     inside = np.linalg.norm(fine.coords - centre, axis=1) < 0.25
     g = inside / max(1, int(inside.sum()))
-    g[fine.dirichlet] = 0.0
-    print(f"# q2 {cells}^3: {n} DOF, nnz {fine.nnz}, {hier.nlevels} levels, ball of "
-          f"{int(inside.sum())} nodes, built in {t_struct:.1f}s", file=sys.stderr, flush=True)
+    nhn = (len(fine.hn_edges), len(fine.hn_faces))
+    print(f"# q2 {cells}^3 + {adaptive} adaptive: {n} DOF, nnz {fine.nnz}, {hier.nlevels} "
+          f"levels, hanging nodes {nhn}, ball of {int(inside.sum())} nodes, built in "
+          f"{t_struct:.1f}s", file=sys.stderr, flush=True)
     ctx = H.Context(0)
     common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
                   ndirect=1000, order=None if args.order == "none" else args.order)
@@ -641,6 +653,15 @@ def bench_q2(args):
     t_setup = time.perf_counter() - t0
     F = H.CudaVectorInterface.from_global(ctx, f)
     G = H.CudaVectorInterface.from_global(ctx, g)
+    if sum(nhn):
+        hn = H.HNStructure(ctx, 23, n, fine.hn_edges, fine.hn_faces)
+        hn.distribute(F)
+        hn.distribute(G)
+        hn.free()
+    mask = H.DirichletMask(ctx, n, 1, fine.dirichlet)
+    mask.zero(F)
+    mask.zero(G)
+    f, g = F.copy_back(), G.copy_back()
     U, Z = H.CudaVectorInterface(ctx, n), H.CudaVectorInterface(ctx, n)
 
     def solve(mg, X, B, tol):
@@ -690,9 +711,13 @@ def bench_q2(args):
         "dtype": f"f64 outer / f32 V-cycle ({args.mixed}-stored matrix values)",
         "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, "
                 "Dirichlet 0; J = mean of u over a seeded ball of radius 0.25",
-        "config": {"workload": f"3D Poisson, triquadratic patch element (Q23d), {cells}^3 cells, "
-                               "uniform (no adaptive levels)", "dof": n, "nnz": fine.nnz,
-                   "levels": hier.nlevels},
+        "config": {"workload": f"3D Poisson, triquadratic patch element (Q23d), {cells}^3 cells"
+                               + (f" + {adaptive} adaptive levels in a ball of radius 0.25 "
+                                  "around a seeded centre, Q2 hanging nodes" if adaptive
+                                  else ", uniform (no adaptive levels)"),
+                   "dof": n, "nnz": fine.nnz, "levels": hier.nlevels,
+                   "hanging_nodes": {"edge": nhn[0], "face": nhn[1]},
+                   "centre": [float(c) for c in centre]},
         "fp64": {"forward_ms": res["fp64"]["forward"]["ms"],
                  "forward_gmres": res["fp64"]["forward"]["gmres"],
                  "adjoint_ms": res["fp64"]["adjoint"]["ms"],

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -41,6 +42,10 @@ struct Mesh {
   // transfer from the next coarser mesh (empty on the coarsest)
   int64_t ncoarse = 0;
   std::vector<int32_t> zweier, vierer, achter;
+  // adaptive patch meshes: refinement generation of each patch (empty:
+  // uniform) and the Q2 hanging-node constraints (4 / 10 entries each)
+  std::vector<int8_t> plev;
+  std::vector<int32_t> hn_e, hn_f;
   int64_t n() const { return (int64_t)X.size() / 3; }
   int64_t ncells() const { return (int64_t)cells.size() / npc; }
 };
@@ -119,8 +124,11 @@ inline uint64_t fkey(const int32_t q[4]) {
   return ekey(q[m], q[(m + 2) & 3]);
 }
 
-// one uniform refinement (nested numbering)
-int refine(const Mesh& C, Mesh& F) {
+// One refinement (nested numbering): every cell, or the cells with mark[c]
+// set (the others are copied; cell order is kept, a refined cell's 8 (4)
+// children take its place, so a mark that is uniform over each group of
+// sibling cells keeps the patch grouping).
+int refine(const Mesh& C, Mesh& F, const std::vector<uint8_t>* mark = nullptr) {
   const int dim = C.dim;
   F.dim = dim;
   F.npc = C.npc;
@@ -128,14 +136,18 @@ int refine(const Mesh& C, Mesh& F) {
   F.X = C.X;
   F.ncoarse = n0;
   std::vector<double>& X = F.X;
+  auto marked = [&](int64_t c) { return !mark || (*mark)[c] != 0; };
+  std::vector<int32_t> cmid(nc, -1);
   // cell midpoints
   for (int64_t c = 0; c < nc; ++c) {
+    if (!marked(c)) continue;
     const int32_t* v = &C.cells[c * C.npc];
     double s[3] = {0, 0, 0};
     for (int m = 0; m < C.npc; ++m)
       for (int d = 0; d < 3; ++d) s[d] += C.X[3 * v[m] + d];
+    const int32_t id = (int32_t)(X.size() / 3);
+    cmid[c] = id;
     for (int d = 0; d < 3; ++d) X.push_back(s[d] / C.npc);
-    const int32_t id = (int32_t)(n0 + c);
     if (dim == 3) {
       F.achter.push_back(id);
       for (int m = 0; m < 8; ++m) F.achter.push_back(v[m]);
@@ -159,6 +171,7 @@ int refine(const Mesh& C, Mesh& F) {
   emap.reserve(nc * (dim == 3 ? 4 : 3));
   if (dim == 3) {
     for (int64_t c = 0; c < nc; ++c) {
+      if (!marked(c)) continue;
       const int32_t* v = &C.cells[c * 8];
       for (int ax = 0; ax < 3; ++ax)
         for (int side = 0; side < 2; ++side) {
@@ -187,6 +200,7 @@ int refine(const Mesh& C, Mesh& F) {
   }
   // edge midpoints
   for (int64_t c = 0; c < nc; ++c) {
+    if (!marked(c)) continue;
     const int32_t* v = &C.cells[c * C.npc];
     const int nedge = dim == 3 ? 12 : 4;
     for (int e = 0; e < nedge; ++e) {
@@ -220,6 +234,10 @@ int refine(const Mesh& C, Mesh& F) {
   F.cells.reserve(nc * C.npc * (dim == 3 ? 8 : 4));
   for (int64_t c = 0; c < nc; ++c) {
     const int32_t* v = &C.cells[c * C.npc];
+    if (!marked(c)) {
+      F.cells.insert(F.cells.end(), v, v + C.npc);
+      continue;
+    }
     if (dim == 3) {
       int32_t G[3][3][3];
       for (int a = 0; a < 3; ++a)
@@ -229,7 +247,7 @@ int refine(const Mesh& C, Mesh& F) {
             if (odd == 0) {
               G[a][b][k] = corner(v, a / 2, b / 2, k / 2);
             } else if (odd == 3) {
-              G[a][b][k] = (int32_t)(n0 + c);
+              G[a][b][k] = cmid[c];
             } else if (odd == 1) {
               int p[3] = {a, b, k}, q[3] = {a, b, k};
               for (int d = 0; d < 3; ++d)
@@ -267,7 +285,7 @@ int refine(const Mesh& C, Mesh& F) {
           if (odd == 0)
             G[a][b] = corner(v, a / 2, b / 2, 0);
           else if (odd == 2)
-            G[a][b] = (int32_t)(n0 + c);
+            G[a][b] = cmid[c];
           else {
             int p[2] = {a, b}, q[2] = {a, b};
             for (int d = 0; d < 2; ++d)
@@ -290,7 +308,12 @@ int refine(const Mesh& C, Mesh& F) {
     for (int64_t f = 0; f < nb; ++f) {
       const int32_t* q = &C.bnd[4 * f];
       auto fit = fmap.find(fkey(q));
-      if (fit == fmap.end()) return fail("boundary quad %lld is not a cell face", (long long)f);
+      if (fit == fmap.end()) {
+        if (!mark) return fail("boundary quad %lld is not a cell face", (long long)f);
+        F.bnd.insert(F.bnd.end(), q, q + 4);  // face of a cell left as it is
+        F.bcol.push_back(C.bcol[f]);
+        continue;
+      }
       const int32_t m = fit->second;
       int32_t e[4];
       for (int t = 0; t < 4; ++t) e[t] = emap.at(ekey(q[t], q[(t + 1) & 3]));
@@ -307,7 +330,12 @@ int refine(const Mesh& C, Mesh& F) {
     for (int64_t f = 0; f < nb; ++f) {
       const int32_t a = C.bnd[2 * f], b = C.bnd[2 * f + 1];
       auto it = emap.find(ekey(a, b));
-      if (it == emap.end()) return fail("boundary line %lld is not a cell edge", (long long)f);
+      if (it == emap.end()) {
+        if (!mark) return fail("boundary line %lld is not a cell edge", (long long)f);
+        F.bnd.insert(F.bnd.end(), {a, b});
+        F.bcol.push_back(C.bcol[f]);
+        continue;
+      }
       F.bnd.insert(F.bnd.end(), {a, it->second, it->second, b});
       F.bcol.push_back(C.bcol[f]);
       F.bcol.push_back(C.bcol[f]);
@@ -578,10 +606,368 @@ int patch_box(const Mesh& M, const int32_t* P, double h[3]) {
   return (h[0] > 0 && h[1] > 0 && h[2] > 0) ? 0 : -1;
 }
 
+// Laplace on the triquadratic patch elements of M, unconstrained, on the patch
+// structure (rp, c) of M.
+int q2_matrix(const Mesh* M, std::vector<int32_t>& rp, std::vector<int32_t>& c,
+              std::vector<double>& vals) {
+  if (M->dim != 3) return fail("Q2 patch assembly is three-dimensional");
+  std::vector<int32_t> P;
+  if (patches(*M, P)) return -1;
+  const int64_t n = M->n(), np = (int64_t)P.size() / 27;
+  if (structure(n, np, 27, P.data(), rp, c)) return -1;
+  // Laplace<3> on the triquadratic patch element (CGDiscQ23d: the 27 patch
+  // nodes carry the Lagrange basis of BaseQ23d; src/Problems/laplace.h,
+  // src/DofHandler/cgdisc.h:1291): per direction the quadratic element on
+  // [0, L], L = 2h, nodes 0, L/2, L, with stiffness (7,-8,1;-8,16,-8;1,-8,7)/(3L)
+  // and mass (4,2,-1;2,16,2;-1,2,4) L/30, exact (the reference's 3x3x3 Gauss
+  // rule is exact for these products)
+  auto q2 = [](double L, double S[3][3], double Mm[3][3]) {
+    const double s[3][3] = {{7, -8, 1}, {-8, 16, -8}, {1, -8, 7}};
+    const double m[3][3] = {{4, 2, -1}, {2, 16, 2}, {-1, 2, 4}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        S[a][b] = s[a][b] / (3.0 * L);
+        Mm[a][b] = m[a][b] * L / 30.0;
+      }
+  };
+  std::vector<int32_t> shape(np);
+  std::vector<std::array<double, 3>> sizes;
+  std::vector<std::vector<double>> elem;  // 27 x 27 per shape
+  for (int64_t p = 0; p < np; ++p) {
+    double hb[3];
+    if (patch_box(*M, &P[p * 27], hb)) return fail("patch %lld is not an axis-aligned box of 8 equal cells", (long long)p);
+    int sidx = 0;
+    while (sidx < (int)sizes.size() && !(sizes[sidx][0] == hb[0] && sizes[sidx][1] == hb[1] && sizes[sidx][2] == hb[2])) ++sidx;
+    if (sidx == (int)sizes.size()) {
+      if (sidx >= 64) return fail("Q2 assembly: more than 64 distinct patch sizes");
+      sizes.push_back({hb[0], hb[1], hb[2]});
+      double SX[3][3], MX[3][3], SY[3][3], MY[3][3], SZ[3][3], MZ[3][3];
+      q2(2 * hb[0], SX, MX);
+      q2(2 * hb[1], SY, MY);
+      q2(2 * hb[2], SZ, MZ);
+      std::vector<double> E(27 * 27);
+      for (int a = 0; a < 27; ++a)
+        for (int b = 0; b < 27; ++b) {
+          const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+          const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+          E[a * 27 + b] = SX[ax][bx] * MY[ay][by] * MZ[az][bz] + MX[ax][bx] * SY[ay][by] * MZ[az][bz] +
+                          MX[ax][bx] * MY[ay][by] * SZ[az][bz];
+        }
+      elem.push_back(std::move(E));
+    }
+    shape[p] = sidx;
+  }
+  std::vector<int32_t> pstart(n + 1, 0), plist(np * 27);
+  std::vector<int8_t> ploc(np * 27);
+  for (int64_t k = 0; k < np * 27; ++k) pstart[P[k] + 1]++;
+  for (int64_t i = 0; i < n; ++i) pstart[i + 1] += pstart[i];
+  {
+    std::vector<int32_t> fill(pstart.begin(), pstart.end() - 1);
+    for (int64_t p = 0; p < np; ++p)
+      for (int a = 0; a < 27; ++a) {
+        const int32_t k = fill[P[p * 27 + a]]++;
+        plist[k] = (int32_t)p;
+        ploc[k] = (int8_t)a;
+      }
+  }
+  vals.assign(rp[n], 0.0);
+  double* val = vals.data();
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : bad)
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t rs = rp[i], re = rp[i + 1];
+    for (int32_t q = rs; q < re; ++q) val[q] = 0.0;
+    for (int32_t k = pstart[i]; k < pstart[i + 1]; ++k) {
+      const int64_t p = plist[k];
+      const int a = ploc[k];
+      const double* E = elem[shape[p]].data() + a * 27;
+      for (int b = 0; b < 27; ++b) {
+        const int32_t j = P[p * 27 + b];
+        const int32_t* it = std::lower_bound(c.data() + rs, c.data() + re, j);
+        if (it == c.data() + re || *it != j) {
+          bad++;
+          continue;
+        }
+        val[it - c.data()] += E[b];
+      }
+    }
+  }
+  if (bad) return fail("patch structure misses %d couplings", bad);
+  return 0;
+}
+
+// ------------------------------------------------ adaptive Q2 patch meshes
+// Integer lattice positions of the nodes (coordinates must be multiples of
+// 2^-20 in [0, 2): the refined unit-box meshes of the benchmark workloads).
+int lattice(const Mesh& M, std::vector<std::array<int32_t, 3>>& L) {
+  const double s = 1048576.0;
+  L.resize(M.n());
+  for (int64_t i = 0; i < M.n(); ++i)
+    for (int d = 0; d < 3; ++d) {
+      const double v = M.X[3 * i + d] * s;
+      const double r = std::nearbyint(v);
+      if (std::fabs(v - r) > 1e-6 || r < 0 || r >= 2097152.0)
+        return fail("adaptive Q2 meshes need node coordinates on the dyadic lattice of [0,2)^3");
+      L[i][d] = (int32_t)r;
+    }
+  return 0;
+}
+inline uint64_t lkey(int64_t a, int64_t b, int64_t c) {
+  return ((uint64_t)a << 42) | ((uint64_t)b << 21) | (uint64_t)c;
+}
+
+// Q2 hanging nodes of a patch mesh: a node at a half position of a patch's
+// face (between two of its Q2 nodes) is interpolated from that patch, as in
+// HNStructureQ23d (src/Discretization/Q2/hnstructureq23d.cc:120-156):
+//   edge constraint [h, near, mid, far] on a Q2 line of the face, weights
+//     3/8, 3/4, -1/8 (the quadratic Lagrange basis at a quarter point);
+//   face constraint [h, g_0..g_8], g_{3a+b} the Q2 node at position a
+//     (near, mid, far) along one face direction and b along the other,
+//     weight w_a w_b.
+int q2_hanging(const Mesh& M, std::vector<int32_t>& E, std::vector<int32_t>& Fc) {
+  E.clear();
+  Fc.clear();
+  std::vector<int32_t> P;
+  if (patches(M, P)) return -1;
+  std::vector<std::array<int32_t, 3>> L;
+  if (lattice(M, L)) return -1;
+  std::unordered_map<uint64_t, int32_t> at;
+  at.reserve(M.n() * 2);
+  for (int64_t i = 0; i < M.n(); ++i) at.emplace(lkey(L[i][0], L[i][1], L[i][2]), (int32_t)i);
+  std::vector<int8_t> done(M.n(), 0);
+  const int64_t np = (int64_t)P.size() / 27;
+  for (int64_t p = 0; p < np; ++p) {
+    const int32_t* pn = &P[p * 27];
+    const std::array<int32_t, 3>& o = L[pn[0]];
+    const int32_t cs = L[pn[1]][0] - o[0];  // Q2 node spacing (a cell) in lattice units
+    if (cs <= 0 || (cs & 1) || L[pn[3]][1] - o[1] != cs || L[pn[9]][2] - o[2] != cs)
+      return fail("patch %lld is not a cube of 8 equal cells on the lattice", (long long)p);
+    auto node = [&](int i0, int i1, int i2) { return pn[i0 + 3 * i1 + 9 * i2]; };
+    for (int ax = 0; ax < 3; ++ax)
+      for (int side = 0; side <= 2; side += 2) {
+        const int u = (ax + 1) % 3, w = (ax + 2) % 3;
+        for (int iu = 0; iu <= 4; ++iu)
+          for (int iw = 0; iw <= 4; ++iw) {
+            if (!(iu & 1) && !(iw & 1)) continue;  // a Q2 node of the patch
+            int64_t q[3];
+            q[ax] = o[ax] + (int64_t)side * cs;
+            q[u] = o[u] + (int64_t)iu * (cs / 2);
+            q[w] = o[w] + (int64_t)iw * (cs / 2);
+            auto it = at.find(lkey(q[0], q[1], q[2]));
+            if (it == at.end()) continue;
+            const int32_t h = it->second;
+            if (done[h]) continue;
+            done[h] = 1;
+            // per in-face direction: near end, middle, far end of the Q2 line
+            auto line = [](int i, int (&idx)[3]) {
+              const int nr = (i == 1) ? 0 : 2;
+              idx[0] = nr;
+              idx[1] = 1;
+              idx[2] = 2 - nr;
+            };
+            int I[3];
+            I[ax] = side;
+            if ((iu & 1) && (iw & 1)) {
+              int U[3], W[3];
+              line(iu, U);
+              line(iw, W);
+              Fc.push_back(h);
+              for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                  I[u] = U[a];
+                  I[w] = W[b];
+                  Fc.push_back(node(I[0], I[1], I[2]));
+                }
+            } else {
+              const int dir = (iu & 1) ? u : w;
+              const int other = (iu & 1) ? w : u;
+              int D[3];
+              line((iu & 1) ? iu : iw, D);
+              I[other] = ((iu & 1) ? iw : iu) / 2;
+              E.push_back(h);
+              for (int a = 0; a < 3; ++a) {
+                I[dir] = D[a];
+                E.push_back(node(I[0], I[1], I[2]));
+              }
+            }
+          }
+      }
+  }
+  // no constraint may depend on another hanging node
+  for (size_t k = 0; k < E.size(); k += 4)
+    for (int t = 1; t < 4; ++t)
+      if (done[E[k + t]]) return fail("hanging node %d has a hanging parent", E[k]);
+  for (size_t k = 0; k < Fc.size(); k += 10)
+    for (int t = 1; t < 10; ++t)
+      if (done[Fc[k + t]]) return fail("hanging node %d has a hanging parent", Fc[k]);
+  return 0;
+}
+
+// One adaptive step on a patch mesh: refine every candidate patch with a
+// node inside the ball (|x - c| < r; MeshAgent::refine_nodes refines the
+// patches of the given nodes).  Candidates are the patches of the newest
+// generation; from the second step on a candidate is refined only when every
+// patch sharing one of its corners is of its generation or newer, so that
+// neighbouring patches differ by at most one generation and no hanging node
+// has a hanging parent.
+int adapt(const Mesh& C, Mesh& F, double r, const double ctr[3]) {
+  std::vector<int32_t> P;
+  if (patches(C, P)) return -1;
+  const int64_t np = (int64_t)P.size() / 27;
+  std::vector<int8_t> lev = C.plev;
+  if ((int64_t)lev.size() != np) lev.assign(np, 0);
+  int8_t top = 0;
+  for (int8_t v : lev) top = std::max(top, v);
+  std::vector<int32_t> nstart(C.n() + 1, 0), nlist(np * 27);
+  for (int64_t k = 0; k < np * 27; ++k) nstart[P[k] + 1]++;
+  for (int64_t i = 0; i < C.n(); ++i) nstart[i + 1] += nstart[i];
+  {
+    std::vector<int32_t> fill(nstart.begin(), nstart.end() - 1);
+    for (int64_t k = 0; k < np * 27; ++k) nlist[fill[P[k]]++] = (int32_t)(k / 27);
+  }
+  const int corners[8] = {0, 2, 6, 8, 18, 20, 24, 26};
+  std::vector<uint8_t> pmark(np, 0), cmark(C.ncells(), 0);
+  int64_t nmark = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    if (lev[p] != top) continue;
+    bool inside = false;
+    for (int a = 0; a < 27 && !inside; ++a) {
+      const double* x = &C.X[3 * (int64_t)P[p * 27 + a]];
+      const double d2 = (x[0] - ctr[0]) * (x[0] - ctr[0]) + (x[1] - ctr[1]) * (x[1] - ctr[1]) +
+                        (x[2] - ctr[2]) * (x[2] - ctr[2]);
+      inside = d2 < r * r;
+    }
+    if (!inside) continue;
+    bool ok = true;
+    for (int t = 0; t < 8 && ok; ++t) {
+      const int32_t v = P[p * 27 + corners[t]];
+      for (int32_t k = nstart[v]; k < nstart[v + 1] && ok; ++k) ok = lev[nlist[k]] >= top;
+    }
+    if (!ok) continue;
+    pmark[p] = 1;
+    nmark++;
+    for (int c = 0; c < 8; ++c) cmark[p * 8 + c] = 1;
+  }
+  if (refine(C, F, &cmark)) return -1;
+  F.plev.clear();
+  for (int64_t p = 0; p < np; ++p) {
+    if (pmark[p])
+      F.plev.insert(F.plev.end(), 8, (int8_t)(lev[p] + 1));
+    else
+      F.plev.push_back(lev[p]);
+  }
+  return q2_hanging(F, F.hn_e, F.hn_f);
+}
+
+// Condensed Q2 operator of a patch mesh with hanging nodes: A_c = T^T A T,
+// T the interpolation that sets hanging values from their parents (the
+// element-wise condensation of CGDisc::LocalToGlobal with CondenseHanging,
+// src/DofHandler/cgdisc.h:353-366, in global form); hanging rows and columns
+// are then identity rows (as in the reference's assembled matrices), and the
+// Dirichlet rows and columns of the given nodes likewise.
+int condense(int64_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& cl,
+             const std::vector<double>& vl, const std::vector<int32_t>& E,
+             const std::vector<int32_t>& Fc, const std::vector<uint8_t>& isdir,
+             std::vector<int32_t>& rpo, std::vector<int32_t>& clo, std::vector<double>& vlo) {
+  // hanging node -> (parent, weight)
+  std::vector<int32_t> hstart(n + 1, 0);
+  std::vector<std::pair<int32_t, double>> hpar;
+  std::vector<std::vector<std::pair<int32_t, double>>> tmp(n);
+  const double w1[3] = {0.375, 0.75, -0.125};
+  for (size_t k = 0; k < E.size(); k += 4)
+    for (int t = 0; t < 3; ++t) tmp[E[k]].push_back({E[k + 1 + t], w1[t]});
+  for (size_t k = 0; k < Fc.size(); k += 10)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) tmp[Fc[k]].push_back({Fc[k + 1 + 3 * a + b], w1[a] * w1[b]});
+  for (int64_t i = 0; i < n; ++i) {
+    hstart[i + 1] = hstart[i] + (int32_t)tmp[i].size();
+    hpar.insert(hpar.end(), tmp[i].begin(), tmp[i].end());
+  }
+  std::vector<std::vector<std::pair<int32_t, double>>>().swap(tmp);
+  auto hanging = [&](int64_t i) { return hstart[i + 1] > hstart[i]; };
+  // parent -> hanging rows that feed it (for the row condensation)
+  std::vector<int32_t> fstart(n + 1, 0), flist;
+  std::vector<double> fw;
+  for (const auto& e : hpar) fstart[e.first + 1]++;
+  for (int64_t i = 0; i < n; ++i) fstart[i + 1] += fstart[i];
+  flist.resize(hpar.size());
+  fw.resize(hpar.size());
+  {
+    std::vector<int32_t> fill(fstart.begin(), fstart.end() - 1);
+    for (int64_t h = 0; h < n; ++h)
+      for (int32_t k = hstart[h]; k < hstart[h + 1]; ++k) {
+        const int32_t q = fill[hpar[k].first]++;
+        flist[q] = (int32_t)h;
+        fw[q] = hpar[k].second;
+      }
+  }
+  // B = A T, row by row (hanging columns replaced by their parents)
+  auto brow = [&](int64_t i, double s, std::vector<std::pair<int32_t, double>>& out) {
+    for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
+      const int32_t j = cl[q];
+      if (hanging(j)) {
+        for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k)
+          out.push_back({hpar[k].first, s * (vl[q] * hpar[k].second)});
+      } else {
+        out.push_back({j, s * vl[q]});
+      }
+    }
+  };
+  std::vector<std::vector<std::pair<int32_t, double>>> rows(n);
+#pragma omp parallel for schedule(dynamic, 512)
+  for (int64_t i = 0; i < n; ++i) {
+    std::vector<std::pair<int32_t, double>>& r = rows[i];
+    if (hanging(i)) {
+      r.push_back({(int32_t)i, 1.0});
+      continue;
+    }
+    brow(i, 1.0, r);
+    for (int32_t k = fstart[i]; k < fstart[i + 1]; ++k) brow(flist[k], fw[k], r);
+    std::stable_sort(r.begin(), r.end(),
+                     [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+                       return a.first < b.first;
+                     });
+    size_t m = 0;
+    for (size_t t = 0; t < r.size(); ++t) {
+      if (m && r[m - 1].first == r[t].first)
+        r[m - 1].second += r[t].second;
+      else
+        r[m++] = r[t];
+    }
+    r.resize(m);
+    if (isdir[i]) {
+      for (auto& e : r) e.second = (e.first == i) ? 1.0 : 0.0;
+    } else {
+      for (auto& e : r)
+        if (isdir[e.first]) e.second = 0.0;
+    }
+  }
+  rpo.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) rpo[i + 1] = rpo[i] + (int32_t)rows[i].size();
+  clo.resize(rpo[n]);
+  vlo.resize(rpo[n]);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t q = rpo[i];
+    for (const auto& e : rows[i]) {
+      clo[q] = e.first;
+      vlo[q++] = e.second;
+    }
+    std::vector<std::pair<int32_t, double>>().swap(rows[i]);
+  }
+  return 0;
+}
+
 }  // namespace
 
 struct ghs_hier {
   std::vector<Mesh> levels;  // 0 = coarsest kept level
+  // condensed Q2 operators between the size query and the copy-out
+  struct Condensed {
+    std::vector<int32_t> rp, col;
+    std::vector<double> val;
+  };
+  std::map<int, Condensed> q2c;
 };
 
 extern "C" {
@@ -598,6 +984,29 @@ int ghs_build(const char* inp_path, int dim, int prerefine, int drop_coarse, ghs
   for (int r = 0; r < prerefine; ++r) {
     all.emplace_back();
     if (refine(all[r], all[r + 1])) return -1;
+  }
+  ghs_hier* h = new ghs_hier();
+  h->levels.assign(std::make_move_iterator(all.begin() + drop_coarse),
+                   std::make_move_iterator(all.end()));
+  *out = h;
+  return 0;
+}
+
+int ghs_build_adaptive(const char* inp_path, int prerefine, int drop_coarse, int nadapt,
+                       double radius, const double* centre, ghs_hier** out) {
+  if (!out || !inp_path || !centre || prerefine < 1 || drop_coarse < 0 ||
+      drop_coarse > prerefine || nadapt < 0 || nadapt > 2 || !(radius > 0))
+    return fail("ghs_build_adaptive: bad arguments (nadapt 0..2, prerefine >= 1)");
+  *out = nullptr;
+  std::vector<Mesh> all(1);
+  if (read_inp(inp_path, 3, all[0])) return -1;
+  for (int r = 0; r < prerefine; ++r) {
+    all.emplace_back();
+    if (refine(all[r], all[r + 1])) return -1;
+  }
+  for (int s = 0; s < nadapt; ++s) {
+    all.emplace_back();
+    if (adapt(all[all.size() - 2], all.back(), radius, centre)) return -1;
   }
   ghs_hier* h = new ghs_hier();
   h->levels.assign(std::make_move_iterator(all.begin() + drop_coarse),
@@ -881,96 +1290,67 @@ int ghs_level_assemble_q2(const ghs_hier* h, int l, int ncolours, const int32_t*
                           double* val) {
   const Mesh* M = lev(h, l);
   if (!M) return -1;
-  if (M->dim != 3) return fail("Q2 patch assembly is three-dimensional");
-  std::vector<int32_t> P, rp, c, d;
-  if (patches(*M, P)) return -1;
-  const int64_t n = M->n(), np = (int64_t)P.size() / 27;
-  if (structure(n, np, 27, P.data(), rp, c)) return -1;
-  // Laplace<3> on the triquadratic patch element (CGDiscQ23d: the 27 patch
-  // nodes carry the Lagrange basis of BaseQ23d; src/Problems/laplace.h,
-  // src/DofHandler/cgdisc.h:1291): per direction the quadratic element on
-  // [0, L], L = 2h, nodes 0, L/2, L, with stiffness (7,-8,1;-8,16,-8;1,-8,7)/(3L)
-  // and mass (4,2,-1;2,16,2;-1,2,4) L/30, exact (the reference's 3x3x3 Gauss
-  // rule is exact for these products)
-  auto q2 = [](double L, double S[3][3], double Mm[3][3]) {
-    const double s[3][3] = {{7, -8, 1}, {-8, 16, -8}, {1, -8, 7}};
-    const double m[3][3] = {{4, 2, -1}, {2, 16, 2}, {-1, 2, 4}};
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) {
-        S[a][b] = s[a][b] / (3.0 * L);
-        Mm[a][b] = m[a][b] * L / 30.0;
-      }
-  };
-  std::vector<int32_t> shape(np);
-  std::vector<std::array<double, 3>> sizes;
-  std::vector<std::vector<double>> elem;  // 27 x 27 per shape
-  for (int64_t p = 0; p < np; ++p) {
-    double hb[3];
-    if (patch_box(*M, &P[p * 27], hb)) return fail("patch %lld is not an axis-aligned box of 8 equal cells", (long long)p);
-    int sidx = 0;
-    while (sidx < (int)sizes.size() && !(sizes[sidx][0] == hb[0] && sizes[sidx][1] == hb[1] && sizes[sidx][2] == hb[2])) ++sidx;
-    if (sidx == (int)sizes.size()) {
-      if (sidx >= 64) return fail("Q2 assembly: more than 64 distinct patch sizes");
-      sizes.push_back({hb[0], hb[1], hb[2]});
-      double SX[3][3], MX[3][3], SY[3][3], MY[3][3], SZ[3][3], MZ[3][3];
-      q2(2 * hb[0], SX, MX);
-      q2(2 * hb[1], SY, MY);
-      q2(2 * hb[2], SZ, MZ);
-      std::vector<double> E(27 * 27);
-      for (int a = 0; a < 27; ++a)
-        for (int b = 0; b < 27; ++b) {
-          const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-          const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
-          E[a * 27 + b] = SX[ax][bx] * MY[ay][by] * MZ[az][bz] + MX[ax][bx] * SY[ay][by] * MZ[az][bz] +
-                          MX[ax][bx] * MY[ay][by] * SZ[az][bz];
-        }
-      elem.push_back(std::move(E));
-    }
-    shape[p] = sidx;
-  }
-  std::vector<int32_t> pstart(n + 1, 0), plist(np * 27);
-  std::vector<int8_t> ploc(np * 27);
-  for (int64_t k = 0; k < np * 27; ++k) pstart[P[k] + 1]++;
-  for (int64_t i = 0; i < n; ++i) pstart[i + 1] += pstart[i];
-  {
-    std::vector<int32_t> fill(pstart.begin(), pstart.end() - 1);
-    for (int64_t p = 0; p < np; ++p)
-      for (int a = 0; a < 27; ++a) {
-        const int32_t k = fill[P[p * 27 + a]]++;
-        plist[k] = (int32_t)p;
-        ploc[k] = (int8_t)a;
-      }
-  }
+  std::vector<int32_t> rp, c, d;
+  std::vector<double> v;
+  if (q2_matrix(M, rp, c, v)) return -1;
+  const int64_t n = M->n();
   dir_nodes(M, ncolours, colours, d);
   std::vector<uint8_t> isdir(n, 0);
   for (int32_t i : d) isdir[i] = 1;
-  int bad = 0;
-#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : bad)
-  for (int64_t i = 0; i < n; ++i) {
-    const int32_t rs = rp[i], re = rp[i + 1];
-    for (int32_t q = rs; q < re; ++q) val[q] = 0.0;
-    for (int32_t k = pstart[i]; k < pstart[i + 1]; ++k) {
-      const int64_t p = plist[k];
-      const int a = ploc[k];
-      const double* E = elem[shape[p]].data() + a * 27;
-      for (int b = 0; b < 27; ++b) {
-        const int32_t j = P[p * 27 + b];
-        const int32_t* it = std::lower_bound(c.data() + rs, c.data() + re, j);
-        if (it == c.data() + re || *it != j) {
-          bad++;
-          continue;
-        }
-        val[it - c.data()] += E[b];
-      }
-    }
-    for (int32_t q = rs; q < re; ++q) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i)
+    for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
       if (isdir[i])
         val[q] = (c[q] == i) ? 1.0 : 0.0;
       else if (isdir[c[q]])
         val[q] = 0.0;
+      else
+        val[q] = v[q];
     }
-  }
-  if (bad) return fail("patch structure misses %d couplings", bad);
+  return 0;
+}
+
+int ghs_level_hn_sizes(const ghs_hier* h, int l, int64_t* nedges, int64_t* nfaces) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  if (nedges) *nedges = (int64_t)M->hn_e.size() / 4;
+  if (nfaces) *nfaces = (int64_t)M->hn_f.size() / 10;
+  return 0;
+}
+
+int ghs_level_hn(const ghs_hier* h, int l, int32_t* edges, int32_t* faces) {
+  const Mesh* M = lev(h, l);
+  if (!M) return -1;
+  if (edges) std::memcpy(edges, M->hn_e.data(), M->hn_e.size() * sizeof(int32_t));
+  if (faces) std::memcpy(faces, M->hn_f.data(), M->hn_f.size() * sizeof(int32_t));
+  return 0;
+}
+
+int ghs_level_q2_condensed_nnz(ghs_hier* h, int l, int ncolours, const int32_t* colours,
+                               int64_t* nnz) {
+  const Mesh* M = lev(h, l);
+  if (!M || !nnz) return fail("ghs_level_q2_condensed_nnz: bad arguments");
+  std::vector<int32_t> rp, c, d;
+  std::vector<double> v;
+  if (q2_matrix(M, rp, c, v)) return -1;
+  const int64_t n = M->n();
+  dir_nodes(M, ncolours, colours, d);
+  std::vector<uint8_t> isdir(n, 0);
+  for (int32_t i : d) isdir[i] = 1;
+  ghs_hier::Condensed& C = h->q2c[l];
+  if (condense(n, rp, c, v, M->hn_e, M->hn_f, isdir, C.rp, C.col, C.val)) return -1;
+  *nnz = C.rp[n];
+  return 0;
+}
+
+int ghs_level_q2_condensed(ghs_hier* h, int l, int32_t* rowptr, int32_t* col, double* val) {
+  auto it = h ? h->q2c.find(l) : h->q2c.end();
+  if (!h || it == h->q2c.end()) return fail("ghs_level_q2_condensed: call ghs_level_q2_condensed_nnz first");
+  const ghs_hier::Condensed& C = it->second;
+  std::memcpy(rowptr, C.rp.data(), C.rp.size() * sizeof(int32_t));
+  std::memcpy(col, C.col.data(), C.col.size() * sizeof(int32_t));
+  std::memcpy(val, C.val.data(), C.val.size() * sizeof(double));
+  h->q2c.erase(it);
   return 0;
 }
 

@@ -167,6 +167,12 @@ STRUCT_SIGNATURES = {
                                             C.c_int, f64p, f64p, f64p]),
     "ghs_level_face_mass": (C.c_int, [vp, C.c_int, C.c_int, f64p]),
     "ghs_level_assemble_q2": (C.c_int, [vp, C.c_int, C.c_int, i32p, f64p]),
+    "ghs_build_adaptive": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, f64p,
+                                     P(vp)]),
+    "ghs_level_hn_sizes": (C.c_int, [vp, C.c_int, i64p, i64p]),
+    "ghs_level_hn": (C.c_int, [vp, C.c_int, i32p, i32p]),
+    "ghs_level_q2_condensed_nnz": (C.c_int, [vp, C.c_int, C.c_int, i32p, i64p]),
+    "ghs_level_q2_condensed": (C.c_int, [vp, C.c_int, i32p, i32p, f64p]),
 }
 
 _libs = {}

@@ -28,6 +28,20 @@ class Mesh:
         check_s(N.structure().ghs_build(path.encode(), dim, prerefine, drop_coarse, C.byref(self.h)))
         self.nlevels = N.structure().ghs_nlevels(self.h)
 
+    @classmethod
+    def adaptive(cls, inp, prerefine, drop_coarse, nadapt, radius, centre):
+        """3D hierarchy of `prerefine` uniform refinements and `nadapt`
+        adaptive patch refinements inside a ball (ghs_build_adaptive)"""
+        m = cls.__new__(cls)
+        path = inp if os.path.isabs(inp) else os.path.join(CONFIGS, inp)
+        m.dim = 3
+        m.h = C.c_void_p()
+        ctr = np.ascontiguousarray(centre, dtype=np.float64)
+        check_s(N.structure().ghs_build_adaptive(path.encode(), prerefine, drop_coarse, nadapt,
+                                                 float(radius), ptr(ctr), C.byref(m.h)))
+        m.nlevels = N.structure().ghs_nlevels(m.h)
+        return m
+
     def sizes(self, l, with_nnz=False):
         n, c, z = C.c_int64(), C.c_int64(), C.c_int64()
         check_s(N.structure().ghs_level_sizes(self.h, l, C.byref(n), C.byref(c),
@@ -132,6 +146,30 @@ class Mesh:
         check_s(N.structure().ghs_level_assemble_q2(self.h, l, len(cols), ptr(cols, N.i32p),
                                                     ptr(val)))
         return val
+
+    def hn(self, l):
+        """Q2 hanging-node constraints: (edges (ne, 4), faces (nf, 10))"""
+        ne, nf = C.c_int64(), C.c_int64()
+        check_s(N.structure().ghs_level_hn_sizes(self.h, l, C.byref(ne), C.byref(nf)))
+        e = np.empty((ne.value, 4), np.int32)
+        f = np.empty((nf.value, 10), np.int32)
+        check_s(N.structure().ghs_level_hn(self.h, l, ptr(e, N.i32p), ptr(f, N.i32p)))
+        return e, f
+
+    def q2_condensed(self, l, colours=(1,)):
+        """Q2 Laplace condensed over the hanging nodes, Dirichlet rows of the
+        given colours: (rowptr, col, val)"""
+        cols = np.ascontiguousarray(colours, dtype=np.int32)
+        nnz = C.c_int64()
+        check_s(N.structure().ghs_level_q2_condensed_nnz(self.h, l, len(cols), ptr(cols, N.i32p),
+                                                         C.byref(nnz)))
+        n, _, _ = self.sizes(l)
+        rp = np.empty(n + 1, np.int32)
+        col = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value)
+        check_s(N.structure().ghs_level_q2_condensed(self.h, l, ptr(rp, N.i32p), ptr(col, N.i32p),
+                                                     ptr(val)))
+        return rp, col, val
 
     def face_mass(self, l, colour):
         n, _, _ = self.sizes(l)
@@ -257,22 +295,35 @@ def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", 
 
 
 def q2_hierarchy(prerefine, inp="cube.inp", drop_coarse=1, colours=(1,), name=None,
-                 keep_coords=True):
+                 keep_coords=True, adaptive=0, radius=0.25, centre=(0.5, 0.5, 0.5)):
     """Poisson with the triquadratic patch element (CGDiscQ23d, Laplace<3>) on a
-    uniformly refined hex mesh: one unknown per mesh node, 2x2x2-cell patches
-    as elements (up to 125 couplings per row), nodal transfers between levels
+    refined hex mesh: one unknown per mesh node, 2x2x2-cell patches as
+    elements (up to 125 couplings per row), nodal transfers between levels
     (MgInterpolatorMatrix, as for every discretisation of the reference's
-    multilevel solver), Dirichlet nodes of the given colours."""
-    M = Mesh(inp, 3, prerefine, drop_coarse)
+    multilevel solver), Dirichlet nodes of the given colours.  With
+    adaptive = 1 or 2 the finest uniform mesh is followed by that many
+    adaptive levels refined inside the ball (radius, centre), with Q2
+    hanging-node constraints and the operator condensed over them."""
+    if adaptive:
+        M = Mesh.adaptive(inp, prerefine, drop_coarse, adaptive, radius, centre)
+    else:
+        M = Mesh(inp, 3, prerefine, drop_coarse)
     levels = []
     for l in range(M.nlevels):
-        rp, col = M.patch_structure(l)
-        val = M.assemble_q2(l, len(col), colours)
+        if adaptive:
+            rp, col, val = M.q2_condensed(l, colours)
+            he, hf = M.hn(l)
+        else:
+            rp, col = M.patch_structure(l)
+            val = M.assemble_q2(l, len(col), colours)
+            he, hf = np.zeros((0, 4), np.int32), np.zeros((0, 10), np.int32)
         levels.append(Level(n=len(rp) - 1, ncomp=1, rowptr=rp, col=col, val=val,
                             transfer=M.transfer(l) if l > 0 else None, hn_kind=23,
+                            hn_edges=he, hn_faces=hf,
                             dirichlet=M.dirichlet(l, colours),
                             dirichlet_comps=np.zeros(1, np.int32),
                             coords=M.coords(l) if keep_coords else None))
-    h = Hierarchy(levels, 3, name or f"q2:{inp}:{prerefine}")
+    tag = f"q2:{inp}:{prerefine}" + (f"+{adaptive}" if adaptive else "")
+    h = Hierarchy(levels, 3, name or tag)
     h.mesh = M
     return h

@@ -99,6 +99,30 @@ int ghs_level_assemble_stokes(const ghs_hier* h, int l, double visc, double alph
  * columns on the nodes of the given colours (identity diagonal). */
 int ghs_level_assemble_q2(const ghs_hier* h, int l, int ncolours, const int32_t* colours,
                           double* val);
+/* Adaptive Q2 patch hierarchy: the .inp mesh refined `prerefine` times, then
+ * `nadapt` (0..2) adaptive steps, each refining (all 8 cells of) every patch
+ * of the newest generation that has a node inside the ball |x - centre| <
+ * radius (MeshAgent::refine_nodes, src/Mesh/meshagent.cc:540-556, refines
+ * the patches of the marked nodes); from the second step on only patches
+ * whose corner-sharing patches are all of that generation or newer, so that
+ * neighbouring patches differ by one generation at most.  Node coordinates
+ * must lie on the dyadic lattice (unit-box meshes).  Levels as in
+ * ghs_build; the adaptive levels carry Q2 hanging-node constraints. */
+int ghs_build_adaptive(const char* inp_path, int prerefine, int drop_coarse, int nadapt,
+                       double radius, const double* centre /* 3 */, ghs_hier** out);
+/* Q2 hanging-node constraints of level l (HNStructureQ23d layout):
+ * edges[ne*4] = hanging node, near, middle, far parent (weights 3/8, 3/4,
+ * -1/8); faces[nf*10] = hanging node, 9 parents g_{3a+b} (weights w_a w_b). */
+int ghs_level_hn_sizes(const ghs_hier* h, int l, int64_t* nedges, int64_t* nfaces);
+int ghs_level_hn(const ghs_hier* h, int l, int32_t* edges, int32_t* faces);
+/* Q2 Laplace on level l condensed over its hanging nodes (T^T A T: hanging
+ * rows/columns become identity rows, as in the reference's assembled
+ * matrices), then strong Dirichlet rows/columns of the given colours.  The
+ * first call forms it and returns nnz; the second copies it out (rowptr[n+1],
+ * col[nnz] ascending, val[nnz]) and releases it. */
+int ghs_level_q2_condensed_nnz(ghs_hier* h, int l, int ncolours, const int32_t* colours,
+                               int64_t* nnz);
+int ghs_level_q2_condensed(ghs_hier* h, int l, int32_t* rowptr, int32_t* col, double* val);
 /* m[i] = integral of the bilinear hat of node i over the rectangular boundary
  * faces of the given colour (boundary mass row sums), m[nnodes] */
 int ghs_level_face_mass(const ghs_hier* h, int l, int colour, double* m);

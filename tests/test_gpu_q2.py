@@ -127,3 +127,105 @@ def test_level_matrix_view(ctx, h8):
             assert np.abs(yi - y[lex]).max() <= tol * np.abs(y).max()
         del Ai
         mg.free()
+
+
+# ------------------------------------------------- adaptive levels, Q2 hanging nodes
+ADA = dict(adaptive=2, radius=0.25, centre=(0.5, 0.47, 0.53))
+
+
+@pytest.fixture(scope="module")
+def a8():
+    return q2_hierarchy(3, **ADA)
+
+
+@pytest.fixture(scope="module")
+def a16():
+    return q2_hierarchy(4, **ADA)
+
+
+def ada_rhs(h, seed=3):
+    """seeded f distributed onto the parents of the hanging nodes
+    (HNDistribute, as the reference's assembly leaves it), Dirichlet 0"""
+    lev = h.fine
+    f = lattice_rhs(lev.coords, seed)
+    O.HN(lev.hn_kind, lev.hn_edges, lev.hn_faces).distribute(f)
+    f[lev.dirichlet] = 0.0
+    return f
+
+
+@pytest.mark.parametrize("ndirect", [1, 200])
+@pytest.mark.parametrize("order", [None, "lexicographic"])
+def test_q2_adaptive_vcycle_vs_oracle(ctx, a8, order, ndirect):
+    """8^3 + 2 adaptive levels: the V-cycle with HNAverage / HNDistribute /
+    HNZero on both adaptive levels, bit for bit with the iterative coarse
+    level, from a zero and from a non-zero start"""
+    print(f"[q2 adaptive V-cycle 8^3+2 order={order} ndirect={ndirect}]", flush=True)
+    f = ada_rhs(a8)
+    om = O.MG(a8, ndirect=ndirect, **MGP)
+    mg = H.CudaMultiLevelSolver(ctx, a8, ndirect=ndirect, order=order, **MGP)
+    x0 = np.random.default_rng(4).uniform(-1, 1, len(f))
+    for start in (np.zeros_like(f), x0):
+        xo = start.copy()
+        om.precondition(xo, f)
+        X = H.CudaVectorInterface.from_global(ctx, start)
+        mg.precondition(X, H.CudaVectorInterface.from_global(ctx, f),
+                        x_is_zero=not start.any())
+        x = X.copy_back()
+        if ndirect == 1:
+            assert np.array_equal(x, xo)
+        assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.norm(xo)
+    assert mg.mgsteps == om.mgsteps
+
+
+def test_q2_adaptive_solve_vs_oracle(ctx, a16):
+    """16^3 + 2 adaptive levels (24,401 DOF, 3040 edge and 1520 face hanging
+    nodes): Newton/GMRES iterates to 1e-12 with iteration counts within one,
+    the true residual of the condensed system, and the mixed-precision solve
+    within 1e-8 of the FP64 oracle solve"""
+    print("[q2 adaptive Newton/GMRES 16^3+2 FP64 and mixed]", flush=True)
+    import scipy.sparse as sp
+    f = ada_rhs(a16)
+    lev = a16.fine
+    om = O.MG(a16, ndirect=200, **MGP)
+    uo = np.zeros_like(f)
+    rc, orep = om.newton(uo, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30),
+                         norm_kind=1)
+    assert rc == 0
+    mg = H.CudaMultiLevelSolver(ctx, a16, ndirect=200, order="lexicographic", **MGP)
+    U = mg.new_vector()
+    F = H.CudaVectorInterface.from_global(ctx, f)
+    rep = mg.Solve(U, F, H.CGInfo(1e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30), norm_kind=1)
+    assert rep.newton_iterations == orep.newton_iterations
+    assert abs(rep.gmres_iterations - orep.gmres_iterations) <= 1
+    u = U.copy_back()
+    assert np.linalg.norm(u - uo) <= 1e-12 * np.linalg.norm(uo)
+    A = sp.csr_matrix((lev.val, lev.col, lev.rowptr), shape=(lev.n, lev.n))
+    r = f - A @ u
+    r[lev.dirichlet] = 0.0
+    assert np.linalg.norm(r) <= 1e-10 * rep.first_residual * 1.0001
+    # the hanging entries leave the solve zeroed (HNZero)
+    assert not u[lev.hn_edges[:, 0]].any() and not u[lev.hn_faces[:, 0]].any()
+    uo2 = np.zeros_like(f)
+    om.newton(uo2, f, O.cginfo(1e-12, 1e-30, 100), O.nlinfo(1e-12, 1e-30, 30), norm_kind=1)
+    mx = H.CudaMultiLevelSolver(ctx, a16, ndirect=200, order="lexicographic", prec="f16",
+                                krylov="fgmres", **MGP)
+    U2 = mx.new_vector()
+    rep2 = mx.Solve(U2, F, H.CGInfo(0.9e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30),
+                    norm_kind=1)
+    assert H.N.NL_STATUS[rep2.status] == "converged"
+    assert np.linalg.norm(U2.copy_back() - uo2) <= 1e-8 * np.linalg.norm(uo2)
+
+
+def test_q2_adaptive_adjoint_vs_oracle(ctx, a16):
+    """the condensed operator is symmetric: the adjoint solve (transposed
+    levels) of g equals the oracle's forward solve of g"""
+    print("[q2 adaptive adjoint 16^3+2]", flush=True)
+    g = ada_rhs(a16, seed=9)
+    om = O.MG(a16, ndirect=200, **MGP)
+    zo = np.zeros_like(g)
+    om.newton(zo, g, O.cginfo(1e-12, 1e-30, 100), O.nlinfo(1e-12, 1e-30, 30), norm_kind=1)
+    mg = H.CudaMultiLevelSolver(ctx, a16, ndirect=200, order="lexicographic", adjoint=True, **MGP)
+    Z = mg.new_vector()
+    mg.Solve(Z, H.CudaVectorInterface.from_global(ctx, g), H.CGInfo(1e-12, 1e-30, 100),
+             H.NLInfo(1e-12, 1e-30, 30), norm_kind=1)
+    assert np.linalg.norm(Z.copy_back() - zo) <= 1e-10 * np.linalg.norm(zo)
