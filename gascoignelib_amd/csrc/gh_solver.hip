@@ -693,9 +693,6 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
          "gh_mg_create: bad smoother");
   GH_ARG(params->krylov == 0 || params->krylov == 1, "gh_mg_create: krylov must be 0 or 1");
   GH_ARG(params->adjoint == 0 || params->adjoint == 1, "gh_mg_create: adjoint must be 0 or 1");
-  if (params->adjoint && H)
-    for (int l = 0; l < nlevels; ++l)
-      GH_ARG(!H[l], "gh_mg_create: the adjoint solver does not support hanging nodes");
   // shapes of the caller's objects
   for (int l = 0; l < nlevels; ++l) {
     if (!A[l] || A[l]->prec != GH_F64)
@@ -746,7 +743,10 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
         Lv.own_A = !rc;
       }
       Lv.T = l > 0 ? T[l] : nullptr;
-      Lv.H = nullptr;
+      // a condensed level operator has identity-only hanging rows and columns,
+      // so its transpose is condensed over the same constraints: the V-cycle
+      // keeps the forward HNAverage / HNDistribute / HNZero
+      Lv.H = H ? H[l] : nullptr;
       Lv.M = M ? M[l] : nullptr;
     } else {
       Lv.A64 = A[l];

@@ -247,8 +247,12 @@ typedef struct {
                            creation, with smoothing, residuals and products by
                            A_l^T and restriction = transpose of prolongation (no
                            reference counterpart: the reference has no adjoint
-                           solve, src/Solver/stdsolver.cc:1068-1090).  Hanging
-                           nodes are not supported in this mode. */
+                           solve, src/Solver/stdsolver.cc:1068-1090).  With
+                           hanging nodes the level matrices are condensed
+                           (hanging rows and columns identity-only); their
+                           transposes are condensed over the same constraints,
+                           so the V-cycle applies the same HNAverage,
+                           HNDistribute and HNZero as the forward solver. */
 } gh_mg_params;
 void gh_mg_params_default(gh_mg_params* p);
 

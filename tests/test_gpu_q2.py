@@ -6,6 +6,7 @@ FP64 oracle solve, and the adjoint solve against the oracle on the transposed
 levels."""
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import oracle_ctypes as O
 from gascoignelib_amd import hip as H
@@ -183,7 +184,6 @@ def test_q2_adaptive_solve_vs_oracle(ctx, a16):
     the true residual of the condensed system, and the mixed-precision solve
     within 1e-8 of the FP64 oracle solve"""
     print("[q2 adaptive Newton/GMRES 16^3+2 FP64 and mixed]", flush=True)
-    import scipy.sparse as sp
     f = ada_rhs(a16)
     lev = a16.fine
     om = O.MG(a16, ndirect=200, **MGP)
@@ -229,3 +229,55 @@ def test_q2_adaptive_adjoint_vs_oracle(ctx, a16):
     mg.Solve(Z, H.CudaVectorInterface.from_global(ctx, g), H.CGInfo(1e-12, 1e-30, 100),
              H.NLInfo(1e-12, 1e-30, 30), norm_kind=1)
     assert np.linalg.norm(Z.copy_back() - zo) <= 1e-10 * np.linalg.norm(zo)
+
+
+def column_scaled(h):
+    """every level matrix scaled by columns, A_l D_l, with d = 1 + 0.4 sin(7x)
+    cos(5y) sin(3z + 1) on free nodes and 1 on hanging and Dirichlet nodes:
+    hanging rows and columns stay identity-only, and A_l D_l is not symmetric"""
+    import dataclasses
+    from gascoignelib_amd.problem import Hierarchy
+    levels, ds = [], []
+    for lev in h.levels:
+        X = lev.coords
+        d = 1.0 + 0.4 * np.sin(7 * X[:, 0]) * np.cos(5 * X[:, 1]) * np.sin(3 * X[:, 2] + 1)
+        d[lev.dirichlet] = 1.0
+        d[lev.hn_edges[:, 0]] = 1.0
+        d[lev.hn_faces[:, 0]] = 1.0
+        levels.append(dataclasses.replace(lev, val=lev.val * d[lev.col]))
+        ds.append(d)
+    return Hierarchy(levels, h.dim, h.name + ":scaled"), ds[-1]
+
+
+def test_q2_adaptive_adjoint_nonsymmetric(ctx, a16):
+    """adjoint with hanging nodes on a non-symmetric condensed operator:
+    (A D)^T z = g on the free nodes is A z = D^-1 g there (A symmetric on the
+    free block), so the GPU adjoint solve of the column-scaled hierarchy must
+    equal the oracle's forward solve of the unscaled one with D^-1 g; the
+    forward solve of A D against the oracle on the same scaled levels"""
+    print("[q2 adaptive adjoint 16^3+2, column-scaled]", flush=True)
+    hs, d = column_scaled(a16)
+    lev = hs.fine
+    A = sp.csr_matrix((lev.val, lev.col, lev.rowptr), shape=(lev.n, lev.n))
+    assert abs(A - A.T).max() > 1e-3 * abs(A).max()
+    g = ada_rhs(a16, seed=9)
+    om = O.MG(a16, ndirect=200, **MGP)
+    zo = np.zeros_like(g)
+    om.newton(zo, g / d, O.cginfo(1e-12, 1e-30, 100), O.nlinfo(1e-12, 1e-30, 30), norm_kind=1)
+    for order in (None, "lexicographic"):
+        mg = H.CudaMultiLevelSolver(ctx, hs, ndirect=200, order=order, adjoint=True, **MGP)
+        Z = mg.new_vector()
+        rep = mg.Solve(Z, H.CudaVectorInterface.from_global(ctx, g), H.CGInfo(1e-12, 1e-30, 100),
+                       H.NLInfo(1e-12, 1e-30, 30), norm_kind=1)
+        assert H.N.NL_STATUS[rep.status] == "converged"
+        assert np.linalg.norm(Z.copy_back() - zo) <= 1e-10 * np.linalg.norm(zo)
+        mg.free()
+    # forward on the scaled levels: bit for bit V-cycle, solve to 1e-12
+    f = ada_rhs(a16)
+    oms = O.MG(hs, ndirect=1, **MGP)
+    xo = np.zeros_like(f)
+    oms.precondition(xo, f)
+    mg = H.CudaMultiLevelSolver(ctx, hs, ndirect=1, **MGP)
+    X = mg.new_vector()
+    mg.precondition(X, H.CudaVectorInterface.from_global(ctx, f), x_is_zero=True)
+    assert np.array_equal(X.copy_back(), xo)
