@@ -737,7 +737,8 @@ def bench_q2(args, adaptive=0):
         "residual_spmv_f64": {"us_per_launch": t_spmv * 1e6, "bytes_per_launch": b64,
                               "gbs": b64 / t_spmv / 1e9,
                               "frac": b64 / t_spmv / 1e9 / HBM_PEAK_GBS,
-                              "stored_entries": lay["stored"], "pattern_entries": fine.nnz},
+                              "stored_entries": lay["stored"], "pattern_entries": fine.nnz,
+                              "layout": lay},
         "setup_s": {"structure": t_struct, "device": t_setup},
     }
     print(json.dumps(line), flush=True)

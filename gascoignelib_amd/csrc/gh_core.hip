@@ -677,62 +677,97 @@ static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const 
   return GH_OK;
 }
 
-// Valued operators: rows with the same relative column pattern are grouped
-// (stable sort by pattern inside windows of kSellWindow rows), so that the
-// slices of the common patterns hold a single pattern (offsets from scalar
-// loads, no padding) -- applied when it makes clearly more slices uniform.
-// Each row keeps its entries and its sum; only the slot it runs in moves.
+// Valued operators: the rows of a window of kSellWindow rows may run in
+// another order (a stable sort), so that a slice holds rows of one length
+// (no lanes idle on padding) and, where rows share a relative column pattern,
+// one pattern (offsets from scalar loads).  Two candidate orders: longest rows
+// first with equal lengths grouped by pattern, or longest rows first only; the
+// first that clearly pays (5% fewer padded slots or 5% more single-pattern
+// slices) and keeps each slice's rows close together is used, else the
+// caller's order.  Each row keeps its entries and its sum; only the slot it
+// runs in moves.
 int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
                const double* vals, int prec, Sell* S) {
   S->perm = nullptr;
   if (!vals || n < 2 * kSlice) return sell_build_impl(ctx, n, rowptr, col, vals, prec, nullptr, S);
   constexpr int64_t kSellWindow = 4096;
   std::vector<uint64_t> key(n);
+  std::vector<int32_t> len(n);
   for (int64_t i = 0; i < n; ++i) {
     uint64_t h = 1469598103934665603ull;
+    int32_t k = 0;
     for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
       if (vals[q] == 0.0) continue;
       h = (h ^ (uint32_t)(int32_t)((int64_t)col[q] - i)) * 1099511628211ull;
+      ++k;
     }
     key[i] = h;
+    len[i] = k;
   }
-  std::vector<int32_t> perm(n);
-  for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-  for (int64_t w0 = 0; w0 < n; w0 += kSellWindow) {
-    const int64_t w1 = std::min<int64_t>(n, w0 + kSellWindow);
-    std::stable_sort(perm.begin() + w0, perm.begin() + w1,
-                     [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-  }
-  auto uniform = [&](bool permuted) {
+  auto window_sort = [&](bool by_pattern) {
+    std::vector<int32_t> p(n);
+    for (int64_t i = 0; i < n; ++i) p[i] = (int32_t)i;
+    for (int64_t w0 = 0; w0 < n; w0 += kSellWindow) {
+      const int64_t w1 = std::min<int64_t>(n, w0 + kSellWindow);
+      std::stable_sort(p.begin() + w0, p.begin() + w1, [&](int32_t a, int32_t b) {
+        if (len[a] != len[b]) return len[a] > len[b];
+        return by_pattern && key[a] < key[b];
+      });
+    }
+    return p;
+  };
+  // slices whose 64 rows share one pattern
+  auto uniform = [&](const std::vector<int32_t>* p) {
     int64_t u = 0;
     for (int64_t s0 = 0; s0 + kSlice <= n; s0 += kSlice) {
-      const uint64_t k0 = key[permuted ? perm[s0] : s0];
+      const uint64_t k0 = key[p ? (*p)[s0] : s0];
       bool same = true;
-      for (int l = 1; l < kSlice && same; ++l) same = key[permuted ? perm[s0 + l] : s0 + l] == k0;
+      for (int l = 1; l < kSlice && same; ++l) same = key[p ? (*p)[s0 + l] : s0 + l] == k0;
       u += same;
     }
     return u;
   };
+  // slots a sweep runs: 64 times the longest row of each slice
+  auto slots = [&](const std::vector<int32_t>* p) {
+    int64_t t = 0;
+    for (int64_t s0 = 0; s0 < n; s0 += kSlice) {
+      int32_t w = 0;
+      for (int64_t t0 = s0; t0 < std::min<int64_t>(n, s0 + kSlice); ++t0)
+        w = std::max(w, len[p ? (*p)[t0] : t0]);
+      t += (int64_t)kSlice * w;
+    }
+    return t;
+  };
   // rows a slice's lanes touch, in groups of 16 (the vectors' 128-byte lines
   // in FP64): 4 or 5 for 64 consecutive rows
-  auto groups = [&]() {
+  auto groups = [&](const std::vector<int32_t>& p) {
     int64_t g = 0;
     std::vector<int32_t> b(kSlice);
     for (int64_t s0 = 0; s0 < n; s0 += kSlice) {
       const int m = (int)std::min<int64_t>(kSlice, n - s0);
-      for (int l = 0; l < m; ++l) b[l] = perm[s0 + l] >> 4;
+      for (int l = 0; l < m; ++l) b[l] = p[s0 + l] >> 4;
       std::sort(b.begin(), b.begin() + m);
       g += std::unique(b.begin(), b.begin() + m) - b.begin();
     }
     return (double)g / (double)((n + kSlice - 1) / kSlice);
   };
-  const int64_t u0 = uniform(false), u1 = uniform(true);
-  // worth it only with clearly more single-pattern slices and rows that stay
-  // close together (rows of two kinds interleaved at stride 2 touch ~8-16
-  // groups per slice and gain; rows of many kinds grouped from all over the
-  // window touch 20+ and lose the vectors' coalescing)
-  if (u1 < u0 + std::max<int64_t>(1, (n / kSlice) / 20) || groups() > 20.0)
-    return sell_build_impl(ctx, n, rowptr, col, vals, prec, nullptr, S);
+  const int64_t u0 = uniform(nullptr), t0 = slots(nullptr);
+  const int64_t du = std::max<int64_t>(1, (n / kSlice) / 20);
+  // worth it only when clearly fewer slots run or clearly more slices hold one
+  // pattern, with rows that stay close together (rows of two kinds interleaved
+  // at stride 2 touch ~8-16 groups per slice and gain; rows of many kinds
+  // grouped from all over the window touch 20+ and lose the vectors'
+  // coalescing)
+  auto pays = [&](const std::vector<int32_t>& p, bool count_patterns) {
+    const bool fewer = slots(&p) * 20 <= t0 * 19;
+    const bool more = count_patterns && uniform(&p) >= u0 + du;
+    return (fewer || more) && groups(p) <= 20.0;
+  };
+  std::vector<int32_t> perm = window_sort(true);
+  if (!pays(perm, true)) {
+    perm = window_sort(false);
+    if (!pays(perm, false)) return sell_build_impl(ctx, n, rowptr, col, vals, prec, nullptr, S);
+  }
   // the operator in slot order
   std::vector<int32_t> rp2(n + 1, 0);
   for (int64_t t = 0; t < n; ++t) rp2[t + 1] = rp2[t] + (rowptr[perm[t] + 1] - rowptr[perm[t]]);
@@ -1273,6 +1308,7 @@ int gh_csr_layout(const gh_csr* A, int64_t* out) {
     out[4] = A->sell.patterned;
     out[5] = A->sell.ntab;
     out[6] = A->sell.col_bytes;
+    out[7] = A->sell.perm != nullptr;
   } else if (A->bsell.B) {
     // component rows: entries are block rows (B values each), 32-bit columns
     out[0] = A->bsell.stored;
@@ -1280,10 +1316,12 @@ int gh_csr_layout(const gh_csr* A, int64_t* out) {
     out[2] = A->bsell.nslices;
     out[3] = out[4] = out[5] = 0;
     out[6] = A->bsell.stored * 4;
+    out[7] = 1;
   } else {
     out[0] = out[1] = A->nnz;
     out[2] = out[3] = out[4] = out[5] = 0;
     out[6] = A->nnz * 4;
+    out[7] = 0;
   }
   return GH_OK;
 }

@@ -151,12 +151,12 @@ class CudaCSRMatrixInterface:
     def layout(self):
         """Device layout: entries stored, padded slots, slices, slices with
         16-bit column offsets, slices in pattern mode, pattern-table entries,
-        column bytes a sweep reads."""
-        out = np.zeros(7, np.int64)
+        column bytes a sweep reads, whether rows run in a permuted order."""
+        out = np.zeros(8, np.int64)
         N.check(N.hip().gh_csr_layout(self.h, N.ptr(out, N.i64p)))
         return dict(stored=int(out[0]), slots=int(out[1]), slices=int(out[2]),
                     narrow=int(out[3]), patterned=int(out[4]), table=int(out[5]),
-                    column_bytes=int(out[6]))
+                    column_bytes=int(out[6]), permuted=bool(out[7]))
 
     def vmult(self, y, x, d=1.0, b=0.0):
         """y = d*A*x + b*y"""

@@ -107,17 +107,20 @@ int gh_csr_destroy(gh_csr* A);
 int64_t gh_csr_n(const gh_csr* A);
 int64_t gh_csr_nnz(const gh_csr* A);
 int gh_csr_prec(const gh_csr* A);
-/* Device layout (no reference counterpart; for byte accounting), out[7].
+/* Device layout (no reference counterpart; for byte accounting), out[8].
  * Scalar matrices: out[0] entries stored (exact zeros of the pattern are not
  * stored; every row sum takes its other terms in the same order, so the values
  * of A x are unchanged), out[1] padded slots, out[2] slices of 64 rows,
  * out[3] slices with 16-bit column offsets, out[4] slices whose rows all follow
  * shared relative column patterns (no column storage), out[5] entries of the
- * pattern table, out[6] column bytes one sweep reads.  Block matrices with
- * 2..4 components (stored as component rows): out[0] block rows kept (row r
- * of a block is left out when it is all zero), out[1] padded slots, out[2]
- * slices, out[3..5] = 0, out[6] = 4 out[0]; other block sizes: out[0] =
- * out[1] = nnz blocks, out[2..5] = 0, out[6] = 4 nnz. */
+ * pattern table, out[6] column bytes one sweep reads, out[7] 1 when the rows
+ * run in slices in another order than their own (rows of one length, and of
+ * one pattern, grouped inside windows of 4096 rows), else 0.  Block matrices
+ * with 2..4 components (stored as component rows, sorted by length inside
+ * windows): out[0] block rows kept (row r of a block is left out when it is
+ * all zero), out[1] padded slots, out[2] slices, out[3..5] = 0, out[6] =
+ * 4 out[0], out[7] = 1; other block sizes: out[0] = out[1] = nnz blocks,
+ * out[2..5] = 0, out[6] = 4 nnz, out[7] = 0. */
 int gh_csr_layout(const gh_csr* A, int64_t* out);
 /* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);

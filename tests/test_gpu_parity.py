@@ -358,32 +358,48 @@ def test_ordered_graph_mixed(ctx, prec):
     assert relerr(Ub.copy_back(), Ua.copy_back()) <= 1e-9
 
 
-def test_sell_column_encodings_and_dropped_zeros(ctx):
-    """A matrix whose slices mix 16-bit and 32-bit column encodings and whose
-    pattern holds exact zeros: SpMV and residual equal the oracle's (same
-    values, same per-row order)."""
-    rng = np.random.default_rng(5)
-    n = 150_000
+def sell_test_matrix(n, seed, equal_lengths):
+    """rows of slices 2 mod 3 couple to far columns (those slices need 32-bit
+    indices); the others are banded: slices 0 mod 3 on a 7-point band (shared
+    patterns), slices 1 mod 3 on that band plus one random column within 2000
+    (16-bit offsets from the slice base).  The pattern holds exact zeros,
+    first entries included.  equal_lengths: every row keeps 5 entries (the
+    others of its off-diagonal entries are zero) -- else a random fifth of
+    all entries is zero"""
+    rng = np.random.default_rng(seed)
+    band = np.array([-300, -17, -1, 0, 1, 17, 300])
     rows = []
     for i in range(n):
-        if (i // 64) % 3 == 2:  # far columns: this slice needs 32-bit indices
+        kind = (i // 64) % 3
+        if kind == 2:
             cols = np.unique(np.concatenate([[i], rng.integers(0, n, 6)]))
-        else:                   # banded: 16-bit offsets from the slice base
-            cols = np.unique(np.clip(i + np.array([-300, -17, -1, 0, 1, 17, 300]), 0, n - 1))
+        elif kind == 1:
+            cols = np.unique(np.clip(np.append(i + band, i + rng.integers(-2000, 2000)), 0, n - 1))
+        else:
+            cols = np.unique(np.clip(i + band, 0, n - 1))
         rows.append(cols)
     rowptr = np.zeros(n + 1, np.int32)
     rowptr[1:] = np.cumsum([len(c) for c in rows])
     col = np.concatenate(rows).astype(np.int32)
     val = rng.uniform(-1, 1, len(col))
-    val[rng.random(len(col)) < 0.2] = 0.0                     # explicit zeros
-    val[rowptr[:-1][np.arange(n) % 5 == 0]] = 0.0             # incl. first entries
-    diag = np.concatenate([np.nonzero(c == i)[0] + rowptr[i] for i, c in enumerate(rows)])
-    val[diag] = 4.0
-    A = H.CudaCSRMatrixInterface(ctx, rowptr, col, val)
-    lay = A.layout()
-    assert lay["stored"] == int(np.count_nonzero(val))
-    assert 0 < lay["narrow"] < lay["slices"]
-    assert 0 < lay["patterned"] < lay["slices"] - lay["narrow"]  # and some 32-bit slices
+    for i, c in enumerate(rows):
+        off = np.flatnonzero(c != i)
+        if equal_lengths:
+            drop = rng.choice(off, max(0, len(c) - 5), replace=False)
+            if i % 5 == 0 and off[0] == 0 and len(drop) and 0 not in drop:
+                drop[0] = 0  # a first entry
+            val[rowptr[i] + drop] = 0.0
+        else:
+            val[rowptr[i] + off[rng.random(len(off)) < 0.2]] = 0.0
+            if i % 5 == 0 and off[0] == 0:
+                val[rowptr[i]] = 0.0
+        val[rowptr[i] + np.flatnonzero(c == i)] = 4.0
+    return rowptr, col, val
+
+
+def check_products_vs_oracle(ctx, A, rowptr, col, val, seed):
+    rng = np.random.default_rng(seed)
+    n = len(rowptr) - 1
     Ao = O.Matrix(rowptr, col, val)
     x, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
     y = np.zeros(n)
@@ -396,6 +412,37 @@ def test_sell_column_encodings_and_dropped_zeros(ctx):
     R = H.CudaVectorInterface(ctx, n)
     A.residual(R, vec(ctx, x), vec(ctx, b))
     assert np.array_equal(R.copy_back(), r)
+
+
+def test_sell_column_encodings_and_dropped_zeros(ctx):
+    """A matrix whose slices mix 16-bit, 32-bit and pattern column encodings
+    and whose pattern holds exact zeros, with rows of one kept length (so the
+    rows run in their own order): SpMV and residual equal the oracle's (same
+    values, same per-row order)."""
+    rowptr, col, val = sell_test_matrix(150_000, 5, equal_lengths=True)
+    A = H.CudaCSRMatrixInterface(ctx, rowptr, col, val)
+    lay = A.layout()
+    assert lay["stored"] == int(np.count_nonzero(val))
+    assert not lay["permuted"]
+    assert 0 < lay["narrow"] < lay["slices"]
+    assert 0 < lay["patterned"] < lay["slices"] - lay["narrow"]  # and some 32-bit slices
+    check_products_vs_oracle(ctx, A, rowptr, col, val, 6)
+
+
+def test_sell_rows_grouped_by_length(ctx):
+    """Rows of varying kept length (a random fifth of the entries are exact
+    zeros) run in slices of equal-length rows: fewer padded slots than the
+    rows' own order needs, the same bits as the oracle's row sums"""
+    rowptr, col, val = sell_test_matrix(150_000, 7, equal_lengths=False)
+    A = H.CudaCSRMatrixInterface(ctx, rowptr, col, val)
+    lay = A.layout()
+    kept = np.add.reduceat(val != 0, rowptr[:-1])
+    pad = np.zeros(lay["slices"] * 64, np.int64)
+    pad[: len(kept)] = kept
+    own = 64 * int(pad.reshape(-1, 64).max(1).sum())
+    assert lay["permuted"] and lay["stored"] == int(np.count_nonzero(val))
+    assert lay["slots"] * 20 <= own * 19
+    check_products_vs_oracle(ctx, A, rowptr, col, val, 8)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
