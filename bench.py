@@ -83,6 +83,29 @@ def sell_bytes(n, layout, sv, sx):
             + n * sx * 3)
 
 
+# pre,post smoothing sweeps per workload, (FP64 path, mixed path): each path at
+# the fastest cycle measured on MI355X (V(2,2), V(3,2), V(3,3), V(4,4), V(6,6);
+# V(1,1) too for the Vanka cycles).  Jacobi is a weak smoother for the
+# triquadratic element and the variable coefficient: the reference's GMRES(10)
+# needs a restart with V(2,2) there (18 steps) and converges in 9 with
+# V(3,2) / V(3,3).  The mixed path's flexible GMRES (restart length 20) is
+# fastest with V(2,2) except on the Q1 Poisson.
+SMOOTH = {
+    "poisson128": ("2,2", "3,2"),
+    "adjoint96": ("3,3", "2,2"),
+    "stokes64": ("2,2", "2,2"),
+    "q2_64": ("3,2", "2,2"),
+    "q2ada": ("3,3", "2,2"),
+}
+
+
+def parse_smooth(text):
+    pre, post = (int(v) for v in text.split(","))
+    if not (0 <= pre <= 32 and 0 <= post <= 32):
+        raise SystemExit(f"bench.py: smoothing sweeps out of range: {text}")
+    return pre, post
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,9 +121,15 @@ def main():
     ap.add_argument("--mixed-linear-tol", type=float, default=None,
                     help="GMRES tolerance of each outer step of the mixed solve "
                          "(default 0.9e-10 with fgmres: its estimate is the true residual; 1e-5 with gmres)")
-    ap.add_argument("--mixed-smooth", default="3,2",
-                    help="pre,post Jacobi sweeps of the mixed V-cycle (the FP64 path keeps the "
-                         "reference's 2,2); the JSON also reports the mixed solve with 2,2")
+    ap.add_argument("--mixed-smooth", default=None,
+                    help="pre,post smoothing sweeps of the mixed V-cycle (default: the fastest "
+                         "measured per workload, SMOOTH); the JSON also reports the reference's 2,2")
+    ap.add_argument("--mixed-vmax", type=int, default=20,
+                    help="restart length of the mixed path's flexible GMRES (the FP64 path keeps "
+                         "the reference's GMRES(10))")
+    ap.add_argument("--fp64-smooth", default=None,
+                    help="pre,post smoothing sweeps of the FP64 V-cycle (default: the fastest "
+                         "measured per workload, SMOOTH)")
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
     ap.add_argument("--workload", default="poisson128",
@@ -117,6 +146,9 @@ def main():
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
     args = ap.parse_args()
+    d64, dmx = SMOOTH[args.workload]
+    args.fp64_smooth = parse_smooth(args.fp64_smooth or d64)
+    args.mixed_smooth = parse_smooth(args.mixed_smooth or dmx)
     if args.mixed_linear_tol is None:
         args.mixed_linear_tol = 0.9e-10 if args.mixed_krylov == "fgmres" else 1e-5
     if args.gpus != 1:
@@ -155,11 +187,12 @@ def main():
                   ndirect=1000, order=None if args.order == "none" else args.order)
     t0 = time.perf_counter()
     mg64 = H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common)
-    mpre, mpost = (int(v) for v in args.mixed_smooth.split(","))
+    mpre, mpost = args.mixed_smooth
+    common = dict(common, iterpre=args.fp64_smooth[0], iterpost=args.fp64_smooth[1])
     mixed_common = dict(common, iterpre=mpre, iterpost=mpost)
     mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, krylov=args.mixed_krylov,
                                   **mixed_common)
-    same = (mpre, mpost) == (2, 2)
+    same = (mpre, mpost) == args.fp64_smooth
     mgmx22 = None if same else H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed,
                                                        krylov=args.mixed_krylov, **common)
     t_setup = time.perf_counter() - t0
@@ -169,7 +202,7 @@ def main():
     def solve(mg, U, linear_tol):
         U.zero()
         rep = mg.Solve(U, F, H.CGInfo(linear_tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
-                       vmax=10, norm_kind=1)
+                       vmax=10 if mg is mg64 else args.mixed_vmax, norm_kind=1)
         if H.N.NL_STATUS[rep.status] != "converged":
             raise RuntimeError(f"solve did not converge: {H.N.NL_STATUS[rep.status]}")
         return rep
@@ -267,7 +300,8 @@ def main():
                   "linear_tol": args.mixed_linear_tol, "krylov": args.mixed_krylov},
         "speedup_mixed_vs_fp64": t64 / tmx,
         "mixed_same_cycle": None if mgmx22 is None else {
-            "cycle": "V(2,2), the FP64 path's", "ms_per_solve": float(np.median(ts22)) * 1e3,
+            "cycle": "V(%d,%d), the FP64 path's" % args.fp64_smooth,
+            "ms_per_solve": float(np.median(ts22)) * 1e3,
             "gmres": rep22.gmres_iterations,
             "speedup_vs_fp64": t64 / float(np.median(ts22))},
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -361,25 +395,28 @@ def bench_adjoint(args):
     ndof = hier.fine.n
     print(f"# 96^3 var-k: {ndof} DOF, {P.ncells} cells, {hier.nlevels} levels, "
           f"setup {t_struct:.1f}s", file=sys.stderr, flush=True)
-    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
-                  ndirect=400, order="lexicographic")
+    common = dict(mgtype="V", iterexact=10, omega=0.8, mgomega=1.0, ndirect=400,
+                  order="lexicographic")
     solvers = {}
-    for tag, prec, kry, tol in (("fp64", "f64", "gmres", 1e-10),
-                                ("mixed", args.mixed, "fgmres", 0.5e-10)):
-        solvers[tag] = (H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, **common),
+    for tag, prec, kry, tol, sm in (("fp64", "f64", "gmres", 1e-10, args.fp64_smooth),
+                                    ("mixed", args.mixed, "fgmres", args.mixed_linear_tol,
+                                     args.mixed_smooth)):
+        c = dict(common, iterpre=sm[0], iterpost=sm[1])
+        solvers[tag] = (H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, **c),
                         H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, adjoint=True,
-                                               **common), tol)
+                                               **c), tol)
     U, Z = H.CudaVectorInterface(ctx, ndof), H.CudaVectorInterface(ctx, ndof)
     Gk = H.CudaVectorInterface(ctx, P.ncells)
 
     def step(tag):
         mgF, mgA, tol = solvers[tag]
+        vmax = 10 if tag == "fp64" else args.mixed_vmax
         U.zero()
         rf = mgF.Solve(U, P.F, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
-                       norm_kind=1)
+                       vmax=vmax, norm_kind=1)
         Z.zero()
         ra = mgA.Solve(Z, P.G, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
-                       norm_kind=1)
+                       vmax=vmax, norm_kind=1)
         P.cellmap.dJdk(Z, U, Gk)
         for r in (rf, ra):
             if H.N.NL_STATUS[r.status] != "converged":
@@ -471,9 +508,9 @@ def bench_stokes(args):
     # forward and adjoint solvers of each precision (the adjoint on the
     # transposed levels, its Vanka factors those of the transposed patches)
     probs = {"fp64": StokesFluxProblem(ctx, hier, prec="f64", krylov="gmres", tol=1e-10,
-                                       order=order),
+                                       order=order, smooth=args.fp64_smooth),
              "mixed": StokesFluxProblem(ctx, hier, prec="f32", krylov="fgmres", tol=1e-10,
-                                        order=order)}
+                                        order=order, smooth=args.mixed_smooth)}
     t_setup = time.perf_counter() - t0
     F = H.CudaVectorInterface.from_global(ctx, f, 4)
     U = H.CudaVectorInterface(ctx, fine.n, 4)
@@ -610,9 +647,10 @@ def bench_q2(args, adaptive=0):
     the unit cube refined to 64^3 cells: 274,625 DOF, 16,974,593 entries (up to
     125 per row), 6 levels, nodal transfers, Dirichlet 0, seeded f.  One step =
     one solve to 1e-10 relative FP64 residual (FP64 GMRES in Newton vs f16-value
-    FP32 V-cycle in FP64 flexible GMRES, V(2,2) Jacobi omega 0.8, levels below
-    1000 nodes direct), then the adjoint solve for J = mean of the nodal values
-    in the ball of radius 0.25 around a seeded centre.
+    FP32 V-cycle in FP64 flexible GMRES, Jacobi omega 0.8 with each path's
+    smoothing counts, levels below 1000 nodes direct), then the adjoint solve
+    for J = mean of the nodal values in the ball of radius 0.25 around a seeded
+    centre.  The forward solves with the reference's V(2,2) are timed beside.
 
     With adaptive = 2 the 64^3 mesh (32^3 triquadratic elements) is followed by
     two levels refined patch-wise inside that same ball: 1,432,971 DOF, 86.6 M
@@ -641,15 +679,17 @@ def bench_q2(args, adaptive=0):
           f"levels, hanging nodes {nhn}, ball of {int(inside.sum())} nodes, built in "
           f"{t_struct:.1f}s", file=sys.stderr, flush=True)
     ctx = H.Context(0)
-    common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
-                  ndirect=1000, order=None if args.order == "none" else args.order)
+    common = dict(mgtype="V", iterexact=10, omega=0.8, mgomega=1.0, ndirect=1000,
+                  order=None if args.order == "none" else args.order)
+    paths = (("fp64", "f64", "gmres", 1e-10, args.fp64_smooth),
+             ("mixed", args.mixed, "fgmres", args.mixed_linear_tol, args.mixed_smooth))
     t0 = time.perf_counter()
     sol = {}
-    for tag, prec, kry, tol in (("fp64", "f64", "gmres", 1e-10),
-                                ("mixed", args.mixed, "fgmres", 0.9e-10)):
-        sol[tag] = (H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, **common),
+    for tag, prec, kry, tol, sm in paths:
+        c = dict(common, iterpre=sm[0], iterpost=sm[1])
+        sol[tag] = (H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, **c),
                     H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, adjoint=True,
-                                           **common), tol)
+                                           **c), tol)
     t_setup = time.perf_counter() - t0
     F = H.CudaVectorInterface.from_global(ctx, f)
     G = H.CudaVectorInterface.from_global(ctx, g)
@@ -666,7 +706,9 @@ def bench_q2(args, adaptive=0):
 
     def solve(mg, X, B, tol):
         X.zero()
-        rep = mg.Solve(X, B, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50), norm_kind=1)
+        vmax = args.mixed_vmax if mg.params.krylov == 1 else 10
+        rep = mg.Solve(X, B, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50), vmax=vmax,
+                       norm_kind=1)
         if H.N.NL_STATUS[rep.status] != "converged":
             raise RuntimeError("Q2 solve did not converge")
         return rep
@@ -690,6 +732,27 @@ def bench_q2(args, adaptive=0):
             out[what] = dict(ms=float(np.median(ts)) * 1e3, gmres=rep.gmres_iterations,
                              x=X.copy_back())
         res[tag] = out
+    # the reference's V(2,2) cycle beside each path's own, forward solves
+    ref_cycle = {}
+    for tag, prec, kry, tol, sm in paths:
+        if sm == (2, 2):
+            ref_cycle[tag] = dict(ms=res[tag]["forward"]["ms"], gmres=res[tag]["forward"]["gmres"])
+            continue
+        mg = H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry,
+                                    **dict(common, iterpre=2, iterpost=2))
+        for _ in range(args.warmup):
+            solve(mg, U, F, tol)
+        ts = []
+        for _ in range(args.steps):
+            ctx.sync()
+            t = time.perf_counter()
+            rep = solve(mg, U, F, tol)
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+        ref_cycle[tag] = dict(ms=float(np.median(ts)) * 1e3, gmres=rep.gmres_iterations)
+        print(f"# {tag} forward with V(2,2): {ref_cycle[tag]['ms']:.2f} ms (gmres "
+              f"{rep.gmres_iterations})", file=sys.stderr, flush=True)
+        mg.free()
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     # the FP64 residual SpMV on the finest level, internal numbering
     order = (np.arange(n, dtype=np.int32) if args.order == "none"
@@ -732,6 +795,11 @@ def bench_q2(args, adaptive=0):
                   "rel_error_z_vs_fp64": rel(res["mixed"]["adjoint"]["x"],
                                              res["fp64"]["adjoint"]["x"])},
         "speedup_mixed_vs_fp64": res["fp64"]["forward"]["ms"] / res["mixed"]["forward"]["ms"],
+        "cycles": {"fp64": "V(%d,%d)" % args.fp64_smooth, "mixed": "V(%d,%d)" % args.mixed_smooth},
+        "reference_cycle": {"cycle": "V(2,2)", "fp64_forward_ms": ref_cycle["fp64"]["ms"],
+                            "fp64_gmres": ref_cycle["fp64"]["gmres"],
+                            "mixed_forward_ms": ref_cycle["mixed"]["ms"],
+                            "mixed_gmres": ref_cycle["mixed"]["gmres"]},
         "J": float(g @ res["fp64"]["forward"]["x"]),
         "J_adjoint": float(f @ res["fp64"]["adjoint"]["x"]),
         "residual_spmv_f64": {"us_per_launch": t_spmv * 1e6, "bytes_per_launch": b64,

@@ -1203,6 +1203,17 @@ int gmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int
 // re-orthogonalisation test, Givens rotations and back-substitution are those
 // of GMRES::solve; its right-preconditioned form (gmres.cc:174-178, 201-205,
 // 252-258) applies M once more to V y at the end, which needs a fixed M.
+// Start a convergence history at residual res: the first residual and the
+// aimed residual res * tol, as gh_cginfo_check records them on its first call.
+void cginfo_begin(gh_cginfo* I, double res) {
+  I->residual = res;
+  I->firstresidual = res;
+  I->previousresidual = res;
+  I->previouscorrection = 0.;
+  I->aimedresidual = res * I->tol;
+  I->status = GH_RUNNING;
+}
+
 int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
                 int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
                 int* reached_out, int* steps_out) {
@@ -1231,6 +1242,11 @@ int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, in
     return GH_OK;
   }
   gamma[0] = rho;
+  // The relative tolerance is measured from this starting residual.  (The
+  // reference's GMRES, reproduced by gmres_impl, takes the residual after its
+  // first step as the base; the flexible variant's estimate is the true FP64
+  // residual, so its base is the residual it starts from.)
+  cginfo_begin(info, rho);
   GH_TRY(vec_scale_to(ctx, GH_F64, n, v, 1. / rho, v));
   const int fl = mg->nlevels - 1;
   const int clevel = coarsest_level(mg, fl, coarselevel < 0 ? fl : coarselevel);

@@ -22,7 +22,7 @@ class StokesFluxProblem:
     J = int_in u0 . e_x ds and dJ/dg_i = int_in phi_i ds."""
 
     def __init__(self, ctx, hier, prec="f64", krylov="gmres", tol=1e-12, order="lexicographic",
-                 omega=0.8, ndirect=200):
+                 omega=0.8, ndirect=200, smooth=(2, 2)):
         if hier.full_val is None:
             raise ValueError("StokesFluxProblem needs stokes_hierarchy(..., with_full=True)")
         self.ctx = ctx
@@ -30,7 +30,8 @@ class StokesFluxProblem:
         fine = hier.fine
         self.n = fine.n
         self.nodes = np.asarray(hier.inflow_nodes, dtype=np.int64)
-        common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=4, omega=omega, mgomega=1.0,
+        common = dict(mgtype="V", iterpre=smooth[0], iterpost=smooth[1], iterexact=4,
+                      omega=omega, mgomega=1.0,
                       ndirect=ndirect, linear_smooth="vanka", patches=hier.patches, order=order,
                       prec=prec, krylov=krylov)
         self.mg = H.CudaMultiLevelSolver(ctx, hier, **common)

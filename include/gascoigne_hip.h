@@ -243,7 +243,10 @@ typedef struct {
                            preconditioned directions kept, so that a V-cycle
                            in lower precision may serve as the preconditioner
                            while the Arnoldi process minimises the FP64
-                           residual (no reference counterpart) */
+                           residual (no reference counterpart).  Its CGInfo
+                           tolerance is relative to the residual it starts
+                           from; GMRES, like the reference's, measures it
+                           from the residual after its first step */
   int adjoint;          /* 1: the solver works on the transposed level matrices:
                            gh_gmres_solve / gh_newton_solve / gh_mg_linear then
                            solve A^T z = g for the finest-level A given at
