@@ -10,6 +10,14 @@ in its own order).  Checked: CSR pattern, every value (to rounding: the
 producer forms the exact integrals as tensor products, the reference sums a
 3x3x3 Gauss rule), Dirichlet rows, and the patch node lists in the
 reference's patch-local order.
+
+The adaptive levels (patch-wise refinement inside a ball, Q2 hanging nodes,
+the operator condensed over them) are pinned by what the discretisation
+fixes: the constraints interpolate quadratics exactly, the condensed operator
+equals an independent scipy T^T A T, is symmetric, annihilates constants and
+is exact on quadratics; a ball holding the whole domain reproduces the uniform
+refinement entry for entry; the oracle's HNStructureQ23d solve converges on
+them.
 """
 import os
 
@@ -192,3 +200,36 @@ def test_adaptive_oracle_solve(ada):
     u = np.zeros_like(f)
     rc, rep = om.newton(u, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 30), norm_kind=1)
     assert rc == 0 and rep.gmres_iterations < 40
+
+
+def test_adaptive_with_every_patch_marked_is_uniform():
+    """a ball that holds the whole domain marks every patch: the adaptive levels
+    are then the uniform refinements, node for node and entry for entry, with
+    no hanging nodes"""
+    a = q2_hierarchy(2, adaptive=2, radius=10.0)
+    u = q2_hierarchy(4)
+    assert a.nlevels == u.nlevels
+    for la, lu in zip(a.levels, u.levels):
+        assert len(la.hn_edges) == 0 and len(la.hn_faces) == 0
+        assert np.array_equal(la.coords, lu.coords)
+        assert np.array_equal(la.rowptr, lu.rowptr) and np.array_equal(la.col, lu.col)
+        assert np.abs(la.val - lu.val).max() <= 1e-14 * np.abs(lu.val).max()
+        assert np.array_equal(np.sort(la.dirichlet), np.sort(lu.dirichlet))
+
+
+def test_adaptive_step_that_marks_nothing_repeats_the_level():
+    """a ball outside the domain marks no patch: each adaptive level repeats the
+    previous one with an identity transfer"""
+    h = q2_hierarchy(2, adaptive=2, radius=0.25, centre=(5.0, 5.0, 5.0))
+    for l in (2, 3):
+        lev, prev = h.levels[l], h.levels[l - 1]
+        c2f, z, v, a = lev.transfer
+        assert lev.n == prev.n and np.array_equal(c2f, np.arange(prev.n))
+        assert len(z) == len(v) == len(a) == 0
+        assert np.array_equal(lev.coords, prev.coords)
+
+
+@pytest.mark.parametrize("kw", [dict(adaptive=3), dict(adaptive=2, radius=-1.0)])
+def test_adaptive_bad_arguments_raise(kw):
+    with pytest.raises(RuntimeError, match="ghs_build_adaptive"):
+        q2_hierarchy(2, **kw)
