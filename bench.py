@@ -510,7 +510,8 @@ def bench_stokes(args):
     probs = {"fp64": StokesFluxProblem(ctx, hier, prec="f64", krylov="gmres", tol=1e-10,
                                        order=order, smooth=args.fp64_smooth),
              "mixed": StokesFluxProblem(ctx, hier, prec="f32", krylov="fgmres", tol=1e-10,
-                                        order=order, smooth=args.mixed_smooth)}
+                                        order=order, smooth=args.mixed_smooth,
+                                        vmax=args.mixed_vmax)}
     t_setup = time.perf_counter() - t0
     F = H.CudaVectorInterface.from_global(ctx, f, 4)
     U = H.CudaVectorInterface(ctx, fine.n, 4)
@@ -520,7 +521,7 @@ def bench_stokes(args):
         P = probs[tag]
         U.zero()
         rep = P.mg.Solve(U, F, H.CGInfo(P.lin, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
-                         norm_kind=1)
+                         vmax=P.vmax, norm_kind=1)
         if H.N.NL_STATUS[rep.status] != "converged":
             raise RuntimeError(f"{tag} Stokes solve did not converge")
         return rep
