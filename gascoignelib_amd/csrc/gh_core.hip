@@ -462,6 +462,13 @@ static int upload(gh_ctx* ctx, T** dptr, const T* host, size_t count) {
   return GH_OK;
 }
 
+// A relative column pattern shared by at least this many rows goes into the
+// pattern table; a slice runs from the table when all its rows do and they
+// follow at most kMaxSlicePatterns different patterns (the per-lane offset
+// loads then touch a few table lines per step).
+constexpr int64_t kMinPatternRows = 16;
+constexpr int kMaxSlicePatterns = 16;
+
 // orig: original row of each slot (nullptr: slot = row); rowptr/col/vals are
 // already in slot order.
 static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
@@ -519,26 +526,35 @@ static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const 
     return GH_OK;
   }
   // Relative column patterns (col - row over a row's kept entries, in order)
-  // that at least one slice worth of rows share.  A slice whose rows all follow
+  // that at least kMinPatternRows rows share.  A slice whose rows all follow
   // table patterns stores no columns: rlen holds the pattern ids.
   std::vector<int32_t> rpid(n, -1), ptab, pstart(1, 0);
   {
-    constexpr size_t kMaxCandidates = 1 << 16;
-    constexpr int64_t kMaxTable = 1 << 16;  // offsets in the table
+    constexpr size_t kMaxCandidates = 1 << 20;
+    constexpr int64_t kMaxTable = 1 << 22;  // offsets in the table
+    auto row_hash = [&](int64_t i) {
+      uint64_t hsh = 1469598103934665603ull;
+      const int64_t orow = orig ? orig[i] : i;
+      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
+        if (keep(q)) hsh = (hsh ^ (uint32_t)(int32_t)((int64_t)col[q] - orow)) * 1099511628211ull;
+      return hsh;
+    };
+    // rows per hash: only patterns frequent enough become candidates
+    std::vector<uint64_t> rh(n);
+    std::unordered_map<uint64_t, int64_t> hcount;
+    hcount.reserve(n / 4 + 16);
+    for (int64_t i = 0; i < n; ++i) hcount[rh[i] = row_hash(i)]++;
     std::unordered_map<uint64_t, std::vector<int32_t>> byhash;
     std::vector<std::vector<int32_t>> cand;
     std::vector<int64_t> count;
     std::vector<int32_t> rcand(n, -1), offs;
     for (int64_t i = 0; i < n; ++i) {
+      const uint64_t hsh = rh[i];
+      if (hcount[hsh] < kMinPatternRows) continue;
       offs.clear();
-      uint64_t hsh = 1469598103934665603ull;
       const int64_t orow = orig ? orig[i] : i;
-      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-        if (!keep(q)) continue;
-        const int32_t o = (int32_t)((int64_t)col[q] - orow);
-        offs.push_back(o);
-        hsh = (hsh ^ (uint32_t)o) * 1099511628211ull;
-      }
+      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
+        if (keep(q)) offs.push_back((int32_t)((int64_t)col[q] - orow));
       std::vector<int32_t>& ids = byhash[hsh];
       int32_t id = -1;
       for (int32_t c : ids)
@@ -554,10 +570,10 @@ static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const 
         rcand[i] = id;
       }
     }
-    // table in order of first appearance: patterns used by >= kSlice rows
+    // table in order of first appearance: patterns used by >= kMinPatternRows rows
     std::vector<int32_t> pid_of(cand.size(), -1);
     for (size_t c = 0; c < cand.size(); ++c) {
-      if (count[c] < kSlice || (int64_t)(ptab.size() + cand[c].size()) > kMaxTable ||
+      if (count[c] < kMinPatternRows || (int64_t)(ptab.size() + cand[c].size()) > kMaxTable ||
           pstart.size() > 65535)
         continue;
       pid_of[c] = (int32_t)pstart.size() - 1;
@@ -577,14 +593,21 @@ static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const 
   S->narrow = 0;
   S->patterned = 0;
   S->col_bytes = 0;
+  std::vector<int32_t> ids(kSlice);
   for (int64_t s = 0; s < S->nslices; ++s) {
     bool all = true;
     int64_t kept = 0;
+    int m = 0;
     for (int lane = 0; lane < kSlice; ++lane) {
       const int64_t i = s * kSlice + lane;
       if (i >= n) break;
       all = all && rpid[i] >= 0;
       kept += rlen[i];
+      ids[m++] = rpid[i];
+    }
+    if (all) {
+      std::sort(ids.begin(), ids.begin() + m);
+      all = (std::unique(ids.begin(), ids.begin() + m) - ids.begin()) <= kMaxSlicePatterns;
     }
     if (all) {
       pat[s] = 1;
@@ -704,6 +727,16 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
     key[i] = h;
     len[i] = k;
   }
+  // rows of patterns frequent enough for the pattern table (by hash)
+  std::vector<uint8_t> frequent(n, 0);
+  {
+    std::unordered_map<uint64_t, int64_t> hcount;
+    hcount.reserve(n / 4 + 16);
+    for (int64_t i = 0; i < n; ++i) hcount[key[i]]++;
+    for (int64_t i = 0; i < n; ++i) frequent[i] = hcount[key[i]] >= kMinPatternRows;
+  }
+  // by_pattern: rows of one length with the frequent patterns first, then by
+  // pattern, so that table rows fill slices of their own
   auto window_sort = [&](bool by_pattern) {
     std::vector<int32_t> p(n);
     for (int64_t i = 0; i < n; ++i) p[i] = (int32_t)i;
@@ -711,7 +744,9 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
       const int64_t w1 = std::min<int64_t>(n, w0 + kSellWindow);
       std::stable_sort(p.begin() + w0, p.begin() + w1, [&](int32_t a, int32_t b) {
         if (len[a] != len[b]) return len[a] > len[b];
-        return by_pattern && key[a] < key[b];
+        if (!by_pattern) return false;
+        if (frequent[a] != frequent[b]) return frequent[a] > frequent[b];
+        return key[a] < key[b];
       });
     }
     return p;
