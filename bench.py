@@ -765,6 +765,14 @@ def bench_q2(args, adaptive=0):
     t_spmv = time_kernel(ctx, lambda: Ak.residual(R, X, Fk), args.spmv_reps)
     lay = Ak.layout()
     b64 = sell_bytes(n, lay, 8, 8)
+    # the mixed V-cycle's finest-level residual on its own matrix (FP32 vectors)
+    Am = sol["mixed"][0].level_matrix(fp64=False)
+    Xm = H.CudaVectorInterface.from_global(ctx, res["fp64"]["forward"]["x"][order], 1, "f32")
+    Fm = H.CudaVectorInterface.from_global(ctx, f[order], 1, "f32")
+    Rm = H.CudaVectorInterface(ctx, n, 1, "f32")
+    t_spmv_mx = time_kernel(ctx, lambda: Am.residual(Rm, Xm, Fm), args.spmv_reps)
+    lay_mx = Am.layout()
+    bmx = sell_bytes(n, lay_mx, 2 if args.mixed == "f16" else 4, 4)
     line = {
         "metric": "DOF/s of a Q2 Poisson solve to 1e-10 rel residual (mixed vs FP64), "
                   "plus the adjoint solve",
@@ -808,6 +816,9 @@ def bench_q2(args, adaptive=0):
                               "frac": b64 / t_spmv / 1e9 / HBM_PEAK_GBS,
                               "stored_entries": lay["stored"], "pattern_entries": fine.nnz,
                               "layout": lay},
+        "residual_spmv_mixed": {"us_per_launch": t_spmv_mx * 1e6, "bytes_per_launch": bmx,
+                                "gbs": bmx / t_spmv_mx / 1e9,
+                                "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS, "layout": lay_mx},
         "setup_s": {"structure": t_struct, "device": t_setup},
     }
     print(json.dumps(line), flush=True)
