@@ -9,7 +9,8 @@ One "step" is one full solve from u = 0, stopping at
     FP64 direct solve on levels below 1000 nodes -- the reference's algorithm;
   * mixed: a V(3,2) cycle (the fastest setting of each path: the FP64 path is
     fastest at its reference V(2,2)) in FP32 arithmetic with FP16-stored matrix
-    values, as the preconditioner of a flexible GMRES(10) whose Arnoldi vectors,
+    values, as the preconditioner of a flexible GMRES (restart length 20,
+    tolerance measured from its starting residual) whose Arnoldi vectors,
     products with A, Givens rotations and update are FP64; the mixed solve
     with the identical V(2,2) cycle is reported beside it.
 Inputs (matrix hierarchy, right-hand side) are resident in HBM before the timed
@@ -287,8 +288,8 @@ def main():
         "config": {"workload": f"3D Q1 Poisson, unit cube, {args.cells}^3 cells",
                    "dof": ndof, "nnz": fine.nnz, "levels": hier.nlevels,
                    "solver": "FP64: Newton/IR + GMRES(10) left-preconditioned, V(2,2); mixed: "
-                             + ("flexible GMRES(10)" if args.mixed_krylov == "fgmres"
-                                else "Newton/IR + GMRES(10)")
+                             + (f"flexible GMRES({args.mixed_vmax})"
+                                if args.mixed_krylov == "fgmres" else "Newton/IR + GMRES(10)")
                              + f", V({mpre},{mpost}); Jacobi omega=0.8, ndirect=1000",
                    "order": args.order},
         "fp64": {"dof_per_s": dofs_64, "ms_per_solve": t64 * 1e3, "linear_tol": 1e-10,
@@ -484,7 +485,7 @@ def bench_stokes(args):
     V(2,2) cycles of the Vanka smoother (omega 0.8, 2x2x2-cell patches, 108
     unknowns each): FP64 (left-preconditioned GMRES(10) in Newton, the
     reference's algorithm) against mixed precision (FP32 matrices, FP32 patch
-    factors and FP32 V-cycle inside FP64 flexible GMRES(10)).  Beside each
+    factors and FP32 V-cycle inside FP64 flexible GMRES(20)).  Beside each
     solve, the gradient of the outflow flux with respect to the inflow data
     (adjoint solve on the transposed levels + one transposed product) is timed
     and checked against the inflow-face masses."""
