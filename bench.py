@@ -203,7 +203,8 @@ def main():
     def solve(mg, U, linear_tol):
         U.zero()
         rep = mg.Solve(U, F, H.CGInfo(linear_tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
-                       vmax=10 if mg is mg64 else args.mixed_vmax, norm_kind=1)
+                       vmax=10 if (mg is mg64 or args.mixed_krylov == "gmres") else args.mixed_vmax,
+                       norm_kind=1)
         if H.N.NL_STATUS[rep.status] != "converged":
             raise RuntimeError(f"solve did not converge: {H.N.NL_STATUS[rep.status]}")
         return rep
