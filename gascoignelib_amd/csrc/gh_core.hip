@@ -1316,6 +1316,44 @@ int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, cons
   return csr_create_impl(ctx, n, ncomp, rowptr, col, val, prec, true, out);
 }
 
+int gh_csr_create_host(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr,
+                       const int32_t* col, const double* val, gh_csr** out) {
+  GH_ARG(ctx && out && rowptr && n >= 0 && ncomp >= 1 && ncomp <= 8,
+         "gh_csr_create_host: bad arguments (ncomp must be 1..8)");
+  GH_ARG(n < (int64_t)kIdxMask, "gh_csr_create_host: too many rows");
+  *out = nullptr;
+  const int64_t nnz = rowptr[n];
+  GH_ARG(rowptr[0] == 0 && nnz >= 0 && (nnz == 0 || (col && val)),
+         "gh_csr_create_host: bad rowptr");
+  for (int64_t i = 0; i < n; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) return fail(GH_EINVAL, "gh_csr_create_host: rowptr not monotone");
+    bool diag = false;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      if (col[p] < 0 || col[p] >= n)
+        return fail(GH_EINVAL, "gh_csr_create_host: column %d out of range in row %lld", col[p],
+                    (long long)i);
+      if (p > rowptr[i] && col[p] <= col[p - 1])
+        return fail(GH_EINVAL, "gh_csr_create_host: columns of row %lld not strictly ascending",
+                    (long long)i);
+      diag = diag || col[p] == i;
+    }
+    if (!diag) return fail(GH_EINVAL, "gh_csr_create_host: row %lld has no diagonal", (long long)i);
+  }
+  gh_csr* A = new gh_csr();
+  A->ctx = ctx;
+  A->n = n;
+  A->ncomp = ncomp;
+  A->nnz = nnz;
+  A->prec = GH_F64;
+  A->host_only = true;
+  A->h_rowptr.assign(rowptr, rowptr + n + 1);
+  A->h_col.assign(col, col + nnz);
+  A->h_val.assign(val, val + nnz * (int64_t)ncomp * ncomp);
+  ctx_retain(ctx);  // balanced by gh_csr_destroy
+  *out = A;
+  return GH_OK;
+}
+
 int gh_csr_destroy(gh_csr* A) {
   if (!A) return GH_OK;
   gh_ctx* ctx = A->ctx;
@@ -1335,6 +1373,7 @@ int64_t gh_csr_nnz(const gh_csr* A) { return A ? A->nnz : -1; }
 int gh_csr_prec(const gh_csr* A) { return A ? A->prec : -1; }
 int gh_csr_layout(const gh_csr* A, int64_t* out) {
   GH_ARG(A && out, "gh_csr_layout: NULL argument");
+  GH_ARG(!A->host_only, "gh_csr_layout: the matrix has no device layout (created by gh_csr_create_host)");
   if (A->ncomp == 1) {
     out[0] = A->sell.stored;
     out[1] = A->sell.slots;
@@ -1367,6 +1406,7 @@ static int vec_matches(const gh_csr* A, const gh_vec* v) {
 
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b) {
   GH_ARG(ctx && A && vec_matches(A, y) && vec_matches(A, x), "gh_spmv: shape mismatch");
+  GH_ARG(!A->host_only, "gh_spmv: the matrix has no device layout (created by gh_csr_create_host)");
   GH_ARG(y->d != x->d, "gh_spmv: x and y must not alias");
   return csr_spmv(ctx, A, y->prec, y->d, x->prec, x->d, d, b);
 }
@@ -1374,6 +1414,7 @@ int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, 
 int gh_residual(gh_ctx* ctx, const gh_csr* A, gh_vec* r, const gh_vec* x, const gh_vec* b) {
   GH_ARG(ctx && A && vec_matches(A, r) && vec_matches(A, x) && vec_matches(A, b),
          "gh_residual: shape mismatch");
+  GH_ARG(!A->host_only, "gh_residual: the matrix has no device layout (created by gh_csr_create_host)");
   GH_ARG(r->prec == x->prec && x->prec == b->prec, "gh_residual: precision mismatch");
   GH_ARG(r->d != x->d, "gh_residual: r and x must not alias");
   if (r->prec == GH_F64 && A->prec != GH_F64)
@@ -1385,6 +1426,7 @@ int gh_smooth(gh_ctx* ctx, const gh_csr* A, gh_vec* x, const gh_vec* b, gh_vec* 
               double omega, int smoother) {
   GH_ARG(ctx && A && vec_matches(A, x) && vec_matches(A, b) && vec_matches(A, h),
          "gh_smooth: shape mismatch");
+  GH_ARG(!A->host_only, "gh_smooth: the matrix has no device layout (created by gh_csr_create_host)");
   GH_ARG(x->prec == b->prec && b->prec == h->prec, "gh_smooth: precision mismatch");
   GH_ARG(smoother == 0 || smoother == 1, "gh_smooth: smoother must be 0 (jacobi) or 1 (richardson)");
   void* cur = x->d;
@@ -1399,6 +1441,7 @@ int gh_smooth(gh_ctx* ctx, const gh_csr* A, gh_vec* x, const gh_vec* b, gh_vec* 
 
 int gh_jacobi(gh_ctx* ctx, const gh_csr* A, gh_vec* h) {
   GH_ARG(ctx && A && vec_matches(A, h), "gh_jacobi: shape mismatch");
+  GH_ARG(!A->host_only, "gh_jacobi: the matrix has no device layout (created by gh_csr_create_host)");
   // h = D^{-1} h, as a sweep from x = 0 with omega = 1 into a temporary
   gh_vec t;
   GH_TRY(vec_alloc(ctx, h->n, h->ncomp, h->prec, &t));

@@ -183,6 +183,11 @@ struct gh_csr {
   // host copies kept for derived structures (transpose, coarse direct solve)
   std::vector<int32_t> h_rowptr, h_col;
   std::vector<double> h_val;
+  // created by gh_csr_create_host: the host copies only, no device layout;
+  // accepted by gh_mg_create_ordered (which builds its own device matrices)
+  // and as the finest-level handle of such a solver, refused by every device
+  // operation
+  bool host_only = false;
   // transposed structure for gh_spmv_t (built on first use)
   int32_t* t_rowptr = nullptr;
   int32_t* t_col = nullptr;

@@ -835,6 +835,11 @@ extern "C" {
 int gh_mg_create(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T,
                  gh_hn* const* H, gh_mask* const* M, const gh_mg_params* params,
                  gh_mg** out) {
+  GH_ARG(A && nlevels >= 1, "gh_mg_create: bad arguments");
+  for (int l = 0; l < nlevels; ++l)
+    GH_ARG(!A[l] || !A[l]->host_only,
+           "gh_mg_create: level matrices need a device layout (host-only matrices serve "
+           "gh_mg_create_ordered)");
   return mg_build(ctx, nlevels, A, T, H, M, nullptr, params, out);
 }
 
@@ -1464,6 +1469,7 @@ int gh_spmv_t(gh_ctx* ctx, const gh_csr* Ac, gh_vec* y, const gh_vec* x, double 
   GH_ARG(ctx && Ac && x && y && x->n == Ac->n && y->n == Ac->n && x->ncomp == Ac->ncomp &&
              y->ncomp == Ac->ncomp,
          "gh_spmv_t: shape mismatch");
+  GH_ARG(!Ac->host_only, "gh_spmv_t: the matrix has no device layout (created by gh_csr_create_host)");
   GH_ARG(Ac->prec == GH_F64 && x->prec == GH_F64 && y->prec == GH_F64,
          "gh_spmv_t: FP64 only");
   GH_ARG(x->d != y->d, "gh_spmv_t: x and y must not alias");

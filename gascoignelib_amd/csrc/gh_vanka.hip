@@ -343,6 +343,8 @@ extern "C" {
 
 int gh_vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize,
                     const int32_t* nodes, int prec, gh_vanka** out) {
+  GH_ARG(A, "gh_vanka_create: NULL matrix");
+  GH_ARG(!A->host_only, "gh_vanka_create: the matrix has no device layout (created by gh_csr_create_host)");
   return vanka_create(ctx, A, npatches, psize, nodes, prec, out);
 }
 

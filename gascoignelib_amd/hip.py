@@ -130,6 +130,26 @@ class CudaCSRMatrixInterface:
                                     ptr(col, N.i32p), ptr(val), N.PREC[prec], C.byref(self.h)))
 
     @classmethod
+    def host_only(cls, ctx, rowptr, col, val, n_comp=1):
+        """An FP64 matrix kept on the host only (gh_csr_create_host): a level
+        of an ordered solver, which builds its own device matrices; device
+        operations on it raise."""
+        m = cls.__new__(cls)
+        m.ctx = ctx
+        m._keep = (np.ascontiguousarray(rowptr, dtype=np.int32),
+                   np.ascontiguousarray(col, dtype=np.int32),
+                   np.ascontiguousarray(val, dtype=np.float64))
+        m._n = len(m._keep[0]) - 1
+        m.n_comp = n_comp
+        m.prec = "f64"
+        m.h = C.c_void_p()
+        check(N.hip().gh_csr_create_host(ctx.h, m._n, n_comp, ptr(m._keep[0], N.i32p),
+                                         ptr(m._keep[1], N.i32p), ptr(m._keep[2]),
+                                         C.byref(m.h)))
+        m._keep = None
+        return m
+
+    @classmethod
     def view(cls, ctx, handle, n_comp, owner):
         """A matrix owned by another object (`owner` is kept alive; the handle
         is never destroyed through the view)."""
@@ -331,9 +351,17 @@ class CudaMultiLevelSolver:
         self.hier = hier
         nl = hier.nlevels
         self.A, self.T, self.H, self.M = [], [], [], []
+        self._fine_device = None
+        self._ordered = order is not None
         for l in range(nl):
             lev = hier.level(l)
-            self.A.append(CudaCSRMatrixInterface(ctx, lev.rowptr, lev.col, lev.val, lev.ncomp, "f64"))
+            if order is None:
+                self.A.append(CudaCSRMatrixInterface(ctx, lev.rowptr, lev.col, lev.val, lev.ncomp,
+                                                     "f64"))
+            else:
+                # the ordered solver builds its own device matrices from these
+                self.A.append(CudaCSRMatrixInterface.host_only(ctx, lev.rowptr, lev.col, lev.val,
+                                                               lev.ncomp))
             if l > 0:
                 self.T.append(CudaMgInterpolatorNested(ctx, hier.level(l - 1).n, *lev.transfer))
             else:
@@ -393,7 +421,15 @@ class CudaMultiLevelSolver:
 
     @property
     def fine_matrix(self):
-        return self.A[-1]
+        """The finest-level matrix in the caller's numbering, on the device
+        (for an ordered solver made on first use: its levels are host-only)."""
+        if self.A[-1].h and not self._ordered:
+            return self.A[-1]
+        if self._fine_device is None:
+            lev = self.hier.level(self.hier.nlevels - 1)
+            self._fine_device = CudaCSRMatrixInterface(self.ctx, lev.rowptr, lev.col, lev.val,
+                                                       lev.ncomp, "f64")
+        return self._fine_device
 
     def level_matrix(self, level=None, fp64=True):
         """The matrix the solver runs on `level` (finest by default), in its
@@ -430,14 +466,14 @@ class CudaMultiLevelSolver:
     def Gmres(self, x, b, info, vmax=10, coarselevel=0):
         reached = C.c_int()
         m = self.fine_mask
-        check(N.hip().gh_gmres_solve(self.ctx.h, self.h, self.fine_matrix.h, m.h if m else None,
+        check(N.hip().gh_gmres_solve(self.ctx.h, self.h, self.A[-1].h, m.h if m else None,
                                      coarselevel, vmax, x.h, b.h, C.byref(info.s), C.byref(reached)))
         return reached.value
 
     def Solve(self, u, f, linfo, nlinfo, vmax=10, coarselevel=0, norm_kind=0):
         rep = N.GhSolveReport()
         m, hn = self.fine_mask, self.fine_hn
-        check(N.hip().gh_newton_solve(self.ctx.h, self.h, self.fine_matrix.h, m.h if m else None,
+        check(N.hip().gh_newton_solve(self.ctx.h, self.h, self.A[-1].h, m.h if m else None,
                                       hn.h if hn else None, coarselevel, vmax, u.h, f.h,
                                       C.byref(linfo.s), C.byref(nlinfo.s), norm_kind,
                                       C.byref(rep)))

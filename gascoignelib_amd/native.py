@@ -90,6 +90,7 @@ HIP_SIGNATURES = {
     "gh_vec_dot": (C.c_int, [vp, vp, vp, f64p]),
     "gh_vec_norm_l8": (C.c_int, [vp, vp, f64p]),
     "gh_csr_create": (C.c_int, [vp, C.c_int64, C.c_int, i32p, i32p, f64p, C.c_int, P(vp)]),
+    "gh_csr_create_host": (C.c_int, [vp, C.c_int64, C.c_int, i32p, i32p, f64p, P(vp)]),
     "gh_csr_destroy": (C.c_int, [vp]),
     "gh_csr_n": (C.c_int64, [vp]),
     "gh_csr_nnz": (C.c_int64, [vp]),

@@ -103,6 +103,16 @@ int gh_vec_norm_l8(gh_ctx* ctx, const gh_vec* x, double* out);
  * computed in FP64 and kept in FP64 (prec GH_F64) or FP32. */
 int gh_csr_create(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr,
                   const int32_t* col, const double* val, int prec, gh_csr** out);
+/* An FP64 level matrix kept on the host only (same arguments and checks as
+ * gh_csr_create, nothing allocated on the device; no reference counterpart).
+ * gh_mg_create_ordered builds its own device matrices from the host copies,
+ * so a caller handing it such matrices skips the device layout of the
+ * caller-numbered levels; the finest one is then also the handle its
+ * gh_gmres_solve / gh_newton_solve take.  Every device operation (products,
+ * residuals, smoothing, gh_csr_layout, gh_vanka_create, gh_mg_create) refuses
+ * it with GH_EINVAL. */
+int gh_csr_create_host(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr,
+                       const int32_t* col, const double* val, gh_csr** out);
 int gh_csr_destroy(gh_csr* A);
 int64_t gh_csr_n(const gh_csr* A);
 int64_t gh_csr_nnz(const gh_csr* A);

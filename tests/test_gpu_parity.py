@@ -159,6 +159,46 @@ def test_spmv_transpose(ctx, golden):
     assert np.allclose(Y.copy_back(), A.T @ x, rtol=1e-13, atol=1e-14)
 
 
+def test_host_only_matrices(ctx):
+    """gh_csr_create_host: validated like gh_csr_create, refused by every
+    device operation (no device layout exists), accepted by the ordered
+    solver, whose solve matches the solver built from device matrices"""
+    import ctypes as C
+    from gascoignelib_amd import native as N
+    from gascoignelib_amd.native import GhError
+    with pytest.raises(GhError):
+        H.CudaCSRMatrixInterface.host_only(ctx, [0, 1, 2], [0, 5], [1.0, 1.0])
+    rowptr, col, val = [0, 2, 4], [0, 1, 0, 1], [2.0, -1.0, -1.0, 2.0]
+    A = H.CudaCSRMatrixInterface.host_only(ctx, rowptr, col, val)
+    assert A.n() == 2 and A.n_entries() == 4
+    X, Y, B = vec(ctx, np.ones(2)), H.CudaVectorInterface(ctx, 2), vec(ctx, np.ones(2))
+    for call in (lambda: A.vmult(Y, X), lambda: A.vmult_transpose(Y, X),
+                 lambda: A.residual(Y, X, B), lambda: A.smooth(Y, B, X, 1, 0.8),
+                 lambda: A.jacobi(Y), lambda: A.layout()):
+        with pytest.raises(GhError):
+            call()
+    nodes = np.array([[0, 1]], np.int32)
+    V = C.c_void_p()
+    assert N.hip().gh_vanka_create(ctx.h, A.h, 1, 2, N.ptr(nodes, N.i32p), 0, C.byref(V)) != 0
+    p = N.GhMgParams()
+    N.hip().gh_mg_params_default(C.byref(p))
+    arr = (C.c_void_p * 1)(A.h)
+    none = (C.c_void_p * 1)(None)
+    mg = C.c_void_p()
+    assert N.hip().gh_mg_create(ctx.h, 1, arr, none, none, none, C.byref(p), C.byref(mg)) != 0
+    # ordered solvers run on host-only levels (the Python wrapper's choice)
+    h = Hierarchy.from_fixture(fixture_path("q1_3d_16"))
+    f = rhs(h)
+    out = []
+    for order in (None, [np.arange(lev.n) for lev in h.levels]):
+        mgs = H.CudaMultiLevelSolver(ctx, h, ndirect=1, order=order, **MGP)
+        U = mgs.new_vector()
+        mgs.Solve(U, vec(ctx, f), H.CGInfo(1e-10, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 30))
+        out.append(U.copy_back())
+        mgs.free()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_invalid_arguments_are_errors(ctx):
     from gascoignelib_amd.native import GhError
     with pytest.raises(GhError):
