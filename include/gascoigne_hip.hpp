@@ -177,6 +177,17 @@ class CudaCSRMatrixInterface {
     check(gh_csr_create(ctx.get(), n, n_comp, rowptr.data(), col.data(), val.data(), prec,
                         h_.out()));
   }
+  // An FP64 level matrix kept on the host only (gh_csr_create_host): for a
+  // CudaMultiLevelSolver given an order, which builds its own device matrices;
+  // products, residuals and smoothing on it throw.
+  struct HostOnly {};
+  CudaCSRMatrixInterface(HostOnly, const Context& ctx, int64_t n,
+                         const std::vector<int32_t>& rowptr, const std::vector<int32_t>& col,
+                         const std::vector<double>& val, int n_comp = 1)
+      : ctx_(&ctx) {
+    check(gh_csr_create_host(ctx.get(), n, n_comp, rowptr.data(), col.data(), val.data(),
+                             h_.out()));
+  }
   int64_t n() const { return gh_csr_n(h_.get()); }
   int64_t n_entries() const { return gh_csr_nnz(h_.get()); }
   // y = d*A*x + b*y (cudacsrmatrixinterface.cc:372-412)

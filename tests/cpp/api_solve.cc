@@ -121,7 +121,12 @@ int main(int argc, char** argv) {
   std::vector<const G::CudaMgInterpolatorNested*> pt;
   std::vector<const G::DirichletMask*> pm;
   for (int l = 0; l < nl; ++l) {
-    A.emplace_back(ctx, L[l].n, L[l].rowptr, L[l].col, L[l].val);
+    // an ordered solver builds its own device matrices: hand it host-only ones
+    if (order == "lexicographic")
+      A.emplace_back(G::CudaCSRMatrixInterface::HostOnly{}, ctx, L[l].n, L[l].rowptr, L[l].col,
+                     L[l].val);
+    else
+      A.emplace_back(ctx, L[l].n, L[l].rowptr, L[l].col, L[l].val);
     pa.push_back(&A.back());
     if (l > 0) {
       T.emplace_back(ctx, L[l].c2f, L[l].zw, L[l].vi, L[l].ac);
