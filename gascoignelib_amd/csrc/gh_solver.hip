@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "gh_sell.h"
@@ -568,6 +569,28 @@ namespace {
 // Renumbered copies of the caller's level operators: internal position k holds
 // the caller's node perm[k].  Every row keeps its entries in the caller's order,
 // so sums are formed in the same order as without the renumbering.
+
+// fn(begin, end) over [0, n) in contiguous chunks on up to 16 host threads
+// (the calling thread takes the first); the chunks write disjoint ranges, so
+// the result does not depend on the thread count.
+template <class F>
+void parallel_rows(int64_t n, F&& fn) {
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  const int T = (int)std::min<int64_t>(std::min<int64_t>(16, hw), std::max<int64_t>(1, n >> 15));
+  if (T <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  const int64_t step = (n + T - 1) / T;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) {
+    const int64_t a = t * step, b = std::min(n, a + step);
+    if (a < b) pool.emplace_back([&fn, a, b] { fn(a, b); });
+  }
+  fn((int64_t)0, std::min(n, step));
+  for (std::thread& th : pool) th.join();
+}
+
 int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* const* A,
                         gh_transfer* const* T, gh_hn* const* H, gh_mask* const* M,
                         const int32_t* const* perm, MgLevel& Lv) {
@@ -594,18 +617,20 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
     s_col = scl.data();
     s_val = svl.data();
   }
-  std::vector<int32_t> rp(n + 1, 0), cl;
-  std::vector<double> vl;
-  cl.reserve(Au->nnz);
-  vl.reserve(Au->nnz * BB);
-  for (int64_t k = 0; k < n; ++k) {
-    const int32_t o = P[k];
-    for (int32_t q = s_rp[o]; q < s_rp[o + 1]; ++q) {
-      cl.push_back(Q[s_col[q]]);
-      vl.insert(vl.end(), s_val + (int64_t)q * BB, s_val + (int64_t)(q + 1) * BB);
+  std::vector<int32_t> rp(n + 1, 0);
+  for (int64_t k = 0; k < n; ++k) rp[k + 1] = rp[k] + (s_rp[P[k] + 1] - s_rp[P[k]]);
+  std::vector<int32_t> cl(rp[n]);
+  std::vector<double> vl((size_t)rp[n] * BB);
+  parallel_rows(n, [&](int64_t a, int64_t b) {
+    for (int64_t k = a; k < b; ++k) {
+      const int32_t o = P[k];
+      int64_t t = rp[k];
+      for (int32_t q = s_rp[o]; q < s_rp[o + 1]; ++q, ++t) {
+        cl[t] = Q[s_col[q]];
+        std::copy(s_val + (int64_t)q * BB, s_val + (int64_t)(q + 1) * BB, vl.begin() + t * BB);
+      }
     }
-    rp[k + 1] = (int32_t)cl.size();
-  }
+  });
   GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cl.data(), vl.data(), GH_F64, false, &Lv.A64));
   Lv.own_A64 = true;
   if (params->prec != GH_F64) {
@@ -613,18 +638,21 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
     // take their entries in ascending internal column order: rows of one kind
     // then share one relative pattern, whatever order the caller's numbering
     // gave them.
-    std::vector<int32_t> idx, cs(cl.size());
+    std::vector<int32_t> cs(cl.size());
     std::vector<double> vs(vl.size());
-    for (int64_t k = 0; k < n; ++k) {
-      idx.resize(rp[k + 1] - rp[k]);
-      for (size_t t = 0; t < idx.size(); ++t) idx[t] = rp[k] + (int32_t)t;
-      std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return cl[a] < cl[b]; });
-      for (size_t t = 0; t < idx.size(); ++t) {
-        cs[rp[k] + t] = cl[idx[t]];
-        std::copy(vl.begin() + (int64_t)idx[t] * BB, vl.begin() + (int64_t)(idx[t] + 1) * BB,
-                  vs.begin() + (int64_t)(rp[k] + t) * BB);
+    parallel_rows(n, [&](int64_t a, int64_t b) {
+      std::vector<int32_t> idx;
+      for (int64_t k = a; k < b; ++k) {
+        idx.resize(rp[k + 1] - rp[k]);
+        for (size_t t = 0; t < idx.size(); ++t) idx[t] = rp[k] + (int32_t)t;
+        std::sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return cl[x] < cl[y]; });
+        for (size_t t = 0; t < idx.size(); ++t) {
+          cs[rp[k] + t] = cl[idx[t]];
+          std::copy(vl.begin() + (int64_t)idx[t] * BB, vl.begin() + (int64_t)(idx[t] + 1) * BB,
+                    vs.begin() + (int64_t)(rp[k] + t) * BB);
+        }
       }
-    }
+    });
     GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cs.data(), vs.data(), params->prec, false, &Lv.A));
     Lv.own_A = true;
   } else {
