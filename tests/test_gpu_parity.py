@@ -511,3 +511,34 @@ def test_flexible_gmres_solve(ctx, name, prec):
     print(f"{name} {prec}: newton {rep.newton_iterations} fgmres {rep.gmres_iterations} "
           f"rel. error vs FP64 oracle {err:.2e}")
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_fgmres_tolerance_base_and_estimate(ctx, prec):
+    """Flexible GMRES measures its relative tolerance from the residual it
+    starts from, and its residual estimate is the true FP64 residual (the
+    products A z_j are FP64); the reference's left-preconditioned GMRES instead
+    takes the residual after its first step as the base"""
+    h = Hierarchy.from_fixture(fixture_path("q1_3d_16"))
+    lev = h.fine
+    f = rhs(h)
+    A = sp.csr_matrix((lev.val, lev.col, lev.rowptr), shape=(lev.n, lev.n))
+    r0 = f.copy()
+    r0[lev.dirichlet] = 0.0
+    mg = H.CudaMultiLevelSolver(ctx, h, ndirect=1, prec=prec, krylov="fgmres", **MGP)
+    info = H.CGInfo(1e-6, 1e-30, 50)
+    X = mg.new_vector()
+    assert mg.Gmres(X, vec(ctx, f), info, vmax=20)
+    r = f - A @ X.copy_back()
+    r[lev.dirichlet] = 0.0
+    assert info.s.firstresidual == pytest.approx(np.linalg.norm(r0), rel=1e-12)
+    assert info.s.residual == pytest.approx(np.linalg.norm(r), rel=1e-6)
+    assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(r0)
+    # the left-preconditioned GMRES works on M^-1 r; its first recorded
+    # residual is the one after its first step, well below |M^-1 r0|
+    mg64 = H.CudaMultiLevelSolver(ctx, h, ndirect=1, **MGP)
+    info64 = H.CGInfo(1e-6, 1e-30, 50)
+    mg64.Gmres(mg64.new_vector(), vec(ctx, f), info64, vmax=10)
+    Z = mg64.new_vector()
+    mg64.precondition(Z, vec(ctx, r0), x_is_zero=True)
+    assert info64.s.firstresidual < 0.5 * np.linalg.norm(Z.copy_back())
