@@ -8,7 +8,9 @@
 // EXACT = true the row sums reproduce the reference's sequential loops bit for
 // bit (SparseBlockMatrix::vmult, src/LinAlg/sparseblockmatrix.xx:378-392;
 // MgInterpolatorMatrix::restrict_zero / prolongate_add,
-// src/Discretization/Q1/mginterpolatormatrix.cc:123-147).
+// src/Discretization/Q1/mginterpolatormatrix.cc:123-147).  Reduced-precision
+// (non-exact) sweeps over levels of few slices may instead split a slice over
+// K wavefronts (k_sell_split).
 #pragma once
 
 #include <type_traits>
@@ -32,6 +34,7 @@ __device__ __forceinline__ T fms_step(T acc, T a, T x) {
 
 template <typename Y, typename T, bool EXACT>
 struct SpmvOp {  // y = d * (A x) + b * y
+  static constexpr bool kExact = EXACT;
   Y* y;
   T d, b;
   bool read_y;
@@ -46,6 +49,7 @@ struct SpmvOp {  // y = d * (A x) + b * y
 
 template <typename T, bool EXACT>
 struct ResidualOp {  // r = b - A x: r starts at b and subtracts each a_ij x_j in order
+  static constexpr bool kExact = EXACT;
   T* r;
   const T* b;
   __device__ __forceinline__ T init(int64_t i) const { return b[i]; }
@@ -55,6 +59,7 @@ struct ResidualOp {  // r = b - A x: r starts at b and subtracts each a_ij x_j i
 
 template <typename T, typename DT, bool EXACT>
 struct JacobiOp {  // xn = x + omega * (dinv * (b - A x))
+  static constexpr bool kExact = EXACT;
   T* xn;
   const T* x;
   const T* b;
@@ -216,45 +221,50 @@ __global__ __launch_bounds__(kBlock) void k_bsell(int64_t nslices, const uint32_
 
 // One row of a valued operator: entries k = 0..len-1 at stride kSlice, column
 // = cb + cp[k * kSlice] (16-bit offsets) or cp[k * kSlice] (32-bit, cb = 0).
-template <typename CT, typename VT, typename XT, typename T, class Op>
+// With K > 1 the caller folds every K-th entry only: len counts those, and
+// cp and vp point at the first of them.
+template <int K = 1, typename CT, typename VT, typename XT, typename T, class Op>
 __device__ __forceinline__ T sell_row(const CT* __restrict__ cp, uint32_t cb,
                                       const VT* __restrict__ vp, int len,
                                       const XT* __restrict__ x, T acc, const Op& op) {
+  constexpr int S = K * kSlice;
   int k = 0;
   for (; k + 4 <= len; k += 4) {
-    const uint32_t c0 = cb + cp[(k + 0) * kSlice], c1 = cb + cp[(k + 1) * kSlice];
-    const uint32_t c2 = cb + cp[(k + 2) * kSlice], c3 = cb + cp[(k + 3) * kSlice];
-    const T a0 = (T)ld(vp + (k + 0) * kSlice), a1 = (T)ld(vp + (k + 1) * kSlice);
-    const T a2 = (T)ld(vp + (k + 2) * kSlice), a3 = (T)ld(vp + (k + 3) * kSlice);
+    const uint32_t c0 = cb + cp[(k + 0) * S], c1 = cb + cp[(k + 1) * S];
+    const uint32_t c2 = cb + cp[(k + 2) * S], c3 = cb + cp[(k + 3) * S];
+    const T a0 = (T)ld(vp + (k + 0) * S), a1 = (T)ld(vp + (k + 1) * S);
+    const T a2 = (T)ld(vp + (k + 2) * S), a3 = (T)ld(vp + (k + 3) * S);
     const T x0 = (T)x[c0], x1 = (T)x[c1], x2 = (T)x[c2], x3 = (T)x[c3];
     acc = op.step(acc, a0, x0);
     acc = op.step(acc, a1, x1);
     acc = op.step(acc, a2, x2);
     acc = op.step(acc, a3, x3);
   }
-  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * kSlice), (T)x[cb + cp[k * kSlice]]);
+  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * S), (T)x[cb + cp[k * S]]);
   return acc;
 }
 
 // One row that follows a shared relative pattern: column = row + off[k]
-// (the offsets come from a table of a few KB, served by the caches).
-template <typename VT, typename XT, typename T, class Op>
+// (the offsets come from a table of a few KB, served by the caches).  With
+// K > 1, as above: every K-th entry, off and vp at the first of them.
+template <int K = 1, typename VT, typename XT, typename T, class Op>
 __device__ __forceinline__ T sell_row_pattern(const int32_t* __restrict__ off, uint32_t row,
                                               const VT* __restrict__ vp, int len,
                                               const XT* __restrict__ x, T acc, const Op& op) {
+  constexpr int S = K * kSlice;
   int k = 0;
   for (; k + 4 <= len; k += 4) {
-    const uint32_t c0 = row + off[k], c1 = row + off[k + 1];
-    const uint32_t c2 = row + off[k + 2], c3 = row + off[k + 3];
-    const T a0 = (T)ld(vp + (k + 0) * kSlice), a1 = (T)ld(vp + (k + 1) * kSlice);
-    const T a2 = (T)ld(vp + (k + 2) * kSlice), a3 = (T)ld(vp + (k + 3) * kSlice);
+    const uint32_t c0 = row + off[(k + 0) * K], c1 = row + off[(k + 1) * K];
+    const uint32_t c2 = row + off[(k + 2) * K], c3 = row + off[(k + 3) * K];
+    const T a0 = (T)ld(vp + (k + 0) * S), a1 = (T)ld(vp + (k + 1) * S);
+    const T a2 = (T)ld(vp + (k + 2) * S), a3 = (T)ld(vp + (k + 3) * S);
     const T x0 = (T)x[c0], x1 = (T)x[c1], x2 = (T)x[c2], x3 = (T)x[c3];
     acc = op.step(acc, a0, x0);
     acc = op.step(acc, a1, x1);
     acc = op.step(acc, a2, x2);
     acc = op.step(acc, a3, x3);
   }
-  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * kSlice), (T)x[row + off[k]]);
+  for (; k < len; ++k) acc = op.step(acc, (T)ld(vp + k * S), (T)x[row + off[k * K]]);
   return acc;
 }
 
@@ -304,6 +314,73 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __re
   if (active) op.finish(row, acc);
 }
 
+// The same sweep for reduced-precision (non-exact) ops on levels with few
+// slices, where one thread per row leaves the chip mostly idle and each row
+// is a long chain of dependent loads (Q2 rows hold up to 125 entries).  A
+// workgroup of K wavefronts takes one slice: wavefront w folds entries
+// w, w + K, w + 2K, ... of each of the slice's 64 rows (every load still
+// covers 64 contiguous slots), and wavefront 0 adds the K partial sums, in
+// order, to the row's start value.  That reorders the row sum, so exact ops
+// are rejected at compile time.  Every thread reaches the barrier: lanes past
+// the last row repeat row n-1, as in k_sell, and only skip the store.
+template <typename VT, typename XT, typename T, class Op, int K>
+__global__ __launch_bounds__(kSlice * K) void k_sell_split(
+    int64_t n, const uint32_t* __restrict__ sptr, const uint16_t* __restrict__ rlen,
+    const uint16_t* __restrict__ cidx, const uint32_t* __restrict__ cptr,
+    const uint32_t* __restrict__ cbase, const int32_t* __restrict__ ptab,
+    const int32_t* __restrict__ pstart, const VT* __restrict__ val,
+    const int32_t* __restrict__ perm, const XT* __restrict__ x, Op op) {
+  static_assert(!Op::kExact, "split sweeps reorder the row sums");
+  __shared__ T part[K][kSlice];
+  const int64_t s = xcd_block(blockIdx.x, gridDim.x);  // one slice per workgroup
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSlice));
+  const int64_t slot0 = s * kSlice + lane;
+  const bool active = slot0 < n;
+  const int64_t slot = active ? slot0 : n - 1;
+  const int64_t row = perm ? (int64_t)perm[slot] : slot;
+  const VT* vp = val + sptr[s] + lane + w * kSlice;
+  const uint32_t meta = cptr[s];
+  // entries w, w + K, ... below len
+  auto mine = [w](int len) { return len > w ? (len - w + K - 1) / K : 0; };
+  T acc = (T)0;
+  if ((meta & 0x60000000u) == 0x60000000u) {
+    const int pid = __builtin_amdgcn_readfirstlane((int)cbase[s]);
+    const int p0 = pstart[pid];
+    acc = sell_row_pattern<K>(ptab + p0 + w, (uint32_t)row, vp, mine(pstart[pid + 1] - p0), x,
+                              acc, op);
+  } else {
+    const int r = rlen[slot];
+    if (meta & 0x40000000u) {
+      const int p0 = pstart[r];
+      acc = sell_row_pattern<K>(ptab + p0 + w, (uint32_t)row, vp, mine(pstart[r + 1] - p0), x,
+                                acc, op);
+    } else if (meta & 0x80000000u) {
+      acc = sell_row<K>((const uint32_t*)(cidx + (meta & 0x1FFFFFFFu)) + lane + w * kSlice, 0u,
+                        vp, mine(r), x, acc, op);
+    } else {
+      acc = sell_row<K>(cidx + meta + lane + w * kSlice, cbase[s], vp, mine(r), x, acc, op);
+    }
+  }
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0) {
+    T sum = op.init(row);
+#pragma unroll
+    for (int i = 0; i < K; ++i) sum += part[i][lane];
+    if (active) op.finish(row, sum);
+  }
+}
+
+// Split factor of a reduced-precision sweep: double K while the level runs
+// fewer than 8192 wavefronts (32 per CU) and every wavefront still gets at
+// least 8 of the longest row's entries, up to 16 wavefronts per slice.
+inline int sell_split(const Sell& S) {
+  int K = 1;
+  while (K < 16 && S.nslices * K < 8192 && S.maxlen >= 16 * K) K *= 2;
+  return K;
+}
+
 // packed-weight operator (grid transfer): w = 2^-code from the column's top bits
 template <typename T, class Op>
 __global__ __launch_bounds__(kBlock) void k_sell_w(int64_t n, const uint32_t* __restrict__ sptr,
@@ -334,9 +411,27 @@ __global__ __launch_bounds__(kBlock) void k_sell_w(int64_t n, const uint32_t* __
   op.finish(row, acc);
 }
 
+template <typename VT, typename XT, typename T, class Op, int K>
+inline int sell_split_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
+  hipLaunchKernelGGL((k_sell_split<VT, XT, T, Op, K>), dim3((unsigned)S.nslices),
+                     dim3(kSlice * K), 0, ctx->stream, S.n, S.sptr, S.rlen, S.cidx, S.cptr,
+                     S.cbase, S.ptab, S.pstart, (const VT*)S.val, S.perm, x, op);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
 template <typename VT, typename XT, typename T, class Op>
 inline int sell_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
   if (S.n == 0) return GH_OK;
+  if constexpr (!Op::kExact) {
+    switch (sell_split(S)) {
+      case 2: return sell_split_launch<VT, XT, T, Op, 2>(ctx, S, x, op);
+      case 4: return sell_split_launch<VT, XT, T, Op, 4>(ctx, S, x, op);
+      case 8: return sell_split_launch<VT, XT, T, Op, 8>(ctx, S, x, op);
+      case 16: return sell_split_launch<VT, XT, T, Op, 16>(ctx, S, x, op);
+      default: break;
+    }
+  }
   hipLaunchKernelGGL((k_sell<VT, XT, T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream,
                      S.n, S.sptr, S.rlen, S.cidx, S.cptr, S.cbase, S.ptab, S.pstart,
                      (const VT*)S.val, S.perm, x, op);

@@ -485,6 +485,90 @@ def test_sell_rows_grouped_by_length(ctx):
     check_products_vs_oracle(ctx, A, rowptr, col, val, 8)
 
 
+def long_row_matrix(n, L, seed):
+    """Rows of up to L entries in four kinds of slices: one band of L offsets
+    for the whole slice, two bands alternating by row, random columns within
+    2000 of the row (16-bit offsets), random columns anywhere (32-bit indices);
+    the last two vary the row length and hold exact zeros (dropped from the
+    layout).  Values are nonzero multiples of 2^-10 in [-1, 1] elsewhere and
+    the diagonal is L + 1, all exact in f16.  The band rows keep every entry,
+    so their slices run from the pattern table in the rows' own order."""
+    rng = np.random.default_rng(seed)
+    band_a = np.arange(-(L // 2), L - L // 2) * 3
+    band_b = np.arange(-(L // 2), L - L // 2) * 5
+    rows = []
+    for i in range(n):
+        kind = (i // 64) % 4
+        if kind == 0:
+            c = i + band_a
+        elif kind == 1:
+            c = i + (band_a if i % 2 == 0 else band_b)
+        elif kind == 2:
+            c = i + rng.integers(-2000, 2000, rng.integers(1, L))
+        else:
+            c = rng.integers(0, n, rng.integers(1, L))
+        rows.append(np.unique(np.append(np.clip(c, 0, n - 1), i)))
+    rowptr = np.zeros(n + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(c) for c in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    val = rng.choice([-1.0, 1.0], len(col)) * rng.integers(1, 1025, len(col)) / 1024.0
+    for i, c in enumerate(rows):
+        if (i // 64) % 4 >= 2:
+            val[rowptr[i] + np.flatnonzero(rng.random(len(c)) < 0.05)] = 0.0
+        val[rowptr[i] + np.flatnonzero(c == i)] = L + 1.0
+    return rowptr, col, val
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("n,L", [(5000, 12), (5000, 24), (5000, 48), (5000, 100), (5000, 200),
+                                 (90, 30), (100, 48), (700, 200), (70000, 100)])
+def test_reduced_precision_sweeps_long_rows(ctx, n, L, prec):
+    """FP32-arithmetic products, residuals and Jacobi / Richardson sweeps on
+    levels of few slices and long rows, where one slice's entries are split
+    over up to 16 wavefronts and the partial sums meet in LDS: each result is
+    within the FP32 summation bound (L + 20) 2^-23 sum|terms| of the exact
+    value (the stored values are exact in f16 and f32), rows past the last
+    full slice included."""
+    print(f"[n={n} L={L} {prec}]", flush=True)
+    rowptr, col, val = long_row_matrix(n, L, seed=L + n)
+    M = sp.csr_matrix((val, col, rowptr), shape=(n, n))
+    absM = abs(M)
+    A = H.CudaCSRMatrixInterface(ctx, rowptr, col, val, prec=prec)
+    lay = A.layout()
+    assert lay["stored"] == int(np.count_nonzero(val))
+    if n >= 5000:
+        assert lay["patterned"] > 0 and lay["narrow"] > 0
+    if n > 65536:  # columns anywhere need 32-bit indices
+        assert lay["patterned"] + lay["narrow"] < lay["slices"]
+    rng = np.random.default_rng(n * L)
+    f32 = lambda v: v.astype(np.float32).astype(np.float64)  # noqa: E731
+    x, b, y0 = (f32(rng.uniform(-1, 1, n)) for _ in range(3))
+    vx = H.CudaVectorInterface.from_global(ctx, x, 1, "f32")
+    vb = H.CudaVectorInterface.from_global(ctx, b, 1, "f32")
+    eps = (L + 20) * 2.0 ** -23
+    ax, sx = M @ x, absM @ np.abs(x)
+
+    Y = H.CudaVectorInterface.from_global(ctx, y0, 1, "f32")
+    A.vmult(Y, vx, -0.5, 1.5)
+    want = -0.5 * ax + 1.5 * y0
+    assert np.all(np.abs(Y.copy_back() - want) <= eps * (0.5 * sx + 1.5 * np.abs(y0)) + 1e-30)
+
+    R = H.CudaVectorInterface(ctx, n, 1, "f32")
+    A.residual(R, vx, vb)
+    r = b - ax
+    bound_r = eps * (sx + np.abs(b))
+    assert np.all(np.abs(R.copy_back() - r) <= bound_r + 1e-30)
+
+    dinv = 1.0 / M.diagonal()
+    for smoother, scale in (("jacobi", 0.8 * dinv), ("richardson", np.full(n, 0.8))):
+        X = H.CudaVectorInterface.from_global(ctx, x, 1, "f32")
+        T = H.CudaVectorInterface(ctx, n, 1, "f32")
+        A.smooth(X, vb, T, 1, 0.8, smoother)
+        want = x + scale * r
+        bound = scale * bound_r + 2.0 ** -21 * (np.abs(x) + np.abs(scale * r))
+        assert np.all(np.abs(X.copy_back() - want) <= bound), smoother
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
 @pytest.mark.parametrize("name", ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada"])
 def test_flexible_gmres_solve(ctx, name, prec):
