@@ -222,13 +222,27 @@ __global__ __launch_bounds__(kBlock) void k_bsell(int64_t nslices, const uint32_
 // One row of a valued operator: entries k = 0..len-1 at stride kSlice, column
 // = cb + cp[k * kSlice] (16-bit offsets) or cp[k * kSlice] (32-bit, cb = 0).
 // With K > 1 the caller folds every K-th entry only: len counts those, and
-// cp and vp point at the first of them.
+// cp and vp point at the first of them.  The loads of 8 (then 4) entries are
+// issued before their adds, which stay in entry order: a long row pays one
+// column -> x round trip per 8 entries, and the sum keeps its bits.
 template <int K = 1, typename CT, typename VT, typename XT, typename T, class Op>
 __device__ __forceinline__ T sell_row(const CT* __restrict__ cp, uint32_t cb,
                                       const VT* __restrict__ vp, int len,
                                       const XT* __restrict__ x, T acc, const Op& op) {
   constexpr int S = K * kSlice;
   int k = 0;
+  for (; k + 8 <= len; k += 8) {
+    uint32_t c[8];
+    T a[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = cb + cp[(k + u) * S];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = (T)ld(vp + (k + u) * S);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = (T)x[c[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = op.step(acc, a[u], xv[u]);
+  }
   for (; k + 4 <= len; k += 4) {
     const uint32_t c0 = cb + cp[(k + 0) * S], c1 = cb + cp[(k + 1) * S];
     const uint32_t c2 = cb + cp[(k + 2) * S], c3 = cb + cp[(k + 3) * S];
@@ -253,6 +267,18 @@ __device__ __forceinline__ T sell_row_pattern(const int32_t* __restrict__ off, u
                                               const XT* __restrict__ x, T acc, const Op& op) {
   constexpr int S = K * kSlice;
   int k = 0;
+  for (; k + 8 <= len; k += 8) {
+    uint32_t c[8];
+    T a[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = row + off[(k + u) * K];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = (T)ld(vp + (k + u) * S);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = (T)x[c[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = op.step(acc, a[u], xv[u]);
+  }
   for (; k + 4 <= len; k += 4) {
     const uint32_t c0 = row + off[(k + 0) * K], c1 = row + off[(k + 1) * K];
     const uint32_t c2 = row + off[(k + 2) * K], c3 = row + off[(k + 3) * K];
