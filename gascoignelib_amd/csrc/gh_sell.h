@@ -346,8 +346,10 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, const uint32_t* __re
 // workgroup of K wavefronts takes one slice: wavefront w folds entries
 // w, w + K, w + 2K, ... of each of the slice's 64 rows (every load still
 // covers 64 contiguous slots), and wavefront 0 adds the K partial sums, in
-// order, to the row's start value.  That reorders the row sum, so exact ops
-// are rejected at compile time.  Every thread reaches the barrier: lanes past
+// order, to the row's start value.  Each partial chain starts from zero,
+// which is right for ops whose step is acc + a x or acc - a x (SpmvOp,
+// ResidualOp, JacobiOp).  That reorders the row sum, so exact ops are
+// rejected at compile time.  Every thread reaches the barrier: lanes past
 // the last row repeat row n-1, as in k_sell, and only skip the store.
 template <typename VT, typename XT, typename T, class Op, int K>
 __global__ __launch_bounds__(kSlice * K) void k_sell_split(
