@@ -1116,13 +1116,13 @@ int gh_vec_upload(gh_ctx* ctx, gh_vec* v, const double* host) {
     GH_HIP(hipStreamSynchronize(ctx->stream));
     return GH_OK;
   }
-  double* tmp = nullptr;
-  GH_HIP(hipMalloc((void**)&tmp, std::max<int64_t>(v->size, 1) * 8));
-  GH_HIP(hipMemcpyAsync(tmp, host, v->size * 8, hipMemcpyHostToDevice, ctx->stream));
-  int rc = vec_convert(ctx, v->size, GH_F32, v->d, GH_F64, tmp);
+  // FP32 vector: rounded to nearest on the host (the same rounding as the
+  // device conversion kernels), so that no device staging is needed
+  std::vector<float> tmp((size_t)v->size);
+  for (int64_t i = 0; i < v->size; ++i) tmp[i] = (float)host[i];
+  GH_HIP(hipMemcpyAsync(v->d, tmp.data(), v->size * 4, hipMemcpyHostToDevice, ctx->stream));
   GH_HIP(hipStreamSynchronize(ctx->stream));
-  GH_HIP(hipFree(tmp));
-  return rc;
+  return GH_OK;
 }
 
 int gh_vec_download(gh_ctx* ctx, const gh_vec* v, double* host) {
@@ -1132,12 +1132,11 @@ int gh_vec_download(gh_ctx* ctx, const gh_vec* v, double* host) {
     GH_HIP(hipStreamSynchronize(ctx->stream));
     return GH_OK;
   }
-  double* tmp = nullptr;
-  GH_HIP(hipMalloc((void**)&tmp, std::max<int64_t>(v->size, 1) * 8));
-  GH_TRY(vec_convert(ctx, v->size, GH_F64, tmp, GH_F32, v->d));
-  GH_HIP(hipMemcpyAsync(host, tmp, v->size * 8, hipMemcpyDeviceToHost, ctx->stream));
+  // FP32 vector: copied as stored and widened (exactly) on the host
+  std::vector<float> tmp((size_t)v->size);
+  GH_HIP(hipMemcpyAsync(tmp.data(), v->d, v->size * 4, hipMemcpyDeviceToHost, ctx->stream));
   GH_HIP(hipStreamSynchronize(ctx->stream));
-  GH_HIP(hipFree(tmp));
+  for (int64_t i = 0; i < v->size; ++i) host[i] = (double)tmp[i];
   return GH_OK;
 }
 
@@ -1209,6 +1208,25 @@ namespace gh {
 // check_sorted = false admits rows whose entries are kept in a given order that
 // is not ascending (operators renumbered for locality keep each row's
 // original summation order)
+template <typename DT>
+static int launch_block_inverse(gh_ctx* ctx, int64_t n, int B, const int32_t* diagpos,
+                                const double* val, DT* dinv, int* singular) {
+  const dim3 g(grid_for(n)), blk(kBlock);
+  switch (B) {
+    case 1: hipLaunchKernelGGL((k_block_inverse<1, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 2: hipLaunchKernelGGL((k_block_inverse<2, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 3: hipLaunchKernelGGL((k_block_inverse<3, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 4: hipLaunchKernelGGL((k_block_inverse<4, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 5: hipLaunchKernelGGL((k_block_inverse<5, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 6: hipLaunchKernelGGL((k_block_inverse<6, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 7: hipLaunchKernelGGL((k_block_inverse<7, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    case 8: hipLaunchKernelGGL((k_block_inverse<8, DT>), g, blk, 0, ctx->stream, n, diagpos, val, dinv, singular); break;
+    default: return fail(GH_EINVAL, "block inverse: block size %d not in 1..8", B);
+  }
+  if (hipGetLastError() != hipSuccess) return fail(GH_EHIP, "block inverse launch");
+  return GH_OK;
+}
+
 int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
                     const double* val, int prec, bool check_sorted, gh_csr** out) {
   GH_ARG(ctx && out && rowptr && n >= 0 && ncomp >= 1 && ncomp <= 8,
@@ -1255,18 +1273,12 @@ int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, co
   // inverse diagonal blocks, FP64 then narrowed for the FP32 paths
   const size_t dsz = (size_t)std::max<int64_t>(n * BB, 1);
   if (!rc) {
-    if (prec == GH_F64) {
-      if (hipMalloc(&A->dinv, dsz * 8) != hipSuccess) rc = fail(GH_ENOMEM, "alloc dinv");
-      else
-        hipLaunchKernelGGL(k_block_inverse<double>, dim3(grid_for(n)), dim3(kBlock), 0,
-                           ctx->stream, n, ncomp, ddiag, dval64, (double*)A->dinv, dsing);
-    } else {
-      if (hipMalloc(&A->dinv, dsz * 4) != hipSuccess) rc = fail(GH_ENOMEM, "alloc dinv");
-      else
-        hipLaunchKernelGGL(k_block_inverse<float>, dim3(grid_for(n)), dim3(kBlock), 0,
-                           ctx->stream, n, ncomp, ddiag, dval64, (float*)A->dinv, dsing);
-    }
-    if (!rc && hipGetLastError() != hipSuccess) rc = fail(GH_EHIP, "block inverse launch");
+    if (hipMalloc(&A->dinv, dsz * (prec == GH_F64 ? 8 : 4)) != hipSuccess)
+      rc = fail(GH_ENOMEM, "alloc dinv");
+    else if (prec == GH_F64)
+      rc = launch_block_inverse(ctx, n, ncomp, ddiag, dval64, (double*)A->dinv, dsing);
+    else
+      rc = launch_block_inverse(ctx, n, ncomp, ddiag, dval64, (float*)A->dinv, dsing);
   }
   if (!rc) {
     if (prec == GH_F64) {

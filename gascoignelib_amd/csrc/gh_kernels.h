@@ -252,9 +252,11 @@ __global__ __launch_bounds__(kBlock) void k_bsr_jacobi(int64_t n, int B,
 
 // block-diagonal inverse by Gauss-Jordan with partial pivoting, one thread per
 // node: NodeMatrix::gauss_jordan (src/LinAlg/nodematrix.h:159-213), the device
-// counterpart of invert_device (cuda/cusparsehelper.cu:24-118).
-template <typename DT>
-__global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n, int B,
+// counterpart of invert_device (cuda/cusparsehelper.cu:24-118).  B is a
+// template parameter so that every index is a constant and the block lives in
+// registers (B*B doubles per lane).
+template <int B, typename DT>
+__global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n,
                                                           const int32_t* __restrict__ diagpos,
                                                           const double* __restrict__ val,
                                                           DT* __restrict__ dinv,
@@ -262,29 +264,38 @@ __global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n, int B,
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double m[64];
-  int p[8];
+  double m[B * B];
+  int p[B];
   const double* src = val + (int64_t)diagpos[i] * B * B;
+#pragma unroll
   for (int k = 0; k < B * B; ++k) m[k] = src[k];
 #define M_(a, c) m[(a) * B + (c)]
+#pragma unroll
   for (int k = 0; k < B; ++k) p[k] = k;
+#pragma unroll
   for (int j = 0; j < B; ++j) {
+    // pivot search and row exchange by predicated selects (constant indices)
     double mx = fabs(M_(j, j));
     int r = j;
+#pragma unroll
     for (int k = j + 1; k < B; ++k)
       if (fabs(M_(k, j)) > mx) {
         mx = fabs(M_(k, j));
         r = k;
       }
-    if (r > j) {
-      for (int k = 0; k < B; ++k) {
-        double t = M_(j, k);
-        M_(j, k) = M_(r, k);
-        M_(r, k) = t;
+#pragma unroll
+    for (int k = j + 1; k < B; ++k) {
+      if (r == k) {
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          double t = M_(j, c);
+          M_(j, c) = M_(k, c);
+          M_(k, c) = t;
+        }
+        int t = p[j];
+        p[j] = p[k];
+        p[k] = t;
       }
-      int t = p[j];
-      p[j] = p[r];
-      p[r] = t;
     }
     if (M_(j, j) == 0.0) {
       atomicExch(singular, 1);
@@ -292,24 +303,28 @@ __global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n, int B,
     }
     double hr = 1. / M_(j, j);
     M_(j, j) = hr;
+#pragma unroll
     for (int k = 0; k < B; ++k) {
       if (k == j) continue;
+#pragma unroll
       for (int a = 0; a < B; ++a) {
         if (a == j) continue;
         M_(a, k) -= M_(a, j) * M_(j, k) * hr;
       }
     }
+#pragma unroll
     for (int a = 0; a < B; ++a) {
       M_(a, j) *= hr;
       M_(j, a) *= -hr;
     }
     M_(j, j) = hr;
   }
-  double hv[8];
-  for (int a = 0; a < B; ++a) {
-    for (int k = 0; k < B; ++k) hv[p[k]] = M_(a, k);
-    for (int k = 0; k < B; ++k) dinv[i * B * B + a * B + k] = (DT)hv[k];
-  }
+  // column k of the eliminated matrix belongs to column p[k] of the inverse
+  DT* dst = dinv + i * B * B;
+#pragma unroll
+  for (int a = 0; a < B; ++a)
+#pragma unroll
+    for (int k = 0; k < B; ++k) dst[a * B + p[k]] = (DT)M_(a, k);
 #undef M_
 }
 

@@ -236,6 +236,10 @@ struct gh_mg {
   // buffers holding caller vectors in the internal numbering
   bool ordered = false;
   const gh_csr* user_fine = nullptr;
+  // the caller's finest-level Dirichlet mask and hanging-node handles (the
+  // only ones gh_gmres_solve / gh_newton_solve accept besides NULL)
+  const gh_mask* user_mask = nullptr;
+  const gh_hn* user_hn = nullptr;
   double* ext_x = nullptr;
   double* ext_y = nullptr;
   // captured V-cycle (fast path from zero initial guess)
@@ -745,6 +749,8 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
   mg->L.resize(nlevels);
   mg->ordered = perm != nullptr;
   mg->user_fine = A[nlevels - 1];
+  mg->user_mask = M ? M[nlevels - 1] : nullptr;
+  mg->user_hn = H ? H[nlevels - 1] : nullptr;
   ctx_retain(ctx);  // balanced by gh_mg_destroy
   int rc = GH_OK;
   for (int l = 0; l < nlevels && !rc; ++l) {
@@ -1350,7 +1356,7 @@ int newton_residual(gh_ctx* ctx, const gh_csr* A, const gh_mask* mask, double* y
 
 extern "C" {
 
-int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mask*/,
+int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
                    int coarselevel, int vmax, gh_vec* x, const gh_vec* b, gh_cginfo* info,
                    int* reached) {
   GH_ARG(ctx && mg && A && x && b && info && vmax >= 2 && vmax <= 256,
@@ -1361,6 +1367,8 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mas
   GH_ARG(x->prec == GH_F64 && b->prec == GH_F64 && x->n == F.n && b->n == F.n,
          "gh_gmres_solve: x and b must be FP64 finest-level vectors");
   GH_ARG(A == mg->user_fine, "gh_gmres_solve: A must be the finest-level matrix of the solver");
+  GH_ARG(!mask || mask == mg->user_mask,
+         "gh_gmres_solve: mask must be NULL or the finest-level mask the solver was created with");
   int rdum = 0, steps = 0;
   double* xd = (double*)x->d;
   const double* bd = (const double*)b->d;
@@ -1378,8 +1386,8 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mas
   return GH_OK;
 }
 
-int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*mask*/,
-                    const gh_hn* /*hn*/, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
+int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* user_mask,
+                    const gh_hn* user_hn, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,
                     gh_cginfo* linfo, gh_nlinfo* nl, int norm_kind, gh_solve_report* rep) {
   GH_ARG(ctx && mg && A && u && f && linfo && nl && vmax >= 2 && vmax <= 256,
          "gh_newton_solve: bad arguments (2 <= vmax <= 256)");
@@ -1389,6 +1397,10 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
   GH_ARG(u->prec == GH_F64 && f->prec == GH_F64 && u->n == F.n && f->n == F.n,
          "gh_newton_solve: u and f must be FP64 finest-level vectors");
   GH_ARG(A == mg->user_fine, "gh_newton_solve: A must be the finest-level matrix of the solver");
+  GH_ARG(!user_mask || user_mask == mg->user_mask,
+         "gh_newton_solve: mask must be NULL or the finest-level mask the solver was created with");
+  GH_ARG(!user_hn || user_hn == mg->user_hn,
+         "gh_newton_solve: hn must be NULL or the finest-level hanging nodes the solver was created with");
   const int64_t n = F.size;
   GH_TRY(ensure_gmres_memory(mg, vmax, mg->P.krylov == 1));
   double* r = mg->r;
@@ -1405,7 +1417,7 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
   const gh_mask* mask = F.M;
   const gh_hn* hn = F.H;
   const int64_t mg0 = mg->mgsteps;
-  int gm = 0;
+  int gm = 0, ndamp = 0, nexpl = 0;
   if (hn) GH_TRY(hn_apply(ctx, hn, u->ncomp, GH_F64, ud, 0));  // Solve: HNAverage(u)
   gh_nlinfo_reset(nl);
   gh_cginfo_reset(linfo);
@@ -1439,11 +1451,13 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
         const int msg = nl_check_damping(nl, iter, rr);
         if (msg == 0) break;
         if (msg == 1) {
+          ndamp++;
           GH_TRY(vec_axpby(ctx, GH_F64, n, ud, 1.0, relax * (omega - 1.), w));
           GH_TRY(newton_residual(ctx, A, mask, r, ud, fd, n, norm_kind, &rr));
           relax *= omega;
           continue;
         }
+        nexpl = 1;
         GH_TRY(vec_axpby(ctx, GH_F64, n, ud, 1.0, -relax, w));
         relax = 0.;
         nl->status = GH_NL_DIVERGED;
@@ -1461,6 +1475,8 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* /*ma
     rep->gmres_iterations = gm;
     rep->mgsteps = mg->mgsteps - mg0;
     rep->status = nl->status;
+    rep->damping_steps = ndamp;
+    rep->damping_exploded = nexpl;
     rep->last_residual = rr;
   }
   return GH_OK;

@@ -56,7 +56,8 @@ class GhMgParams(C.Structure):
 class GhSolveReport(C.Structure):
     _fields_ = [("newton_iterations", C.c_int), ("gmres_iterations", C.c_int),
                 ("mgsteps", C.c_int64), ("status", C.c_int),
-                ("first_residual", C.c_double), ("last_residual", C.c_double)]
+                ("first_residual", C.c_double), ("last_residual", C.c_double),
+                ("damping_steps", C.c_int), ("damping_exploded", C.c_int)]
 
 
 CG_STATUS = {0: "waiting", 1: "running", 2: "converged", 3: "too many iterations",

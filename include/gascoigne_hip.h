@@ -325,8 +325,8 @@ int gh_mg_precondition(gh_ctx* ctx, gh_mg* mg, int coarselevel, gh_vec* x,
  * gh_mg_precondition, FP64 Arnoldi (modified Gram-Schmidt with the reference's
  * re-orthogonalisation test), Givens rotations and back-substitution.
  * A must be the finest-level matrix the solver was created with; the solve
- * uses that level's Dirichlet mask and hanging nodes given at creation (mask
- * is accepted for the reference's call shape and not read).
+ * uses that level's Dirichlet mask and hanging nodes given at creation; mask
+ * must be NULL or that same handle (anything else is GH_EINVAL).
  * Returns the solve's `reached` flag in *reached. */
 int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
                    int coarselevel, int vmax, gh_vec* x, const gh_vec* b, gh_cginfo* info,
@@ -336,14 +336,19 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
  * (stdmultilevelsolver.cc:548-570, 646-673, 693-758, 832-856): FP64 iterative
  * refinement whose correction is one GMRES(vmax) cycle.  The element residual
  * `Form` is the assembled matrix residual f - A u with Dirichlet rows zeroed.
- * A must be the solver's finest-level matrix; mask and hn are taken from the
- * solver's finest level (the arguments are accepted and not read).
+ * A must be the solver's finest-level matrix; mask and hn are the solver's
+ * finest-level handles given at creation: each argument must be NULL or that
+ * same handle (anything else is GH_EINVAL).
  * norm_kind 0: NewtonNorm is the max norm (reference); 1: Euclidean norm. */
 typedef struct {
   int newton_iterations, gmres_iterations;
   int64_t mgsteps;
   int status;
   double first_residual, last_residual;
+  /* NewtonUpdate damping (stdmultilevelsolver.cc:735-752): "continue" answers
+   * of NLInfo::check_damping over all steps, and 1 when one answered
+   * "exploded" (the step was taken back and the status set to diverged) */
+  int damping_steps, damping_exploded;
 } gh_solve_report;
 int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
                     const gh_hn* hn, int coarselevel, int vmax, gh_vec* u, const gh_vec* f,

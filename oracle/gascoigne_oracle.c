@@ -910,7 +910,7 @@ static void mgstep(or_mg* M, double* res, double* rw, int l, int finelevel, int 
     smooth(M, l, M->iterpre, u, b, v);
     or_residual(A, v, u, b);
     or_restrict_zero(M->L[l].T, A->ncomp, Bv[l - 1], v);
-    hn_distribute(M, l - 1, Bv[l - 1]);
+    if (!(M->plant & OR_PLANT_NO_HN_DISTRIBUTE)) hn_distribute(M, l - 1, Bv[l - 1]);
     bnd_zero(M, l - 1, Bv[l - 1]);
     or_zero(U[l - 1], lsize(M, l - 1));
     int j = 0;
@@ -922,10 +922,10 @@ static void mgstep(or_mg* M, double* res, double* rw, int l, int finelevel, int 
     if (l == 0 && p == 'F') *p0 = 'W';
     rw[l] = or_norm(U[l - 1], lsize(M, l - 1));
     or_zero(v, lsize(M, l));
-    hn_average(M, l - 1, U[l - 1]);
+    if (!(M->plant & OR_PLANT_NO_HN_AVERAGE)) hn_average(M, l - 1, U[l - 1]);
     or_prolongate_add(M->L[l].T, A->ncomp, v, U[l - 1]);
     hn_zero(M, l - 1, U[l - 1]);
-    hn_zero(M, l, v);
+    if (!(M->plant & OR_PLANT_NO_HN_ZERO_FINE)) hn_zero(M, l, v);
     bnd_zero(M, l, v);
     or_add(u, M->mgomega, v, lsize(M, l));
     smooth(M, l, M->iterpost, u, b, v);
@@ -1097,7 +1097,7 @@ int or_newton_solve(or_mg* M, int coarselevel, int vmax, double* u, const double
   double* r = (double*)calloc((size_t)n, sizeof(double));
   double* w = (double*)calloc((size_t)n, sizeof(double));
   int64_t mg0 = M->mgsteps;
-  int gm = 0, it;
+  int gm = 0, it, ndamp = 0, nexpl = 0;
   hn_average(M, fl, u);
   or_nlinfo_reset(nl);
   or_cginfo_reset(linfo);
@@ -1128,11 +1128,13 @@ int or_newton_solve(or_mg* M, int coarselevel, int vmax, double* u, const double
         int msg = or_nlinfo_check_damping(nl, iter, rr);
         if (msg == 0) break;
         if (msg == 1) {
+          ndamp++;
           or_add(u, relax * (omega - 1.), w, n);
           rr = newton_residual(M, r, u, f, norm_kind);
           relax *= omega;
           continue;
         }
+        nexpl = 1;
         or_add(u, -relax, w, n);
         relax = 0.;
         nl->status = OR_NL_DIVERGED;
@@ -1149,6 +1151,8 @@ int or_newton_solve(or_mg* M, int coarselevel, int vmax, double* u, const double
     rep->mgsteps = M->mgsteps - mg0;
     rep->status = nl->status;
     rep->last_residual = rr;
+    rep->damping_steps = ndamp;
+    rep->damping_exploded = nexpl;
   }
   free(r);
   free(w);

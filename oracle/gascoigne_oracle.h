@@ -162,7 +162,16 @@ typedef struct {
   double prec_tol, prec_globaltol;
   int64_t mgsteps;  /* counter ("--> mgstep") */
   double **u, **b, **v; /* per level work vectors */
+  /* planted mistakes, for tests that show a check fails on them (0 in every
+   * other use): OR_PLANT_* bits drop or reorder one step of mgstep */
+  int plant;
 } or_mg;
+
+enum {
+  OR_PLANT_NO_HN_DISTRIBUTE = 1, /* restriction without HNDistribute */
+  OR_PLANT_NO_HN_AVERAGE = 2,    /* prolongation without HNAverage */
+  OR_PLANT_NO_HN_ZERO_FINE = 4   /* prolonged correction without HNZero(v) */
+};
 
 or_mg* or_mg_create(int nlevels);
 void or_mg_free(or_mg* M);
@@ -182,6 +191,8 @@ typedef struct {
   int64_t mgsteps;
   int status;
   double first_residual, last_residual;
+  int damping_steps;    /* "continue" answers of check_damping, all steps */
+  int damping_exploded; /* 1 when check_damping answered "exploded" */
 } or_solve_report;
 
 int or_newton_solve(or_mg* M, int coarselevel, int vmax, double* u, const double* f,

@@ -65,14 +65,19 @@ class OrMG(C.Structure):
                 ("omega", C.c_double), ("mgomega", C.c_double), ("smoother", C.c_int),
                 ("prec_maxiter", C.c_int), ("prec_tol", C.c_double),
                 ("prec_globaltol", C.c_double), ("mgsteps", C.c_int64),
-                ("u", C.POINTER(f64p)), ("b", C.POINTER(f64p)), ("v", C.POINTER(f64p))]
+                ("u", C.POINTER(f64p)), ("b", C.POINTER(f64p)), ("v", C.POINTER(f64p)),
+                ("plant", C.c_int)]
 
 
 class OrSolveReport(C.Structure):
     _fields_ = [("newton_iterations", C.c_int), ("gmres_iterations", C.c_int),
                 ("mgsteps", C.c_int64), ("status", C.c_int),
-                ("first_residual", C.c_double), ("last_residual", C.c_double)]
+                ("first_residual", C.c_double), ("last_residual", C.c_double),
+                ("damping_steps", C.c_int), ("damping_exploded", C.c_int)]
 
+
+# planted-mistake switches of or_mg.plant (gascoigne_oracle.h)
+PLANT_NO_HN_DISTRIBUTE, PLANT_NO_HN_AVERAGE, PLANT_NO_HN_ZERO_FINE = 1, 2, 4
 
 CG_STATUS = {0: "waiting", 1: "running", 2: "converged", 3: "too many iterations",
              4: "exploded", 5: "diverging"}

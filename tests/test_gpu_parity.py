@@ -270,10 +270,12 @@ def test_linear_mg_iterations_vs_oracle(ctx, name):
     ig = H.CGInfo(1e-6, 1e-30, 50)
     U = mg.new_vector()
     mg.LinearMg(U, vec(ctx, f), ig)
-    assert abs(ig.iteration - io.iteration) <= 1
+    # the V-cycles are bit-exact and the stopping test is a ratio of norms
+    # (sums in another order, a few ulps apart): the counts are equal unless a
+    # ratio falls within rounding of the tolerance, which these cases do not
+    assert ig.iteration == io.iteration
     assert ig.status == O.CG_STATUS[io.status]
-    if ig.iteration == io.iteration:
-        assert relerr(U.copy_back(), uo) <= 1e-12
+    assert relerr(U.copy_back(), uo) <= 1e-12
 
 
 @pytest.mark.parametrize("name", FIXTURES)
@@ -288,10 +290,9 @@ def test_gmres_vs_oracle(ctx, name):
     ig = H.CGInfo(1e-8, 1e-30, 100)
     X = mg.new_vector()
     mg.Gmres(X, vec(ctx, f), ig, vmax=10)
-    assert abs(ig.iteration - io.iteration) <= 1
-    if ig.iteration == io.iteration:
-        assert relerr(X.copy_back(), xo) <= 1e-12
-        assert ig.residual == pytest.approx(io.residual, rel=1e-9)
+    assert ig.iteration == io.iteration
+    assert relerr(X.copy_back(), xo) <= 1e-12
+    assert ig.residual == pytest.approx(io.residual, rel=1e-9)
 
 
 @pytest.mark.parametrize("name", FIXTURES)
