@@ -363,6 +363,8 @@ int csr_jacobi_sweep(gh_ctx* ctx, const gh_csr* A, int prec, void* xnew, const v
     return jacobi_impl<double, double>(ctx, A, (double*)xnew, (const double*)x,
                                        (const double*)b, omega, smoother, x_zero);
   }
+  // the inverse diagonal is kept in FP64 for FP64 matrices, in FP32 otherwise
+  if (A->prec == GH_F64) return fail(GH_EINVAL, "FP32 sweep needs an FP32 or FP16 matrix");
   return jacobi_impl<float, float>(ctx, A, (float*)xnew, (const float*)x, (const float*)b,
                                    (float)omega, smoother, x_zero);
 }
