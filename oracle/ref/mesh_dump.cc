@@ -26,6 +26,18 @@
 
 using namespace Gascoigne;
 
+// distance of node i from (cx, cy, cz); 2D meshes ignore cz (the
+// AdaptiveElasticity2d examples refine the nodes inside a circle)
+static double node_distance(const GascoigneMesh* M, int i, double cx, double cy, double cz) {
+  if (M->dimension() == 2) {
+    const Vertex2d& x = M->vertex2d(i);
+    return std::sqrt((x.x() - cx) * (x.x() - cx) + (x.y() - cy) * (x.y() - cy));
+  }
+  const Vertex3d& x = M->vertex3d(i);
+  return std::sqrt((x.x() - cx) * (x.x() - cx) + (x.y() - cy) * (x.y() - cy) +
+                   (x.z() - cz) * (x.z() - cz));
+}
+
 static void dump_level(const MeshAgent& MA, int l, const std::string& out) {
   const GascoigneMesh* M = MA.GetMesh(l);
   const int dim = M->dimension();
@@ -132,12 +144,8 @@ int main(int argc, char** argv) {
     for (int s = 0; s < steps; ++s) {
       const GascoigneMesh* M = MA.GetMesh(0);
       IntVector ref;
-      for (int i = 0; i < M->nnodes(); ++i) {
-        const Vertex3d& x = M->vertex3d(i);
-        double d = std::sqrt((x.x() - cx) * (x.x() - cx) + (x.y() - cy) * (x.y() - cy) +
-                             (x.z() - cz) * (x.z() - cz));
-        if (d < r) ref.push_back(i);
-      }
+      for (int i = 0; i < M->nnodes(); ++i)
+        if (node_distance(M, i, cx, cy, cz) < r) ref.push_back(i);
       MA.refine_nodes(ref);
     }
   }

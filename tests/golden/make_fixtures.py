@@ -33,6 +33,10 @@ FIXTURES = {
     "q2_3d_8_ada": (3, "cube.inp", 3, 1, "Q23d", "1", (0.08, 0.41, 0.53, 0.47, 2)),
     # 3D Q1 on the 3x3x3-hex cube, 12^3 cells
     "q1_3d_cube3_12": (3, "cube3.inp", 2, 1, "Q13d", "1", None),
+    # 2D Q1 and Q2 (patches) with two adaptive circle refinements: hanging
+    # edge nodes with 2 parents (HNStructureQ12d) and 3 parents (HNStructureQ22d)
+    "q1_2d_16_ada": (2, "square.inp", 4, 1, "Q12d", "1", (0.2, 0.41, 0.53, 0.0, 2)),
+    "q2_2d_16_ada": (2, "square.inp", 4, 1, "Q22d", "1", (0.2, 0.41, 0.53, 0.0, 2)),
 }
 
 

@@ -22,7 +22,8 @@ from gascoignelib_amd.problem import Hierarchy, splitmix64_uniform
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada", "q1_3d_cube3_12"]
+FIXTURES = ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada", "q1_3d_cube3_12",
+            "q1_2d_16_ada", "q2_2d_16_ada"]
 EPS = np.finfo(np.float64).eps
 
 
@@ -99,7 +100,8 @@ def test_transfer_bitexact_vs_reference(ctx, golden, name):
         assert np.array_equal(UF.copy_back(), z[p + "prolongate"])
 
 
-@pytest.mark.parametrize("name,kind", [("q1_3d_8_ada", 13), ("q2_3d_8_ada", 23)])
+@pytest.mark.parametrize("name,kind", [("q1_3d_8_ada", 13), ("q2_3d_8_ada", 23),
+                                       ("q1_2d_16_ada", 12), ("q2_2d_16_ada", 22)])
 def test_hanging_nodes_bitexact_vs_reference(ctx, golden, name, kind):
     z = golden(name)
     for l in range(int(z["nlevels"])):

@@ -18,7 +18,8 @@ import scipy.sparse.linalg as spla
 import oracle_ctypes as O
 from gascoignelib_amd.problem import Hierarchy, splitmix64_uniform
 
-FIXTURES = ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada", "q1_3d_cube3_12"]
+FIXTURES = ["q1_2d_128", "q1_3d_16", "q1_3d_8_ada", "q2_3d_8_ada", "q1_3d_cube3_12",
+            "q1_2d_16_ada", "q2_2d_16_ada"]
 
 
 def levels(z):
@@ -78,7 +79,8 @@ def test_transfer_bitexact(golden, name):
         assert np.allclose(P @ np.ones(nc), 1.0, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("name,kind", [("q1_3d_8_ada", 13), ("q2_3d_8_ada", 23)])
+@pytest.mark.parametrize("name,kind", [("q1_3d_8_ada", 13), ("q2_3d_8_ada", 23),
+                                       ("q1_2d_16_ada", 12), ("q2_2d_16_ada", 22)])
 def test_hanging_nodes_bitexact(golden, name, kind):
     z = golden(name)
     seen = 0
@@ -95,7 +97,7 @@ def test_hanging_nodes_bitexact(golden, name, kind):
     assert seen >= 2
 
 
-@pytest.mark.parametrize("name", ["q1_3d_8_ada", "q2_3d_8_ada"])
+@pytest.mark.parametrize("name", ["q1_3d_8_ada", "q2_3d_8_ada", "q1_2d_16_ada", "q2_2d_16_ada"])
 def test_hanging_parents_are_regular(golden, name):
     """The gather-form device kernels rely on it: no parent of a hanging node
     hangs itself, and every hanging node has a single entry."""

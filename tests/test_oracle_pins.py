@@ -62,11 +62,14 @@ def vcycle_checks(h, plant=0, seed=1):
     return abs(a - b) / abs(a), float(np.abs(Mx[fixed]).max())
 
 
-@pytest.mark.parametrize("name", ["q1_3d_8_ada", "q2_3d_8_ada"])
+ADAPTIVE = ["q1_3d_8_ada", "q2_3d_8_ada", "q1_2d_16_ada", "q2_2d_16_ada"]
+
+
+@pytest.mark.parametrize("name", ADAPTIVE)
 def test_vcycle_with_hanging_nodes_is_symmetric(name):
     h = fixture(name)
     lev = h.fine
-    assert len(lev.hn_edges) and len(lev.hn_faces)
+    assert len(lev.hn_edges) and (len(lev.hn_faces) or h.dim == 2)
     A = sp.csr_matrix((lev.val, lev.col, lev.rowptr))
     assert abs(A - A.T).max() <= 1e-15 * abs(A).max()
     asym, fixed_max = vcycle_checks(h)
@@ -74,7 +77,7 @@ def test_vcycle_with_hanging_nodes_is_symmetric(name):
     assert fixed_max == 0.0
 
 
-@pytest.mark.parametrize("name", ["q1_3d_8_ada", "q2_3d_8_ada"])
+@pytest.mark.parametrize("name", ADAPTIVE)
 @pytest.mark.parametrize("plant", [O.PLANT_NO_HN_DISTRIBUTE, O.PLANT_NO_HN_AVERAGE,
                                    O.PLANT_NO_HN_ZERO_FINE])
 def test_vcycle_checks_fail_on_planted_hn_mistakes(name, plant):
