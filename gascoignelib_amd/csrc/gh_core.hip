@@ -12,6 +12,16 @@
 
 #include "gh_sell.h"
 
+// Scratch of gh_jacobi_noalloc: 8 MiB per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (in host code the name
+// stands for a host-side shadow, not a device address): gh_ctx_create looks up
+// each device's address with hipGetSymbolAddress, and only that address is used.
+static constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kMaxDevices] = {};
+
 namespace gh {
 
 static thread_local std::string g_err;
@@ -1044,6 +1054,17 @@ int gh_ctx_create(int device, gh_ctx** out) {
   }
   GH_HIP(hipMemsetAsync(c->scal, 0, sizeof(double) * kScalars, c->stream));
   GH_HIP(hipStreamSynchronize(c->stream));
+  {
+    // `device` is current here: look up its copy of the no-allocation
+    // scratch once, and keep that address for every later context on it
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (device < kMaxDevices && !g_noalloc_ptr[device]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_noalloc_ptr[device] = p;
+      else (void)hipGetLastError();  // gh_jacobi_noalloc then reports the failure
+    }
+    if (device < kMaxDevices) c->noalloc_scratch = g_noalloc_ptr[device];
+  }
   *out = c;
   return GH_OK;
 }
@@ -1464,6 +1485,23 @@ int gh_jacobi(gh_ctx* ctx, const gh_csr* A, gh_vec* h) {
   (void)hipStreamSynchronize(ctx->stream);
   vec_free(&t);
   return rc;
+}
+
+int gh_jacobi_noalloc(gh_ctx* ctx, const gh_csr* A, gh_vec* h) {
+  GH_ARG(ctx && A && vec_matches(A, h), "gh_jacobi_noalloc: shape mismatch");
+  GH_ARG(!A->host_only,
+         "gh_jacobi_noalloc: the matrix has no device layout (created by gh_csr_create_host)");
+  const size_t bytes = (size_t)h->size * prec_bytes(h->prec);
+  if (bytes > kNoAllocScratchBytes)
+    return fail(GH_EINVAL,
+                "gh_jacobi_noalloc: h holds %zu bytes, more than its %zu-byte scratch (use gh_jacobi)",
+                bytes, kNoAllocScratchBytes);
+  if (!ctx->noalloc_scratch)
+    return fail(GH_EHIP, "gh_jacobi_noalloc: no scratch address for device %d", ctx->device);
+  // the same two launches as gh_jacobi, with the scratch as the temporary: the
+  // sweep from zero writes every entry of it, so nothing needs clearing
+  GH_TRY(csr_jacobi_sweep(ctx, A, h->prec, ctx->noalloc_scratch, nullptr, h->d, 1.0, 0, true));
+  return vec_convert(ctx, h->size, h->prec, h->d, h->prec, ctx->noalloc_scratch);
 }
 
 // ---------------------------------------------------------------- transfer

@@ -156,6 +156,9 @@ struct gh_ctx {
   double* scal = nullptr;          // device result slots
   double* host_scal = nullptr;     // pinned mirror of scal
   int num_cu = 256;
+  // device address of this device's gh_jacobi_noalloc scratch (nullptr when
+  // the lookup failed at creation)
+  void* noalloc_scratch = nullptr;
 };
 
 struct gh_vec {

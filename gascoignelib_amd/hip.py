@@ -195,6 +195,10 @@ class CudaCSRMatrixInterface:
     def jacobi(self, h):
         check(N.hip().gh_jacobi(self.ctx.h, self.h, h.h))
 
+    def jacobi_noalloc(self, h):
+        """h = D^-1 h through the fixed per-GPU scratch (no device allocation)"""
+        check(N.hip().gh_jacobi_noalloc(self.ctx.h, self.h, h.h))
+
     def free(self):
         if getattr(self, "_owner", None) is not None:  # a view: the owner frees
             self.h = C.c_void_p()

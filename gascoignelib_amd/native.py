@@ -102,6 +102,7 @@ HIP_SIGNATURES = {
     "gh_residual": (C.c_int, [vp, vp, vp, vp, vp]),
     "gh_smooth": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, C.c_double, C.c_int]),
     "gh_jacobi": (C.c_int, [vp, vp, vp]),
+    "gh_jacobi_noalloc": (C.c_int, [vp, vp, vp]),
     "gh_transfer_create": (C.c_int, [vp, C.c_int64, i32p, C.c_int64, i32p, C.c_int64, i32p,
                                      C.c_int64, i32p, P(vp)]),
     "gh_transfer_destroy": (C.c_int, [vp]),

@@ -147,8 +147,18 @@ int gh_residual(gh_ctx* ctx, const gh_csr* A, gh_vec* r, const gh_vec* x, const 
  * h is scratch of x's shape; on return it holds no defined value. */
 int gh_smooth(gh_ctx* ctx, const gh_csr* A, gh_vec* x, const gh_vec* b, gh_vec* h,
               int niter, double omega, int smoother);
-/* h = D^{-1} h: SparseBlockMatrix::Jacobi (src/LinAlg/sparseblockmatrix.xx:47-56) */
+/* h = D^{-1} h: SparseBlockMatrix::Jacobi (src/LinAlg/sparseblockmatrix.xx:47-56).
+ * Allocates (and frees) a temporary of h's size on every call. */
 int gh_jacobi(gh_ctx* ctx, const gh_csr* A, gh_vec* h);
+/* gh_jacobi without any allocation; no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The temporary is a
+ * fixed 8 MiB device array per GPU, held until the process ends: h may hold at
+ * most 8 MiB (1,048,576 FP64 or 2,097,152 FP32 entries), larger h is GH_EINVAL.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (several contexts or host threads) race; calls on one context's stream are
+ * ordered.  Nothing synchronises; the result equals gh_jacobi's bit for bit. */
+int gh_jacobi_noalloc(gh_ctx* ctx, const gh_csr* A, gh_vec* h);
 
 /* ----------------------------------------------------------------- Vanka */
 /* VankaSmoother (src/LinAlg/vankasmoother.h:65-125, .cc:7-237): patches of
