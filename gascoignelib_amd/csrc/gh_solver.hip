@@ -182,6 +182,11 @@ struct MgLevel {
   gh_csr* A64 = nullptr;   // FP64 matrix (the caller's, or an owned renumbered copy)
   gh_csr* A = nullptr;     // matrix of the V-cycle (A64 on the FP64 path)
   bool own_A = false;
+  // finest level of a renumbered flexible-GMRES solver: FP64 copy of A64 whose
+  // rows take their entries in ascending internal column order (shared
+  // relative patterns), for the outer products and residuals, which reproduce
+  // no reference sums
+  gh_csr* A64s = nullptr;
   bool own_A64 = false, own_T = false, own_H = false, own_M = false;
   int32_t* d_perm = nullptr;  // internal position -> caller's node (nullptr: identity)
   gh_transfer* T = nullptr;
@@ -597,7 +602,7 @@ void parallel_rows(int64_t n, F&& fn) {
 
 int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* const* A,
                         gh_transfer* const* T, gh_hn* const* H, gh_mask* const* M,
-                        const int32_t* const* perm, MgLevel& Lv) {
+                        const int32_t* const* perm, bool sorted64, MgLevel& Lv) {
   const gh_csr* Au = A[l];
   const int64_t n = Au->n;
   const int B = Au->ncomp, BB = B * B;
@@ -637,11 +642,11 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
   });
   GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cl.data(), vl.data(), GH_F64, false, &Lv.A64));
   Lv.own_A64 = true;
-  if (params->prec != GH_F64) {
+  if (params->prec != GH_F64 || sorted64) {
     // The reduced-precision V-cycle reproduces no reference bits, so its rows
     // take their entries in ascending internal column order: rows of one kind
     // then share one relative pattern, whatever order the caller's numbering
-    // gave them.
+    // gave them.  The same holds for the outer FP64 products of flexible GMRES.
     std::vector<int32_t> cs(cl.size());
     std::vector<double> vs(vl.size());
     parallel_rows(n, [&](int64_t a, int64_t b) {
@@ -657,8 +662,14 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
         }
       }
     });
-    GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cs.data(), vs.data(), params->prec, false, &Lv.A));
-    Lv.own_A = true;
+    if (sorted64)
+      GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cs.data(), vs.data(), GH_F64, false, &Lv.A64s));
+    if (params->prec != GH_F64) {
+      GH_TRY(csr_create_impl(ctx, n, B, rp.data(), cs.data(), vs.data(), params->prec, false, &Lv.A));
+      Lv.own_A = true;
+    } else {
+      Lv.A = Lv.A64;
+    }
   } else {
     Lv.A = Lv.A64;
   }
@@ -759,7 +770,8 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
     Lv.ncomp = A[l]->ncomp;
     Lv.size = Lv.n * Lv.ncomp;
     if (perm) {
-      rc = build_ordered_level(ctx, params, l, A, T, H, M, perm, Lv);
+      rc = build_ordered_level(ctx, params, l, A, T, H, M, perm,
+                               params->krylov == 1 && l == nlevels - 1, Lv);
     } else if (params->adjoint) {
       // the transposed level operator, once; everything downstream runs on it
       std::vector<int32_t> rpT, clT;
@@ -890,6 +902,7 @@ int gh_mg_destroy(gh_mg* mg) {
   for (MgLevel& L : mg->L) {
     if (L.own_A && L.A) gh_csr_destroy(L.A);
     if (L.own_A64 && L.A64) gh_csr_destroy(L.A64);
+    if (L.A64s) gh_csr_destroy(L.A64s);
     if (L.own_T && L.T) gh_transfer_destroy(L.T);
     if (L.own_H && L.H) gh_hn_destroy(L.H);
     if (L.own_M && L.M) gh_mask_destroy(L.M);
@@ -1378,7 +1391,7 @@ int gh_gmres_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask,
     xd = mg->ext_x;
     bd = mg->ext_y;
   }
-  GH_TRY(krylov_impl(ctx, mg, F.A64, F.M, coarselevel, vmax, xd, bd, false, info,
+  GH_TRY(krylov_impl(ctx, mg, F.A64s ? F.A64s : F.A64, F.M, coarselevel, vmax, xd, bd, false, info,
                      reached ? reached : &rdum, &steps));
   if (F.d_perm)
     GH_TRY(vec_permute(ctx, F.n, F.ncomp, F.d_perm, GH_F64, x->d, GH_F64, mg->ext_x, false));
@@ -1413,7 +1426,7 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* user
     ud = mg->ext_x;
     fd = mg->ext_y;
   }
-  A = F.A64;
+  A = F.A64s ? F.A64s : F.A64;
   const gh_mask* mask = F.M;
   const gh_hn* hn = F.H;
   const int64_t mg0 = mg->mgsteps;
@@ -1422,7 +1435,21 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* user
   gh_nlinfo_reset(nl);
   gh_cginfo_reset(linfo);
   double rr;
-  GH_TRY(newton_residual(ctx, A, mask, r, ud, fd, n, norm_kind, &rr));
+  {
+    // a zero start (the usual case) needs no product: every row of f - A u is
+    // then f_i (up to the sign of an exactly zero f_i)
+    int rc = 0;
+    const double u8 = vec_norm_kind(ctx, ud, n, 0, &rc);
+    GH_TRY(rc);
+    if (u8 == 0.0) {
+      GH_TRY(vec_convert(ctx, n, GH_F64, r, GH_F64, fd));
+      if (mask) GH_TRY(mask_apply(ctx, mask, GH_F64, r));
+      rr = vec_norm_kind(ctx, r, n, norm_kind, &rc);
+      GH_TRY(rc);
+    } else {
+      GH_TRY(newton_residual(ctx, A, mask, r, ud, fd, n, norm_kind, &rr));
+    }
+  }
   if (rep) rep->first_residual = rr;
   int reached = gh_nlinfo_check(nl, 0, rr, 0.);
   for (int it = 1; !reached; it++) {
@@ -1437,7 +1464,9 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* user
     int rc = 0;
     const double nn = vec_norm_kind(ctx, w, n, norm_kind, &rc);
     GH_TRY(rc);
-    const double nr = vec_norm_kind(ctx, r, n, 1, &rc);
+    // r is unchanged since its norm was taken: with the Euclidean NewtonNorm
+    // that norm is the one needed here
+    const double nr = norm_kind == 1 ? rr : vec_norm_kind(ctx, r, n, 1, &rc);
     GH_TRY(rc);
     if (nn > 1.e30 || !(nn >= 0.) || nr > 1.e30 || !(nr >= 0.)) lex = 1;
     if (lex) {
