@@ -279,6 +279,8 @@ def main():
     t64, tmx, t64f = float(np.median(ts64)), float(np.median(tsmx)), float(np.median(ts64f))
     dofs_mx, dofs_64 = ndof / tmx, ndof / t64
 
+    t_asm = assembly_times(ctx, hier, mg64, mgmx)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hier, f)
@@ -330,10 +332,49 @@ def main():
                      "csr_equivalent_gbs": csr64 / t_spmv / 1e9},
         "spmv_mixed": {"gbs": bmx / t_spmv_mx / 1e9, "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS,
                        "bytes_per_launch": bmx, "us_per_launch": t_spmv_mx * 1e6},
-        "setup_s": {"structure": t_struct, "device_upload": t_setup},
+        "setup_s": dict({"structure": t_struct, "device_upload": t_setup}, **t_asm),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def assembly_times(ctx, hier, mg64, mgmx, reps=3):
+    """Seconds to assemble the Q1 operator (k = 1): the finest level on the
+    host producer (its threads) and on the device into the finest FP64 matrix
+    in the caller's numbering (gh_assemble_q1), and the device re-assembly of
+    every level of the FP64 and of the mixed solver (gh_mg_assemble_q1:
+    operators, reduced-precision and flexible-GMRES copies, inverse diagonals,
+    dense coarse inverses).  Each the median of `reps` after one warm-up; the
+    values are those the solves ran on."""
+    from gascoignelib_amd import hip as H
+    M = hier.mesh
+    fl = M.nlevels - 1
+    nnz = hier.fine.nnz
+    th = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        M.assemble(fl, nnz)
+        th.append(time.perf_counter() - t)
+    cells = [H.CoefficientCells(ctx, hier.dim, M.coords(l), M.cells(l)) for l in range(M.nlevels)]
+    A = mg64.fine_matrix
+    mask = mg64.fine_mask
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    td = timed(lambda: cells[-1].assemble(A, mask=mask))
+    t64 = timed(lambda: mg64.assemble_q1(cells))
+    tmx = timed(lambda: mgmx.assemble_q1(cells))
+    return {"assembly_host_fine": float(np.median(th)), "assembly_device_fine": td,
+            "reassembly_device_fp64_solver": t64, "reassembly_device_mixed_solver": tmx}
 
 
 def measured_traffic(layout):

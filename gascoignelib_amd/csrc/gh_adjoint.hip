@@ -101,6 +101,8 @@ int gh_cells_create(gh_ctx* ctx, int dim, int64_t n, const double* xyz, int64_t 
   GH_ARG(ctx && out && xyz && (cells || ncells == 0) && n >= 0 && ncells >= 0,
          "gh_cells_create: bad arguments");
   GH_ARG(dim == 2 || dim == 3, "gh_cells_create: dim must be 2 or 3");
+  GH_ARG(ncells < (int64_t(1) << 28) && n < (int64_t(1) << 31),
+         "gh_cells_create: too many cells or nodes");
   const int npc = dim == 3 ? 8 : 4;
   static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                    {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
@@ -124,6 +126,15 @@ int gh_cells_create(gh_ctx* ctx, int dim, int64_t n, const double* xyz, int64_t 
     if (h[3 * c] <= 0 || h[3 * c + 1] <= 0 || h[3 * c + 2] <= 0)
       return fail(GH_EINVAL, "gh_cells_create: cell %lld has inverted orientation", (long long)c);
   }
+  // node -> (cell, corner), ascending cell per node (counting sort)
+  std::vector<int32_t> nptr(n + 1, 0), nlist((size_t)ncells * npc);
+  for (int64_t q = 0; q < ncells * npc; ++q) nptr[cells[q] + 1]++;
+  for (int64_t i = 0; i < n; ++i) nptr[i + 1] += nptr[i];
+  {
+    std::vector<int32_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int64_t c = 0; c < ncells; ++c)
+      for (int m = 0; m < npc; ++m) nlist[fill[cells[c * npc + m]]++] = (int32_t)(c * 8 + m);
+  }
   gh_cells* C = new gh_cells();
   C->ctx = ctx;
   ctx_retain(ctx);  // balanced by gh_cells_destroy
@@ -142,6 +153,13 @@ int gh_cells_create(gh_ctx* ctx, int dim, int64_t n, const double* xyz, int64_t 
         hipMemcpy(C->h, h.data(), sizeof(double) * 3 * ncells, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(GH_EHIP, "gh_cells_create: upload failed"));
   }
+  if (hipMalloc(&C->nptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&C->nlist, sizeof(int32_t) * std::max<int64_t>(ncells * npc, 1)) != hipSuccess)
+    return cleanup(fail(GH_ENOMEM, "gh_cells_create: allocation failed"));
+  if (hipMemcpy(C->nptr, nptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      (ncells > 0 && hipMemcpy(C->nlist, nlist.data(), sizeof(int32_t) * ncells * npc,
+                               hipMemcpyHostToDevice) != hipSuccess))
+    return cleanup(fail(GH_EHIP, "gh_cells_create: upload failed"));
   *out = C;
   return GH_OK;
 }
@@ -150,6 +168,8 @@ int gh_cells_destroy(gh_cells* C) {
   if (!C) return GH_OK;
   if (C->cells) (void)hipFree(C->cells);
   if (C->h) (void)hipFree(C->h);
+  if (C->nptr) (void)hipFree(C->nptr);
+  if (C->nlist) (void)hipFree(C->nlist);
   gh_ctx* ctx = C->ctx;
   delete C;
   if (ctx) ctx_release(ctx);

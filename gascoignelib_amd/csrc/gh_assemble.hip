@@ -1,0 +1,239 @@
+// Device assembly of Q1 operators and of the element residual (Form) for
+// -div(k grad u) = f on axis-aligned box cells, one coefficient per cell.
+//
+// Reference: CGDisc::Matrix (src/DofHandler/cgdisc.h:599-635) adds every
+// cell's element matrix (elementintegrator.xx:350-397) into the global matrix
+// in ascending cell order, then SparseBlockMatrix::dirichlet
+// (src/LinAlg/sparseblockmatrix.xx:205-218) makes the Dirichlet rows and
+// columns identity; CudaSolver::AssembleMatrix (cuda/cudasolver.cc:114-128)
+// then uploads the host matrix and inverts its diagonal on every assembly.
+// Here the values are formed on the device, directly in the matrix's own
+// numbering and layout: one thread per row sums, for each of the row's
+// entries, the contributions of the row node's cells in ascending cell order,
+// k_c * K^c_ab with K^c the unit-coefficient Q1 element matrix formed exactly
+// (tensor products of the 1D stiffness s and mass m matrices:
+// K = s m m + m s m + m m s in 3D, s m + m s in 2D, each product and sum
+// rounded separately in that order), which is what the host producer
+// (structure.cc, assemble) computes; the two agree bit for bit, and with the
+// reference's Gauss-rule values to rounding.
+//
+// CGDisc::Form (cgdisc.h:415-448) is the residual counterpart: y += d * sum
+// over cells of the element residual.  gh_form_q1 forms r = f - A(u) cell by
+// cell from u without any matrix.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gh_kernels.h"
+
+using namespace gh;
+
+namespace {
+
+// Corner a of a hex at (cx, cy, cz): 0 (0,0,0), 1 (1,0,0), 2 (1,1,0), 3 (0,1,0),
+// then the same at z = 1 (Gascoigne's order); quads use corners 0..3.
+__device__ __forceinline__ int cx(int a) { return (a & 1) ^ ((a >> 1) & 1); }
+__device__ __forceinline__ int cy(int a) { return (a >> 1) & 1; }
+__device__ __forceinline__ int cz(int a) { return (a >> 2) & 1; }
+
+// 1D Q1 stiffness and mass on an interval of length h
+__device__ __forceinline__ double s1(int p, int q, double h) { return (p == q ? 1.0 : -1.0) / h; }
+__device__ __forceinline__ double m1(int p, int q, double h) {
+  return mul_rn(h, p == q ? 2.0 : 1.0) / 6.0;
+}
+
+template <int DIM>
+__device__ __forceinline__ double q1_entry(int a, int b, double hx, double hy, double hz) {
+  if constexpr (DIM == 3) {
+    const double t1 = mul_rn(mul_rn(s1(cx(a), cx(b), hx), m1(cy(a), cy(b), hy)), m1(cz(a), cz(b), hz));
+    const double t2 = mul_rn(mul_rn(m1(cx(a), cx(b), hx), s1(cy(a), cy(b), hy)), m1(cz(a), cz(b), hz));
+    const double t3 = mul_rn(mul_rn(m1(cx(a), cx(b), hx), m1(cy(a), cy(b), hy)), s1(cz(a), cz(b), hz));
+    return add_rn(add_rn(t1, t2), t3);
+  } else {
+    (void)hz;
+    return add_rn(mul_rn(s1(cx(a), cx(b), hx), m1(cy(a), cy(b), hy)),
+                  mul_rn(m1(cx(a), cx(b), hx), s1(cy(a), cy(b), hy)));
+  }
+}
+
+// One thread per row i of A (A's numbering; the cells' node of row i is
+// perm[i], or i).  Entry (i, j): sum over the cells c of node perm[i], in
+// ascending order, of k_c K^c_ab where a, b are the local corners of the two
+// nodes in c.  Dirichlet rows are unit rows, Dirichlet columns zero.  Each
+// (cell, corner) pair of the row must meet exactly one entry of the pattern,
+// else flags[2] is set.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_assemble_q1(
+    int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ nptr,
+    const int32_t* __restrict__ nlist, const int32_t* __restrict__ cells,
+    const double* __restrict__ h, const double* __restrict__ kc,
+    const uint8_t* __restrict__ isdir, double* __restrict__ val, int* flags) {
+  constexpr int NPC = DIM == 3 ? 8 : 4;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t ic = perm ? perm[i] : (int32_t)i;
+  const int32_t c0 = nptr[ic], c1 = nptr[ic + 1];
+  const bool drow = isdir && isdir[i];
+  int matched = 0;
+  for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+    const int32_t j = col[q];
+    const int32_t jc = perm ? perm[j] : j;
+    double acc = 0.0;
+    for (int32_t e = c0; e < c1; ++e) {
+      const int32_t ce = nlist[e];
+      const int64_t c = ce >> 3;
+      const int a = ce & 7;
+      const int32_t* v = cells + c * NPC;
+#pragma unroll
+      for (int b = 0; b < NPC; ++b) {
+        if (v[b] == jc) {
+          const double k = kc ? kc[c] : 1.0;
+          acc = add_rn(acc, mul_rn(k, q1_entry<DIM>(a, b, h[3 * c], h[3 * c + 1], h[3 * c + 2])));
+          ++matched;
+        }
+      }
+    }
+    if (drow)
+      acc = (j == i) ? 1.0 : 0.0;
+    else if (isdir && isdir[j])
+      acc = 0.0;
+    val[q] = acc;
+  }
+  if (matched != (c1 - c0) * NPC) flags[2] = 1;
+}
+
+// One thread per node: r_i = f_i - sum over the cells c of node i (ascending)
+// of k_c sum_b K^c_ab u(v_b); Dirichlet rows zero.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_form_q1(
+    int64_t n, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const int32_t* __restrict__ cells, const double* __restrict__ h,
+    const double* __restrict__ kc, const uint8_t* __restrict__ isdir,
+    const double* __restrict__ u, const double* __restrict__ f, double* __restrict__ r) {
+  constexpr int NPC = DIM == 3 ? 8 : 4;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (isdir && isdir[i]) {
+    r[i] = 0.0;
+    return;
+  }
+  double acc = f[i];
+  for (int32_t e = nptr[i]; e < nptr[i + 1]; ++e) {
+    const int32_t ce = nlist[e];
+    const int64_t c = ce >> 3;
+    const int a = ce & 7;
+    const int32_t* v = cells + c * NPC;
+    const double hx = h[3 * c], hy = h[3 * c + 1], hz = h[3 * c + 2];
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < NPC; ++b) s = add_rn(s, mul_rn(q1_entry<DIM>(a, b, hx, hy, hz), u[v[b]]));
+    acc = add_rn(acc, -mul_rn(kc ? kc[c] : 1.0, s));
+  }
+  r[i] = acc;
+}
+
+}  // namespace
+
+namespace gh {
+
+int assemble_q1_launch(gh_ctx* ctx, const gh_cells* C, const double* kcell, const uint8_t* isdir,
+                       const int32_t* perm, gh_csr* A, double* val64) {
+  if (!A->d_flags) {
+    if (hipMalloc((void**)&A->d_flags, sizeof(int) * 4) != hipSuccess)
+      return fail(GH_ENOMEM, "device assembly: flags");
+    GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
+  }
+  if (A->n == 0) return GH_OK;
+  const dim3 g(grid_for(A->n));
+  if (C->dim == 3)
+    hipLaunchKernelGGL(k_assemble_q1<3>, g, dim3(kBlock), 0, ctx->stream, A->n, A->rowptr, A->col,
+                       perm, C->nptr, C->nlist, C->cells, C->h, kcell, isdir, val64, A->d_flags);
+  else
+    hipLaunchKernelGGL(k_assemble_q1<2>, g, dim3(kBlock), 0, ctx->stream, A->n, A->rowptr, A->col,
+                       perm, C->nptr, C->nlist, C->cells, C->h, kcell, isdir, val64, A->d_flags);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+}  // namespace gh
+
+extern "C" {
+
+int gh_assemble_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* dirichlet,
+                   gh_csr* A) {
+  GH_ARG(ctx && C && A, "gh_assemble_q1: NULL argument");
+  GH_ARG(!A->host_only && A->ncomp == 1,
+         "gh_assemble_q1: needs a scalar matrix with a device layout");
+  GH_ARG(A->n == C->n, "gh_assemble_q1: the matrix and the cells have different node counts");
+  GH_ARG(!k || (k->prec == GH_F64 && k->size == C->ncells),
+         "gh_assemble_q1: k must be an FP64 vector with one entry per cell");
+  GH_ARG(!dirichlet || (dirichlet->n == A->n && dirichlet->ncomp == 1),
+         "gh_assemble_q1: the Dirichlet mask does not match the matrix");
+  const uint8_t* isdir = nullptr;
+  if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
+  double* v64 = nullptr;
+  GH_TRY(csr_asm_values(ctx, A, &v64));
+  GH_TRY(assemble_q1_launch(ctx, C, k ? (const double*)k->d : nullptr, isdir, nullptr, A, v64));
+  return csr_asm_commit(ctx, A, "gh_assemble_q1");
+}
+
+int gh_form_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* dirichlet,
+               const gh_vec* u, const gh_vec* f, gh_vec* r) {
+  GH_ARG(ctx && C && u && f && r, "gh_form_q1: NULL argument");
+  GH_ARG(u->prec == GH_F64 && f->prec == GH_F64 && r->prec == GH_F64,
+         "gh_form_q1: vectors must be FP64");
+  GH_ARG(u->ncomp == 1 && f->ncomp == 1 && r->ncomp == 1 && u->n == C->n && f->n == C->n &&
+             r->n == C->n,
+         "gh_form_q1: u, f and r must be scalar nodal vectors of the mesh");
+  GH_ARG(r->d != u->d, "gh_form_q1: r and u must not alias");
+  GH_ARG(!k || (k->prec == GH_F64 && k->size == C->ncells),
+         "gh_form_q1: k must be an FP64 vector with one entry per cell");
+  GH_ARG(!dirichlet || (dirichlet->n == C->n && dirichlet->ncomp == 1),
+         "gh_form_q1: the Dirichlet mask does not match the mesh");
+  const uint8_t* isdir = nullptr;
+  if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
+  if (C->n == 0) return GH_OK;
+  const dim3 g(grid_for(C->n));
+  const double* kc = k ? (const double*)k->d : nullptr;
+  if (C->dim == 3)
+    hipLaunchKernelGGL(k_form_q1<3>, g, dim3(kBlock), 0, ctx->stream, C->n, C->nptr, C->nlist,
+                       C->cells, C->h, kc, isdir, (const double*)u->d, (const double*)f->d,
+                       (double*)r->d);
+  else
+    hipLaunchKernelGGL(k_form_q1<2>, g, dim3(kBlock), 0, ctx->stream, C->n, C->nptr, C->nlist,
+                       C->cells, C->h, kc, isdir, (const double*)u->d, (const double*)f->d,
+                       (double*)r->d);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int gh_csr_download_values(gh_ctx* ctx, const gh_csr* A, double* val) {
+  GH_ARG(ctx && A && (val || A->nnz == 0), "gh_csr_download_values: NULL argument");
+  GH_ARG(!A->host_only, "gh_csr_download_values: the matrix has no device layout");
+  const int64_t m = A->nnz * (int64_t)A->ncomp * A->ncomp;
+  if (m == 0) return GH_OK;
+  if (A->prec == GH_F64) {
+    GH_HIP(hipMemcpyAsync(val, A->val, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    return GH_OK;
+  }
+  double* tmp = nullptr;
+  if (hipMalloc((void**)&tmp, sizeof(double) * m) != hipSuccess)
+    return fail(GH_ENOMEM, "gh_csr_download_values: staging buffer");
+  const unsigned g = (unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 8192);
+  if (A->prec == GH_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kBlock), 0, ctx->stream, m, tmp,
+                       (const float*)A->val);
+  else
+    hipLaunchKernelGGL((k_convert<double, __half>), dim3(g), dim3(kBlock), 0, ctx->stream, m, tmp,
+                       (const __half*)A->val);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(val, tmp, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(GH_EHIP, "gh_csr_download_values: %s", hipGetErrorString(e));
+  return GH_OK;
+}
+
+}  // extern "C"

@@ -1250,6 +1250,129 @@ static int launch_block_inverse(gh_ctx* ctx, int64_t n, int B, const int32_t* di
   return GH_OK;
 }
 
+// ------------------------------------------------------- device assembly
+// The SELL value stream of a scalar matrix from its FP64 CSR values: slot t
+// (row perm[t]) stores the entries of its row that the layout kept, in CSR
+// order, at sptr[s] + lane + 64 k; an entry left out at creation (an exact
+// zero then) must still be zero, else flag[1] is set.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_sell_refresh(int64_t n, const uint32_t* __restrict__ sptr,
+                                                         const int32_t* __restrict__ perm,
+                                                         const int32_t* __restrict__ rowptr,
+                                                         const uint32_t* __restrict__ kept,
+                                                         const double* __restrict__ val64,
+                                                         VT* __restrict__ sval, int* flags) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int64_t row = perm ? (int64_t)perm[t] : t;
+  VT* vp = sval + sptr[t >> 6] + (t & (kSlice - 1));
+  int k = 0;
+  for (int32_t q = rowptr[row]; q < rowptr[row + 1]; ++q) {
+    const double v = val64[q];
+    if ((kept[q >> 5] >> (q & 31)) & 1u) {
+      vp[(int64_t)k * kSlice] = (VT)v;
+      ++k;
+    } else if (v != 0.0) {
+      flags[1] = 1;
+    }
+  }
+}
+
+int csr_asm_values(gh_ctx* ctx, gh_csr* A, double** val64) {
+  GH_ARG(A && A->ncomp == 1 && !A->host_only,
+         "device assembly: needs a scalar matrix with a device layout");
+  if (A->prec == GH_F64) {
+    *val64 = (double*)A->val;
+    return GH_OK;
+  }
+  if (!A->d_val64) {
+    if (hipMalloc((void**)&A->d_val64, sizeof(double) * std::max<int64_t>(A->nnz, 1)) != hipSuccess)
+      return fail(GH_ENOMEM, "device assembly: FP64 value buffer");
+  }
+  *val64 = A->d_val64;
+  (void)ctx;
+  return GH_OK;
+}
+
+int csr_asm_commit(gh_ctx* ctx, gh_csr* A, const char* who) {
+  const int64_t n = A->n, nnz = A->nnz;
+  double* v64 = nullptr;
+  GH_TRY(csr_asm_values(ctx, A, &v64));
+  if (!A->d_flags) {
+    if (hipMalloc((void**)&A->d_flags, sizeof(int) * 4) != hipSuccess)
+      return fail(GH_ENOMEM, "%s: flags", who);
+    GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
+  }
+  // host staging of the uploads below, alive until the stream is synchronised
+  std::vector<int32_t> diag;
+  std::vector<uint32_t> kept;
+  if (!A->d_diag || !A->d_kept) {
+    // from the host copy, which still holds the values of creation here
+    if (!A->h_val_ok) return fail(GH_EINVAL, "%s: layout data lost", who);
+    diag.assign(n, -1);
+    for (int64_t i = 0; i < n; ++i)
+      for (int32_t q = A->h_rowptr[i]; q < A->h_rowptr[i + 1]; ++q)
+        if (A->h_col[q] == i && diag[i] < 0) diag[i] = q;
+    kept.assign((nnz + 31) / 32 + 1, 0u);
+    for (int64_t q = 0; q < nnz; ++q)
+      if (A->h_val[q] != 0.0) kept[q >> 5] |= 1u << (q & 31);
+    GH_TRY(upload(ctx, &A->d_diag, diag.data(), n));
+    GH_TRY(upload(ctx, &A->d_kept, kept.data(), (int64_t)kept.size()));
+  }
+  // the assembly has written v64 on the stream; everything below follows it
+  if (A->prec == GH_F64)
+    GH_TRY(launch_block_inverse(ctx, n, 1, A->d_diag, v64, (double*)A->dinv, A->d_flags));
+  else
+    GH_TRY(launch_block_inverse(ctx, n, 1, A->d_diag, v64, (float*)A->dinv, A->d_flags));
+  if (A->prec == GH_F32)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(nnz)), dim3(kBlock), 0,
+                       ctx->stream, nnz, (float*)A->val, (const double*)v64);
+  else if (A->prec == GH_F16)
+    hipLaunchKernelGGL((k_convert<__half, double>), dim3(grid_stride(nnz)), dim3(kBlock), 0,
+                       ctx->stream, nnz, (__half*)A->val, (const double*)v64);
+  GH_HIP(hipGetLastError());
+  const Sell& S = A->sell;
+  const dim3 g(grid_for(n));
+  if (A->prec == GH_F64)
+    hipLaunchKernelGGL(k_sell_refresh<double>, g, dim3(kBlock), 0, ctx->stream, n, S.sptr, S.perm,
+                       A->rowptr, A->d_kept, v64, (double*)S.val, A->d_flags);
+  else if (A->prec == GH_F32)
+    hipLaunchKernelGGL(k_sell_refresh<float>, g, dim3(kBlock), 0, ctx->stream, n, S.sptr, S.perm,
+                       A->rowptr, A->d_kept, v64, (float*)S.val, A->d_flags);
+  else
+    hipLaunchKernelGGL(k_sell_refresh<__half>, g, dim3(kBlock), 0, ctx->stream, n, S.sptr, S.perm,
+                       A->rowptr, A->d_kept, v64, (__half*)S.val, A->d_flags);
+  GH_HIP(hipGetLastError());
+  int flags[4] = {0, 0, 0, 0};
+  GH_HIP(hipMemcpyAsync(flags, A->d_flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  A->h_val_ok = false;
+  if (flags[0] || flags[1] || flags[2]) {
+    GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    GH_HIP(hipStreamSynchronize(ctx->stream));
+    if (flags[2])
+      return fail(GH_EINVAL, "%s: a cell coupling has no entry in the matrix pattern", who);
+    if (flags[1])
+      return fail(GH_EINVAL,
+                  "%s: an entry that was zero at creation (left out of the device layout) "
+                  "became nonzero",
+                  who);
+    return fail(GH_ESINGULAR, "%s: singular diagonal", who);
+  }
+  return GH_OK;
+}
+
+int mask_node_flags(gh_ctx* ctx, gh_mask* M, const uint8_t** out) {
+  if (!M->d_nodeflag) {
+    std::vector<uint8_t> f(std::max<int64_t>(M->n, 1), 0);
+    for (int32_t v : M->h_nodes) f[v] = 1;
+    GH_TRY(upload(ctx, &M->d_nodeflag, f.data(), (int64_t)f.size()));
+    GH_HIP(hipStreamSynchronize(ctx->stream));  // f is host staging
+  }
+  *out = M->d_nodeflag;
+  return GH_OK;
+}
+
 int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
                     const double* val, int prec, bool check_sorted, gh_csr** out) {
   GH_ARG(ctx && out && rowptr && n >= 0 && ncomp >= 1 && ncomp <= 8,
@@ -1394,7 +1517,8 @@ int gh_csr_destroy(gh_csr* A) {
   gh_ctx* ctx = A->ctx;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void* p : {(void*)A->rowptr, (void*)A->col, A->val, A->dinv, (void*)A->t_rowptr,
-                  (void*)A->t_col, (void*)A->t_pos})
+                  (void*)A->t_col, (void*)A->t_pos, (void*)A->d_diag, (void*)A->d_kept,
+                  (void*)A->d_val64, (void*)A->d_flags})
     if (p) (void)hipFree(p);
   sell_free(&A->sell);
   bsell_free(&A->bsell);
@@ -1790,6 +1914,7 @@ int gh_mask_destroy(gh_mask* M) {
   gh_ctx* ctx = M->ctx;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (M->idx) (void)hipFree(M->idx);
+  if (M->d_nodeflag) (void)hipFree(M->d_nodeflag);
   delete M;
   ctx_release(ctx);
   return GH_OK;

@@ -36,6 +36,10 @@ struct gh_cells {
   int64_t ncells = 0;
   int32_t* cells = nullptr;  // ncells * (4 | 8) node indices, local corner order
   double* h = nullptr;       // ncells * 3 box edge lengths (hz = 1 in 2D)
+  // node -> cells: the cells of node i are nlist[nptr[i] .. nptr[i+1]), each
+  // entry cell * 8 + local corner, in ascending cell order
+  int32_t* nptr = nullptr;
+  int32_t* nlist = nullptr;
 };
 
 namespace gh {
@@ -195,6 +199,15 @@ struct gh_csr {
   int32_t* t_rowptr = nullptr;
   int32_t* t_col = nullptr;
   int32_t* t_pos = nullptr;  // position of the transposed block in val
+  // device assembly (scalar matrices): the host copy h_val keeps the values of
+  // creation only until the first device assembly; constructors that read it
+  // refuse the matrix afterwards
+  bool h_val_ok = true;
+  int32_t* d_diag = nullptr;   // position of each row's diagonal entry
+  uint32_t* d_kept = nullptr;  // bit q: CSR entry q is stored in the SELL layout
+  double* d_val64 = nullptr;   // FP64 CSR values of a reduced-precision matrix
+  int* d_flags = nullptr;      // [0] singular diagonal, [1] a left-out entry became nonzero,
+                               // [2] a cell coupling has no entry in the pattern
 };
 
 struct gh_transfer {
@@ -237,6 +250,7 @@ struct gh_mask {
   int ncomps = 0;
   int32_t* idx = nullptr;      // nidx * ncomps flat entries (node*ncomp+comp)
   std::vector<int32_t> h_nodes, h_comps;
+  uint8_t* d_nodeflag = nullptr;  // per node 1 if listed (device assembly; built on first use)
 };
 
 namespace gh {
@@ -277,6 +291,22 @@ int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslo
 int finalize_slots(gh_ctx* ctx, int first, int count, bool max);
 int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
                     const double* val, int prec, bool check_sorted, gh_csr** out);
+// Device assembly of scalar matrices.  csr_asm_values: the FP64 CSR value
+// array an assembly kernel writes (the matrix's own values in FP64, else a
+// buffer allocated on first use).  csr_asm_commit: after that array was
+// written on the stream, derive everything else from it (inverse diagonal,
+// reduced-precision values, the SELL value stream) and report a singular
+// diagonal, a left-out entry that became nonzero or a coupling missing from
+// the pattern (flag [2], set by the assembly kernel); synchronises.
+int csr_asm_values(gh_ctx* ctx, gh_csr* A, double** val64);
+int csr_asm_commit(gh_ctx* ctx, gh_csr* A, const char* who);
+// per-node flags of a Dirichlet mask (built on first use)
+int mask_node_flags(gh_ctx* ctx, gh_mask* M, const uint8_t** out);
+// Q1 assembly of -div(k grad u) on box cells into the FP64 CSR values of A
+// (rows and columns in A's numbering; perm[i] = the cells' node of row i,
+// nullptr: the same numbering); isdir per row of A (nullptr: none)
+int assemble_q1_launch(gh_ctx* ctx, const gh_cells* C, const double* kcell, const uint8_t* isdir,
+                       const int32_t* perm, gh_csr* A, double* val64);
 int transfer_from_csr(gh_ctx* ctx, int64_t ncoarse, int64_t nfine, std::vector<int32_t> rrow,
                       std::vector<uint32_t> rcol, std::vector<int32_t> prow,
                       std::vector<uint32_t> pcol, gh_transfer** out);

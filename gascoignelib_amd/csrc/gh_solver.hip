@@ -275,6 +275,8 @@ int dupload(gh_ctx* ctx, T** d, const std::vector<T>& h) {
 
 // dense inverse of a direct level's matrix (LU with partial pivoting, FP64)
 int dense_inverse(const gh_csr* A, std::vector<double>& inv) {
+  if (!A->h_val_ok)
+    return fail(GH_EINVAL, "dense coarse solve: the matrix values were assembled on the device");
   const int B = A->ncomp;
   const int64_t m = A->n * B;
   std::vector<double> a(m * m, 0.0);
@@ -604,6 +606,8 @@ int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* 
                         gh_transfer* const* T, gh_hn* const* H, gh_mask* const* M,
                         const int32_t* const* perm, bool sorted64, MgLevel& Lv) {
   const gh_csr* Au = A[l];
+  if (!Au->h_val_ok)
+    return fail(GH_EINVAL, "gh_mg_create_ordered: level %d was assembled on the device (no host values)", l);
   const int64_t n = Au->n;
   const int B = Au->ncomp, BB = B * B;
   const int32_t* P = perm[l];
@@ -740,6 +744,10 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
   for (int l = 0; l < nlevels; ++l) {
     if (!A[l] || A[l]->prec != GH_F64)
       return fail(GH_EINVAL, "gh_mg_create: level %d needs an FP64 matrix", l);
+    if (!A[l]->h_val_ok && (params->adjoint || params->prec != GH_F64 || perm))
+      return fail(GH_EINVAL,
+                  "gh_mg_create: level %d was assembled on the device; its copies cannot be formed",
+                  l);
     if (H && H[l] && H[l]->n != A[l]->n)
       return fail(GH_EINVAL, "gh_mg_create: HN size mismatch on level %d", l);
     if (M && M[l] && (M[l]->n != A[l]->n || M[l]->ncomp != A[l]->ncomp))
@@ -928,6 +936,47 @@ int gh_mg_destroy(gh_mg* mg) {
   gh_ctx* ctx = mg->ctx;
   delete mg;
   ctx_release(ctx);
+  return GH_OK;
+}
+
+int gh_mg_assemble_q1(gh_ctx* ctx, gh_mg* mg, gh_cells* const* C, gh_vec* const* k) {
+  GH_ARG(ctx && mg && C, "gh_mg_assemble_q1: NULL argument");
+  GH_ARG(!mg->P.adjoint, "gh_mg_assemble_q1: the adjoint solver's transposed levels are not assembled");
+  GH_ARG(mg->P.smoother != 2, "gh_mg_assemble_q1: Vanka patch factors are not re-formed");
+  for (int l = 0; l < mg->nlevels; ++l) {
+    const MgLevel& L = mg->L[l];
+    if (!C[l] || C[l]->n != L.n || L.ncomp != 1)
+      return fail(GH_EINVAL, "gh_mg_assemble_q1: cells of level %d do not match a scalar level", l);
+    if (k && k[l] && (k[l]->prec != GH_F64 || k[l]->size != C[l]->ncells))
+      return fail(GH_EINVAL, "gh_mg_assemble_q1: k of level %d needs one FP64 entry per cell", l);
+    if (L.H)
+      return fail(GH_EINVAL, "gh_mg_assemble_q1: level %d has hanging nodes (not assembled here)", l);
+  }
+  for (int l = 0; l < mg->nlevels; ++l) {
+    MgLevel& L = mg->L[l];
+    const uint8_t* isdir = nullptr;
+    if (L.M) GH_TRY(mask_node_flags(ctx, L.M, &isdir));
+    const double* kc = (k && k[l]) ? (const double*)k[l]->d : nullptr;
+    gh_csr* mats[3] = {L.A64, L.A != L.A64 ? L.A : nullptr, L.A64s};
+    for (gh_csr* M : mats) {
+      if (!M) continue;
+      double* v64 = nullptr;
+      GH_TRY(csr_asm_values(ctx, M, &v64));
+      GH_TRY(assemble_q1_launch(ctx, C[l], kc, isdir, L.d_perm, M, v64));
+      GH_TRY(csr_asm_commit(ctx, M, "gh_mg_assemble_q1"));
+    }
+    if (L.direct) {
+      // the dense inverse of a direct level is formed on the host, from its
+      // new FP64 values (a few thousand entries)
+      gh_csr* A = L.A64;
+      GH_HIP(hipMemcpy(A->h_val.data(), A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
+      A->h_val_ok = true;
+      std::vector<double> inv;
+      GH_TRY(dense_inverse(A, inv));
+      GH_HIP(hipMemcpy(L.Ainv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    }
+  }
+  GH_HIP(hipStreamSynchronize(ctx->stream));
   return GH_OK;
 }
 

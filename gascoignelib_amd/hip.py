@@ -195,6 +195,13 @@ class CudaCSRMatrixInterface:
     def jacobi(self, h):
         check(N.hip().gh_jacobi(self.ctx.h, self.h, h.h))
 
+    def values(self):
+        """the values on the device, in CSR order, as FP64"""
+        nnz = N.hip().gh_csr_nnz(self.h)
+        out = np.empty(nnz * self.n_comp * self.n_comp)
+        check(N.hip().gh_csr_download_values(self.ctx.h, self.h, ptr(out)))
+        return out
+
     def jacobi_noalloc(self, h):
         """h = D^-1 h through the fixed per-GPU scratch (no device allocation)"""
         check(N.hip().gh_jacobi_noalloc(self.ctx.h, self.h, h.h))
@@ -483,6 +490,17 @@ class CudaMultiLevelSolver:
                                       C.byref(rep)))
         return rep
 
+    def assemble_q1(self, cells, k=None):
+        """AssembleMatrix + ComputeIlu on the device: every level's operator
+        re-formed from its cells (CoefficientCells per level, caller numbering)
+        and per-cell coefficients (CudaVectorInterface per level, or None)"""
+        nl = self.hier.nlevels
+        carr = (C.c_void_p * nl)(*[c.h for c in cells])
+        karr = None
+        if k is not None:
+            karr = (C.c_void_p * nl)(*[(v.h if v is not None else None) for v in k])
+        check(N.hip().gh_mg_assemble_q1(self.ctx.h, self.h, carr, karr))
+
     def free(self):
         if self.h:
             N.hip().gh_mg_destroy(self.h)
@@ -514,6 +532,17 @@ class CoefficientCells:
 
     def dJdk(self, z, u, g):
         check(N.hip().gh_dJdk_cells(self.ctx.h, self.h, z.h, u.h, g.h))
+
+    def assemble(self, A, k=None, mask=None):
+        """A <- Q1 operator of -div(k grad u) on these cells, formed on the
+        device (k: CudaVectorInterface with one entry per cell, or None)"""
+        check(N.hip().gh_assemble_q1(self.ctx.h, self.h, k.h if k is not None else None,
+                                     mask.h if mask is not None else None, A.h))
+
+    def form(self, r, u, f, k=None, mask=None):
+        """r = f - A(u), cell by cell (CGDisc::Form), mask rows zero"""
+        check(N.hip().gh_form_q1(self.ctx.h, self.h, k.h if k is not None else None,
+                                 mask.h if mask is not None else None, u.h, f.h, r.h))
 
     def __del__(self):
         try:

@@ -144,6 +144,10 @@ HIP_SIGNATURES = {
     "gh_cells_create": (C.c_int, [vp, C.c_int, C.c_int64, f64p, C.c_int64, i32p, P(vp)]),
     "gh_cells_destroy": (C.c_int, [vp]),
     "gh_dJdk_cells": (C.c_int, [vp, vp, vp, vp, vp]),
+    "gh_assemble_q1": (C.c_int, [vp, vp, vp, vp, vp]),
+    "gh_csr_download_values": (C.c_int, [vp, vp, f64p]),
+    "gh_form_q1": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "gh_mg_assemble_q1": (C.c_int, [vp, vp, P(vp), P(vp)]),
 }
 
 STRUCT_SIGNATURES = {

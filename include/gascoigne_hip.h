@@ -132,6 +132,10 @@ int gh_csr_prec(const gh_csr* A);
  * 4 out[0], out[7] = 1; other block sizes: out[0] = out[1] = nnz blocks,
  * out[2..5] = 0, out[6] = 4 nnz, out[7] = 0. */
 int gh_csr_layout(const gh_csr* A, int64_t* out);
+/* The values the device holds, in A's CSR order (nnz*B*B, reduced-precision
+ * values widened to FP64); the D2H half of operator<< (print of the device
+ * matrix, cuda/cudacsrmatrixinterface.cc:439).  Synchronises. */
+int gh_csr_download_values(gh_ctx* ctx, const gh_csr* A, double* val);
 /* y = d*A*x + b*y: CudaCSRMatrixInterface::vmult (cudacsrmatrixinterface.cc:372-412) */
 int gh_spmv(gh_ctx* ctx, const gh_csr* A, gh_vec* y, const gh_vec* x, double d, double b);
 /* y = d*A^T*x + b*y, FP64; block matrices are transposed blockwise
@@ -384,6 +388,43 @@ int gh_cells_destroy(gh_cells* C);
  * and z the adjoint (A^T z = dJ/du).  u and z must vanish on Dirichlet nodes
  * (then only free-node couplings contribute, as in the eliminated matrix). */
 int gh_dJdk_cells(gh_ctx* ctx, const gh_cells* C, const gh_vec* z, const gh_vec* u, gh_vec* g);
+
+/* ------------------------------------------------------ device assembly */
+/* CudaSolver::AssembleMatrix (cuda/cudasolver.cc:114-128): StdSolver's CPU
+ * assembly CGDisc::Matrix (src/DofHandler/cgdisc.h:599-635, element matrices of
+ * src/DofHandler/elementintegrator.xx:350-397) with SparseBlockMatrix::dirichlet
+ * (src/LinAlg/sparseblockmatrix.xx:205-218), uploaded and its diagonal inverted
+ * (get_inverse_diagonal, cuda/cudacsrmatrixinterface.cc:119-141) -- here formed
+ * on the device.  The Q1 operator of -div(k grad u) on the box cells C with one
+ * coefficient per cell (k: FP64 vector of C's cell count, NULL: k = 1) is
+ * written into the existing scalar matrix A (any precision, any layout; its
+ * pattern must hold every coupling of the cells), with the rows and columns of
+ * the mask's nodes made identity, and A's inverse diagonal re-formed.  Each
+ * entry sums its cells' contributions in ascending cell order, so the values
+ * equal a host assembly in that order bit for bit.  An entry that was an exact
+ * zero when A was created (left out of the device layout) must stay zero, else
+ * GH_EINVAL.  Afterwards A's creation values are gone: gh_mg_create of a
+ * non-FP64, ordered or adjoint solver refuses A.  Synchronises. */
+int gh_assemble_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* dirichlet,
+                   gh_csr* A);
+/* CGDisc::Form (src/DofHandler/cgdisc.h:415-448) as used by
+ * StdMultiLevelSolver::NewtonResidual (src/Solver/stdmultilevelsolver.cc:608-621):
+ * r = f - A(u) formed cell by cell from u (no matrix), each node summing its
+ * cells in ascending order; the mask's rows set to zero (SetBoundaryVectorZero).
+ * u enters at every node, Dirichlet nodes included. */
+int gh_form_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* dirichlet,
+               const gh_vec* u, const gh_vec* f, gh_vec* r);
+/* StdMultiLevelSolver::AssembleMatrix + ComputeIlu
+ * (src/Solver/stdmultilevelsolver.cc:763-790): gh_assemble_q1 on every level
+ * of a solver, into the level operators it runs on (the FP64 operator, the
+ * reduced-precision V-cycle copy and the flexible-GMRES copy; for a solver
+ * made by gh_mg_create those FP64 operators are the caller's matrices),
+ * followed by the inverse diagonals and, for directly solved levels, the dense
+ * inverse (formed on the host from the new values).  C[l] and k[l] (k or k[l]
+ * NULL: k = 1) in the caller's numbering of level l; the solver's own Dirichlet
+ * masks apply.  Levels with hanging nodes, the Vanka smoother and adjoint
+ * solvers are refused (GH_EINVAL).  Synchronises. */
+int gh_mg_assemble_q1(gh_ctx* ctx, gh_mg* mg, gh_cells* const* C, gh_vec* const* k);
 
 #ifdef __cplusplus
 }
