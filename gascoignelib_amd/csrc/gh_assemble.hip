@@ -61,7 +61,10 @@ __device__ __forceinline__ double q1_entry(int a, int b, double hx, double hy, d
 // ascending order, of k_c K^c_ab where a, b are the local corners of the two
 // nodes in c.  Dirichlet rows are unit rows, Dirichlet columns zero.  Each
 // (cell, corner) pair of the row must meet exactly one entry of the pattern,
-// else flags[2] is set.
+// else flags[2] is set.  The row's cells (at most MAXC; rows of more cells
+// take the generic loop) are read once into registers.
+constexpr int kMaxRowCells = 8;
+
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_assemble_q1(
     int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -74,33 +77,72 @@ __global__ __launch_bounds__(kBlock) void k_assemble_q1(
   if (i >= n) return;
   const int32_t ic = perm ? perm[i] : (int32_t)i;
   const int32_t c0 = nptr[ic], c1 = nptr[ic + 1];
+  const int nc = c1 - c0;
   const bool drow = isdir && isdir[i];
+  const int32_t q0 = rowptr[i], q1 = rowptr[i + 1];
   int matched = 0;
-  for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-    const int32_t j = col[q];
-    const int32_t jc = perm ? perm[j] : j;
-    double acc = 0.0;
-    for (int32_t e = c0; e < c1; ++e) {
-      const int32_t ce = nlist[e];
-      const int64_t c = ce >> 3;
-      const int a = ce & 7;
-      const int32_t* v = cells + c * NPC;
+  if (nc <= kMaxRowCells) {
+    int32_t node[kMaxRowCells][NPC];
+    int loc[kMaxRowCells];
+    double kk[kMaxRowCells], hx[kMaxRowCells], hy[kMaxRowCells], hz[kMaxRowCells];
 #pragma unroll
-      for (int b = 0; b < NPC; ++b) {
-        if (v[b] == jc) {
-          const double k = kc ? kc[c] : 1.0;
-          acc = add_rn(acc, mul_rn(k, q1_entry<DIM>(a, b, h[3 * c], h[3 * c + 1], h[3 * c + 2])));
-          ++matched;
+    for (int e = 0; e < kMaxRowCells; ++e) {
+      const int32_t ce = e < nc ? nlist[c0 + e] : 0;
+      const int64_t c = ce >> 3;
+      loc[e] = ce & 7;
+      kk[e] = (e < nc && kc) ? kc[c] : 1.0;
+      hx[e] = e < nc ? h[3 * c] : 1.0;
+      hy[e] = e < nc ? h[3 * c + 1] : 1.0;
+      hz[e] = e < nc ? h[3 * c + 2] : 1.0;
+#pragma unroll
+      for (int b = 0; b < NPC; ++b) node[e][b] = e < nc ? cells[c * NPC + b] : -1;
+    }
+    for (int32_t q = q0; q < q1; ++q) {
+      const int32_t j = col[q];
+      const int32_t jc = perm ? perm[j] : j;
+      double acc = 0.0;
+#pragma unroll
+      for (int e = 0; e < kMaxRowCells; ++e) {
+#pragma unroll
+        for (int b = 0; b < NPC; ++b) {
+          if (node[e][b] == jc) {
+            acc = add_rn(acc, mul_rn(kk[e], q1_entry<DIM>(loc[e], b, hx[e], hy[e], hz[e])));
+            ++matched;
+          }
         }
       }
+      if (drow)
+        acc = (j == i) ? 1.0 : 0.0;
+      else if (isdir && isdir[j])
+        acc = 0.0;
+      val[q] = acc;
     }
-    if (drow)
-      acc = (j == i) ? 1.0 : 0.0;
-    else if (isdir && isdir[j])
-      acc = 0.0;
-    val[q] = acc;
+  } else {
+    for (int32_t q = q0; q < q1; ++q) {
+      const int32_t j = col[q];
+      const int32_t jc = perm ? perm[j] : j;
+      double acc = 0.0;
+      for (int32_t e = c0; e < c1; ++e) {
+        const int32_t ce = nlist[e];
+        const int64_t c = ce >> 3;
+        const int32_t* v = cells + c * NPC;
+#pragma unroll
+        for (int b = 0; b < NPC; ++b) {
+          if (v[b] == jc) {
+            const double k = kc ? kc[c] : 1.0;
+            acc = add_rn(acc, mul_rn(k, q1_entry<DIM>(ce & 7, b, h[3 * c], h[3 * c + 1], h[3 * c + 2])));
+            ++matched;
+          }
+        }
+      }
+      if (drow)
+        acc = (j == i) ? 1.0 : 0.0;
+      else if (isdir && isdir[j])
+        acc = 0.0;
+      val[q] = acc;
+    }
   }
-  if (matched != (c1 - c0) * NPC) flags[2] = 1;
+  if (matched != nc * NPC) flags[2] = 1;
 }
 
 // One thread per node: r_i = f_i - sum over the cells c of node i (ascending)
