@@ -167,6 +167,78 @@ __global__ __launch_bounds__(kBlock) void k_mgs_part(int64_t n, double* __restri
   if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
+// Classical Gram-Schmidt pieces of the flexible GMRES Arnoldi step: one pass
+// forms the partials of <w, basis[k]> for up to kCgsGroup basis vectors (w
+// read once), a second pass subtracts sum_k h_k basis[k] from w (k ascending)
+// and forms the partials of <w, w>.  Per Arnoldi step that is two passes over
+// w against MGS's one per basis vector.
+constexpr int kCgsGroup = 8;
+
+__global__ __launch_bounds__(kBlock) void k_cgs_dot(int64_t n, const double* __restrict__ w,
+                                                    const double* const* __restrict__ basis,
+                                                    int k0, int cnt, double* __restrict__ part) {
+  const double* v[kCgsGroup];
+#pragma unroll
+  for (int g = 0; g < kCgsGroup; ++g) v[g] = g < cnt ? basis[k0 + g] : nullptr;
+  double acc[kCgsGroup];
+#pragma unroll
+  for (int g = 0; g < kCgsGroup; ++g) acc[g] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double wi = w[i];
+#pragma unroll
+    for (int g = 0; g < kCgsGroup; ++g)
+      if (g < cnt) acc[g] += wi * v[g][i];
+  }
+#pragma unroll
+  for (int g = 0; g < kCgsGroup; ++g) {
+    if (g >= cnt) break;  // cnt is uniform: every thread leaves at the same g
+    const double a = block_reduce<false>(acc[g]);
+    if (threadIdx.x == 0) part[(int64_t)(k0 + g) * kRedBlocks + blockIdx.x] = a;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, double* __restrict__ w,
+                                                       const double* const* __restrict__ basis,
+                                                       int dim, const double* __restrict__ h,
+                                                       double* __restrict__ part) {
+  __shared__ double hs[256];
+  __shared__ const double* vs[256];
+  for (int k = threadIdx.x; k < dim; k += kBlock) {
+    hs[k] = h[k];
+    vs[k] = basis[k];
+  }
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double x = w[i];
+    for (int k = 0; k < dim; ++k) x -= hs[k] * vs[k][i];
+    w[i] = x;
+    acc += x * x;
+  }
+  const double a = block_reduce<false>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+int cgs_dots(gh_ctx* ctx, int64_t size, const double* w, const double* const* dbasis, int dim,
+             int slot0) {
+  for (int k0 = 0; k0 < dim; k0 += kCgsGroup)
+    hipLaunchKernelGGL(k_cgs_dot, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size, w, dbasis,
+                       k0, std::min(kCgsGroup, dim - k0), ctx->partials + (int64_t)slot0 * kRedBlocks);
+  GH_HIP(hipGetLastError());
+  return finalize_slots(ctx, slot0, dim, false);
+}
+
+int cgs_update(gh_ctx* ctx, int64_t size, double* w, const double* const* dbasis, int dim,
+               int hslot0, int oslot) {
+  if (dim > 256) return fail(GH_EINVAL, "cgs_update: at most 256 basis vectors");
+  hipLaunchKernelGGL(k_cgs_update, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size, w, dbasis,
+                     dim, ctx->scal + hslot0, ctx->partials + (int64_t)oslot * kRedBlocks);
+  GH_HIP(hipGetLastError());
+  return finalize_slots(ctx, oslot, 1, false);
+}
+
 int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
              const double* next, int oslot) {
   hipLaunchKernelGGL(k_mgs_part, dim3(kRedBlocks), dim3(kBlock), 0, ctx->stream, size, vv, prev,

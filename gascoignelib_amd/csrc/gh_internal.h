@@ -289,6 +289,12 @@ int mask_apply(gh_ctx* ctx, const gh_mask* M, int prec, void* x);
 int mgs_step(gh_ctx* ctx, int64_t size, double* vv, const double* prev, int dslot,
              const double* next, int oslot);
 int finalize_slots(gh_ctx* ctx, int first, int count, bool max);
+// classical Gram-Schmidt: scal[slot0 + k] = <w, basis[k]> (k < dim); then
+// w -= sum_k scal[hslot0 + k] basis[k] and scal[oslot] = <w, w>
+int cgs_dots(gh_ctx* ctx, int64_t size, const double* w, const double* const* dbasis, int dim,
+             int slot0);
+int cgs_update(gh_ctx* ctx, int64_t size, double* w, const double* const* dbasis, int dim,
+               int hslot0, int oslot);
 int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, const int32_t* col,
                     const double* val, int prec, bool check_sorted, gh_csr** out);
 // Device assembly of scalar matrices.  csr_asm_values: the FP64 CSR value

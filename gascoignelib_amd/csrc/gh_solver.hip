@@ -528,7 +528,14 @@ int fast_vcycle(gh_mg* mg, int clevel, void* xdev, int xprec, const void* ydev, 
     GH_TRY(vec_convert(ctx, F.size, mg->wprec, mg->fine_b, yprec, ydev));
     F.b = mg->fine_b;
   }
-  if (direct_u)
+  // Every Jacobi / Richardson sweep of the finest level swaps its u and v
+  // buffers, iterpre + iterpost times in all.  From a zero start the caller's
+  // x may take the role that ends holding the result, so no copy follows.
+  const bool flip = direct_u && x_zero && mg->P.smoother != 2 && fl != clevel &&
+                    ((mg->P.iterpre + mg->P.iterpost) & 1);
+  if (flip)
+    F.v = xdev;
+  else if (direct_u)
     F.u = xdev;
   else if (!x_zero)
     GH_TRY(vec_convert(ctx, F.size, mg->wprec, F.u, xprec, xdev));
@@ -1195,11 +1202,21 @@ int precond_raw(gh_mg* mg, int coarselevel, double* x, const double* y) {
 // `dim-1`.  Returns |gamma[dim]|.
 int arnoldi_step(gh_mg* mg, int dim, double* vv, std::vector<double>& h,
                  std::vector<double>& gamma, std::vector<double>& ci, std::vector<double>& si,
-                 std::vector<double>& d, double* rho) {
+                 std::vector<double>& d, double* rho, bool classical = false) {
   gh_ctx* ctx = mg->ctx;
   const int64_t n = mg->L[mg->nlevels - 1].size;
   const int last = dim - 1;
-  for (int pass = 0; pass < 2; ++pass) {
+  if (classical) {
+    // flexible GMRES (no reference sums to keep): classical Gram-Schmidt,
+    // h_k = <vv, v_k> for all k at once, then vv -= sum_k h_k v_k
+    const double* const* basis = (const double* const*)mg->d_basis;
+    GH_TRY(cgs_dots(ctx, n, vv, basis, dim, 8));
+    GH_TRY(cgs_update(ctx, n, vv, basis, dim, 8, 8 + dim));
+    GH_TRY(fetch_scalars(ctx, 8, dim + 1, d.data()));
+    for (int i = 0; i < dim; ++i) h[i] += d[i];
+    h[dim] = std::sqrt(d[dim]);
+  }
+  for (int pass = 0; pass < 2 && !classical; ++pass) {
     GH_TRY(mgs_step(ctx, n, vv, nullptr, 0, mg->mem[0], 8));
     for (int i = 1; i < dim; ++i) GH_TRY(mgs_step(ctx, n, vv, mg->mem[i - 1], 8 + i - 1, mg->mem[i], 8 + i));
     GH_TRY(mgs_step(ctx, n, vv, mg->mem[dim - 1], 8 + dim - 1, nullptr, 8 + dim));
@@ -1369,7 +1386,7 @@ int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, in
       GH_TRY(csr_spmv(ctx, A, GH_F64, vv, GH_F64, z, 1.0, 0.0));
     else
       GH_TRY(csr_spmv_x32(ctx, A, vv, (const float*)z));
-    GH_TRY(arnoldi_step(mg, dim, vv, H[last], gamma, ci, si, d, &rho));
+    GH_TRY(arnoldi_step(mg, dim, vv, H[last], gamma, ci, si, d, &rho, true));
     reached = gh_cginfo_check(info, rho, (double)(k0 + last));
   }
   if (dim > 0) {
