@@ -439,6 +439,72 @@ __global__ __launch_bounds__(kBlock) void k_sell_w(int64_t n, const uint32_t* __
   op.finish(row, acc);
 }
 
+// The grid transfers of an FP32 V-cycle on levels of few slices, split as
+// k_sell_split splits the sweeps: wavefront w of a slice's K folds entries
+// w, w + K, ... of each row from zero, and wavefront 0 adds the K partial sums,
+// in order, to op.init(row).  Every transfer op steps acc + w x, so only the
+// order of the row sum changes; the FP64 transfers keep one thread per row and
+// their bits.
+template <typename T, class Op, int K>
+__global__ __launch_bounds__(kSlice * K) void k_sell_w_split(int64_t n,
+                                                             const uint32_t* __restrict__ sptr,
+                                                             const uint16_t* __restrict__ rlen,
+                                                             const uint32_t* __restrict__ col,
+                                                             const T* __restrict__ x, Op op) {
+  __shared__ T part[K][kSlice];
+  const int64_t s = xcd_block(blockIdx.x, gridDim.x);  // one slice per workgroup
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSlice));
+  const int64_t row0 = s * kSlice + lane;
+  const bool active = row0 < n;
+  const int64_t row = active ? row0 : n - 1;
+  const uint32_t* cp = col + sptr[s] + lane + w * kSlice;
+  const int len = rlen[row];
+  const int mine = len > w ? (len - w + K - 1) / K : 0;
+  constexpr int S = K * kSlice;
+  T acc = (T)0;
+  int k = 0;
+  for (; k + 4 <= mine; k += 4) {
+    const uint32_t e0 = cp[(k + 0) * S], e1 = cp[(k + 1) * S];
+    const uint32_t e2 = cp[(k + 2) * S], e3 = cp[(k + 3) * S];
+    const T x0 = x[e0 & kIdxMask], x1 = x[e1 & kIdxMask], x2 = x[e2 & kIdxMask],
+            x3 = x[e3 & kIdxMask];
+    acc = op.step(acc, code_weight<T>(e0 >> 29), x0);
+    acc = op.step(acc, code_weight<T>(e1 >> 29), x1);
+    acc = op.step(acc, code_weight<T>(e2 >> 29), x2);
+    acc = op.step(acc, code_weight<T>(e3 >> 29), x3);
+  }
+  for (; k < mine; ++k) {
+    const uint32_t e = cp[k * S];
+    acc = op.step(acc, code_weight<T>(e >> 29), x[e & kIdxMask]);
+  }
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0) {
+    T sum = op.init(row);
+#pragma unroll
+    for (int i = 0; i < K; ++i) sum += part[i][lane];
+    if (active) op.finish(row, sum);
+  }
+}
+
+// Split factor of an FP32 transfer: double K while the level runs fewer than
+// 8192 wavefronts and every wavefront still gets at least 4 of the longest
+// row's entries, up to 16.
+inline int sell_w_split(const Sell& S) {
+  int K = 1;
+  while (K < 16 && S.nslices * K < 8192 && S.maxlen >= 4 * K) K *= 2;
+  return K;
+}
+
+template <typename T, class Op, int K>
+inline int sell_w_split_launch(gh_ctx* ctx, const Sell& S, const T* x, Op op) {
+  hipLaunchKernelGGL((k_sell_w_split<T, Op, K>), dim3((unsigned)S.nslices), dim3(kSlice * K), 0,
+                     ctx->stream, S.n, S.sptr, S.rlen, S.col, x, op);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
 template <typename VT, typename XT, typename T, class Op, int K>
 inline int sell_split_launch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
   hipLaunchKernelGGL((k_sell_split<VT, XT, T, Op, K>), dim3((unsigned)S.nslices),
@@ -477,6 +543,15 @@ inline int sell_dispatch(gh_ctx* ctx, const Sell& S, const XT* x, Op op) {
 template <typename T, class Op>
 inline int sell_w_launch(gh_ctx* ctx, const Sell& S, const T* x, Op op) {
   if (S.n == 0) return GH_OK;
+  if constexpr (std::is_same<T, float>::value) {
+    switch (sell_w_split(S)) {
+      case 2: return sell_w_split_launch<T, Op, 2>(ctx, S, x, op);
+      case 4: return sell_w_split_launch<T, Op, 4>(ctx, S, x, op);
+      case 8: return sell_w_split_launch<T, Op, 8>(ctx, S, x, op);
+      case 16: return sell_w_split_launch<T, Op, 16>(ctx, S, x, op);
+      default: break;
+    }
+  }
   hipLaunchKernelGGL((k_sell_w<T, Op>), dim3(grid_for(S.n)), dim3(kBlock), 0, ctx->stream, S.n,
                      S.sptr, S.rlen, S.col, x, op);
   GH_HIP(hipGetLastError());
