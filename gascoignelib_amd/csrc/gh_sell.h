@@ -488,12 +488,13 @@ __global__ __launch_bounds__(kSlice * K) void k_sell_w_split(int64_t n,
   }
 }
 
-// Split factor of an FP32 transfer: double K while the level runs fewer than
-// 8192 wavefronts and every wavefront still gets at least 4 of the longest
-// row's entries, up to 16.
+// Split factor of an FP32 transfer: double K while the doubled level still
+// runs at most 8192 wavefronts and every wavefront still gets at least 4 of
+// the longest row's entries, up to 16.  (Levels of more than 4096 slices stay
+// unsplit: at 64^3 a split of 2 measured slower than one thread per row.)
 inline int sell_w_split(const Sell& S) {
   int K = 1;
-  while (K < 16 && S.nslices * K < 8192 && S.maxlen >= 4 * K) K *= 2;
+  while (K < 16 && S.nslices * 2 * K <= 8192 && S.maxlen >= 4 * K) K *= 2;
   return K;
 }
 

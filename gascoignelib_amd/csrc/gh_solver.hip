@@ -1307,7 +1307,7 @@ int gmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int
     const std::vector<double> hs = gmres_backward(H, gamma, dim);
     GH_TRY(basis_update(mg, x, false, dim, hs.data()));
   }
-  GH_HIP(hipStreamSynchronize(ctx->stream));
+  // the update stays queued: the caller's next fetched scalar orders it
   *reached_out = reached;
   *steps_out = dim;
   return GH_OK;
@@ -1334,7 +1334,7 @@ void cginfo_begin(gh_cginfo* I, double res) {
 
 int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
                 int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
-                int* reached_out, int* steps_out) {
+                int* reached_out, int* steps_out, double known_rho) {
   const int64_t n = mg->L[mg->nlevels - 1].size;
   GH_TRY(ensure_gmres_memory(mg, vmax, true));
   std::vector<std::vector<double>> H(vmax);
@@ -1349,10 +1349,13 @@ int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, in
   else
     GH_TRY(csr_residual(ctx, A, GH_F64, v, x, b));
   if (mask) GH_TRY(mask_apply(ctx, mask, GH_F64, v));
-  GH_TRY(reduce_dot(ctx, GH_F64, n, v, v, 2));
-  double rr;
-  GH_TRY(fetch_scalars(ctx, 2, 1, &rr));
-  double rho = std::sqrt(rr);
+  double rho = known_rho;
+  if (!(x_zero && known_rho >= 0.)) {
+    GH_TRY(reduce_dot(ctx, GH_F64, n, v, v, 2));
+    double rr;
+    GH_TRY(fetch_scalars(ctx, 2, 1, &rr));
+    rho = std::sqrt(rr);
+  }
   int dim = 0;
   if (rho == 0.) {
     *reached_out = 1;
@@ -1393,18 +1396,19 @@ int fgmres_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, in
     const std::vector<double> hs = gmres_backward(H, gamma, dim);
     GH_TRY(basis_update(mg, x, true, dim, hs.data()));
   }
-  GH_HIP(hipStreamSynchronize(ctx->stream));
   *reached_out = reached;
   *steps_out = dim;
   return GH_OK;
 }
 
+// known_rho >= 0: the Euclidean norm of b with the mask applied, when the
+// caller has it (flexible GMRES from x = 0 then skips that reduction)
 int krylov_impl(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* mask, int coarselevel,
                 int vmax, double* x, const double* b, bool x_zero, gh_cginfo* info,
-                int* reached_out, int* steps_out) {
+                int* reached_out, int* steps_out, double known_rho = -1.0) {
   if (mg->P.krylov == 1)
     return fgmres_impl(ctx, mg, A, mask, coarselevel, vmax, x, b, x_zero, info, reached_out,
-                       steps_out);
+                       steps_out, known_rho);
   return gmres_impl(ctx, mg, A, mask, coarselevel, vmax, x, b, x_zero, info, reached_out,
                     steps_out);
 }
@@ -1523,7 +1527,9 @@ int gh_newton_solve(gh_ctx* ctx, gh_mg* mg, const gh_csr* A, const gh_mask* user
     gh_cginfo_reset(linfo);
     int greached = 0, steps = 0;
     GH_TRY(vec_fill_zero(ctx, GH_F64, n, w));
-    GH_TRY(krylov_impl(ctx, mg, A, mask, coarselevel, vmax, w, r, true, linfo, &greached, &steps));
+    // r is masked and, with the Euclidean NewtonNorm, rr is its norm
+    GH_TRY(krylov_impl(ctx, mg, A, mask, coarselevel, vmax, w, r, true, linfo, &greached, &steps,
+                       norm_kind == 1 ? rr : -1.0));
     gm += steps;
     // NewtonUpdate
     int lex = linfo->status == GH_EXPLODED;
