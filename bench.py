@@ -883,21 +883,48 @@ def bench_q2(args, adaptive=0):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(hier, f):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hier, f, reps=3):
     """The oracle's FP64 Newton/GMRES/MG solve of the same problem on the host
-    cores (OpenMP over the loops the reference parallelises), one solve."""
+    cores, the median of `reps` solves per variant: the reference's threading
+    (OpenMP over the matrix-vector products and grid transfers, vector
+    operations serial as in nvector.h) and all loops parallel."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as O
     om = O.MG(hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
               ndirect=1000)
-    u = np.zeros_like(f)
-    t = time.perf_counter()
-    rc, rep = om.newton(u, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 50),
-                        norm_kind=1)
-    dt = time.perf_counter() - t
-    return {"value": len(f) / dt, "unit": "DOF/s", "cores": int(O.lib().or_num_threads()),
-            "kind": "port", "sample": f"one full FP64 solve of the same {len(f)}-DOF problem "
-            f"({dt:.1f} s, newton {rep.newton_iterations}, gmres {rep.gmres_iterations})"}
+    out = {}
+    for name, par in (("reference_threading", 0), ("all_loops_parallel", 1)):
+        O.lib().or_set_parallel_vectors(par)
+        ts = []
+        for _ in range(reps):
+            u = np.zeros_like(f)
+            t = time.perf_counter()
+            rc, rep = om.newton(u, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 50),
+                                norm_kind=1)
+            ts.append(time.perf_counter() - t)
+        dt = float(np.median(ts))
+        out[name] = {"dof_per_s": len(f) / dt, "s_per_solve": dt,
+                     "solves_s": [round(t, 3) for t in ts],
+                     "newton": rep.newton_iterations, "gmres": rep.gmres_iterations}
+    O.lib().or_set_parallel_vectors(0)
+    ref = out["reference_threading"]
+    return {"value": ref["dof_per_s"], "unit": "DOF/s", "cores": int(O.lib().or_num_threads()),
+            "kind": "port",
+            "sample": f"full FP64 solves of the same {len(f)}-DOF problem, median of {reps} "
+                      f"({ref['s_per_solve']:.1f} s each, newton {ref['newton']}, "
+                      f"gmres {ref['gmres']}); reference threading",
+            "variants": out, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 if __name__ == "__main__":

@@ -161,6 +161,13 @@ void or_dJdk_cells(int dim, const double* xyz, int64_t ncells, const int32_t* ce
   }
 }
 
+/* 0: the reference's threading (OpenMP over the matrix-vector products and
+ * grid transfers only); 1: also the vector operations and the Jacobi
+ * application (a timing variant: the sums of norms and dots then run in
+ * another order, so iterates differ from the reference's by rounding). */
+static int g_parvec = 0;
+void or_set_parallel_vectors(int on) { g_parvec = on; }
+
 int or_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
@@ -294,8 +301,9 @@ void or_block_gauss_jordan(int N, double* m) {
  * Ad = diag block; Ad.inverse(); Ad.vmult(x_i)  (fmatrixblock.h:194-213). */
 void or_jacobi(const or_matrix* A, double* h) {
   const int B = A->ncomp, BB = B * B;
-  double Ad[64 * 64], vh[64];
+#pragma omp parallel for schedule(static) if (g_parvec)
   for (int i = 0; i < A->n; ++i) {
+    double Ad[64 * 64], vh[64];
     memcpy(Ad, A->val + (size_t)A->diag[i] * BB, sizeof(double) * BB);
     or_block_gauss_jordan(B, Ad);
     double* p = h + (size_t)i * B;
@@ -313,26 +321,32 @@ void or_jacobi(const or_matrix* A, double* h) {
  * 451-460 (add), 507-517 (sadd); serial as in the reference. */
 double or_norm(const double* x, int64_t n) {
   double s = 0.;
+#pragma omp parallel for schedule(static) reduction(+ : s) if (g_parvec)
   for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
   return sqrt(s);
 }
 double or_norm_l8(const double* x, int64_t n) {
   double s = 0.;
+#pragma omp parallel for schedule(static) reduction(max : s) if (g_parvec)
   for (int64_t i = 0; i < n; ++i) s = fmax(s, fabs(x[i]));
   return s;
 }
 double or_dot(const double* x, const double* y, int64_t n) {
   double d = 0.;
+#pragma omp parallel for schedule(static) reduction(+ : d) if (g_parvec)
   for (int64_t i = 0; i < n; ++i) d += x[i] * y[i];
   return d;
 }
 void or_equ(double* dst, double s, const double* src, int64_t n) {
+#pragma omp parallel for schedule(static) if (g_parvec)
   for (int64_t i = 0; i < n; ++i) dst[i] = s * src[i];
 }
 void or_add(double* dst, double s, const double* src, int64_t n) {
+#pragma omp parallel for schedule(static) if (g_parvec)
   for (int64_t i = 0; i < n; ++i) dst[i] += s * src[i];
 }
 void or_sadd(double* y, double a, double d, const double* v, int64_t n) {
+#pragma omp parallel for schedule(static) if (g_parvec)
   for (int64_t i = 0; i < n; ++i) y[i] = a * y[i] + d * v[i];
 }
 void or_zero(double* x, int64_t n) { memset(x, 0, sizeof(double) * (size_t)n); }

@@ -228,6 +228,9 @@ void or_dJdk_cells(int dim, const double* xyz, int64_t ncells, const int32_t* ce
 
 /* OpenMP threads used by the loops the reference parallelises */
 int or_num_threads(void);
+/* timing variant: 1 also runs the vector operations and the Jacobi
+ * application on all threads (norm and dot sums then change order) */
+void or_set_parallel_vectors(int on);
 
 #ifdef __cplusplus
 }

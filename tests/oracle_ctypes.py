@@ -127,6 +127,7 @@ def _load():
                                       C.POINTER(OrCGInfo), C.POINTER(OrNLInfo), C.c_int,
                                       C.POINTER(OrSolveReport)]),
         "or_num_threads": (C.c_int, []),
+        "or_set_parallel_vectors": (None, [C.c_int]),
         "or_dJdk_cells": (None, [C.c_int, f64p, C.c_int64, i32p, f64p, f64p, f64p]),
         "or_vanka_create": (C.c_void_p, [C.POINTER(OrMatrix), C.c_int, C.c_int, i32p]),
         "or_vanka_free": (None, [C.c_void_p]),
