@@ -1,10 +1,10 @@
 // Vanka patch smoother (src/LinAlg/vankasmoother.cc): the patch matrices of a
 // node-block matrix, LU with partial pivoting on the device, and the solve
 //   x <- (1/count_i) sum_p R_p^T A_p^-1 R_p x.
-// Factoring: one workgroup per patch, the m x m patch matrix in LDS (m <= 128,
-// 128 KiB at most), the LU's operations uncontracted so that the factors equal
-// the oracle's; the recorded row exchanges are composed on the host into one
-// gather per factor row.  Solving: one wavefront per patch, the local vector
+// Factoring: one workgroup of 1024 threads per patch, the m x m patch matrix
+// (m <= 128) in LDS, the LU's operations uncontracted so that the factors
+// equal the oracle's; the recorded row exchanges are composed on the host into
+// one gather per factor row.  Solving: one wavefront per patch, the local vector
 // gathered in pivoted order into the lanes' registers (rows lane and
 // lane + 64), column-oriented triangular solves (each pass reads only its
 // triangle) with the solved values broadcast by readlane.  Averaging: one
@@ -23,85 +23,181 @@ namespace {
 constexpr int kVankaMaxM = 128;       // local unknowns per patch
 constexpr int kVankaMaxNodes = 128;   // nodes per patch
 
-__global__ __launch_bounds__(kBlock) void k_vanka_factor(int64_t npatches, int psize, int B,
-                                                         const int32_t* __restrict__ nodes,
-                                                         const int32_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ col,
-                                                         const double* __restrict__ val,
-                                                         double* __restrict__ lu64,
-                                                         float* __restrict__ lu32,
-                                                         int32_t* __restrict__ piv) {
+// Factoring one patch: the m x m matrix in LDS (93 KB at m = 108, so one
+// patch per CU), 1024 threads, two barriers per column (below).
+constexpr int kFactorThreads = 1024;
+
+// DPP move of a double (both halves); lanes the masks leave out read 0.
+template <int CTRL, int ROWMASK, int BANKMASK>
+__device__ __forceinline__ double dpp_move(double x) {
+  const int64_t b = __builtin_bit_cast(int64_t, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xFFFFFFFF), CTRL, ROWMASK, BANKMASK, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, BANKMASK, true);
+  return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ double lane_read(double x, int lane) {
+  const int64_t b = __builtin_bit_cast(int64_t, x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFF), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+// Maximum over the 64 lanes of a wavefront of values >= 0 (or -1 for none):
+// row shifts inside each row of 16 lanes, then the row broadcasts; lane 63
+// ends with the maximum.
+__device__ __forceinline__ double wave_max_nonneg(double x) {
+  x = fmax(x, dpp_move<0x111, 0xf, 0xf>(x));  // row_shr:1
+  x = fmax(x, dpp_move<0x112, 0xf, 0xf>(x));  // row_shr:2
+  x = fmax(x, dpp_move<0x114, 0xf, 0xe>(x));  // row_shr:4
+  x = fmax(x, dpp_move<0x118, 0xf, 0xc>(x));  // row_shr:8
+  x = fmax(x, dpp_move<0x142, 0xa, 0xf>(x));  // row_bcast:15
+  x = fmax(x, dpp_move<0x143, 0xc, 0xf>(x));  // row_bcast:31
+  return lane_read(x, 63);
+}
+
+__global__ __launch_bounds__(kFactorThreads) void k_vanka_factor(
+    int64_t npatches, int psize, int B, const int32_t* __restrict__ nodes,
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const double* __restrict__ val, double* __restrict__ lu64, float* __restrict__ lu32,
+    int32_t* __restrict__ piv) {
   extern __shared__ double a[];  // m * m, row-major
   __shared__ int32_t pn[kVankaMaxNodes];
   __shared__ int s_piv;
+  __shared__ double s_pv, s_akk;
+  constexpr int NT = kFactorThreads;
   const int64_t p = blockIdx.x;
   const int m = psize * B;
-  for (int e = threadIdx.x; e < m * m; e += kBlock) a[e] = 0.0;
-  for (int r = threadIdx.x; r < psize; r += kBlock) pn[r] = nodes[p * psize + r];
+  const int t = threadIdx.x;
+  for (int e = t; e < m * m; e += NT) a[e] = 0.0;
+  for (int r = t; r < psize; r += NT) pn[r] = nodes[p * psize + r];
   __syncthreads();
-  // copy_entries_sparseblockmatrix: entries whose column is a patch node
-  for (int r = threadIdx.x; r < psize; r += kBlock) {
-    const int32_t row = pn[r];
-    for (int32_t q = rowptr[row]; q < rowptr[row + 1]; ++q) {
-      const int32_t c0 = col[q];
-      int c = -1;
-      for (int t = 0; t < psize; ++t)
-        if (pn[t] == c0) c = t;
-      if (c < 0) continue;
-      for (int cr = 0; cr < B; ++cr)
-        for (int cc = 0; cc < B; ++cc)
-          a[(B * r + cr) * m + B * c + cc] = val[((int64_t)q * B + cr) * B + cc];
+  // copy_entries_sparseblockmatrix: entries whose column is a patch node.
+  // The patch's rows share the threads: NT / psize threads walk each row's
+  // entries (every entry lands in its own place, so the order is free).
+  {
+    const int per = NT / psize;  // psize <= 128, so at least 8 threads per row
+    const int r = t / per, u0 = t - r * per;
+    if (r < psize) {
+      const int32_t row = pn[r];
+      for (int32_t q = rowptr[row] + u0; q < rowptr[row + 1]; q += per) {
+        const int32_t c0 = col[q];
+        int c = -1;
+        for (int u = 0; u < psize; ++u)
+          if (pn[u] == c0) c = u;
+        if (c < 0) continue;
+        for (int cr = 0; cr < B; ++cr)
+          for (int cc = 0; cc < B; ++cc)
+            a[(B * r + cr) * m + B * c + cc] = val[((int64_t)q * B + cr) * B + cc];
+      }
     }
   }
   __syncthreads();
+  // LU with partial pivoting (first largest |a_ik|), two barriers per column:
+  //   phase 1: swap rows k and pr (every column but k), divide column k by the
+  //            pivot d (the division of row pr takes the old a_kk, saved by
+  //            the search, since the swap moves that row's column-k entry);
+  //   phase 2: a_ij -= l_i a_kj; wavefront 0 updates column k + 1 itself and
+  //            searches it for the next pivot in the same phase, the others
+  //            update columns k + 2, ... (every product and difference
+  //            rounded separately, as the oracle computes them).
+  // The search (wavefront 0; lanes hold rows k + lane and k + lane + 64): the
+  // largest |a_ik| by a DPP maximum, then the first row holding it (rows
+  // k..k+63 before k+64..) by ballots.
+  auto search = [&](int k, double v0, double v1) {
+    const bool in0 = k + t < m, in1 = k + t + 64 < m;
+    const double best = wave_max_nonneg(fmax(in0 ? fabs(v0) : -1.0, in1 ? fabs(v1) : -1.0));
+    const uint64_t b0 = __ballot(in0 && fabs(v0) == best);
+    int bi;
+    double bv;
+    if (b0) {
+      const int l = __builtin_ctzll(b0);
+      bi = k + l;
+      bv = lane_read(v0, l);
+    } else {
+      const uint64_t b1 = __ballot(in1 && fabs(v1) == best);
+      const int l = __builtin_ctzll(b1);
+      bi = k + 64 + l;
+      bv = lane_read(v1, l);
+    }
+    const double akk = lane_read(v0, 0);  // row k itself
+    if (t == 0) {
+      s_piv = bi;
+      s_pv = bv;
+      s_akk = akk;
+    }
+  };
+  if (t < 64) search(0, t < m ? a[t * m] : 0.0, t + 64 < m ? a[(t + 64) * m] : 0.0);
+  __syncthreads();
+  constexpr int kUpd = NT - 64;            // threads updating columns k + 2 ...
+  constexpr int kGroups = kUpd / 128;      // 7 row groups of 128 columns
+  const int tu = t - 64, tj = tu & 127, tg = tu >> 7;
   for (int k = 0; k < m; ++k) {
-    if (threadIdx.x < 64) {  // first largest |a_ik|, i >= k
-      double best = -1.0;
-      int bi = m;
-      for (int i = k + (int)threadIdx.x; i < m; i += 64) {
-        const double v = fabs(a[i * m + k]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_down(best, off, 64);
-        const int oi = __shfl_down(bi, off, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (threadIdx.x == 0) s_piv = bi;
-    }
-    __syncthreads();
     const int pr = s_piv;
+    const double d = s_pv, akk = s_akk;
     if (pr != k)
-      for (int j = threadIdx.x; j < m; j += kBlock) {
-        const double t = a[k * m + j];
+      for (int j = t; j < m; j += NT) {
+        if (j == k) continue;
+        const double x = a[k * m + j];
         a[k * m + j] = a[pr * m + j];
-        a[pr * m + j] = t;
+        a[pr * m + j] = x;
       }
-    __syncthreads();
-    const double d = a[k * m + k];
-    if (d != 0.0) {
-      const int rem = m - k - 1;
-      for (int e = threadIdx.x; e < rem * rem; e += kBlock) {
-        const int i = k + 1 + e / rem, j = k + 1 + e % rem;
-        const double l = a[i * m + k] / d;
-        a[i * m + j] = add_rn(a[i * m + j], -mul_rn(l, a[k * m + j]));
-      }
-      __syncthreads();
-      for (int i = k + 1 + (int)threadIdx.x; i < m; i += kBlock) a[i * m + k] = a[i * m + k] / d;
+    if (t == 0) {
+      a[k * m + k] = (pr != k) ? d : akk;
+      piv[p * m + k] = pr;
     }
-    if (threadIdx.x == 0) piv[p * m + k] = pr;
+    if (d != 0.0) {
+      for (int i = k + 1 + t; i < m; i += NT) {
+        const double x = (i == pr) ? akk : a[i * m + k];
+        a[i * m + k] = x / d;
+      }
+    } else if (pr != k && t == 0) {
+      a[pr * m + k] = akk;
+    }
+    __syncthreads();
+    if (t < 64) {
+      // column k + 1: update (when d != 0), then search it
+      const int j = k + 1;
+      if (j < m) {
+        double v0 = 0.0, v1 = 0.0;
+        const double akj = a[k * m + j];
+        const int i0 = j + t, i1 = j + t + 64;
+        if (i0 < m) {
+          v0 = a[i0 * m + j];
+          if (d != 0.0) {
+            v0 = add_rn(v0, -mul_rn(a[i0 * m + k], akj));
+            a[i0 * m + j] = v0;
+          }
+        }
+        if (i1 < m) {
+          v1 = a[i1 * m + j];
+          if (d != 0.0) {
+            v1 = add_rn(v1, -mul_rn(a[i1 * m + k], akj));
+            a[i1 * m + j] = v1;
+          }
+        }
+        search(j, v0, v1);
+      }
+    } else if (d != 0.0 && tg < kGroups) {
+      const int j = k + 2 + tj;
+      if (j < m) {
+        const double akj = a[k * m + j];
+        int i = k + 1 + tg;
+        for (; i + kGroups < m; i += 2 * kGroups) {
+          const double l0 = a[i * m + k], l1 = a[(i + kGroups) * m + k];
+          const double x0 = a[i * m + j], x1 = a[(i + kGroups) * m + j];
+          a[i * m + j] = add_rn(x0, -mul_rn(l0, akj));
+          a[(i + kGroups) * m + j] = add_rn(x1, -mul_rn(l1, akj));
+        }
+        if (i < m) a[i * m + j] = add_rn(a[i * m + j], -mul_rn(a[i * m + k], akj));
+      }
+    }
     __syncthreads();
   }
   // column-major factors for coalesced column reads in the solve
   const int64_t base = p * (int64_t)m * m;
-  for (int e = threadIdx.x; e < m * m; e += kBlock) {
-    const int i = e % m, k = e / m;
+  for (int e = t; e < m * m; e += NT) {
+    const int k = e / m, i = e - k * m;
     if (lu64) lu64[base + e] = a[i * m + k];
     else lu32[base + e] = (float)a[i * m + k];
   }
@@ -307,7 +403,8 @@ int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, cons
     if (hipFuncSetAttribute((const void*)k_vanka_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
       return bail(fail(GH_EHIP, "gh_vanka_create: %zu bytes of LDS per patch not available", lds));
-    hipLaunchKernelGGL(k_vanka_factor, dim3((unsigned)npatches), dim3(kBlock), lds, ctx->stream,
+    hipLaunchKernelGGL(k_vanka_factor, dim3((unsigned)npatches), dim3(kFactorThreads), lds,
+                       ctx->stream,
                        npatches, psize, B, V->nodes, A->rowptr, A->col, (const double*)A->val,
                        prec == GH_F64 ? (double*)V->lu : nullptr,
                        prec == GH_F64 ? nullptr : (float*)V->lu, V->piv);
