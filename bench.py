@@ -433,6 +433,39 @@ def kernel_only(ctx, hier, f, args):
                       "copy_bytes_each_way": m * 8}), flush=True)
 
 
+def q2_kernel_only(ctx, sol, n, f, args):
+    """The finest-level sweeps of the Q2 solvers alone (for counter runs):
+    FP64 residual and Jacobi sweep on the FP64 solver's matrix, f16-value /
+    FP32-vector residual and sweep on the mixed V-cycle's matrix, each
+    args.spmv_reps times, then a plain copy of 2^25 doubles (counter scale)."""
+    from gascoignelib_amd import hip as H
+    out = {"kernel_only": True, "reps": args.spmv_reps, "n": n}
+    for tag, prec, vp, sv in (("f64", "fp64", "f64", 8), ("mixed", "mixed", "f32", 2)):
+        mg = sol[prec][0]
+        A = mg.level_matrix(fp64=(tag == "f64"))
+        sx = 8 if vp == "f64" else 4
+        X = H.CudaVectorInterface.from_global(ctx, f, 1, vp)
+        B = H.CudaVectorInterface.from_global(ctx, f, 1, vp)
+        R = H.CudaVectorInterface(ctx, n, 1, vp)
+        for _ in range(args.spmv_reps):
+            A.residual(R, X, B)
+        for _ in range(args.spmv_reps):
+            A.smooth(X, B, R, 1, 0.8)
+        lay = A.layout()
+        rb = sell_bytes(n, lay, sv, sx)
+        out[tag] = {"residual_bytes": rb, "jacobi_bytes": rb + n * (sx + sx),
+                    "stored_entries": lay["stored"], "slots": lay["slots"],
+                    "slices": lay["slices"], "patterned_slices": lay["patterned"],
+                    "column_bytes": lay["column_bytes"], "permuted": lay["permuted"]}
+    m = 1 << 25
+    C1, C2 = H.CudaVectorInterface(ctx, m), H.CudaVectorInterface(ctx, m)
+    for _ in range(args.spmv_reps):
+        C2.copy_from(C1)
+    ctx.sync()
+    out["copy_bytes_each_way"] = m * 8
+    print(json.dumps(out), flush=True)
+
+
 def bench_adjoint(args):
     """Variable-coefficient diffusion -div(k grad u) = f on the 3x3x3-hex unit
     cube refined to 96^3 cells (912,673 DOF, 5 levels), k = exp(g) per cell with
@@ -751,6 +784,8 @@ def bench_q2(args, adaptive=0):
                     H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=kry, adjoint=True,
                                            **c), tol)
     t_setup = time.perf_counter() - t0
+    if args.kernel_only:
+        return q2_kernel_only(ctx, sol, n, f, args)
     F = H.CudaVectorInterface.from_global(ctx, f)
     G = H.CudaVectorInterface.from_global(ctx, g)
     if sum(nhn):
