@@ -316,6 +316,147 @@ void or_jacobi(const or_matrix* A, double* h) {
   }
 }
 
+/* ------------------------------------------------------------------ ILU(0) */
+
+static const int32_t* g_sort_key;
+static int key_cmp(const void* a, const void* b) {
+  const int32_t x = g_sort_key[*(const int32_t*)a], y = g_sort_key[*(const int32_t*)b];
+  return (x > y) - (x < y);
+}
+
+int or_ilu_init(or_ilu* I, const or_matrix* A, const int32_t* perm) {
+  const int n = A->n, B = A->ncomp, BB = B * B;
+  const int64_t nnz = A->rowptr[n];
+  memset(I, 0, sizeof(*I));
+  I->n = n;
+  I->ncomp = B;
+  I->p = (int32_t*)malloc(sizeof(int32_t) * (n ? n : 1));
+  I->q = (int32_t*)malloc(sizeof(int32_t) * (n ? n : 1));
+  I->rowptr = (int32_t*)malloc(sizeof(int32_t) * (n + 1));
+  I->col = (int32_t*)malloc(sizeof(int32_t) * (nnz ? nnz : 1));
+  I->diag = (int32_t*)malloc(sizeof(int32_t) * (n ? n : 1));
+  I->val = (double*)calloc((size_t)(nnz ? nnz : 1) * BB, sizeof(double));
+  I->y = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1) * B);
+  /* ConstructStructure: p = perm, q its inverse; row i lists q[col] of row
+   * p[i], sorted ascending; diag = position of column i */
+  for (int i = 0; i < n; ++i) I->p[i] = perm ? perm[i] : i;
+  for (int i = 0; i < n; ++i) I->q[I->p[i]] = i;
+  int32_t zmax = 1;
+  for (int i = 0; i < n; ++i) {
+    const int32_t r = A->rowptr[i + 1] - A->rowptr[i];
+    if (r > zmax) zmax = r;
+  }
+  int32_t* picol = (int32_t*)malloc(sizeof(int32_t) * zmax);
+  int32_t* ppi = (int32_t*)malloc(sizeof(int32_t) * zmax);
+  I->rowptr[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    const int pi = I->p[i];
+    const int ni = A->rowptr[pi + 1] - A->rowptr[pi];
+    I->rowptr[i + 1] = I->rowptr[i] + ni;
+    for (int c = 0; c < ni; ++c) picol[c] = I->q[A->col[A->rowptr[pi] + c]];
+    for (int c = 0; c < ni; ++c) ppi[c] = c;
+    g_sort_key = picol;
+    qsort(ppi, (size_t)ni, sizeof(int32_t), key_cmp);
+    for (int c = 0; c < ni; ++c) I->col[I->rowptr[i] + c] = picol[ppi[c]];
+    I->diag[i] = -1;
+    for (int32_t pos = I->rowptr[i]; pos < I->rowptr[i + 1]; ++pos)
+      if (I->col[pos] == i) I->diag[i] = pos;
+  }
+  free(picol);
+  free(ppi);
+  /* copy_entries: entry (p[i], j) of A adds into (i, q[j]) */
+  for (int i = 0; i < n; ++i) {
+    const int pi = I->p[i];
+    for (int32_t posA = A->rowptr[pi]; posA < A->rowptr[pi + 1]; ++posA) {
+      const int pj = I->q[A->col[posA]];
+      for (int32_t pos = I->rowptr[i]; pos < I->rowptr[i + 1]; ++pos)
+        if (I->col[pos] == pj) {
+          for (int e = 0; e < BB; ++e) I->val[(size_t)pos * BB + e] += A->val[(size_t)posA * BB + e];
+          break;
+        }
+    }
+  }
+  /* compute_ilu: a(i,k) *= a(k,k)^-1 (operator*=), a(i,j) -= a(i,k) a(k,j)
+   * (submult), then a(i,i) <- a(i,i)^-1 (gauss_jordan) */
+  double vhelp[8];
+  for (int i = 0; i < n; ++i) {
+    if (I->diag[i] < 0) return -1;
+    for (int32_t pk = I->rowptr[i]; pk < I->diag[i]; ++pk) {
+      const int k = I->col[pk];
+      double* M = I->val + (size_t)pk * BB;
+      const double* D = I->val + (size_t)I->diag[k] * BB;
+      for (int r = 0; r < B; ++r) {
+        for (int c = 0; c < B; ++c) vhelp[c] = M[r * B + c];
+        for (int c = 0; c < B; ++c) {
+          M[r * B + c] = 0.;
+          for (int t = 0; t < B; ++t) M[r * B + c] += vhelp[t] * D[t * B + c];
+        }
+      }
+      for (int32_t pj = I->diag[k] + 1; pj < I->rowptr[k + 1]; ++pj) {
+        const int j = I->col[pj];
+        for (int32_t ph = I->rowptr[i]; ph < I->rowptr[i + 1]; ++ph) {
+          if (I->col[ph] == j) {
+            double* H = I->val + (size_t)ph * BB;
+            const double* C = I->val + (size_t)pj * BB;
+            for (int r = 0; r < B; ++r)
+              for (int c = 0; c < B; ++c)
+                for (int t = 0; t < B; ++t) H[r * B + c] -= M[r * B + t] * C[t * B + c];
+            break;
+          }
+        }
+      }
+    }
+    or_block_gauss_jordan(B, I->val + (size_t)I->diag[i] * BB);
+  }
+  return 0;
+}
+
+void or_ilu_solve(const or_ilu* I, double* x) {
+  const int n = I->n, B = I->ncomp, BB = B * B;
+  double* y = I->y;
+  double vh[8];
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < B; ++c) y[(size_t)i * B + c] = 1. * x[(size_t)I->p[i] * B + c];
+  /* forward (:193-203) */
+  for (int i = 1; i < n; ++i)
+    for (int32_t pos = I->rowptr[i]; pos < I->diag[i]; ++pos) {
+      const double* M = I->val + (size_t)pos * BB;
+      double* yi = y + (size_t)i * B;
+      const double* yc = y + (size_t)I->col[pos] * B;
+      for (int r = 0; r < B; ++r)
+        for (int c = 0; c < B; ++c) yi[r] -= M[r * B + c] * yc[c];
+    }
+  /* backward (:207-222) */
+  for (int i = n - 1; i >= 0; --i) {
+    double* yi = y + (size_t)i * B;
+    for (int32_t pos = I->rowptr[i + 1] - 1; pos > I->diag[i]; --pos) {
+      const double* M = I->val + (size_t)pos * BB;
+      const double* yc = y + (size_t)I->col[pos] * B;
+      for (int r = 0; r < B; ++r)
+        for (int c = 0; c < B; ++c) yi[r] -= M[r * B + c] * yc[c];
+    }
+    const double* D = I->val + (size_t)I->diag[i] * BB;
+    for (int c = 0; c < B; ++c) vh[c] = yi[c];
+    for (int r = 0; r < B; ++r) {
+      yi[r] = 0.;
+      for (int c = 0; c < B; ++c) yi[r] += D[r * B + c] * vh[c];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < B; ++c) x[(size_t)i * B + c] = 1. * y[(size_t)I->q[i] * B + c];
+}
+
+void or_ilu_free(or_ilu* I) {
+  free(I->p);
+  free(I->q);
+  free(I->rowptr);
+  free(I->col);
+  free(I->diag);
+  free(I->val);
+  free(I->y);
+  memset(I, 0, sizeof(*I));
+}
+
 /* ------------------------------------------------------------- nvector ops */
 /* src/Common/nvector.h:169-180 (norm), 332-342 (operator*), 365-374 (equ),
  * 451-460 (add), 507-517 (sadd); serial as in the reference. */

@@ -72,6 +72,23 @@ void or_add(double* dst, double s, const double* src, int64_t n);
 void or_sadd(double* y, double a, double d, const double* v, int64_t n);
 void or_zero(double* x, int64_t n);
 
+/* SparseBlockIlu<FMatrixBlock<B>> (src/LinAlg/sparseblockilu.xx): ILU(0) in
+ * the node order perm (ILU row i = node perm[i]; NULL: identity), on A's
+ * pattern renumbered with each row's columns ascending.  or_ilu_init runs
+ * ConstructStructure (:70-127), copy_entries (:131-166) and compute_ilu
+ * (:240-275); or_ilu_solve is solve (:226-236): hin, forward, backward, her.
+ * Pinned by the compiled reference's factors and solves (golden fixtures). */
+typedef struct {
+  int n, ncomp;
+  int32_t *p, *q;
+  int32_t *rowptr, *col, *diag;
+  double* val;
+  double* y;
+} or_ilu;
+int or_ilu_init(or_ilu* I, const or_matrix* A, const int32_t* perm);
+void or_ilu_solve(const or_ilu* I, double* x);
+void or_ilu_free(or_ilu* I);
+
 /* grid transfer built from c2f / 2er / 4er / 8er maps */
 typedef struct {
   int ncoarse, nfine;

@@ -10,6 +10,9 @@
 //     (src/Discretization/Q1/mginterpolatormatrix.cc:34-147)
 //   * HNStructureQ1x / Q2x Average / Distribute / Zero via CGDisc::HN*
 //   * nvector::norm / operator* (src/Common/nvector.h)
+//   * SparseBlockIlu<FMatrixBlock<1>>: ConstructStructure / copy_entries /
+//     compute_ilu / solve (src/LinAlg/sparseblockilu.xx:70-246) on levels of
+//     at most 1100 nodes, with the identity order and the reversed order
 // The strong-Dirichlet loop mirrors CGDisc::StrongDirichletMatrix
 // (cgdisc.h:1139-1150) for colour list `colours` and component 0, since the
 // ProblemDescriptor plumbing lives in the unbuildable solver sources.
@@ -25,6 +28,7 @@
 #include "sparseblockmatrix.h"
 #include "fmatrixblock.h"
 #include "sparsestructure.h"
+#include "sparseblockilu.h"
 #include "columndiagstencil.h"
 
 #include <cmath>
@@ -131,6 +135,35 @@ static void dump(MeshAgent& MA, const ParamFile& pf, const std::vector<int>& col
     save_gv(p + "jac.npy", h);
     double nd[2] = {x.norm(), x * b};
     npy::save(p + "norm_dot.npy", nd, {2});
+
+    // ILU(0) in a given node order: factors and one solve of a seeded vector
+    if (n <= 1100) {
+      GlobalVector s0 = seeded(n, 5000 + l);
+      save_gv(p + "ilu_in.npy", s0);
+      for (int kind = 0; kind < 2; ++kind) {
+        nvector<int> perm(n);
+        for (int i = 0; i < n; ++i) perm[i] = kind == 0 ? i : n - 1 - i;
+        SparseBlockIlu<FMatrixBlock<1>> I;
+        I.ConstructStructure(perm, A);
+        I.zero();
+        I.copy_entries(A);
+        I.compute_ilu();
+        const ColumnDiagStencil* IS = dynamic_cast<const ColumnDiagStencil*>(I.GetStencil());
+        std::vector<int32_t> irp(IS->start().begin(), IS->start().end());
+        std::vector<int32_t> icol(IS->col().begin(), IS->col().end());
+        std::vector<int32_t> idiag(IS->diag().begin(), IS->diag().end());
+        std::vector<double> ival(I.size());
+        for (int pos = 0; pos < I.size(); ++pos) ival[pos] = (*I.mat(pos))(0, 0);
+        const std::string q = p + "ilu" + std::to_string(kind) + "_";
+        npy::save(q + "rowptr.npy", irp);
+        npy::save(q + "col.npy", icol);
+        npy::save(q + "diag.npy", idiag);
+        npy::save(q + "val.npy", ival);
+        GlobalVector s1(s0);
+        I.solve(s1);
+        save_gv(q + "solve.npy", s1);
+      }
+    }
 
     // hanging-node operators on a seeded vector
     if (disc.nhanging() > 0) {

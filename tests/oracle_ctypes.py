@@ -53,6 +53,11 @@ class OrNLInfo(C.Structure):
                 ("rate", C.c_double), ("lastrate", C.c_double)]
 
 
+class OrIlu(C.Structure):
+    _fields_ = [("n", C.c_int), ("ncomp", C.c_int), ("p", i32p), ("q", i32p),
+                ("rowptr", i32p), ("col", i32p), ("diag", i32p), ("val", f64p), ("y", f64p)]
+
+
 class OrLevel(C.Structure):
     _fields_ = [("A", OrMatrix), ("T", C.POINTER(OrTransfer)), ("hn", OrHN),
                 ("ndir", C.c_int), ("dir", i32p), ("ndircomp", C.c_int), ("dircomp", i32p),
@@ -127,6 +132,9 @@ def _load():
                                       C.POINTER(OrCGInfo), C.POINTER(OrNLInfo), C.c_int,
                                       C.POINTER(OrSolveReport)]),
         "or_num_threads": (C.c_int, []),
+        "or_ilu_init": (C.c_int, [C.POINTER(OrIlu), C.POINTER(OrMatrix), i32p]),
+        "or_ilu_solve": (None, [C.POINTER(OrIlu), f64p]),
+        "or_ilu_free": (None, [C.POINTER(OrIlu)]),
         "or_set_parallel_vectors": (None, [C.c_int]),
         "or_dJdk_cells": (None, [C.c_int, f64p, C.c_int64, i32p, f64p, f64p, f64p]),
         "or_vanka_create": (C.c_void_p, [C.POINTER(OrMatrix), C.c_int, C.c_int, i32p]),
@@ -186,6 +194,34 @@ class Matrix:
 
     def jacobi(self, h):
         lib().or_jacobi(C.byref(self.s), ptr(h))
+
+
+class Ilu:
+    """SparseBlockIlu in the node order perm (None: identity)"""
+
+    def __init__(self, A, perm=None):
+        self.A = A
+        self.perm = None if perm is None else np.ascontiguousarray(perm, dtype=np.int32)
+        self.s = OrIlu()
+        rc = lib().or_ilu_init(C.byref(self.s), C.byref(A.s),
+                               ptr(self.perm, i32p) if self.perm is not None else None)
+        if rc:
+            raise ValueError(f"or_ilu_init failed ({rc})")
+        n, B = A.n, A.ncomp
+        nnz = int(A.rowptr[-1])
+        self.rowptr = np.ctypeslib.as_array(self.s.rowptr, (n + 1,)).copy()
+        self.col = np.ctypeslib.as_array(self.s.col, (max(nnz, 1),))[:nnz].copy()
+        self.diag = np.ctypeslib.as_array(self.s.diag, (max(n, 1),))[:n].copy()
+        self.val = np.ctypeslib.as_array(self.s.val, (max(nnz, 1) * B * B,))[:nnz * B * B].copy()
+
+    def solve(self, x):
+        lib().or_ilu_solve(C.byref(self.s), ptr(x))
+
+    def __del__(self):
+        try:
+            lib().or_ilu_free(C.byref(self.s))
+        except Exception:
+            pass
 
 
 class Transfer:
