@@ -250,25 +250,13 @@ __global__ __launch_bounds__(kBlock) void k_bsr_jacobi(int64_t n, int B,
   }
 }
 
-// block-diagonal inverse by Gauss-Jordan with partial pivoting, one thread per
-// node: NodeMatrix::gauss_jordan (src/LinAlg/nodematrix.h:159-213), the device
-// counterpart of invert_device (cuda/cusparsehelper.cu:24-118).  B is a
-// template parameter so that every index is a constant and the block lives in
-// registers (B*B doubles per lane).
-template <int B, typename DT>
-__global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n,
-                                                          const int32_t* __restrict__ diagpos,
-                                                          const double* __restrict__ val,
-                                                          DT* __restrict__ dinv,
-                                                          int* __restrict__ singular) {
+// In place: m (B x B, row-major, in registers) becomes its inverse, by the
+// operations of NodeMatrix::gauss_jordan (src/LinAlg/nodematrix.h:159-213),
+// uncontracted.  Returns false for a zero pivot (m is then undefined).
+template <int B>
+__device__ __forceinline__ bool gauss_jordan(double (&m)[B * B]) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double m[B * B];
   int p[B];
-  const double* src = val + (int64_t)diagpos[i] * B * B;
-#pragma unroll
-  for (int k = 0; k < B * B; ++k) m[k] = src[k];
 #define M_(a, c) m[(a) * B + (c)]
 #pragma unroll
   for (int k = 0; k < B; ++k) p[k] = k;
@@ -297,10 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n,
         p[k] = t;
       }
     }
-    if (M_(j, j) == 0.0) {
-      atomicExch(singular, 1);
-      return;
-    }
+    if (M_(j, j) == 0.0) return false;
     double hr = 1. / M_(j, j);
     M_(j, j) = hr;
 #pragma unroll
@@ -320,12 +305,41 @@ __global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n,
     M_(j, j) = hr;
   }
   // column k of the eliminated matrix belongs to column p[k] of the inverse
-  DT* dst = dinv + i * B * B;
+  double t[B * B];
 #pragma unroll
   for (int a = 0; a < B; ++a)
 #pragma unroll
-    for (int k = 0; k < B; ++k) dst[a * B + p[k]] = (DT)M_(a, k);
+    for (int k = 0; k < B; ++k) t[a * B + p[k]] = M_(a, k);
+#pragma unroll
+  for (int e = 0; e < B * B; ++e) m[e] = t[e];
 #undef M_
+  return true;
+}
+
+// Inverse diagonal blocks (B x B, row-major, in FP64 or narrowed to DT) of a
+// node-block matrix: NodeMatrix::gauss_jordan on each block, the device
+// counterpart of invert_device (cuda/cusparsehelper.cu:24-118).  B is a
+// template parameter so that every index is a constant and the block lives in
+// registers (B*B doubles per lane).
+template <int B, typename DT>
+__global__ __launch_bounds__(kBlock) void k_block_inverse(int64_t n,
+                                                          const int32_t* __restrict__ diagpos,
+                                                          const double* __restrict__ val,
+                                                          DT* __restrict__ dinv,
+                                                          int* __restrict__ singular) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double m[B * B];
+  const double* src = val + (int64_t)diagpos[i] * B * B;
+#pragma unroll
+  for (int k = 0; k < B * B; ++k) m[k] = src[k];
+  if (!gauss_jordan<B>(m)) {
+    atomicExch(singular, 1);
+    return;
+  }
+  DT* dst = dinv + i * B * B;
+#pragma unroll
+  for (int k = 0; k < B * B; ++k) dst[k] = (DT)m[k];
 }
 
 // --------------------------------------------------------------- transfers

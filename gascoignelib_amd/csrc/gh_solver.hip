@@ -204,6 +204,7 @@ struct MgLevel {
   bool fused_correction = true;  // mask covers every component
   bool mask_in_restrict = true;  // coarse Dirichlet rows emptied in r_row (set on level l)
   gh_vanka* vanka = nullptr;     // patch smoother (smoother 2), owned
+  gh_ilu* ilu = nullptr;         // ILU(0) smoother (smoother 3), owned
   bool pre_fused = false;        // the first pre-smoothing sweep already ran (in the
                                  // restriction that produced b); consumed by mgstep
   void* u = nullptr;
@@ -318,15 +319,21 @@ int dense_inverse(const gh_csr* A, std::vector<double>& inv) {
 
 // ------------------------------------------------------------- V-cycle pieces
 int sweep(gh_mg* mg, MgLevel& L, int niter, bool& u_zero, void* keep_u) {
-  if (mg->P.smoother == 2) {
-    // StdSolver::smooth with the Vanka "ilu": h = b - A x; solve(h); x += omega h
-    if (niter > 0 && !L.vanka) return fail(GH_EINVAL, "vanka smoother: level without patches");
+  if (mg->P.smoother >= 2) {
+    // StdSolver::smooth with an "ilu" smoother (the Vanka patches or the
+    // ILU(0) factors): h = b - A x; solve(h); x += omega h
+    const bool vanka = mg->P.smoother == 2;
+    if (niter > 0 && vanka && !L.vanka) return fail(GH_EINVAL, "vanka smoother: level without patches");
+    if (niter > 0 && !vanka && !L.ilu) return fail(GH_EINVAL, "ilu smoother: level without factors");
     for (int it = 0; it < niter; ++it) {
       if (u_zero)
         GH_TRY(vec_convert(mg->ctx, L.size, mg->wprec, L.v, mg->wprec, L.b));
       else
         GH_TRY(csr_residual(mg->ctx, L.A, mg->wprec, L.v, L.u, L.b));
-      GH_TRY(vanka_solve(mg->ctx, L.vanka, mg->wprec, L.v));
+      if (vanka)
+        GH_TRY(vanka_solve(mg->ctx, L.vanka, mg->wprec, L.v));
+      else
+        GH_TRY(ilu_solve(mg->ctx, L.ilu, mg->wprec, L.v));
       if (u_zero)
         GH_TRY(vec_scale_to(mg->ctx, mg->wprec, L.size, L.u, mg->P.omega, L.v));
       else
@@ -445,7 +452,7 @@ int mgstep(gh_mg* mg, int l, int finelevel, int clevel, char& p0, char p, bool u
   // hanging nodes to distribute, Dirichlet rows emptied in R) and its first
   // pre-smoothing sweep starts from zero: that sweep runs in the restriction's
   // epilogue, writing the buffer the sweep would write.
-  const bool fuse = Cc.ncomp == 1 && mg->P.smoother != 2 && !Cc.H && !(Cc.M && !L.mask_in_restrict) &&
+  const bool fuse = Cc.ncomp == 1 && mg->P.smoother < 2 && !Cc.H && !(Cc.M && !L.mask_in_restrict) &&
                     l - 1 != clevel && mg->P.iterpre >= 1 &&
                     (p0 == 'V' || p0 == 'W' || p0 == 'F');
   if (Cc.ncomp == 1) {
@@ -531,7 +538,7 @@ int fast_vcycle(gh_mg* mg, int clevel, void* xdev, int xprec, const void* ydev, 
   // Every Jacobi / Richardson sweep of the finest level swaps its u and v
   // buffers, iterpre + iterpost times in all.  From a zero start the caller's
   // x may take the role that ends holding the result, so no copy follows.
-  const bool flip = direct_u && x_zero && mg->P.smoother != 2 && fl != clevel &&
+  const bool flip = direct_u && x_zero && mg->P.smoother < 2 && fl != clevel &&
                     ((mg->P.iterpre + mg->P.iterpost) & 1);
   if (flip)
     F.v = xdev;
@@ -743,8 +750,7 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
          "gh_mg_create: bad precision");
   GH_ARG(params->mgtype == 'V' || params->mgtype == 'W' || params->mgtype == 'F',
          "gh_mg_create: mgtype must be V, W or F");
-  GH_ARG(params->smoother == 0 || params->smoother == 1 || params->smoother == 2,
-         "gh_mg_create: bad smoother");
+  GH_ARG(params->smoother >= 0 && params->smoother <= 3, "gh_mg_create: bad smoother");
   GH_ARG(params->krylov == 0 || params->krylov == 1, "gh_mg_create: krylov must be 0 or 1");
   GH_ARG(params->adjoint == 0 || params->adjoint == 1, "gh_mg_create: adjoint must be 0 or 1");
   // shapes of the caller's objects
@@ -923,6 +929,7 @@ int gh_mg_destroy(gh_mg* mg) {
     if (L.own_M && L.M) gh_mask_destroy(L.M);
     if (L.d_perm) (void)hipFree(L.d_perm);
     if (L.vanka) gh_vanka_destroy(L.vanka);
+    if (L.ilu) gh_ilu_destroy(L.ilu);
     for (void* p : {(void*)L.r_row, (void*)L.r_col, (void*)L.p_row, (void*)L.p_col, L.u, L.b,
                     L.v, (void*)L.Ainv})
       if (p) (void)hipFree(p);
@@ -949,7 +956,7 @@ int gh_mg_destroy(gh_mg* mg) {
 int gh_mg_assemble_q1(gh_ctx* ctx, gh_mg* mg, gh_cells* const* C, gh_vec* const* k) {
   GH_ARG(ctx && mg && C, "gh_mg_assemble_q1: NULL argument");
   GH_ARG(!mg->P.adjoint, "gh_mg_assemble_q1: the adjoint solver's transposed levels are not assembled");
-  GH_ARG(mg->P.smoother != 2, "gh_mg_assemble_q1: Vanka patch factors are not re-formed");
+  GH_ARG(mg->P.smoother < 2, "gh_mg_assemble_q1: Vanka and ILU factors are not re-formed");
   for (int l = 0; l < mg->nlevels; ++l) {
     const MgLevel& L = mg->L[l];
     if (!C[l] || C[l]->n != L.n || L.ncomp != 1)
@@ -1007,6 +1014,34 @@ int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int
   GH_TRY(vanka_create(mg->ctx, L.A64, npatches, psize, internal.data(), mg->wprec, &V));
   if (L.vanka) gh_vanka_destroy(L.vanka);
   L.vanka = V;
+  if (mg->graph) (void)hipGraphExecDestroy(mg->graph);  // it holds the old smoother
+  mg->graph = nullptr;
+  mg->graph_ok = false;
+  return GH_OK;
+}
+
+int gh_mg_set_ilu(gh_mg* mg, int level, const int32_t* perm) {
+  GH_ARG(mg && level >= 0 && level < mg->nlevels, "gh_mg_set_ilu: bad arguments");
+  MgLevel& L = mg->L[level];
+  // ILU row i is the caller's node perm[i]: in the level's internal numbering
+  // the node at internal position Q[perm[i]]
+  std::vector<int32_t> internal(L.n);
+  std::vector<int32_t> P(L.n), Q(L.n);
+  if (L.d_perm) {
+    GH_HIP(hipMemcpy(P.data(), L.d_perm, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < L.n; ++k) Q[P[k]] = (int32_t)k;
+  } else {
+    for (int64_t k = 0; k < L.n; ++k) Q[k] = (int32_t)k;
+  }
+  for (int64_t i = 0; i < L.n; ++i) {
+    const int32_t v = perm ? perm[i] : (int32_t)i;
+    if (v < 0 || v >= L.n) return fail(GH_EINVAL, "gh_mg_set_ilu: node out of range");
+    internal[i] = Q[v];
+  }
+  gh_ilu* I = nullptr;
+  GH_TRY(gh_ilu_create(mg->ctx, L.A64, internal.data(), &I));
+  if (L.ilu) gh_ilu_destroy(L.ilu);
+  L.ilu = I;
   if (mg->graph) (void)hipGraphExecDestroy(mg->graph);  // it holds the old smoother
   mg->graph = nullptr;
   mg->graph_ok = false;

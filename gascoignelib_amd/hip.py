@@ -344,8 +344,10 @@ class CudaMultiLevelSolver:
 
     Parameters follow the reference's MultiLevelSolver / Solver blocks:
     mgtype, iterpre, iterpost, iterexact, omega, mgomega, linear_smooth
-    ("jacobi", "richardson", or "vanka" with `patches`: per level an
-    (npatches, psize) node array or None),
+    ("jacobi", "richardson", "vanka" with `patches`: per level an
+    (npatches, psize) node array or None, or "ilu": ILU(0) factors on every
+    level, ILU row i = node ilu_orders[l][i], the identity without
+    `ilu_orders`),
     ndirect, prec_maxiter; `prec` selects the V-cycle storage ("f64", "f32"
     or "f16" matrix values with FP32 arithmetic); `krylov` the outer solver of
     Gmres/Solve: "gmres" (left-preconditioned, as GMRES::solve) or "fgmres"
@@ -354,7 +356,8 @@ class CudaMultiLevelSolver:
 
     def __init__(self, ctx, hier, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
                  mgomega=1.0, linear_smooth="jacobi", ndirect=1, prec_maxiter=1, prec="f64",
-                 use_graph=False, order=None, krylov="gmres", adjoint=False, patches=None):
+                 use_graph=False, order=None, krylov="gmres", adjoint=False, patches=None,
+                 ilu_orders=None):
         """order: None (keep the hierarchy's numbering), "lexicographic" (sort
         each level's nodes by their coordinates) or a list of per-level
         permutations (internal position -> node)."""
@@ -390,7 +393,7 @@ class CudaMultiLevelSolver:
         p.mgtype = mgtype.encode()
         p.iterpre, p.iterpost, p.iterexact = iterpre, iterpost, iterexact
         p.omega, p.mgomega = omega, mgomega
-        p.smoother = {"jacobi": 0, "richardson": 1, "vanka": 2}[linear_smooth]
+        p.smoother = {"jacobi": 0, "richardson": 1, "vanka": 2, "ilu": 3}[linear_smooth]
         p.ndirect = ndirect
         p.prec_maxiter = prec_maxiter
         p.prec = N.PREC[prec]
@@ -429,6 +432,12 @@ class CudaMultiLevelSolver:
                 pt = np.ascontiguousarray(pt, dtype=np.int32)
                 check(N.hip().gh_mg_set_vanka(self.h, l, pt.shape[0], pt.shape[1],
                                               ptr(pt, N.i32p)))
+        if linear_smooth == "ilu":
+            for l in range(nl):
+                po = None
+                if ilu_orders is not None and ilu_orders[l] is not None:
+                    po = np.ascontiguousarray(ilu_orders[l], dtype=np.int32)
+                check(N.hip().gh_mg_set_ilu(self.h, l, None if po is None else ptr(po, N.i32p)))
 
     @property
     def fine_matrix(self):
@@ -548,6 +557,45 @@ class CoefficientCells:
         try:
             if self.h:
                 N.hip().gh_cells_destroy(self.h)
+        except Exception:
+            pass
+
+
+class SparseBlockIlu:
+    """SparseBlockIlu: ILU(0) factors of an FP64 matrix, ILU row i = node
+    perm[i]; solve(x) in place (FP64 or FP32 vector)."""
+
+    def __init__(self, ctx, A, perm=None):
+        self.ctx = ctx
+        self._perm = None if perm is None else np.ascontiguousarray(perm, dtype=np.int32)
+        self.h = C.c_void_p()
+        self.n, self.ncomp = A._n, A.n_comp
+        check(N.hip().gh_ilu_create(ctx.h, A.h, None if self._perm is None else
+                                    ptr(self._perm, N.i32p), C.byref(self.h)))
+
+    def solve(self, x):
+        check(N.hip().gh_ilu_solve(self.ctx.h, self.h, x.h))
+
+    def levels(self):
+        f, b = C.c_int32(), C.c_int32()
+        check(N.hip().gh_ilu_levels(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def factors(self, nnz):
+        """(rowptr, col, diag, val) of the factors in ILU numbering."""
+        B = self.ncomp
+        rp = np.empty(self.n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        dg = np.empty(self.n, np.int32)
+        val = np.empty(nnz * B * B)
+        check(N.hip().gh_ilu_download(self.ctx.h, self.h, ptr(rp, N.i32p), ptr(col, N.i32p),
+                                      ptr(dg, N.i32p), ptr(val)))
+        return rp, col, dg, val
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.hip().gh_ilu_destroy(self.h)
         except Exception:
             pass
 

@@ -141,6 +141,12 @@ HIP_SIGNATURES = {
     "gh_vanka_destroy": (C.c_int, [vp]),
     "gh_vanka_solve": (C.c_int, [vp, vp, vp]),
     "gh_mg_set_vanka": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int, i32p]),
+    "gh_ilu_create": (C.c_int, [vp, vp, i32p, P(vp)]),
+    "gh_ilu_destroy": (C.c_int, [vp]),
+    "gh_ilu_solve": (C.c_int, [vp, vp, vp]),
+    "gh_ilu_levels": (C.c_int, [vp, i32p, i32p]),
+    "gh_ilu_download": (C.c_int, [vp, vp, i32p, i32p, i32p, f64p]),
+    "gh_mg_set_ilu": (C.c_int, [vp, C.c_int, i32p]),
     "gh_cells_create": (C.c_int, [vp, C.c_int, C.c_int64, f64p, C.c_int64, i32p, P(vp)]),
     "gh_cells_destroy": (C.c_int, [vp]),
     "gh_dJdk_cells": (C.c_int, [vp, vp, vp, vp, vp]),

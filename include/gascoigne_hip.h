@@ -47,6 +47,7 @@ typedef struct gh_mask gh_mask;
 typedef struct gh_mg gh_mg;
 typedef struct gh_cells gh_cells;
 typedef struct gh_vanka gh_vanka;
+typedef struct gh_ilu gh_ilu;
 
 const char* gh_last_error(void);
 const char* gh_version(void);
@@ -178,6 +179,29 @@ int gh_vanka_destroy(gh_vanka* V);
  * patches of a node summed in ascending order.  x in the factors' precision. */
 int gh_vanka_solve(gh_ctx* ctx, const gh_vanka* V, gh_vec* x);
 
+/* ------------------------------------------------------------------ ILU(0) */
+/* SparseBlockIlu (src/LinAlg/sparseblockilu.xx), the CPU path's default
+ * smoother "ilu" (src/Solver/solverdata.cc:55).  ConstructStructure +
+ * copy_entries (:70-166): ILU row i is node perm[i] (NULL: the identity), its
+ * columns the matrix row's columns in the inverse order, sorted ascending;
+ * compute_ilu (:240-275) on the device, rows of one level of the lower part's
+ * dependency graph at a time, each row's operations in the reference's order
+ * (the factors equal the CPU factors bit for bit); the diagonal blocks hold
+ * their inverses.  A is FP64 with blocks of 1..4 components and its values
+ * from creation (not device-assembled).  GH_ESINGULAR on a zero pivot. */
+int gh_ilu_create(gh_ctx* ctx, const gh_csr* A, const int32_t* perm, gh_ilu** out);
+int gh_ilu_destroy(gh_ilu* I);
+/* solve (:226-236), in place: x <- (LU)^-1 x through the order perm, FP64 x
+ * bit for bit as the CPU ILU, FP32 x on an FP32 copy of the factors.  The
+ * triangular solves run level by level. */
+int gh_ilu_solve(gh_ctx* ctx, const gh_ilu* I, gh_vec* x);
+/* Level counts of the forward (lower) and backward (upper) schedules. */
+int gh_ilu_levels(const gh_ilu* I, int32_t* forward, int32_t* backward);
+/* The factors in ILU numbering (n+1 rowptr, nnz col, n diagonal positions,
+ * nnz*B*B values, row-major blocks); any pointer may be NULL. */
+int gh_ilu_download(gh_ctx* ctx, const gh_ilu* I, int32_t* rowptr, int32_t* col, int32_t* diag,
+                    double* val);
+
 /* ---------------------------------------------------------------- transfer */
 /* CudaMgInterpolatorNested::BasicInit / check_matrices
  * (cuda/cudamgintepolatornested.cc:27-110) and the CPU MgInterpolatorMatrix
@@ -254,7 +278,9 @@ typedef struct {
   double omega, mgomega;
   int smoother;         /* 0 jacobi, 1 richardson, 2 vanka ("ilu" with the Vanka
                            patch smoother: h = b - A x; Vanka solve of h;
-                           x += omega h; patches from gh_mg_set_vanka) */
+                           x += omega h; patches from gh_mg_set_vanka),
+                           3 ilu (the same with SparseBlockIlu, the reference's
+                           default smoother; factors from gh_mg_set_ilu) */
   int ndirect;          /* levels with fewer nodes get a dense FP64 direct solve */
   int prec_maxiter;
   double prec_tol, prec_globaltol;
@@ -311,6 +337,11 @@ int gh_mg_create_ordered(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer
  * smoother 2; built on the solver's FP64 level matrix, factors in the V-cycle's
  * precision.  Levels solved directly need none. */
 int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int32_t* nodes);
+/* The ILU(0) factors of one level for smoother 3, ILU row i = the caller's
+ * node perm[i] (NULL: the caller's numbering), built on the solver's FP64 level
+ * matrix (StdSolver::ComputeIlu, src/Solver/stdsolver.cc:1547-1573, without
+ * the optional modify_ilu).  Levels solved directly need none. */
+int gh_mg_set_ilu(gh_mg* mg, int level, const int32_t* perm);
 int gh_mg_destroy(gh_mg* mg);
 int gh_mg_nlevels(const gh_mg* mg);
 int64_t gh_mg_mgsteps(const gh_mg* mg);  /* the "--> mgstep" counter */

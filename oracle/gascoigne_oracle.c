@@ -1013,6 +1013,10 @@ static void smooth(or_mg* M, int l, int niter, double* x, const double* y, doubl
       or_residual(A, h, x, y);
       or_vanka_solve((const or_vanka*)M->L[l].vanka, h);
       or_add(x, M->omega, h, n);
+    } else if (M->smoother == 4) {
+      or_residual(A, h, x, y);
+      or_ilu_solve((const or_ilu*)M->L[l].ilu, h);
+      or_add(x, M->omega, h, n);
     }
   }
 }

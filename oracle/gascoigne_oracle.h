@@ -166,6 +166,7 @@ typedef struct {
   double* lu;       /* dense LU of A (direct levels) */
   int32_t* piv;
   void* vanka;      /* or_vanka* of the level (smoother 3), not owned */
+  void* ilu;        /* or_ilu* of the level (smoother 4), not owned */
 } or_level;
 
 typedef struct {
@@ -174,7 +175,8 @@ typedef struct {
   char mgtype;      /* 'V','W','F' */
   int iterpre, iterpost, iterexact;
   double omega, mgomega;
-  int smoother;     /* 0 jacobi, 1 richardson, 2 none, 3 vanka ("ilu" = VankaSmoother) */
+  int smoother;     /* 0 jacobi, 1 richardson, 2 none, 3 vanka ("ilu" = VankaSmoother),
+                       4 ilu (SparseBlockIlu) */
   int prec_maxiter;
   double prec_tol, prec_globaltol;
   int64_t mgsteps;  /* counter ("--> mgstep") */

@@ -61,7 +61,8 @@ class OrIlu(C.Structure):
 class OrLevel(C.Structure):
     _fields_ = [("A", OrMatrix), ("T", C.POINTER(OrTransfer)), ("hn", OrHN),
                 ("ndir", C.c_int), ("dir", i32p), ("ndircomp", C.c_int), ("dircomp", i32p),
-                ("direct", C.c_int), ("lu", f64p), ("piv", i32p), ("vanka", C.c_void_p)]
+                ("direct", C.c_int), ("lu", f64p), ("piv", i32p), ("vanka", C.c_void_p),
+                ("ilu", C.c_void_p)]
 
 
 class OrMG(C.Structure):
@@ -293,7 +294,7 @@ class MG:
     object (solver numbering: level 0 = coarsest)."""
 
     def __init__(self, hier, *, mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8,
-                 mgomega=1.0, smoother="jacobi", ndirect=1, patches=None):
+                 mgomega=1.0, smoother="jacobi", ndirect=1, patches=None, ilu_orders=None):
         nl = hier.nlevels
         self.hier = hier
         self.p = lib().or_mg_create(nl)
@@ -301,8 +302,9 @@ class MG:
         M.mgtype = mgtype.encode()
         M.iterpre, M.iterpost, M.iterexact = iterpre, iterpost, iterexact
         M.omega, M.mgomega = omega, mgomega
-        M.smoother = {"jacobi": 0, "richardson": 1, "none": 2, "vanka": 3}[smoother]
+        M.smoother = {"jacobi": 0, "richardson": 1, "none": 2, "vanka": 3, "ilu": 4}[smoother]
         self.vankas = []
+        self.ilus = []
         self.keep = []
         self.mats, self.transfers, self.hns = [], [], []
         for l in range(nl):
@@ -328,6 +330,10 @@ class MG:
                 V = Vanka(A, patches[l])
                 self.vankas.append(V)
                 L.vanka = V.p
+            if smoother == "ilu":
+                I = Ilu(A, None if ilu_orders is None else ilu_orders[l])
+                self.ilus.append(I)
+                L.ilu = C.cast(C.pointer(I.s), C.c_void_p)
         if lib().or_mg_setup(self.p):
             raise RuntimeError("or_mg_setup failed")
 
