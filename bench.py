@@ -101,6 +101,9 @@ SMOOTH = {
     "stokes64": ("2,2", "2,2"),
     "q2_64": ("3,2", "2,2"),
     "q2ada": ("3,3", "2,2"),
+    # ILU(0) smoother, omega 1: V(1,1) is the fastest of V(1,1), V(2,2) and
+    # V(4,4) (the reference's default counts, timed beside) on both paths
+    "ilu128": ("1,1", "1,1"),
 }
 
 
@@ -138,7 +141,7 @@ def main():
     ap.add_argument("--order", default="lexicographic", choices=["lexicographic", "none"],
                     help="node order inside the solver (the caller's vectors keep theirs)")
     ap.add_argument("--workload", default="poisson128",
-                    choices=["poisson128", "adjoint96", "stokes64", "q2_64", "q2ada"],
+                    choices=["poisson128", "adjoint96", "stokes64", "q2_64", "q2ada", "ilu128"],
                     help="poisson128: 3D Q1 Poisson on 128^3 cells (the headline); adjoint96: "
                          "variable-coefficient diffusion on 96^3 cells, one gradient evaluation "
                          "(forward solve + adjoint solve + dJ/dk per cell) per step; stokes64: "
@@ -146,7 +149,9 @@ def main():
                          "Poisson with the triquadratic patch element on 64^3 cells (125-point "
                          "rows), forward and adjoint solves; q2ada: the same on 64^3 cells "
                          "(32^3 triquadratic elements) plus 2 adaptive levels inside a seeded "
-                         "ball of radius 0.25, with hanging-node constraints (1,432,971 DOF)")
+                         "ball of radius 0.25, with hanging-node constraints (1,432,971 DOF); "
+                         "ilu128: the poisson128 problem with the ILU(0) smoother (the CPU "
+                         "path's default) in each level's colour order")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the roofline kernel (FP64 residual SpMV, then FP16/FP32) "
                          "and a plain copy of known size, for counter runs under rocprofv3")
@@ -169,6 +174,8 @@ def main():
         return bench_q2(args)
     if args.workload == "q2ada":
         return bench_q2(args, adaptive=2)
+    if args.workload == "ilu128":
+        return bench_ilu(args)
 
     from gascoignelib_amd import hip as H
     from gascoignelib_amd.problem import lattice_rhs
@@ -916,6 +923,185 @@ def bench_q2(args, adaptive=0):
         "setup_s": {"structure": t_struct, "device": t_setup},
     }
     print(json.dumps(line), flush=True)
+
+
+def ilu_bytes(n, nnz, sv, sx):
+    """bytes of one ILU(0) solve in the sliced layout: the off-diagonal factor
+    values and their 32-bit columns, the diagonal inverses, per slot a row
+    index and an entry count in each triangle, and the vectors: x read, y
+    written (forward), y read, the coupled y values once per sweep, y and x
+    written (backward)"""
+    return (nnz - n) * (sv + 4) + n * sv + n * 16 + n * sx * 7
+
+
+def bench_ilu(args):
+    """The headline problem (3D Q1 Poisson, 128^3 cells, 2,146,689 DOF) solved
+    with the ILU(0) smoother (SparseBlockIlu, src/LinAlg/sparseblockilu.xx),
+    the CPU path's default linear_smooth, omega = 1 (its default): h = b - A x;
+    ILU solve of h; x += h.  Each level's ILU order is the greedy colour order
+    of its matrix graph (ghs_colour_order, 8 colours: 8 forward and 8 backward
+    level sweeps); the reference's default order comes from METIS, which is not
+    in this image.  One step = one solve to 1e-10: FP64 (Newton + GMRES(10),
+    FP64 V-cycle and factors) and mixed (f16-value residuals, FP32 factors and
+    vectors in the V-cycle, FP64 flexible GMRES(20)).  Also measured: one ILU
+    solve on the finest level alone (HIP events), the factorisation, and on the
+    host the oracle's FP64 ILU factorisation, ILU solve and a whole FP64 solve
+    (cpu_baseline)."""
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd.problem import lattice_rhs
+    from gascoignelib_amd.structure import colour_order, uniform_hierarchy
+
+    pre = int(round(np.log2(args.cells)))
+    t0 = time.perf_counter()
+    hier = uniform_hierarchy("cube.inp", 3, pre, drop_coarse=1)
+    t_struct = time.perf_counter() - t0
+    fine = hier.fine
+    n = fine.n
+    f = lattice_rhs(fine.coords, seed=2024)
+    f[fine.dirichlet] = 0.0
+    t0 = time.perf_counter()
+    orders = [colour_order(hier.level(l).rowptr, hier.level(l).col) for l in range(hier.nlevels)]
+    t_order = time.perf_counter() - t0
+    io = [o for o, _ in orders]
+    print(f"# ilu {args.cells}^3: {n} DOF, nnz {fine.nnz}, {hier.nlevels} levels, colours "
+          f"{[c for _, c in orders]}", file=sys.stderr, flush=True)
+    ctx = H.Context(0)
+    common = dict(mgtype="V", iterexact=10, omega=1.0, mgomega=1.0, ndirect=1000,
+                  linear_smooth="ilu", ilu_orders=io,
+                  order=None if args.order == "none" else args.order)
+    F = H.CudaVectorInterface.from_global(ctx, f)
+
+    def run(prec, krylov, sm, steps, warmup, label):
+        t0 = time.perf_counter()
+        mg = H.CudaMultiLevelSolver(ctx, hier, prec=prec, krylov=krylov, iterpre=sm[0],
+                                    iterpost=sm[1], **common)
+        t_setup = time.perf_counter() - t0
+        U = mg.new_vector()
+        tol = 1e-10 if krylov == "gmres" else args.mixed_linear_tol
+        ts, rep = [], None
+        for k in range(warmup + steps):
+            U.zero()
+            ctx.sync()
+            t = time.perf_counter()
+            rep = mg.Solve(U, F, H.CGInfo(tol, 1e-30, 100), H.NLInfo(1e-10, 1e-30, 50),
+                           vmax=10 if krylov == "gmres" else args.mixed_vmax, norm_kind=1)
+            ctx.sync()
+            if k >= warmup:
+                ts.append(time.perf_counter() - t)
+                print(f"# {label} step {k - warmup}: {ts[-1] * 1e3:.1f} ms, gmres "
+                      f"{rep.gmres_iterations}", file=sys.stderr, flush=True)
+            if H.N.NL_STATUS[rep.status] != "converged":
+                raise RuntimeError(f"{label}: {H.N.NL_STATUS[rep.status]}")
+        A64 = mg.fine_matrix
+        R = mg.new_vector()
+        A64.residual(R, U, F)
+        if mg.fine_mask:
+            mg.fine_mask.zero(R)
+        out = {"ms_per_solve": float(np.median(ts)) * 1e3, "gmres": rep.gmres_iterations,
+               "newton": rep.newton_iterations, "true_rel_residual": R.norm() / F.norm(),
+               "setup_s": t_setup, "cycle": "V(%d,%d)" % sm, "u": U.copy_back()}
+        del mg
+        return out
+
+    r64 = run("f64", "gmres", args.fp64_smooth, args.steps, args.warmup, "fp64")
+    rmx = run(args.mixed, "fgmres", args.mixed_smooth, args.steps, args.warmup, "mixed")
+    r64f = run("f64", "fgmres", args.mixed_smooth, args.steps, args.warmup, "fp64 fgmres")
+    ref = {p: run(prec, kr, (4, 4), max(1, args.steps // 2), 1, f"{p} V(4,4)")
+           for p, prec, kr in (("fp64", "f64", "gmres"), ("mixed", args.mixed, "fgmres"))}
+    err_mx = float(np.linalg.norm(rmx["u"] - r64["u"]) / np.linalg.norm(r64["u"]))
+
+    # the ILU(0) solve alone on the finest level, colour order
+    A = H.CudaCSRMatrixInterface(ctx, fine.rowptr, fine.col, fine.val)
+    ctx.sync()
+    t0 = time.perf_counter()
+    I = H.SparseBlockIlu(ctx, A, io[-1])
+    ctx.sync()
+    t_factor = time.perf_counter() - t0
+    X = H.CudaVectorInterface.from_global(ctx, f)
+    X32 = H.CudaVectorInterface.from_global(ctx, f, 1, "f32")
+    t_ilu = time_kernel(ctx, lambda: I.solve(X), args.spmv_reps)
+    t_ilu32 = time_kernel(ctx, lambda: I.solve(X32), args.spmv_reps)
+    levels = I.levels()
+    del I
+    b64, b32 = ilu_bytes(n, fine.nnz, 8, 8), ilu_bytes(n, fine.nnz, 4, 4)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_ilu(hier, f, io, args.fp64_smooth)
+
+    gbs = b64 / t_ilu / 1e9
+    line = {
+        "metric": "DOF/s of a Q1 Poisson solve to 1e-10 rel residual with the ILU(0) smoother "
+                  "(mixed vs FP64 MG-GMRES)",
+        "value": n / (rmx["ms_per_solve"] * 1e-3), "unit": "DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": rmx["ms_per_solve"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"f64 outer / f32 V-cycle and ILU factors ({args.mixed}-stored matrix values)",
+        "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, Dirichlet 0",
+        "config": {"workload": f"3D Q1 Poisson, unit cube, {args.cells}^3 cells, ILU(0) smoother "
+                               "in the colour order of each level",
+                   "dof": n, "nnz": fine.nnz, "levels": hier.nlevels,
+                   "colours": [c for _, c in orders], "omega": 1.0},
+        "fp64": {k: v for k, v in r64.items() if k != "u"},
+        "mixed": dict({k: v for k, v in rmx.items() if k != "u"}, rel_error_vs_fp64=err_mx),
+        "fp64_same_stop": {k: v for k, v in r64f.items() if k != "u"},
+        "speedup_mixed_vs_fp64": r64["ms_per_solve"] / rmx["ms_per_solve"],
+        "speedup_mixed_vs_fp64_same_stop": r64f["ms_per_solve"] / rmx["ms_per_solve"],
+        "reference_cycle": {p: {k: v for k, v in r.items() if k != "u"} for p, r in ref.items()},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one FP64 ILU(0) solve on the finest level (16 level sweeps of "
+                               "k_ilu_sweep, HIP events around the whole solve)",
+                     "bytes_per_launch": b64, "us_per_launch": t_ilu * 1e6,
+                     "levels": {"forward": levels[0], "backward": levels[1]}},
+        "ilu_fp32": {"us_per_solve": t_ilu32 * 1e6, "bytes": b32,
+                     "gbs": b32 / t_ilu32 / 1e9},
+        "setup_s": {"structure": t_struct, "colour_orders": t_order,
+                    "ilu_factor_finest_device": t_factor},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_ilu(hier, f, io, sm, reps=2):
+    """The oracle on the host: the FP64 ILU(0) factorisation and one ILU solve
+    of the finest level in the colour order (serial, as SparseBlockIlu), and
+    whole FP64 Newton/GMRES/MG solves with the ILU smoother (reference
+    threading: OpenMP over the matrix-vector products and grid transfers)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as O
+    fine = hier.fine
+    t = time.perf_counter()
+    I = O.Ilu(O.Matrix(fine.rowptr, fine.col, fine.val), io[-1])
+    t_factor = time.perf_counter() - t
+    x = f.copy()
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        I.solve(x)
+        ts.append(time.perf_counter() - t)
+    del I
+    O.lib().or_set_parallel_vectors(0)
+    t = time.perf_counter()
+    om = O.MG(hier, mgtype="V", iterpre=sm[0], iterpost=sm[1], iterexact=10, omega=1.0,
+              mgomega=1.0, ndirect=1000, smoother="ilu", ilu_orders=io)
+    t_setup = time.perf_counter() - t
+    tsol = []
+    for _ in range(reps):
+        u = np.zeros_like(f)
+        t = time.perf_counter()
+        rc, rep = om.newton(u, f, O.cginfo(1e-10, 1e-30, 100), O.nlinfo(1e-10, 1e-30, 50),
+                            norm_kind=1)
+        tsol.append(time.perf_counter() - t)
+    dt = float(np.median(tsol))
+    return {"value": len(f) / dt, "unit": "DOF/s", "cores": int(O.lib().or_num_threads()),
+            "kind": "port",
+            "sample": f"full FP64 solves with the ILU smoother V({sm[0]},{sm[1]}) of the same "
+                      f"{len(f)}-DOF problem, median of {reps} ({dt:.1f} s each, gmres "
+                      f"{rep.gmres_iterations}); reference threading",
+            "ilu_factor_finest_s": t_factor, "ilu_solve_finest_ms": float(np.median(ts)) * 1e3,
+            "mg_setup_s": t_setup, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def cpu_model():

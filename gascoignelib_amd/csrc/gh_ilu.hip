@@ -10,11 +10,19 @@
 // level scheduling: row i of the forward DAG waits only for the rows its lower
 // part references, so the rows of one level (1 + the largest level they
 // reference) are independent and run as one launch; the backward solve uses
-// the levels of the upper part (level 0: rows without upper entries).  Each row's operations run
-// in one thread in the reference's order, every product and difference
-// rounded separately, so the factors and the FP64 solves equal the CPU ILU's
-// bit for bit.  FP32 solves (the reduced-precision V-cycle) use an FP32 copy
-// of the factors.
+// the levels of the upper part (level 0: rows without upper entries).  Each
+// row's operations run in one thread in the reference's order, every product
+// and difference rounded separately, so the factors and the FP64 solves equal
+// the CPU ILU's bit for bit.
+//
+// The factorisation runs on the factors in CSR form (one thread walks its
+// row).  The solves run on a sliced copy of each triangle: the rows of a level
+// in slices of 64 (a level starts a new slice), entry j of the 64 rows of a
+// slice stored side by side, the lower part in ascending and the upper part in
+// descending column order (the order the solves take them), so a wavefront's
+// loads of one step are contiguous.  The diagonal inverses are a separate
+// array indexed by ILU row.  FP32 solves (the reduced-precision V-cycle) use
+// FP32 copies of the sliced values.
 #include <algorithm>
 #include <functional>
 #include <numeric>
@@ -23,6 +31,21 @@
 #include "gh_kernels.h"
 
 using namespace gh;
+
+namespace gh {
+// one triangle of the factors in slices of 64 rows, levels aligned to slices
+struct IluSlices {
+  int64_t nslices = 0, nentries = 0;
+  std::vector<int64_t> lslice;  // level L covers slices [lslice[L], lslice[L+1]) (host)
+  int32_t* row = nullptr;       // nslices*64 ILU rows, -1 in padding
+  int32_t* len = nullptr;       // nslices*64 entry counts
+  int64_t* base = nullptr;      // nslices first entry
+  int32_t* col = nullptr;       // nentries
+  int32_t* map = nullptr;       // nentries: CSR block position, -1 in padding
+  double* v64 = nullptr;        // nentries*B*B
+  float* v32 = nullptr;
+};
+}  // namespace gh
 
 struct gh_ilu {
   gh_ctx* ctx = nullptr;
@@ -34,10 +57,11 @@ struct gh_ilu {
   int32_t* col = nullptr;
   int32_t* diag = nullptr;
   double* val = nullptr;      // factors, nnz blocks of B*B (row-major), FP64
-  float* val32 = nullptr;     // the same in FP32
+  double* dinv64 = nullptr;   // n diagonal inverses (from val) by ILU row
+  float* dinv32 = nullptr;
+  gh::IluSlices lo, up;       // sliced lower (forward) and upper (backward) parts
   void* y = nullptr;          // work vector, n*B doubles (or floats)
-  int32_t* rows_f = nullptr;  // rows grouped by forward level
-  int32_t* rows_b = nullptr;  // rows grouped by backward level
+  int32_t* rows_f = nullptr;  // rows grouped by forward level (factorisation)
   std::vector<int32_t> lptr_f, lptr_b;  // level boundaries (host)
   int* d_flag = nullptr;
 };
@@ -110,17 +134,6 @@ __global__ __launch_bounds__(kBlock) void k_ilu_factor(int32_t cnt, const int32_
   for (int x = 0; x < BB; ++x) Dm[x] = d[x];
 }
 
-// y_i = x_{order[i]} (hin: yp(i) = x(p[i]); her: x(i) = yp(q[i]))
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ilu_gather(int64_t n, int B, const int32_t* __restrict__ order,
-                                                       const T* __restrict__ x, T* __restrict__ y) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n * B) return;
-  const int64_t i = t / B;
-  const int c = (int)(t - i * B);
-  y[t] = x[(int64_t)order[i] * B + c];
-}
-
 template <int B, typename T, typename VT>
 __device__ __forceinline__ void block_subtract(T* yi, const VT* M, const T* yc) {
   // FMatrixBlock::subtract (fmatrixblock.h:136-146): y_i -= M y_c, row by row
@@ -133,57 +146,103 @@ __device__ __forceinline__ void block_subtract(T* yi, const VT* M, const T* yc) 
   }
 }
 
-// forward (:193-203) for the rows of one level
-template <int B, typename T, typename VT>
-__global__ __launch_bounds__(kBlock) void k_ilu_forward(int32_t cnt, const int32_t* __restrict__ rows,
-                                                        const int32_t* __restrict__ rowptr,
-                                                        const int32_t* __restrict__ col,
-                                                        const int32_t* __restrict__ diag,
-                                                        const VT* __restrict__ val, T* __restrict__ y) {
-  const int32_t t = (int32_t)(blockIdx.x * kBlock + threadIdx.x);
-  if (t >= cnt) return;
-  const int32_t i = rows[t];
+// forward (:193-203) or backward (:207-222) for the slices [s0, s1) of one
+// level: y_i -= a(i,c) y_c over the row's entries in the sliced order, then,
+// backward, y_i = a(i,i)^-1 y_i (FMatrixBlock::vmult).  The permutations of
+// solve (:226-236) are folded in: the forward pass starts row i from
+// xin[perm[i]] (hin), the backward pass also writes its result to
+// xout[perm[i]] (her), and the smoother's update x += omega h
+// (StdSolver::smooth, src/Solver/stdsolver.cc:962-967) can follow there:
+// mode 0 xout = h, 1 xout += omega h, 2 xout = omega h.
+template <int B, typename T, typename VT, bool kUpper>
+__global__ __launch_bounds__(kBlock) void k_ilu_sweep(int64_t s0, int64_t s1,
+                                                      const int32_t* __restrict__ row,
+                                                      const int32_t* __restrict__ len,
+                                                      const int64_t* __restrict__ base,
+                                                      const int32_t* __restrict__ col,
+                                                      const VT* __restrict__ val,
+                                                      const VT* __restrict__ dinv,
+                                                      const int32_t* __restrict__ perm,
+                                                      const T* xin, T* __restrict__ y, T* xout,
+                                                      int mode, T omega) {
+  constexpr int BB = B * B;
+  // entries whose loads are issued before their updates (fewer for larger
+  // blocks, whose values take B*B registers each)
+  constexpr int kGroup = B == 1 ? 8 : (B == 2 ? 4 : 2);
+  const int64_t s = s0 + (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (s >= s1) return;
+  const int lane = threadIdx.x & 63;
+  const int32_t i = row[s * 64 + lane];
+  if (i < 0) return;
+  const int32_t cnt = len[s * 64 + lane];
+  const int64_t b = base[s] + lane;
   T yi[B];
+  if (kUpper) {
 #pragma unroll
-  for (int r = 0; r < B; ++r) yi[r] = y[(int64_t)i * B + r];
-  for (int32_t pos = rowptr[i]; pos < diag[i]; ++pos) {
+    for (int r = 0; r < B; ++r) yi[r] = y[(int64_t)i * B + r];
+  } else {
+    const int64_t src = (int64_t)perm[i] * B;
+#pragma unroll
+    for (int r = 0; r < B; ++r) yi[r] = xin[src + r];
+  }
+  int32_t j = 0;
+  for (; j + kGroup <= cnt; j += kGroup) {
+    T yc[kGroup][B];
+    VT m[kGroup][BB];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) {
+      const int64_t e = b + (int64_t)(j + g) * 64;
+      const int32_t c = col[e];
+#pragma unroll
+      for (int r = 0; r < B; ++r) yc[g][r] = y[(int64_t)c * B + r];
+#pragma unroll
+      for (int x = 0; x < BB; ++x) m[g][x] = val[e * BB + x];
+    }
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) block_subtract<B>(yi, m[g], yc[g]);
+  }
+  for (; j < cnt; ++j) {
+    const int64_t e = b + (int64_t)j * 64;
+    const int32_t c = col[e];
     T yc[B];
 #pragma unroll
-    for (int c = 0; c < B; ++c) yc[c] = y[(int64_t)col[pos] * B + c];
-    block_subtract<B>(yi, val + (int64_t)pos * B * B, yc);
+    for (int r = 0; r < B; ++r) yc[r] = y[(int64_t)c * B + r];
+    block_subtract<B>(yi, val + e * BB, yc);
   }
+  if (kUpper) {
+    const VT* D = dinv + (int64_t)i * BB;
+    const int64_t dst = (int64_t)perm[i] * B;
 #pragma unroll
-  for (int r = 0; r < B; ++r) y[(int64_t)i * B + r] = yi[r];
+    for (int r = 0; r < B; ++r) {
+      T a = (T)0;
+#pragma unroll
+      for (int c = 0; c < B; ++c) a = add_rn(a, mul_rn((T)D[r * B + c], yi[c]));
+      y[(int64_t)i * B + r] = a;
+      if (mode == 0)
+        xout[dst + r] = a;
+      else if (mode == 1)
+        xout[dst + r] = add_rn(xout[dst + r], mul_rn(omega, a));
+      else
+        xout[dst + r] = mul_rn(omega, a);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < B; ++r) y[(int64_t)i * B + r] = yi[r];
+  }
 }
 
-// backward (:207-222) for the rows of one level: the upper entries from the
-// last to the first, then y_i = a(i,i)^-1 y_i (FMatrixBlock::vmult)
-template <int B, typename T, typename VT>
-__global__ __launch_bounds__(kBlock) void k_ilu_backward(int32_t cnt, const int32_t* __restrict__ rows,
-                                                         const int32_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ col,
-                                                         const int32_t* __restrict__ diag,
-                                                         const VT* __restrict__ val, T* __restrict__ y) {
-  const int32_t t = (int32_t)(blockIdx.x * kBlock + threadIdx.x);
-  if (t >= cnt) return;
-  const int32_t i = rows[t];
-  const int32_t di = diag[i];
-  T yi[B];
-#pragma unroll
-  for (int r = 0; r < B; ++r) yi[r] = y[(int64_t)i * B + r];
-  for (int32_t pos = rowptr[i + 1] - 1; pos > di; --pos) {
-    T yc[B];
-#pragma unroll
-    for (int c = 0; c < B; ++c) yc[c] = y[(int64_t)col[pos] * B + c];
-    block_subtract<B>(yi, val + (int64_t)pos * B * B, yc);
-  }
-  const VT* D = val + (int64_t)di * B * B;
-#pragma unroll
-  for (int r = 0; r < B; ++r) {
-    T a = (T)0;
-#pragma unroll
-    for (int c = 0; c < B; ++c) a = add_rn(a, mul_rn((T)D[r * B + c], yi[c]));
-    y[(int64_t)i * B + r] = a;
+// sliced values (and the diagonal inverses) from the CSR factors
+__global__ __launch_bounds__(kBlock) void k_ilu_fill(int64_t m, int BB, const int32_t* __restrict__ map,
+                                                     const double* __restrict__ val,
+                                                     double* __restrict__ v64,
+                                                     float* __restrict__ v32) {
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += (int64_t)gridDim.x * kBlock) {
+    const int32_t pos = map[k];
+    for (int x = 0; x < BB; ++x) {
+      const double v = pos >= 0 ? val[(int64_t)pos * BB + x] : 0.0;
+      v64[k * BB + x] = v;
+      v32[k * BB + x] = (float)v;
+    }
   }
 }
 
@@ -199,16 +258,24 @@ int factor_levels(gh_ctx* ctx, gh_ilu* I) {
 }
 
 template <int B, typename T, typename VT>
-int solve_levels(gh_ctx* ctx, const gh_ilu* I, const VT* val, T* y) {
-  for (size_t L = 0; L + 1 < I->lptr_f.size(); ++L) {
-    const int32_t a = I->lptr_f[L], cnt = I->lptr_f[L + 1] - a;
-    hipLaunchKernelGGL((k_ilu_forward<B, T, VT>), dim3(grid_for(cnt)), dim3(kBlock), 0,
-                       ctx->stream, cnt, I->rows_f + a, I->rowptr, I->col, I->diag, val, y);
-  }
-  for (size_t L = 0; L + 1 < I->lptr_b.size(); ++L) {
-    const int32_t a = I->lptr_b[L], cnt = I->lptr_b[L + 1] - a;
-    hipLaunchKernelGGL((k_ilu_backward<B, T, VT>), dim3(grid_for(cnt)), dim3(kBlock), 0,
-                       ctx->stream, cnt, I->rows_b + a, I->rowptr, I->col, I->diag, val, y);
+int solve_levels(gh_ctx* ctx, const gh_ilu* I, bool fp64, const T* xin, T* y, T* xout, int mode,
+                 T omega) {
+  for (int part = 0; part < 2; ++part) {
+    const IluSlices& S = part == 0 ? I->lo : I->up;
+    const VT* val = fp64 ? (const VT*)S.v64 : (const VT*)S.v32;
+    const VT* dinv = fp64 ? (const VT*)I->dinv64 : (const VT*)I->dinv32;
+    for (size_t L = 0; L + 1 < S.lslice.size(); ++L) {
+      const int64_t s0 = S.lslice[L], s1 = S.lslice[L + 1];
+      const unsigned g = (unsigned)((s1 - s0 + kBlock / 64 - 1) / (kBlock / 64));
+      if (part == 0)
+        hipLaunchKernelGGL((k_ilu_sweep<B, T, VT, false>), dim3(g), dim3(kBlock), 0, ctx->stream,
+                           s0, s1, S.row, S.len, S.base, S.col, val, dinv, I->p, xin, y, xout,
+                           mode, omega);
+      else
+        hipLaunchKernelGGL((k_ilu_sweep<B, T, VT, true>), dim3(g), dim3(kBlock), 0, ctx->stream,
+                           s0, s1, S.row, S.len, S.base, S.col, val, dinv, I->p, xin, y, xout,
+                           mode, omega);
+    }
   }
   GH_HIP(hipGetLastError());
   return GH_OK;
@@ -248,10 +315,76 @@ void schedule(int64_t n, const std::vector<int32_t>& order_rows,
   for (int64_t i = 0; i < n; ++i) rows[fill[lev[i]]++] = (int32_t)i;
 }
 
+// one triangle in slices: the rows of each level (in `rows`, levels at
+// lptr) in slices of 64; entry j of a row is CSR position rowptr[i] + j
+// (lower part, ascending) or rowptr[i+1] - 1 - j (upper part, descending)
+int build_slices(gh_ctx* ctx, const std::vector<int32_t>& rows, const std::vector<int32_t>& lptr,
+                 const std::vector<int32_t>& rowptr, const std::vector<int32_t>& col,
+                 const std::vector<int32_t>& diag, bool upper, int BB, IluSlices* S) {
+  const size_t nl = lptr.size() - 1;
+  S->lslice.assign(nl + 1, 0);
+  for (size_t L = 0; L < nl; ++L) S->lslice[L + 1] = S->lslice[L] + (lptr[L + 1] - lptr[L] + 63) / 64;
+  const int64_t ns = S->lslice[nl];
+  std::vector<int32_t> hrow(ns * 64, -1), hlen(ns * 64, 0);
+  std::vector<int64_t> hbase(ns + 1, 0);
+  auto count = [&](int32_t i) { return upper ? rowptr[i + 1] - 1 - diag[i] : diag[i] - rowptr[i]; };
+  for (size_t L = 0; L < nl; ++L)
+    for (int32_t k = lptr[L]; k < lptr[L + 1]; ++k) {
+      const int64_t slot = S->lslice[L] * 64 + (k - lptr[L]);
+      hrow[slot] = rows[k];
+      hlen[slot] = count(rows[k]);
+    }
+  for (int64_t sl = 0; sl < ns; ++sl) {
+    int32_t w = 0;
+    for (int l = 0; l < 64; ++l) w = std::max(w, hlen[sl * 64 + l]);
+    hbase[sl + 1] = hbase[sl] + (int64_t)w * 64;
+  }
+  const int64_t ne = hbase[ns];
+  std::vector<int32_t> hcol(ne, 0), hmap(ne, -1);
+  for (int64_t sl = 0; sl < ns; ++sl)
+    for (int l = 0; l < 64; ++l) {
+      const int32_t i = hrow[sl * 64 + l];
+      for (int32_t j = 0; j < hlen[sl * 64 + l]; ++j) {
+        const int32_t pos = upper ? rowptr[i + 1] - 1 - j : rowptr[i] + j;
+        hcol[hbase[sl] + (int64_t)j * 64 + l] = col[pos];
+        hmap[hbase[sl] + (int64_t)j * 64 + l] = pos;
+      }
+    }
+  hbase.pop_back();
+  S->nslices = ns;
+  S->nentries = ne;
+  GH_TRY(dupload(ctx, &S->row, hrow));
+  GH_TRY(dupload(ctx, &S->len, hlen));
+  GH_TRY(dupload(ctx, &S->base, hbase));
+  GH_TRY(dupload(ctx, &S->col, hcol));
+  GH_TRY(dupload(ctx, &S->map, hmap));
+  if (hipMalloc((void**)&S->v64, sizeof(double) * std::max<int64_t>(ne * BB, 1)) != hipSuccess ||
+      hipMalloc((void**)&S->v32, sizeof(float) * std::max<int64_t>(ne * BB, 1)) != hipSuccess)
+    return fail(GH_ENOMEM, "gh_ilu_create: allocation failed");
+  // the host vectors die with this call: the uploads must have landed
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+void free_slices(IluSlices* S) {
+  for (void* ptr : {(void*)S->row, (void*)S->len, (void*)S->base, (void*)S->col, (void*)S->map,
+                    (void*)S->v64, (void*)S->v32})
+    if (ptr) (void)hipFree(ptr);
+  *S = IluSlices();
+}
+
+void fill_launch(gh_ctx* ctx, int64_t m, int BB, const int32_t* map, const double* val, double* v64,
+                 float* v32) {
+  if (m <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_ilu_fill, dim3(g), dim3(kBlock), 0, ctx->stream, m, BB, map, val, v64, v32);
+}
+
 }  // namespace
 
 namespace gh {
-int ilu_solve(gh_ctx* ctx, const gh_ilu* I, int prec, void* x);
+int ilu_apply(gh_ctx* ctx, const gh_ilu* I, int prec, const void* in, void* out, int mode,
+              double omega);
 }
 
 extern "C" {
@@ -321,8 +454,10 @@ int gh_ilu_create(gh_ctx* ctx, const gh_csr* A, const int32_t* perm, gh_ilu** ou
   if (!rc) rc = dupload(ctx, &I->diag, diag);
   if (!rc) rc = dupload(ctx, &I->val, val);
   if (!rc) rc = dupload(ctx, &I->rows_f, rows_f);
-  if (!rc) rc = dupload(ctx, &I->rows_b, rows_b);
-  if (!rc && (hipMalloc((void**)&I->val32, sizeof(float) * std::max<size_t>(val.size(), 1)) != hipSuccess ||
+  if (!rc) rc = build_slices(ctx, rows_f, I->lptr_f, rowptr, col, diag, false, BB, &I->lo);
+  if (!rc) rc = build_slices(ctx, rows_b, I->lptr_b, rowptr, col, diag, true, BB, &I->up);
+  if (!rc && (hipMalloc((void**)&I->dinv64, sizeof(double) * std::max<int64_t>(n * BB, 1)) != hipSuccess ||
+              hipMalloc((void**)&I->dinv32, sizeof(float) * std::max<int64_t>(n * BB, 1)) != hipSuccess ||
               hipMalloc(&I->y, sizeof(double) * std::max<int64_t>(n * B, 1)) != hipSuccess ||
               hipMalloc((void**)&I->d_flag, sizeof(int)) != hipSuccess))
     rc = fail(GH_ENOMEM, "gh_ilu_create: allocation failed");
@@ -336,11 +471,11 @@ int gh_ilu_create(gh_ctx* ctx, const gh_csr* A, const int32_t* perm, gh_ilu** ou
       default: rc = factor_levels<4>(ctx, I); break;
     }
   }
-  if (!rc && nnz > 0) {
-    const int64_t m = nnz * BB;
-    hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 8192)),
-                       dim3(kBlock), 0, ctx->stream, m, I->val32, (const double*)I->val);
-    if (hipGetLastError() != hipSuccess) rc = fail(GH_EHIP, "gh_ilu_create: convert");
+  if (!rc) {
+    fill_launch(ctx, I->lo.nentries, BB, I->lo.map, I->val, I->lo.v64, I->lo.v32);
+    fill_launch(ctx, I->up.nentries, BB, I->up.map, I->val, I->up.v64, I->up.v32);
+    fill_launch(ctx, n, BB, I->diag, I->val, I->dinv64, I->dinv32);
+    if (hipGetLastError() != hipSuccess) rc = fail(GH_EHIP, "gh_ilu_create: fill");
   }
   int flag = 0;
   if (!rc && hipMemcpyAsync(&flag, I->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
@@ -360,9 +495,11 @@ int gh_ilu_destroy(gh_ilu* I) {
   gh_ctx* ctx = I->ctx;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void* ptr : {(void*)I->p, (void*)I->q, (void*)I->rowptr, (void*)I->col, (void*)I->diag,
-                    (void*)I->val, (void*)I->val32, I->y, (void*)I->rows_f, (void*)I->rows_b,
+                    (void*)I->val, (void*)I->dinv64, (void*)I->dinv32, I->y, (void*)I->rows_f,
                     (void*)I->d_flag})
     if (ptr) (void)hipFree(ptr);
+  free_slices(&I->lo);
+  free_slices(&I->up);
   delete I;
   if (ctx) ctx_release(ctx);
   return GH_OK;
@@ -390,45 +527,39 @@ int gh_ilu_download(gh_ctx* ctx, const gh_ilu* I, int32_t* rowptr, int32_t* col,
 int gh_ilu_solve(gh_ctx* ctx, const gh_ilu* I, gh_vec* x) {
   GH_ARG(ctx && I && x, "gh_ilu_solve: NULL argument");
   GH_ARG(x->n == I->n && x->ncomp == I->B, "gh_ilu_solve: vector shape does not match");
-  return ilu_solve(ctx, I, x->prec, x->d);
+  return ilu_apply(ctx, I, x->prec, x->d, x->d, 0, 1.0);
 }
 
 }  // extern "C"
 
 namespace gh {
 
-// SparseBlockIlu::solve (:226-236), in place on x (FP64 or FP32)
-int ilu_solve(gh_ctx* ctx, const gh_ilu* I, int prec, void* x) {
-  const int64_t n = I->n;
-  const int B = I->B;
-  if (n == 0) return GH_OK;
-  const unsigned g = grid_for(n * B);
+// SparseBlockIlu::solve (:226-236) of `in`, written to `out` (mode 0; `in`
+// and `out` may be one vector), or added to it scaled by omega (mode 1), or
+// stored scaled (mode 2); FP64 or FP32 vectors
+int ilu_apply(gh_ctx* ctx, const gh_ilu* I, int prec, const void* in, void* out, int mode,
+              double omega) {
+  if (I->n == 0) return GH_OK;
   if (prec == GH_F64) {
-    double* y = (double*)I->y;
-    hipLaunchKernelGGL(k_ilu_gather<double>, dim3(g), dim3(kBlock), 0, ctx->stream, n, B, I->p,
-                       (const double*)x, y);
-    switch (B) {
-      case 1: GH_TRY((solve_levels<1, double, double>(ctx, I, I->val, y))); break;
-      case 2: GH_TRY((solve_levels<2, double, double>(ctx, I, I->val, y))); break;
-      case 3: GH_TRY((solve_levels<3, double, double>(ctx, I, I->val, y))); break;
-      default: GH_TRY((solve_levels<4, double, double>(ctx, I, I->val, y))); break;
+    const double* xi = (const double*)in;
+    double *y = (double*)I->y, *xo = (double*)out;
+    switch (I->B) {
+      case 1: GH_TRY((solve_levels<1, double, double>(ctx, I, true, xi, y, xo, mode, omega))); break;
+      case 2: GH_TRY((solve_levels<2, double, double>(ctx, I, true, xi, y, xo, mode, omega))); break;
+      case 3: GH_TRY((solve_levels<3, double, double>(ctx, I, true, xi, y, xo, mode, omega))); break;
+      default: GH_TRY((solve_levels<4, double, double>(ctx, I, true, xi, y, xo, mode, omega))); break;
     }
-    hipLaunchKernelGGL(k_ilu_gather<double>, dim3(g), dim3(kBlock), 0, ctx->stream, n, B, I->q,
-                       (const double*)y, (double*)x);
   } else {
-    float* y = (float*)I->y;
-    hipLaunchKernelGGL(k_ilu_gather<float>, dim3(g), dim3(kBlock), 0, ctx->stream, n, B, I->p,
-                       (const float*)x, y);
-    switch (B) {
-      case 1: GH_TRY((solve_levels<1, float, float>(ctx, I, I->val32, y))); break;
-      case 2: GH_TRY((solve_levels<2, float, float>(ctx, I, I->val32, y))); break;
-      case 3: GH_TRY((solve_levels<3, float, float>(ctx, I, I->val32, y))); break;
-      default: GH_TRY((solve_levels<4, float, float>(ctx, I, I->val32, y))); break;
+    const float* xi = (const float*)in;
+    float *y = (float*)I->y, *xo = (float*)out;
+    const float om = (float)omega;
+    switch (I->B) {
+      case 1: GH_TRY((solve_levels<1, float, float>(ctx, I, false, xi, y, xo, mode, om))); break;
+      case 2: GH_TRY((solve_levels<2, float, float>(ctx, I, false, xi, y, xo, mode, om))); break;
+      case 3: GH_TRY((solve_levels<3, float, float>(ctx, I, false, xi, y, xo, mode, om))); break;
+      default: GH_TRY((solve_levels<4, float, float>(ctx, I, false, xi, y, xo, mode, om))); break;
     }
-    hipLaunchKernelGGL(k_ilu_gather<float>, dim3(g), dim3(kBlock), 0, ctx->stream, n, B, I->q,
-                       (const float*)y, (float*)x);
   }
-  GH_HIP(hipGetLastError());
   return GH_OK;
 }
 

@@ -269,8 +269,10 @@ int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x);
 int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, const int32_t* nodes,
                  int prec, gh_vanka** out);
 int vanka_solve(gh_ctx* ctx, const gh_vanka* V, int prec, void* x);
-// SparseBlockIlu::solve in place on x (FP64 or FP32, the storage of x)
-int ilu_solve(gh_ctx* ctx, const gh_ilu* I, int prec, void* x);
+// SparseBlockIlu::solve of `in` (FP64 or FP32 storage): mode 0 out = h,
+// 1 out += omega h, 2 out = omega h (`in` and `out` may alias in mode 0)
+int ilu_apply(gh_ctx* ctx, const gh_ilu* I, int prec, const void* in, void* out, int mode,
+              double omega);
 // host transpose of a node-block CSR (blocks transposed too); rows of the
 // result list their columns ascending
 void csr_transpose_host(int64_t n, int B, const int32_t* rowptr, const int32_t* col,

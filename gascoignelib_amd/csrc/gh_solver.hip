@@ -326,14 +326,19 @@ int sweep(gh_mg* mg, MgLevel& L, int niter, bool& u_zero, void* keep_u) {
     if (niter > 0 && vanka && !L.vanka) return fail(GH_EINVAL, "vanka smoother: level without patches");
     if (niter > 0 && !vanka && !L.ilu) return fail(GH_EINVAL, "ilu smoother: level without factors");
     for (int it = 0; it < niter; ++it) {
+      if (!vanka) {
+        // the ILU solve reads h through its order and adds omega h to x itself
+        if (!u_zero) GH_TRY(csr_residual(mg->ctx, L.A, mg->wprec, L.v, L.u, L.b));
+        GH_TRY(ilu_apply(mg->ctx, L.ilu, mg->wprec, u_zero ? L.b : L.v, L.u, u_zero ? 2 : 1,
+                         mg->P.omega));
+        u_zero = false;
+        continue;
+      }
       if (u_zero)
         GH_TRY(vec_convert(mg->ctx, L.size, mg->wprec, L.v, mg->wprec, L.b));
       else
         GH_TRY(csr_residual(mg->ctx, L.A, mg->wprec, L.v, L.u, L.b));
-      if (vanka)
-        GH_TRY(vanka_solve(mg->ctx, L.vanka, mg->wprec, L.v));
-      else
-        GH_TRY(ilu_solve(mg->ctx, L.ilu, mg->wprec, L.v));
+      GH_TRY(vanka_solve(mg->ctx, L.vanka, mg->wprec, L.v));
       if (u_zero)
         GH_TRY(vec_scale_to(mg->ctx, mg->wprec, L.size, L.u, mg->P.omega, L.v));
       else

@@ -1151,6 +1151,42 @@ int ghs_assemble_on(int dim, int64_t n, const double* xyz, int64_t ncells, const
   return assemble(dim, n, xyz, ncells, cells, kcell, rowptr, col, ndir, dir, val);
 }
 
+int ghs_colour_order(int64_t n, const int32_t* rowptr, const int32_t* col, int32_t* perm,
+                     int32_t* ncolours) {
+  if (n < 0 || (n > 0 && (!rowptr || !col || !perm)) || !ncolours) {
+    g_err = "ghs_colour_order: bad arguments";
+    return -1;
+  }
+  // greedy colouring in node order: the smallest colour no coupled node
+  // already holds
+  std::vector<int32_t> colour(n, -1), seen;
+  int32_t nc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const int32_t j = col[q];
+      if (j < 0 || j >= n) {
+        g_err = "ghs_colour_order: column out of range";
+        return -1;
+      }
+      const int32_t c = colour[j];
+      if (c >= 0) {
+        if ((int32_t)seen.size() <= c) seen.resize(c + 1, -1);
+        seen[c] = (int32_t)i;
+      }
+    }
+    int32_t c = 0;
+    while (c < (int32_t)seen.size() && seen[c] == (int32_t)i) ++c;
+    colour[i] = c;
+    nc = std::max(nc, c + 1);
+  }
+  std::vector<int64_t> start(nc + 1, 0);
+  for (int64_t i = 0; i < n; ++i) start[colour[i] + 1]++;
+  for (int32_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+  for (int64_t i = 0; i < n; ++i) perm[start[colour[i]]++] = (int32_t)i;
+  *ncolours = nc;
+  return 0;
+}
+
 int ghs_level_patch_sizes(const ghs_hier* h, int l, int64_t* npatches, int64_t* nnz) {
   const Mesh* M = lev(h, l);
   if (!M) return -1;

@@ -185,6 +185,20 @@ class Mesh:
             pass
 
 
+def colour_order(rowptr, col):
+    """ILU node order with few triangular-solve levels (ghs_colour_order):
+    the nodes grouped by a greedy colouring of the matrix graph.  Returns
+    (perm, ncolours); ILU row i is node perm[i]."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    n = len(rowptr) - 1
+    perm = np.empty(n, np.int32)
+    nc = C.c_int32()
+    check_s(N.structure().ghs_colour_order(n, ptr(rowptr, N.i32p), ptr(col, N.i32p),
+                                           ptr(perm, N.i32p), C.byref(nc)))
+    return perm, nc.value
+
+
 def coarse_coefficients(k_fine, cells_per_coarse=8):
     """Per-cell coefficients of the next coarser level: the mean over the
     children (children of coarse cell c are fine cells 8c..8c+7 in 3D)."""

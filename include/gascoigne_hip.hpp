@@ -383,6 +383,12 @@ class CudaMultiLevelSolver {
   void SetVankaPatches(int level, int64_t npatches, int psize, const int32_t* nodes) {
     check(gh_mg_set_vanka(h_.get(), level, npatches, psize, nodes));
   }
+  // ILU(0) factors of a level (smoother 3), ILU row i = node perm[i] in the
+  // caller's numbering, nullptr for the identity (StdSolver::ComputeIlu,
+  // src/Solver/stdsolver.cc:1547-1573)
+  void SetIluOrder(int level, const int32_t* perm) {
+    check(gh_mg_set_ilu(h_.get(), level, perm));
+  }
   // The matrix the solver runs on a level (GetSolver(l)->GetMatrix(A)),
   // owned by the solver: FP64, or the V-cycle's precision
   const gh_csr* LevelMatrix(int level, bool fp64 = true) const {
@@ -414,6 +420,23 @@ class VankaSmoother {
  private:
   const Context* ctx_;
   Handle<gh_vanka, gh_vanka_destroy> h_;
+};
+
+// SparseBlockIlu (src/LinAlg/sparseblockilu.xx): ILU(0) of an FP64 node-block
+// matrix (blocks of 1..4) in the order perm (nullptr: identity); Solve(x) in
+// place, FP64 or FP32 x
+class SparseBlockIlu {
+ public:
+  SparseBlockIlu(const Context& ctx, const CudaCSRMatrixInterface& A, const int32_t* perm = nullptr)
+      : ctx_(&ctx) {
+    check(gh_ilu_create(ctx.get(), A.get(), perm, h_.out()));
+  }
+  void Solve(CudaVectorInterface& x) const { check(gh_ilu_solve(ctx_->get(), h_.get(), x.get())); }
+  gh_ilu* get() const { return h_.get(); }
+
+ private:
+  const Context* ctx_;
+  Handle<gh_ilu, gh_ilu_destroy> h_;
 };
 
 }  // namespace gascoigne_hip

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
-WORKLOADS = ["poisson128", "adjoint96", "stokes64", "q2_64", "q2ada"]
+WORKLOADS = ["poisson128", "adjoint96", "stokes64", "q2_64", "q2ada", "ilu128"]
 
 
 def test_more_than_one_gpu_is_refused():

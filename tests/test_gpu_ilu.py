@@ -9,10 +9,13 @@
 * The level schedules are the longest paths of the lower and upper
   dependency graphs (computed here by brute force), and a tridiagonal matrix
   (one chain) has n levels.
+* In the colour order (ghs_colour_order) a uniform level's triangles have 8
+  levels each, and the factors and solves still equal the oracle's.
 * The V-cycle with the ILU smoother (StdSolver::smooth "ilu",
   src/Solver/stdsolver.cc:962-967) equals the oracle's bit for bit, in the
-  caller's and in the lexicographic device numbering; Newton / GMRES iterate
-  with the oracle's counts; the reduced-precision solver converges.
+  caller's and in the lexicographic device numbering, for the identity, the
+  reversed and the colour orders; Newton / GMRES iterate with the oracle's
+  counts; the reduced-precision solver converges.
 * A zero pivot and an order that is not a permutation are errors.
 """
 import ctypes as C
@@ -24,7 +27,7 @@ import oracle_ctypes as O
 from gascoignelib_amd import hip as H
 from gascoignelib_amd import native as N
 from gascoignelib_amd.problem import Hierarchy, Level, block_operator
-from gascoignelib_amd.structure import Mesh, assemble_hierarchy
+from gascoignelib_amd.structure import Mesh, assemble_hierarchy, colour_order, uniform_hierarchy
 
 pytestmark = pytest.mark.gpu
 
@@ -118,6 +121,28 @@ def test_block_ilu_random_order(ctx, B):
     assert np.abs(X32.copy_back() - xo).max() <= 1e-5 * np.abs(xo).max()
 
 
+def test_colour_order_levels_and_solve(ctx):
+    """in the colour order each triangle has one level per colour; factors and
+    solves still equal the oracle's, and the FP32 solve is close"""
+    print("[ilu colour order]", flush=True)
+    lev = uniform_hierarchy("cube.inp", 3, 5, drop_coarse=1).fine
+    perm, nc = colour_order(lev.rowptr, lev.col)
+    A = H.CudaCSRMatrixInterface(ctx, lev.rowptr, lev.col, lev.val)
+    I = H.SparseBlockIlu(ctx, A, perm)
+    assert I.levels() == (nc, nc) == (8, 8)
+    Io = O.Ilu(O.Matrix(lev.rowptr, lev.col, lev.val), perm)
+    assert np.array_equal(I.factors(len(lev.col))[3], Io.val)
+    x = np.random.default_rng(5).uniform(-1, 1, lev.n)
+    X = H.CudaVectorInterface.from_global(ctx, x)
+    I.solve(X)
+    xo = x.copy()
+    Io.solve(xo)
+    assert np.array_equal(X.copy_back(), xo)
+    X32 = H.CudaVectorInterface.from_global(ctx, x, 1, "f32")
+    I.solve(X32)
+    assert np.abs(X32.copy_back() - xo).max() <= 1e-5 * np.abs(xo).max()
+
+
 def test_tridiagonal_chain(ctx):
     """one dependency chain: n levels each way, and ILU(0) is the exact LU"""
     n = 300
@@ -186,13 +211,15 @@ def mg_hierarchy(pre, B):
 
 @pytest.mark.parametrize("B", [1, 4])
 @pytest.mark.parametrize("order", [None, "lexicographic"])
-@pytest.mark.parametrize("ilu_order", ["identity", "reversed"])
+@pytest.mark.parametrize("ilu_order", ["identity", "reversed", "colour"])
 def test_vcycle_with_ilu_vs_oracle(ctx, B, order, ilu_order):
     print(f"[ilu V-cycle B={B} order={order} ilu={ilu_order}]", flush=True)
     h = mg_hierarchy(4 if B == 1 else 3, B)
     orders = None
     if ilu_order == "reversed":
         orders = [np.arange(h.level(l).n, dtype=np.int32)[::-1].copy() for l in range(h.nlevels)]
+    elif ilu_order == "colour":
+        orders = [colour_order(h.level(l).rowptr, h.level(l).col)[0] for l in range(h.nlevels)]
     f = np.random.default_rng(9).uniform(-1, 1, h.fine.n * B)
     for c in range(B):
         f[h.fine.dirichlet * B + c] = 0.0
