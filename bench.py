@@ -543,6 +543,16 @@ def bench_adjoint(args):
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     t_dj = time_kernel(ctx, lambda: P.cellmap.dJdk(Z, U, Gk), args.spmv_reps)
     bytes_dj = P.ncells * (8 * 4 + 3 * 8 + 8) + ndof * 16
+    # DWR indicators of the FP64 state and adjoint (the cells of each finest
+    # patch are the producer's cells 8p..8p+7, so k is already in patch order)
+    patches = P.mesh.patches(P.mesh.nlevels - 1)
+    Dw = H.DwrPatches(ctx, 3, P.coords, patches)
+    Kc = H.CudaVectorInterface.from_global(ctx, k)
+    U64 = H.CudaVectorInterface.from_global(ctx, res["fp64"]["u"])
+    Z64 = H.CudaVectorInterface.from_global(ctx, res["fp64"]["z"])
+    eta = Dw.estimate(U64, Z64, P.F, Kc)
+    t_dwr = time_kernel(ctx, lambda: Dw.estimate(U64, Z64, P.F, Kc, eta), args.spmv_reps)
+    eta_h = eta.copy_back()
     line = {
         "metric": "gradient evaluations: DOF/s of forward + adjoint solve to 1e-10 rel residual "
                   "+ dJ/dk per cell (mixed vs FP64)",
@@ -568,6 +578,11 @@ def bench_adjoint(args):
         "J": J, "euler_identity_residual": float(abs(k @ g64 + J) / abs(J)),
         "dJdk_kernel": {"us_per_launch": t_dj * 1e6, "bytes_per_launch": bytes_dj,
                         "gbs": bytes_dj / t_dj / 1e9},
+        "dwr_indicators": {"us_per_estimate": t_dwr * 1e6, "cells": int(len(eta_h)),
+                           "patches": int(len(patches)), "sum_eta": float(eta_h.sum()),
+                           "sum_abs_eta": float(np.abs(eta_h).sum()),
+                           "note": "f is the nodal load vector used as data: timing of the "
+                                   "estimator on the FP64 state and adjoint"},
         "setup_s": t_struct,
     }
     print(json.dumps(line), flush=True)

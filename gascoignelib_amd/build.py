@@ -24,7 +24,8 @@ HIP_FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["gh_core.hip", "gh_solver.hip", "gh_adjoint.hip", "gh_vanka.hip", "gh_assemble.hip", "gh_ilu.hip"]
+HIP_SOURCES = ["gh_core.hip", "gh_solver.hip", "gh_adjoint.hip", "gh_vanka.hip", "gh_assemble.hip", "gh_ilu.hip",
+               "gh_dwr.hip"]
 HIP_LIB = os.path.join(PKG, "libgascoigne_hip.so")
 
 CXX = os.environ.get("CXX", "g++")

@@ -101,24 +101,26 @@ __global__ __launch_bounds__(kBlock) void k_ilu_factor(int32_t cnt, const int32_
     }
 #pragma unroll
     for (int x = 0; x < BB; ++x) M[x] = m[x];
-    // a(i,j) -= a(i,k) a(k,j) for the upper entries j of row k (submult)
+    // a(i,j) -= a(i,k) a(k,j) for the upper entries j of row k that row i
+    // holds (submult); both rows are sorted, so one merge finds them (the
+    // entries of row i right of column k start at pk + 1)
+    int32_t ph = pk + 1;
     for (int32_t pj = diag[k] + 1; pj < rowptr[k + 1]; ++pj) {
       const int32_t j = col[pj];
-      for (int32_t ph = rowptr[i]; ph < e; ++ph) {
-        if (col[ph] != j) continue;
-        double* Hm = val + (int64_t)ph * BB;
-        const double* Cm = val + (int64_t)pj * BB;
+      while (ph < e && col[ph] < j) ++ph;
+      if (ph == e) break;
+      if (col[ph] != j) continue;
+      double* Hm = val + (int64_t)ph * BB;
+      const double* Cm = val + (int64_t)pj * BB;
 #pragma unroll
-        for (int r = 0; r < B; ++r)
+      for (int r = 0; r < B; ++r)
 #pragma unroll
-          for (int c = 0; c < B; ++c) {
-            double h = Hm[r * B + c];
+        for (int c = 0; c < B; ++c) {
+          double h = Hm[r * B + c];
 #pragma unroll
-            for (int u = 0; u < B; ++u) h = add_rn(h, -mul_rn(m[r * B + u], Cm[u * B + c]));
-            Hm[r * B + c] = h;
-          }
-        break;
-      }
+          for (int u = 0; u < B; ++u) h = add_rn(h, -mul_rn(m[r * B + u], Cm[u * B + c]));
+          Hm[r * B + c] = h;
+        }
     }
   }
   // a(i,i) <- a(i,i)^-1 (FMatrixBlock::inverse = NodeMatrix::gauss_jordan)

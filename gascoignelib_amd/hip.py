@@ -561,6 +561,41 @@ class CoefficientCells:
             pass
 
 
+class DwrPatches:
+    """Patches of a refined mesh level for dual-weighted residual indicators
+    (gh_patches_create / gh_dwr_estimate): estimate(u, z, f, k) returns the
+    per-cell eta_K = int_K f w - k_K int_K grad u_h . grad w, w = I2 z_h - z_h,
+    as a device vector of npatches * 2^dim values (each patch's cells in
+    lexicographic order); their sum estimates J(u) - J(u_h)."""
+
+    def __init__(self, ctx, dim, coords, patches):
+        self.ctx = ctx
+        xyz = np.zeros((len(coords), 3))
+        xyz[:, : np.asarray(coords).shape[1]] = coords
+        self._xyz = np.ascontiguousarray(xyz)
+        self._nodes = np.ascontiguousarray(patches, dtype=np.int32)
+        self.dim = dim
+        self.npatches = len(self._nodes)
+        self.ncells = self.npatches * 2 ** dim
+        self.h = C.c_void_p()
+        check(N.hip().gh_patches_create(ctx.h, dim, len(xyz), ptr(self._xyz), self.npatches,
+                                        ptr(self._nodes, N.i32p), C.byref(self.h)))
+
+    def estimate(self, u, z, f, k=None, eta=None):
+        if eta is None:
+            eta = CudaVectorInterface(self.ctx, self.ncells)
+        check(N.hip().gh_dwr_estimate(self.ctx.h, self.h, u.h, z.h, f.h,
+                                      k.h if k is not None else None, eta.h))
+        return eta
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.hip().gh_patches_destroy(self.h)
+        except Exception:
+            pass
+
+
 class SparseBlockIlu:
     """SparseBlockIlu: ILU(0) factors of an FP64 matrix, ILU row i = node
     perm[i]; solve(x) in place (FP64 or FP32 vector)."""

@@ -147,6 +147,9 @@ HIP_SIGNATURES = {
     "gh_ilu_levels": (C.c_int, [vp, i32p, i32p]),
     "gh_ilu_download": (C.c_int, [vp, vp, i32p, i32p, i32p, f64p]),
     "gh_mg_set_ilu": (C.c_int, [vp, C.c_int, i32p]),
+    "gh_patches_create": (C.c_int, [vp, C.c_int, C.c_int64, f64p, C.c_int64, i32p, P(vp)]),
+    "gh_patches_destroy": (C.c_int, [vp]),
+    "gh_dwr_estimate": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "gh_cells_create": (C.c_int, [vp, C.c_int, C.c_int64, f64p, C.c_int64, i32p, P(vp)]),
     "gh_cells_destroy": (C.c_int, [vp]),
     "gh_dJdk_cells": (C.c_int, [vp, vp, vp, vp, vp]),

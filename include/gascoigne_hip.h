@@ -46,6 +46,7 @@ typedef struct gh_hn gh_hn;
 typedef struct gh_mask gh_mask;
 typedef struct gh_mg gh_mg;
 typedef struct gh_cells gh_cells;
+typedef struct gh_patches gh_patches;
 typedef struct gh_vanka gh_vanka;
 typedef struct gh_ilu gh_ilu;
 
@@ -419,6 +420,31 @@ int gh_cells_destroy(gh_cells* C);
  * and z the adjoint (A^T z = dJ/du).  u and z must vanish on Dirichlet nodes
  * (then only free-node couplings contribute, as in the eliminated matrix). */
 int gh_dJdk_cells(gh_ctx* ctx, const gh_cells* C, const gh_vec* z, const gh_vec* u, gh_vec* g);
+
+/* ------------------------------------------------- DWR error indicators */
+/* The dual-weighted residual estimator of a Q1 solution with the patch-wise
+ * Q2 interpolation of the adjoint: what StdLoop::Estimator
+ * (src/Solver/stdloop.cc:221-273) would compute from the adjoint solve; the
+ * reference holds only stubs there (every estimator type aborts).
+ *
+ * gh_patches_create: the patches of a refined mesh (the 2^dim children of one
+ * coarser cell), npatches x 3^dim node indices in lexicographic order (x
+ * fastest), node coordinates xyz[nnodes*3]; every patch must be an
+ * axis-aligned box whose middle nodes are midpoints (GH_EINVAL otherwise).
+ *
+ * gh_dwr_estimate: per cell K (npatches x 2^dim values, each patch's cells in
+ * lexicographic order),
+ *   eta_K = int_K f w - k_K int_K grad u_h . grad w,   w = I2 z_h - z_h,
+ * u_h, z_h the Q1 functions of the nodal state u and adjoint z (A^T z = g for
+ * J(u) = g^T u), I2 the Q2 interpolation on the patch, f the Q2 interpolant of
+ * the nodal data f (the load b_i = int f phi_i), k the per-cell coefficient
+ * (NULL: 1), all scalar FP64.  sum_K eta_K estimates J(u) - J(u_h).  With
+ * hanging nodes pass u and z after HNAverage. */
+int gh_patches_create(gh_ctx* ctx, int dim, int64_t nnodes, const double* xyz, int64_t npatches,
+                      const int32_t* nodes, gh_patches** out);
+int gh_patches_destroy(gh_patches* P);
+int gh_dwr_estimate(gh_ctx* ctx, const gh_patches* P, const gh_vec* u, const gh_vec* z,
+                    const gh_vec* f, const gh_vec* k, gh_vec* eta);
 
 /* ------------------------------------------------------ device assembly */
 /* CudaSolver::AssembleMatrix (cuda/cudasolver.cc:114-128): StdSolver's CPU
