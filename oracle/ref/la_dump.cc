@@ -13,15 +13,21 @@
 //   * SparseBlockIlu<FMatrixBlock<1>>: ConstructStructure / copy_entries /
 //     compute_ilu / solve (src/LinAlg/sparseblockilu.xx:70-246) on levels of
 //     at most 1100 nodes, with the identity order and the reversed order
+//   * disc Q13dLps: NavierStokesLps<3> (src/Problems/navierstokes.h:92-270,
+//     parameters from the param file's Equation block) linearised at U = 0 on
+//     CGDiscQ13dLps (cgdisc.h:1314-1319), 4x4 blocks (p, u, v, w), on every
+//     level with patches: structure and block values only
 // The strong-Dirichlet loop mirrors CGDisc::StrongDirichletMatrix
-// (cgdisc.h:1139-1150) for colour list `colours` and component 0, since the
-// ProblemDescriptor plumbing lives in the unbuildable solver sources.
+// (cgdisc.h:1139-1150) for colour list `colours` and component 0 (the
+// velocity components 1..3 for Q13dLps), since the ProblemDescriptor plumbing
+// lives in the unbuildable solver sources.
 //
-// usage: la_dump <paramfile> <outdir> <disc Q12d|Q13d|Q22d|Q23d> <colours,...>
+// usage: la_dump <paramfile> <outdir> <disc Q12d|Q13d|Q22d|Q23d|Q13dLps> <colours,...>
 //        [r cx cy cz nsteps]   (adaptive node refinement inside a sphere)
 #include "cgdisc.h"
 #include "gascoignemesh.h"
 #include "laplace.h"
+#include "navierstokes.h"
 #include "meshagent.h"
 #include "mginterpolatormatrix.h"
 #include "npy.h"
@@ -196,6 +202,47 @@ static void dump(MeshAgent& MA, const ParamFile& pf, const std::vector<int>& col
   }
 }
 
+// NavierStokesLps<3> at U = 0 on the patch levels: rowptr, col and the 4x4
+// blocks (row-major) after the velocity Dirichlet rows of `colours`
+static void dump_stokes(MeshAgent& MA, const ParamFile& pf, const std::vector<int>& colours,
+                        const std::string& out) {
+  NavierStokesData nd;
+  nd.BasicInit(pf);
+  NavierStokesLps<3> eq(nd);
+  for (int l = 0; l < MA.nlevels(); ++l) {
+    const GascoigneMesh* M = MA.GetMesh(l);
+    if (!M->HasPatch()) continue;
+    CGDiscQ13dLps disc;
+    disc.BasicInit(pf);
+    disc.ReInit(M);
+    const int n = disc.ndofs();
+    const std::string p = out + "/L" + std::to_string(l) + "_";
+    SparseStructure S;
+    disc.Structure(&S);
+    SparseBlockMatrix<FMatrixBlock<4>> A;
+    A.ReInit(&S);
+    A.zero();
+    GlobalVector u(4, n);
+    u.zero();
+    disc.Matrix(A, u, eq, 1.0);
+    std::vector<int> vel = {1, 2, 3};
+    for (int col : colours) {
+      const IntVector& bv = *M->VertexOnBoundary(col);
+      for (size_t i = 0; i < bv.size(); ++i) A.dirichlet(bv[i], vel);
+    }
+    const ColumnDiagStencil* ST = dynamic_cast<const ColumnDiagStencil*>(A.GetStencil());
+    std::vector<int32_t> rowptr(ST->start().begin(), ST->start().end());
+    std::vector<int32_t> col(ST->col().begin(), ST->col().end());
+    std::vector<double> val((size_t)A.size() * 16);
+    for (int pos = 0; pos < A.size(); ++pos)
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) val[(size_t)pos * 16 + r * 4 + c] = (*A.mat(pos))(r, c);
+    npy::save(p + "stokes_rowptr.npy", rowptr);
+    npy::save(p + "stokes_col.npy", col);
+    npy::save(p + "stokes_val.npy", val.data(), {(size_t)A.size(), (size_t)16});
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc < 5) {
     std::fprintf(stderr, "usage: %s paramfile outdir disc colours [r cx cy cz nsteps]\n", argv[0]);
@@ -227,6 +274,7 @@ int main(int argc, char** argv) {
   else if (disc == "Q22d") dump<CGDiscQ22d, 2>(MA, pf, colours, out);
   else if (disc == "Q13d") dump<CGDiscQ13d, 3>(MA, pf, colours, out);
   else if (disc == "Q23d") dump<CGDiscQ23d, 3>(MA, pf, colours, out);
+  else if (disc == "Q13dLps") dump_stokes(MA, pf, colours, out);
   else {
     std::fprintf(stderr, "unknown disc %s\n", disc.c_str());
     return 2;

@@ -37,7 +37,13 @@ FIXTURES = {
     # edge nodes with 2 parents (HNStructureQ12d) and 3 parents (HNStructureQ22d)
     "q1_2d_16_ada": (2, "square.inp", 4, 1, "Q12d", "1", (0.2, 0.41, 0.53, 0.0, 2)),
     "q2_2d_16_ada": (2, "square.inp", 4, 1, "Q22d", "1", (0.2, 0.41, 0.53, 0.0, 2)),
+    # Stokes-LPS (NavierStokesLps<3> at U = 0 on CGDiscQ13dLps, visc 1, alpha 0.1)
+    # on the channel, 8^3 cells, velocity Dirichlet on colours 1 and 2
+    "stokes_3d_8": (3, "channel.inp", 3, 1, "Q13dLps", "1,2", None),
 }
+
+# Equation block of the param file per discretisation (the Stokes workload's)
+EQUATION = {"Q13dLps": "//Block Equation\nvisc 1.0\nalpha 0.1\n"}
 
 
 def build():
@@ -52,6 +58,7 @@ def run(name):
             f.write("//Block Mesh\n")
             f.write(f"dimension {dim}\nprerefine {pre}\npatchdepth {pdepth}\n")
             f.write(f"gridname {os.path.join(CONFIGS, inp)}\n")
+            f.write(EQUATION.get(disc, ""))
         extra = [] if sphere is None else [str(v) for v in sphere]
         mdir = os.path.join(td, "mesh")
         ldir = os.path.join(td, "la")
