@@ -25,7 +25,8 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["gh_core.hip", "gh_solver.hip", "gh_adjoint.hip", "gh_vanka.hip", "gh_assemble.hip", "gh_ilu.hip",
-               "gh_dwr.hip"]
+               "gh_dwr.hip", "gh_direct.hip"]
+HIP_LIBS = ["-lrocsolver", "-lrocblas"]
 HIP_LIB = os.path.join(PKG, "libgascoigne_hip.so")
 
 CXX = os.environ.get("CXX", "g++")
@@ -41,7 +42,7 @@ def run(cmd):
 def build_hip():
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = HIP_LIB + ".tmp"
-    run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp])
+    run([HIPCC] + HIP_FLAGS + srcs + HIP_LIBS + ["-o", tmp])
     os.replace(tmp, HIP_LIB)
 
 

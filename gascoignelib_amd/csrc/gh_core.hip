@@ -1146,6 +1146,7 @@ int gh_ctx_create(int device, gh_ctx** out) {
 namespace gh {
 static void ctx_free(gh_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  direct_ctx_free(ctx);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->scal) (void)hipFree(ctx->scal);
   if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);

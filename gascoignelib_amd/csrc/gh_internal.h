@@ -163,6 +163,9 @@ struct gh_ctx {
   // device address of this device's gh_jacobi_noalloc scratch (nullptr when
   // the lookup failed at creation)
   void* noalloc_scratch = nullptr;
+  // rocBLAS handle on `stream` for the direct coarse factorisation (created
+  // on first use, destroyed with the context)
+  void* blas = nullptr;
 };
 
 struct gh_vec {
@@ -269,6 +272,10 @@ int csr_spmv_x32(gh_ctx* ctx, const gh_csr* A, double* y, const float* x);
 int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, const int32_t* nodes,
                  int prec, gh_vanka** out);
 int vanka_solve(gh_ctx* ctx, const gh_vanka* V, int prec, void* x);
+// the inverse of an FP64 matrix (scalar or node blocks) as a dense row-major
+// m x m array on the device (m = n * ncomp), factorised on the device
+int dense_inverse_device(gh_ctx* ctx, const gh_csr* A, double* inv);
+void direct_ctx_free(gh_ctx* ctx);
 // SparseBlockIlu::solve of `in` (FP64 or FP32 storage): mode 0 out = h,
 // 1 out += omega h, 2 out = omega h (`in` and `out` may alias in mode 0)
 int ilu_apply(gh_ctx* ctx, const gh_ilu* I, int prec, const void* in, void* out, int mode,

@@ -274,49 +274,6 @@ int dupload(gh_ctx* ctx, T** d, const std::vector<T>& h) {
   return GH_OK;
 }
 
-// dense inverse of a direct level's matrix (LU with partial pivoting, FP64)
-int dense_inverse(const gh_csr* A, std::vector<double>& inv) {
-  if (!A->h_val_ok)
-    return fail(GH_EINVAL, "dense coarse solve: the matrix values were assembled on the device");
-  const int B = A->ncomp;
-  const int64_t m = A->n * B;
-  std::vector<double> a(m * m, 0.0);
-  for (int64_t i = 0; i < A->n; ++i)
-    for (int32_t pos = A->h_rowptr[i]; pos < A->h_rowptr[i + 1]; ++pos)
-      for (int r = 0; r < B; ++r)
-        for (int c = 0; c < B; ++c)
-          a[(i * B + r) * m + A->h_col[pos] * B + c] += A->h_val[(int64_t)pos * B * B + r * B + c];
-  inv.assign(m * m, 0.0);
-  for (int64_t i = 0; i < m; ++i) inv[i * m + i] = 1.0;
-  for (int64_t k = 0; k < m; ++k) {
-    int64_t p = k;
-    double mx = std::fabs(a[k * m + k]);
-    for (int64_t i = k + 1; i < m; ++i)
-      if (std::fabs(a[i * m + k]) > mx) {
-        mx = std::fabs(a[i * m + k]);
-        p = i;
-      }
-    if (mx == 0.0) return fail(GH_ESINGULAR, "direct coarse level: singular matrix");
-    if (p != k) {
-      for (int64_t j = 0; j < m; ++j) {
-        std::swap(a[k * m + j], a[p * m + j]);
-        std::swap(inv[k * m + j], inv[p * m + j]);
-      }
-    }
-    const double piv = a[k * m + k];
-    for (int64_t i = 0; i < m; ++i) {
-      if (i == k) continue;
-      const double l = a[i * m + k] / piv;
-      if (l == 0.0) continue;
-      for (int64_t j = k; j < m; ++j) a[i * m + j] -= l * a[k * m + j];
-      for (int64_t j = 0; j < m; ++j) inv[i * m + j] -= l * inv[k * m + j];
-    }
-    for (int64_t j = 0; j < m; ++j) inv[k * m + j] /= piv;
-    for (int64_t j = k; j < m; ++j) a[k * m + j] /= piv;
-  }
-  return GH_OK;
-}
-
 // ------------------------------------------------------------- V-cycle pieces
 int sweep(gh_mg* mg, MgLevel& L, int niter, bool& u_zero, void* keep_u) {
   if (mg->P.smoother >= 2) {
@@ -841,9 +798,8 @@ int mg_build(gh_ctx* ctx, int nlevels, gh_csr* const* A, gh_transfer* const* T, 
     if (!rc) rc = vec_fill_zero(ctx, mg->wprec, Lv.size, Lv.v);
     Lv.direct = Lv.n < params->ndirect;
     if (!rc && Lv.direct) {
-      std::vector<double> inv;
-      rc = dense_inverse(Lv.A64, inv);
-      if (!rc) rc = dupload(ctx, &Lv.Ainv, inv);
+      rc = dalloc((void**)&Lv.Ainv, sizeof(double) * Lv.size * Lv.size);
+      if (!rc) rc = dense_inverse_device(ctx, Lv.A64, Lv.Ainv);
     }
   }
   // transfer operators with the constraints folded in
@@ -984,16 +940,7 @@ int gh_mg_assemble_q1(gh_ctx* ctx, gh_mg* mg, gh_cells* const* C, gh_vec* const*
       GH_TRY(assemble_q1_launch(ctx, C[l], kc, isdir, L.d_perm, M, v64));
       GH_TRY(csr_asm_commit(ctx, M, "gh_mg_assemble_q1"));
     }
-    if (L.direct) {
-      // the dense inverse of a direct level is formed on the host, from its
-      // new FP64 values (a few thousand entries)
-      gh_csr* A = L.A64;
-      GH_HIP(hipMemcpy(A->h_val.data(), A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
-      A->h_val_ok = true;
-      std::vector<double> inv;
-      GH_TRY(dense_inverse(A, inv));
-      GH_HIP(hipMemcpy(L.Ainv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
-    }
+    if (L.direct) GH_TRY(dense_inverse_device(ctx, L.A64, L.Ainv));
   }
   GH_HIP(hipStreamSynchronize(ctx->stream));
   return GH_OK;
