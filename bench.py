@@ -149,7 +149,7 @@ def main():
                          "Poisson with the triquadratic patch element on 64^3 cells (125-point "
                          "rows), forward and adjoint solves; q2ada: the same on 64^3 cells "
                          "(32^3 triquadratic elements) plus 2 adaptive levels inside a seeded "
-                         "ball of radius 0.25, with hanging-node constraints (1,432,971 DOF); "
+                         "ball of radius 0.25, with hanging-node constraints (1,986,413 DOF); "
                          "ilu128: the poisson128 problem with the ILU(0) smoother (the CPU "
                          "path's default) in each level's colour order")
     ap.add_argument("--kernel-only", action="store_true",
@@ -768,9 +768,11 @@ def bench_q2(args, adaptive=0):
     centre.  The forward solves with the reference's V(2,2) are timed beside.
 
     With adaptive = 2 the 64^3 mesh (32^3 triquadratic elements) is followed by
-    two levels refined patch-wise inside that same ball: 1,432,971 DOF, 86.6 M
-    entries, Q2 hanging-node constraints on the refinement interfaces (49,840
-    edge and 24,920 face nodes on the finest level) with the operator condensed
+    two refinements of the nodes inside that same ball as MeshAgent::refine_nodes
+    makes them, with the multigrid levels Gascoigne derives from the result:
+    1,986,413 DOF, 120.8 M entries, Q2 hanging-node constraints on the
+    refinement interfaces (63,008 edge and 31,504 face nodes on the finest
+    level) with the operator condensed
     over them; f and the functional are distributed onto the parents of the
     hanging nodes (HNDistribute) before the solves."""
     from gascoignelib_amd import hip as H

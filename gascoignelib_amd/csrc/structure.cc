@@ -165,6 +165,20 @@ int refine(const Mesh& C, Mesh& F, const std::vector<uint8_t>* mark = nullptr) {
     }
     return -1;
   };
+  // With marks, a midpoint may already be a node of C (a hanging node where
+  // a neighbour is finer): it is found by its exact coordinates, which the
+  // refinement that made it computed the same way, and kept (a coarse node,
+  // no 2er/4er entry).
+  std::unordered_map<std::string, int32_t> at;
+  if (mark) {
+    at.reserve(n0 * 2);
+    for (int64_t i = 0; i < n0; ++i) at.emplace(std::string((const char*)&C.X[3 * i], 24), (int32_t)i);
+  }
+  auto existing = [&](const double* x) -> int32_t {
+    if (!mark) return -1;
+    auto it = at.find(std::string((const char*)x, 24));
+    return it == at.end() ? -1 : it->second;
+  };
   // face midpoints (3D)
   std::unordered_map<uint64_t, int32_t> fmap, emap;
   fmap.reserve(dim == 3 ? nc * 4 : 1);
@@ -187,12 +201,18 @@ int refine(const Mesh& C, Mesh& F, const std::vector<uint8_t>* mark = nullptr) {
           }
           const uint64_t key = fkey(q);
           if (fmap.find(key) != fmap.end()) continue;
-          const int32_t id = (int32_t)(X.size() / 3);
-          fmap.emplace(key, id);
           double s[3] = {0, 0, 0};
           for (int t = 0; t < 4; ++t)
             for (int d = 0; d < 3; ++d) s[d] += C.X[3 * q[t] + d];
-          for (int d = 0; d < 3; ++d) X.push_back(s[d] / 4.0);
+          for (int d = 0; d < 3; ++d) s[d] /= 4.0;
+          const int32_t old = existing(s);
+          if (old >= 0) {
+            fmap.emplace(key, old);
+            continue;
+          }
+          const int32_t id = (int32_t)(X.size() / 3);
+          fmap.emplace(key, id);
+          for (int d = 0; d < 3; ++d) X.push_back(s[d]);
           F.vierer.push_back(id);
           for (int t = 0; t < 4; ++t) F.vierer.push_back(q[t]);
         }
@@ -220,9 +240,16 @@ int refine(const Mesh& C, Mesh& F, const std::vector<uint8_t>* mark = nullptr) {
       }
       const uint64_t key = ekey(a, b);
       if (emap.find(key) != emap.end()) continue;
+      double mid[3];
+      for (int d = 0; d < 3; ++d) mid[d] = 0.5 * (C.X[3 * a + d] + C.X[3 * b + d]);
+      const int32_t old = existing(mid);
+      if (old >= 0) {
+        emap.emplace(key, old);
+        continue;
+      }
       const int32_t id = (int32_t)(X.size() / 3);
       emap.emplace(key, id);
-      for (int d = 0; d < 3; ++d) X.push_back(0.5 * (C.X[3 * a + d] + C.X[3 * b + d]));
+      for (int d = 0; d < 3; ++d) X.push_back(mid[d]);
       F.zweier.push_back(id);
       F.zweier.push_back(a);
       F.zweier.push_back(b);
@@ -803,51 +830,98 @@ int q2_hanging(const Mesh& M, std::vector<int32_t>& E, std::vector<int32_t>& Fc)
   return 0;
 }
 
-// One adaptive step on a patch mesh: refine every candidate patch with a
-// node inside the ball (|x - c| < r; MeshAgent::refine_nodes refines the
-// patches of the given nodes).  Candidates are the patches of the newest
-// generation; from the second step on a candidate is refined only when every
-// patch sharing one of its corners is of its generation or newer, so that
-// neighbouring patches differ by at most one generation and no hanging node
-// has a hanging parent.
+// One adaptive step on a patch mesh, marking as MeshAgent::refine_nodes does
+// for the nodes inside the ball (|x - c| < r) with patch depth 1 and the
+// default etapatcher (HierarchicalMesh::vertex_patch_refine,
+// src/Mesh/hierarchicalmesh.cc:272-290; HierarchicalMesh3d::FillVertexLevels,
+// RefineCoarseNodes, VertexToCells, LoadFathers3d, patch_refine,
+// regular_grid3d_three_refine, src/Mesh/hierarchicalmesh3d.cc:1938-1998,
+// 2081-2254; CoarseHierarchicalMesh3d::refine,
+// src/Mesh/coarsehierarchicalmesh3d.cc:94-130).  The cells of one patch are
+// the children of one father and share its generation (level), so on the
+// patch mesh the steps read:
+//   1. vertex level: the lowest level of the cells at a node;
+//   2. for every cell with a marked node, its patch's corners of the lowest
+//      vertex level become coarse marks;
+//   3. a cell with a coarse-marked corner of its own level marks its patch;
+//   4. on the mesh of patches (the coarsened mesh, corners as vertices) the
+//      closure: wherever the levels at a corner would differ by 2 or more
+//      after the marked patches are refined, the patches there more than one
+//      level below the finest are marked too, until nothing changes;
+//   5. every cell of a marked patch is refined.
+// Refining the children of the marked patches on the full mesh then needs no
+// further closure: at any corner the patch levels differ by at most one.
 int adapt(const Mesh& C, Mesh& F, double r, const double ctr[3]) {
   std::vector<int32_t> P;
   if (patches(C, P)) return -1;
   const int64_t np = (int64_t)P.size() / 27;
+  const int64_t nc = C.ncells();
   std::vector<int8_t> lev = C.plev;
   if ((int64_t)lev.size() != np) lev.assign(np, 0);
-  int8_t top = 0;
-  for (int8_t v : lev) top = std::max(top, v);
-  std::vector<int32_t> nstart(C.n() + 1, 0), nlist(np * 27);
-  for (int64_t k = 0; k < np * 27; ++k) nstart[P[k] + 1]++;
-  for (int64_t i = 0; i < C.n(); ++i) nstart[i + 1] += nstart[i];
-  {
-    std::vector<int32_t> fill(nstart.begin(), nstart.end() - 1);
-    for (int64_t k = 0; k < np * 27; ++k) nlist[fill[P[k]]++] = (int32_t)(k / 27);
-  }
   const int corners[8] = {0, 2, 6, 8, 18, 20, 24, 26};
-  std::vector<uint8_t> pmark(np, 0), cmark(C.ncells(), 0);
-  int64_t nmark = 0;
-  for (int64_t p = 0; p < np; ++p) {
-    if (lev[p] != top) continue;
-    bool inside = false;
-    for (int a = 0; a < 27 && !inside; ++a) {
-      const double* x = &C.X[3 * (int64_t)P[p * 27 + a]];
-      const double d2 = (x[0] - ctr[0]) * (x[0] - ctr[0]) + (x[1] - ctr[1]) * (x[1] - ctr[1]) +
-                        (x[2] - ctr[2]) * (x[2] - ctr[2]);
-      inside = d2 < r * r;
+  // 1. vertex levels
+  std::vector<int32_t> vlev(C.n(), 1 << 20);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int m = 0; m < 8; ++m) {
+      const int32_t v = C.cells[c * 8 + m];
+      vlev[v] = std::min<int32_t>(vlev[v], lev[c / 8]);
     }
-    if (!inside) continue;
-    bool ok = true;
-    for (int t = 0; t < 8 && ok; ++t) {
-      const int32_t v = P[p * 27 + corners[t]];
-      for (int32_t k = nstart[v]; k < nstart[v + 1] && ok; ++k) ok = lev[nlist[k]] >= top;
-    }
-    if (!ok) continue;
-    pmark[p] = 1;
-    nmark++;
-    for (int c = 0; c < 8; ++c) cmark[p * 8 + c] = 1;
+  std::vector<uint8_t> refnode(C.n(), 0);
+  for (int64_t i = 0; i < C.n(); ++i) {
+    const double* x = &C.X[3 * i];
+    const double d2 = (x[0] - ctr[0]) * (x[0] - ctr[0]) + (x[1] - ctr[1]) * (x[1] - ctr[1]) +
+                      (x[2] - ctr[2]) * (x[2] - ctr[2]);
+    refnode[i] = d2 < r * r;
   }
+  // 2. coarse marks: the lowest-level corners of the patch of a marked cell
+  std::vector<uint8_t> coarse(C.n(), 0);
+  for (int64_t c = 0; c < nc; ++c) {
+    bool any = false;
+    for (int m = 0; m < 8 && !any; ++m) any = refnode[C.cells[c * 8 + m]];
+    if (!any) continue;
+    const int32_t* pc = &P[(c / 8) * 27];
+    int32_t mn = vlev[pc[corners[0]]];
+    for (int t = 1; t < 8; ++t) mn = std::min(mn, vlev[pc[corners[t]]]);
+    for (int t = 0; t < 8; ++t)
+      if (vlev[pc[corners[t]]] == mn) coarse[pc[corners[t]]] = 1;
+  }
+  // 3. cells with a coarse-marked corner of their own level mark their patch
+  std::vector<uint8_t> pmark(np, 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int m = 0; m < 8; ++m) {
+      const int32_t v = C.cells[c * 8 + m];
+      if (coarse[v] && vlev[v] == lev[c / 8]) pmark[c / 8] = 1;
+    }
+  // 4. closure on the mesh of patches
+  std::vector<int32_t> vmax(C.n()), vmin(C.n());
+  for (;;) {
+    std::fill(vmax.begin(), vmax.end(), -1);
+    std::fill(vmin.begin(), vmin.end(), 1 << 20);
+    for (int64_t p = 0; p < np; ++p) {
+      const int32_t l = lev[p] + (pmark[p] ? 1 : 0);
+      for (int t = 0; t < 8; ++t) {
+        const int32_t v = P[p * 27 + corners[t]];
+        vmax[v] = std::max(vmax[v], l);
+        vmin[v] = std::min(vmin[v], l);
+      }
+    }
+    int64_t added = 0;
+    for (int64_t p = 0; p < np; ++p) {
+      if (pmark[p]) continue;
+      for (int t = 0; t < 8; ++t) {
+        const int32_t v = P[p * 27 + corners[t]];
+        if (vmax[v] >= vmin[v] + 2 && vmax[v] >= lev[p] + 2) {
+          pmark[p] = 1;
+          ++added;
+          break;
+        }
+      }
+    }
+    if (!added) break;
+  }
+  // 5. refine the cells of the marked patches
+  std::vector<uint8_t> cmark(nc, 0);
+  for (int64_t c = 0; c < nc; ++c) cmark[c] = pmark[c / 8];
   if (refine(C, F, &cmark)) return -1;
   F.plev.clear();
   for (int64_t p = 0; p < np; ++p) {
@@ -970,6 +1044,172 @@ struct ghs_hier {
   std::map<int, Condensed> q2c;
 };
 
+// ------------------------------------ multigrid levels of an adaptive mesh
+// A hexahedral cell of the refined unit box is (level L, i, j, k): the cube
+// [i, i+1] x [j, j+1] x [k, k+1] * 2^-L.  Packed: 5 bits level, 19 bits each.
+inline uint64_t ckey(int L, int64_t i, int64_t j, int64_t k) {
+  return ((uint64_t)L << 57) | ((uint64_t)i << 38) | ((uint64_t)j << 19) | (uint64_t)k;
+}
+inline int ck_level(uint64_t c) { return (int)(c >> 57); }
+inline int64_t ck_i(uint64_t c, int d) { return (int64_t)((c >> (38 - 19 * d)) & 0x7FFFF); }
+
+int cell_keys(const Mesh& M, std::vector<uint64_t>& K) {
+  std::vector<std::array<int32_t, 3>> L;
+  if (lattice(M, L)) return -1;
+  K.resize(M.ncells());
+  for (int64_t c = 0; c < M.ncells(); ++c) {
+    const int32_t* v = &M.cells[c * 8];
+    const std::array<int32_t, 3>& o = L[v[0]];
+    const int32_t sz = L[v[1]][0] - o[0];
+    int lev = 0;
+    while (lev <= 20 && (1 << (20 - lev)) != sz) ++lev;
+    if (lev > 19 || o[0] % sz || o[1] % sz || o[2] % sz || L[v[6]][1] - o[1] != sz ||
+        L[v[6]][2] - o[2] != sz)
+      return fail("adaptive levels need cubic cells of the dyadic unit box (cell %lld)",
+                  (long long)c);
+    K[c] = ckey(lev, o[0] / sz, o[1] / sz, o[2] / sz);
+  }
+  return 0;
+}
+
+// The next coarser multigrid level of a patch mesh given as its cells, as
+// GascoigneMeshConstructor builds it (LevelMesh3d::construct_lists,
+// src/Mesh/levelmesh3d.cc:703-831, with LevelJumper, src/Mesh/leveljumper.h):
+// a grandfather whose 64 grandchildren are all cells of the level is
+// represented by its 8 children, every other cell is kept; then, while some
+// corner carries cells whose levels differ by more than one, the coarsened
+// groups with a corner where a cell is more than one level finer are undone.
+void coarser_level(const std::unordered_map<uint64_t, int>& S, std::vector<uint64_t>& out) {
+  std::unordered_map<uint64_t, int> opis, oldc, newc;
+  for (const auto& kv : S) {
+    const uint64_t c = kv.first;
+    const int L = ck_level(c);
+    if (L < 2) {
+      oldc[c] = 1;
+      continue;
+    }
+    opis[ckey(L - 2, ck_i(c, 0) >> 2, ck_i(c, 1) >> 2, ck_i(c, 2) >> 2)] = 1;
+  }
+  auto grandchildren = [&](uint64_t g, bool present_only, std::vector<uint64_t>& v) {
+    v.clear();
+    const int L = ck_level(g) + 2;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        for (int e = 0; e < 4; ++e) {
+          const uint64_t c = ckey(L, 4 * ck_i(g, 0) + a, 4 * ck_i(g, 1) + b, 4 * ck_i(g, 2) + e);
+          if (!present_only || S.count(c)) v.push_back(c);
+        }
+  };
+  std::vector<uint64_t> gc;
+  for (const auto& kv : opis) {
+    const uint64_t g = kv.first;
+    grandchildren(g, false, gc);
+    bool all = true;
+    for (uint64_t c : gc) all = all && S.count(c);
+    if (all) {
+      const int L = ck_level(g) + 1;
+      for (int a = 0; a < 8; ++a)
+        newc[ckey(L, 2 * ck_i(g, 0) + (a & 1), 2 * ck_i(g, 1) + ((a >> 1) & 1),
+                  2 * ck_i(g, 2) + (a >> 2))] = 1;
+    } else {
+      grandchildren(g, true, gc);
+      for (uint64_t c : gc) oldc[c] = 1;
+    }
+  }
+  // corners in units of the finest lattice (level 19)
+  auto corner = [](uint64_t c, int m) {
+    const int s = 19 - ck_level(c);
+    return ckey(0, (ck_i(c, 0) + kHexCorner[m][0]) << s, (ck_i(c, 1) + kHexCorner[m][1]) << s,
+                (ck_i(c, 2) + kHexCorner[m][2]) << s);
+  };
+  for (;;) {
+    std::unordered_map<uint64_t, std::pair<int, int>> phi;  // corner -> (min, max) level
+    auto upd = [&](uint64_t c) {
+      for (int m = 0; m < 8; ++m) {
+        auto it = phi.emplace(corner(c, m), std::make_pair(1000, 0)).first;
+        it->second.first = std::min(it->second.first, ck_level(c));
+        it->second.second = std::max(it->second.second, ck_level(c));
+      }
+    };
+    for (const auto& kv : newc) upd(kv.first);
+    for (const auto& kv : oldc) upd(kv.first);
+    bool jump = false;
+    for (const auto& kv : phi) jump = jump || kv.second.second > kv.second.first + 1;
+    if (!jump) break;
+    std::vector<uint64_t> undo;
+    for (const auto& kv : newc) {
+      const uint64_t q = kv.first;
+      for (int m = 0; m < 8; ++m)
+        if (phi[corner(q, m)].second > ck_level(q) + 1) {
+          undo.push_back(ckey(ck_level(q) - 1, ck_i(q, 0) >> 1, ck_i(q, 1) >> 1, ck_i(q, 2) >> 1));
+          break;
+        }
+    }
+    for (uint64_t f : undo) {
+      const int L = ck_level(f) + 1;
+      for (int a = 0; a < 8; ++a)
+        newc.erase(ckey(L, 2 * ck_i(f, 0) + (a & 1), 2 * ck_i(f, 1) + ((a >> 1) & 1),
+                        2 * ck_i(f, 2) + (a >> 2)));
+      grandchildren(f, true, gc);
+      for (uint64_t c : gc) oldc[c] = 1;
+    }
+  }
+  out.clear();
+  for (const auto& kv : newc) out.push_back(kv.first);
+  for (const auto& kv : oldc) out.push_back(kv.first);
+  std::sort(out.begin(), out.end());
+}
+
+// The multigrid levels of the adaptive mesh `fine` (all its cells at levels
+// >= 1 of the unit box), coarsest first, from the level-1 mesh `one`: each
+// level is the previous one with the cells refined that the next finer cell
+// set no longer holds, so nodes, transfer maps and patch grouping come from
+// refine().
+int adaptive_levels(const Mesh& one, const Mesh& fine, std::vector<Mesh>& out) {
+  std::vector<uint64_t> K;
+  if (cell_keys(fine, K)) return -1;
+  int top = 0;
+  for (uint64_t c : K) top = std::max(top, ck_level(c));
+  std::vector<std::vector<uint64_t>> sets(1, K);
+  std::sort(sets[0].begin(), sets[0].end());
+  for (int l = 1; l < top; ++l) {
+    std::unordered_map<uint64_t, int> S;
+    for (uint64_t c : sets.back()) S[c] = 1;
+    std::vector<uint64_t> next;
+    coarser_level(S, next);
+    sets.push_back(next);
+  }
+  // sets[top-1] must be the level-1 mesh
+  std::vector<uint64_t> K1;
+  if (cell_keys(one, K1)) return -1;
+  std::sort(K1.begin(), K1.end());
+  if (K1 != sets.back()) return fail("adaptive levels: the coarsest level is not the level-1 mesh");
+  out.assign(1, one);
+  out[0].plev.clear();
+  for (int l = top - 2; l >= 0; --l) {
+    const Mesh& C = out.back();
+    std::vector<uint64_t> KC;
+    if (cell_keys(C, KC)) return -1;
+    std::unordered_map<uint64_t, int> want;
+    for (uint64_t c : sets[l]) want[c] = 1;
+    std::vector<uint8_t> mark(C.ncells(), 0);
+    for (int64_t c = 0; c < C.ncells(); ++c) mark[c] = want.count(KC[c]) ? 0 : 1;
+    for (int64_t p = 0; p < C.ncells() / 8; ++p)
+      for (int t = 1; t < 8; ++t)
+        if (mark[p * 8 + t] != mark[p * 8])
+          return fail("adaptive levels: a refinement splits a patch");
+    Mesh F;
+    if (refine(C, F, &mark)) return -1;
+    std::vector<uint64_t> KF;
+    if (cell_keys(F, KF)) return -1;
+    std::sort(KF.begin(), KF.end());
+    if (KF != sets[l]) return fail("adaptive levels: level %d is not reached by refinement", l);
+    if (q2_hanging(F, F.hn_e, F.hn_f)) return -1;
+    out.push_back(std::move(F));
+  }
+  return 0;
+}
+
 extern "C" {
 
 const char* ghs_last_error(void) { return g_err.c_str(); }
@@ -1007,6 +1247,14 @@ int ghs_build_adaptive(const char* inp_path, int prerefine, int drop_coarse, int
   for (int s = 0; s < nadapt; ++s) {
     all.emplace_back();
     if (adapt(all[all.size() - 2], all.back(), radius, centre)) return -1;
+  }
+  if (nadapt > 0) {
+    // the multigrid levels GascoigneMeshConstructor derives from the final
+    // mesh (not the meshes after each step)
+    std::vector<Mesh> lv;
+    if (adaptive_levels(all[1], all.back(), lv)) return -1;
+    all.resize(1);
+    for (Mesh& m : lv) all.push_back(std::move(m));
   }
   ghs_hier* h = new ghs_hier();
   h->levels.assign(std::make_move_iterator(all.begin() + drop_coarse),

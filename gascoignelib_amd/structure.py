@@ -315,9 +315,12 @@ def q2_hierarchy(prerefine, inp="cube.inp", drop_coarse=1, colours=(1,), name=No
     elements (up to 125 couplings per row), nodal transfers between levels
     (MgInterpolatorMatrix, as for every discretisation of the reference's
     multilevel solver), Dirichlet nodes of the given colours.  With
-    adaptive = 1 or 2 the finest uniform mesh is followed by that many
-    adaptive levels refined inside the ball (radius, centre), with Q2
-    hanging-node constraints and the operator condensed over them."""
+    adaptive = 1 or 2 the finest uniform mesh is refined that many times as
+    MeshAgent::refine_nodes refines the nodes inside the ball (radius,
+    centre), and the multigrid levels are those Gascoigne's mesh constructor
+    derives from the result (coarser levels coarsen the whole mesh where they
+    can), with Q2 hanging-node constraints and the operator condensed over
+    them."""
     if adaptive:
         M = Mesh.adaptive(inp, prerefine, drop_coarse, adaptive, radius, centre)
     else:

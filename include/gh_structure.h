@@ -110,14 +110,18 @@ int ghs_level_assemble_stokes(const ghs_hier* h, int l, double visc, double alph
 int ghs_level_assemble_q2(const ghs_hier* h, int l, int ncolours, const int32_t* colours,
                           double* val);
 /* Adaptive Q2 patch hierarchy: the .inp mesh refined `prerefine` times, then
- * `nadapt` (0..2) adaptive steps, each refining (all 8 cells of) every patch
- * of the newest generation that has a node inside the ball |x - centre| <
- * radius (MeshAgent::refine_nodes, src/Mesh/meshagent.cc:540-556, refines
- * the patches of the marked nodes); from the second step on only patches
- * whose corner-sharing patches are all of that generation or newer, so that
- * neighbouring patches differ by one generation at most.  Node coordinates
- * must lie on the dyadic lattice (unit-box meshes).  Levels as in
- * ghs_build; the adaptive levels carry Q2 hanging-node constraints. */
+ * `nadapt` (0..2) steps of MeshAgent::refine_nodes (src/Mesh/meshagent.cc:
+ * 540-556) on the nodes inside the ball |x - centre| < radius, with patch
+ * depth 1: the nodes' patches marked as HierarchicalMesh::vertex_patch_refine
+ * marks them, closed so that the levels at any patch corner differ by one at
+ * most (src/Mesh/hierarchicalmesh.cc:272-290, hierarchicalmesh3d.cc:
+ * 1938-2254), every cell of a marked patch refined.  The multigrid levels
+ * are those GascoigneMeshConstructor derives from the refined mesh
+ * (LevelMesh3d::construct_lists, src/Mesh/levelmesh3d.cc:703-831): each
+ * coarser level merges every group of 64 cells of one grandfather into its 8
+ * children where the level jumps allow.  The unit box only (node coordinates
+ * on the dyadic lattice).  Levels otherwise as in ghs_build; levels with
+ * hanging nodes carry the Q2 constraints. */
 int ghs_build_adaptive(const char* inp_path, int prerefine, int drop_coarse, int nadapt,
                        double radius, const double* centre /* 3 */, ghs_hier** out);
 /* Q2 hanging-node constraints of level l (HNStructureQ23d layout):

@@ -12,8 +12,11 @@ Two kinds, computed by the same code for the compiled reference's output
   sorted), the cells (corners renumbered, corner order kept, rows sorted),
   the Dirichlet nodes, the mesh-transfer maps (c2f indexed by canonical coarse
   node; 2er/4er/8er rows of a canonical fine node and its sorted canonical
-  parents); the matrix values, which two assemblies produce only to rounding,
-  through weighted sums compared with a tolerance.
+  parents; the Q2 hanging-node constraints: edge rows (node, 3 parents) as
+  they are, face rows as the set of (parent, weight) pairs, since the 3x3
+  parents of a face may be listed transposed under the symmetric weights);
+  the matrix values, which two assemblies produce only to rounding, through
+  weighted sums compared with a tolerance.
 """
 import hashlib
 
@@ -86,3 +89,23 @@ def canonical_transfer(c2f, zweier, vierer, achter, rank_f, rank_c):
         out[name + "_sha256"] = sha(rows.astype(np.int64))
         out["n" + name] = int(len(a))
     return out
+
+
+Q2_EDGE_W = (0.375, 0.75, -0.125)
+
+
+def canonical_hn(edges, faces, rank):
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 4)
+    f = np.asarray(faces, dtype=np.int64).reshape(-1, 10)
+    er = rank[e] if len(e) else e
+    er = er[np.lexsort(tuple(er[:, k] for k in range(3, -1, -1)))] if len(er) else er
+    # face rows: the hanging node, then its parents ordered by weight class
+    # (corner 1/64-type weights, edge, centre) and rank
+    w = np.outer(Q2_EDGE_W, Q2_EDGE_W).ravel()
+    rows = []
+    for r in f:
+        pairs = sorted(zip(np.round(w, 12), rank[r[1:]]))
+        rows.append([rank[r[0]]] + [int(p) for _, p in pairs])
+    fr = np.array(sorted(rows), dtype=np.int64).reshape(-1, 10)
+    return {"hn_edge_sha256": sha(er), "nhn_edge": int(len(e)),
+            "hn_face_sha256": sha(fr), "nhn_face": int(len(f))}

@@ -24,17 +24,26 @@ sys.path.insert(0, HERE)
 import digests as D  # noqa: E402
 from make_fixtures import CONFIGS, EQUATION, REF_BIN, build  # noqa: E402
 
-# name: (dimension, coarse mesh, prerefine, patchdepth, disc, dirichlet colours)
+# the centre of the adaptive workload's ball (seeded: 0.25 + 0.25 (u + 1),
+# u = SplitMix64 uniforms of seed 4242)
+ADA_CENTRE = (0.6705279178912067, 0.4294083221038335, 0.4314458695334776)
+
+# name: (dimension, coarse mesh, prerefine, patchdepth, disc, dirichlet colours,
+#        sphere refinement (radius, centre x, y, z, steps) or None)
 CASES = {
     # config 2: 3D Q1 Poisson on 128^3 cells (2,146,689 DOF)
-    "q1_3d_128": (3, "cube.inp", 7, 1, "Q13d", "1"),
+    "q1_3d_128": (3, "cube.inp", 7, 1, "Q13d", "1", None),
     # config 5: Stokes-LPS channel on 64^3 cells (274,625 nodes, 4x4 blocks)
-    "stokes_3d_64": (3, "channel.inp", 6, 1, "Q13dLps", "1,2"),
+    "stokes_3d_64": (3, "channel.inp", 6, 1, "Q13dLps", "1,2", None),
+    # config 4: Q2 Poisson, 64^3 cells + 2 refine_nodes steps of the nodes in
+    # the ball of radius 0.25 around ADA_CENTRE, Q2 hanging nodes
+    "q2_3d_64_ada": (3, "cube.inp", 6, 1, "Q23d", "1", (0.25,) + ADA_CENTRE + (2,)),
 }
 
 
 def run(name):
-    dim, inp, pre, pdepth, disc, cols = CASES[name]
+    dim, inp, pre, pdepth, disc, cols, sphere = CASES[name]
+    extra = [] if sphere is None else [repr(float(v)) for v in sphere[:4]] + [str(sphere[4])]
     colours = [int(c) for c in cols.split(",")]
     with tempfile.TemporaryDirectory() as td:
         param = os.path.join(td, "p.param")
@@ -46,9 +55,9 @@ def run(name):
         mdir, ldir = os.path.join(td, "mesh"), os.path.join(td, "la")
         os.makedirs(mdir)
         os.makedirs(ldir)
-        subprocess.check_call([os.path.join(REF_BIN, "mesh_dump"), param, mdir],
+        subprocess.check_call([os.path.join(REF_BIN, "mesh_dump"), param, mdir] + extra,
                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-        subprocess.check_call([os.path.join(REF_BIN, "la_dump"), param, ldir, disc, cols],
+        subprocess.check_call([os.path.join(REF_BIN, "la_dump"), param, ldir, disc, cols] + extra,
                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         nl = int(open(os.path.join(mdir, "meta.txt")).read().split()[1])
         pre_m = lambda l, k: np.load(os.path.join(mdir, f"L{l}_{k}.npy"))  # noqa: E731
@@ -68,6 +77,10 @@ def run(name):
                 val = np.load(os.path.join(ldir, f"L{l}_{csr}val.npy"))
                 rec["raw"].update(rowptr=D.raw(rp), col=D.raw(col), val=D.raw(val))
             can, rank = D.canonical_level(coords, rp, col, val, cells, bnd)
+            if disc == "Q23d":
+                he, hf = pre_m(l, "hn_edge"), pre_m(l, "hn_face")
+                rec["raw"].update(hn_edge=D.raw(he), hn_face=D.raw(hf))
+                can.update(D.canonical_hn(he, hf, rank))
             rec["canonical"] = can
             levels.append(rec)
             ranks.append(rank)
@@ -80,6 +93,7 @@ def run(name):
                 *t, ranks[l], ranks[l + 1])
         out = {"case": name, "dimension": dim, "mesh": inp, "prerefine": pre,
                "patchdepth": pdepth, "disc": disc, "dirichlet_colours": colours,
+               "sphere": sphere,
                "nlevels": nl, "levels": levels}
         path = os.path.join(HERE, f"digests_{name}.json")
         with open(path, "w") as f:

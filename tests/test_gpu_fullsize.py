@@ -11,7 +11,7 @@ gradient, and the duality g.u = z.f of forward and adjoint solves.
 Sizes: 3D Q1 Poisson 128^3 cells (2,146,689 DOF); Stokes-LPS channel 64^3
 cells (1,098,500 unknowns); variable-coefficient diffusion 96^3 cells
 (912,673 DOF); Q2 patch-element Poisson 64^3 cells (274,625 DOF), and the
-same with two adaptive levels and Q2 hanging nodes (1,432,971 DOF).
+same with two adaptive levels and Q2 hanging nodes (1,986,413 DOF).
 """
 import numpy as np
 import pytest
@@ -172,8 +172,8 @@ def test_q2_64_duality(ctx):
 
 def test_q2_adaptive_64_solves(ctx):
     """Q2 64^3 cells + 2 adaptive levels in the ball of radius 0.25 around the
-    seeded centre (1,432,971 DOF, 86.6 M entries, 49,840 edge and 24,920 face
-    hanging nodes): one V-cycle bit for bit against the oracle; FP64 and mixed
+    seeded centre (1,986,413 DOF, 120.8 M entries, 63,008 edge and 31,504 face
+    hanging nodes, the reference's refinement and levels): one V-cycle bit for bit against the oracle; FP64 and mixed
     solves to the true FP64 residual 1e-10, mixed within 1e-8 of FP64; forward
     and adjoint solves satisfy g.u = z.f"""
     print("[q2 64^3 + 2 adaptive levels]", flush=True)
@@ -181,7 +181,7 @@ def test_q2_adaptive_64_solves(ctx):
     centre = 0.25 + 0.5 * (0.5 * (splitmix64_uniform(4242, 3) + 1.0))
     h = q2_hierarchy(6, adaptive=2, radius=0.25, centre=tuple(centre))
     lev = h.fine
-    assert lev.n == 1432971 and len(lev.hn_edges) == 49840 and len(lev.hn_faces) == 24920
+    assert lev.n == 1986413 and len(lev.hn_edges) == 63008 and len(lev.hn_faces) == 31504
     hn = O.HN(lev.hn_kind, lev.hn_edges, lev.hn_faces)
     f = lattice_rhs(lev.coords, seed=2024)
     hn.distribute(f)

@@ -179,7 +179,7 @@ def test_q2_adaptive_vcycle_vs_oracle(ctx, a8, order, ndirect):
 
 
 def test_q2_adaptive_solve_vs_oracle(ctx, a16):
-    """16^3 + 2 adaptive levels (24,401 DOF, 3040 edge and 1520 face hanging
+    """16^3 + 2 adaptive levels (77,485 DOF, 6740 edge and 3352 face hanging
     nodes): Newton/GMRES iterates to 1e-12 with iteration counts within one,
     the true residual of the condensed system, and the mixed-precision solve
     within 1e-8 of the FP64 oracle solve"""

@@ -11,11 +11,17 @@ producer forms the exact integrals as tensor products, the reference sums a
 3x3x3 Gauss rule), Dirichlet rows, and the patch node lists in the
 reference's patch-local order.
 
-The adaptive levels (patch-wise refinement inside a ball, Q2 hanging nodes,
-the operator condensed over them) are pinned by what the discretisation
-fixes: the constraints interpolate quadratics exactly, the condensed operator
-equals an independent scipy T^T A T, is symmetric, annihilates constants and
-is exact on quadratics; a ball holding the whole domain reproduces the uniform
+The adaptive levels (MeshAgent::refine_nodes of the nodes inside a ball,
+the multigrid levels GascoigneMeshConstructor derives from the refined mesh,
+Q2 hanging nodes, the operator condensed over them) equal the reference's on
+the fixture's refinement (two steps in a sphere on 8^3 cells): under the
+coordinate match every level's nodes, cells and transfer maps, the hanging
+edge constraints row for row and the hanging face constraints as (parent,
+weight) sets (some faces list the 3x3 parents transposed; the weights are
+symmetric).  Pinned further by what the discretisation fixes: the
+constraints interpolate quadratics exactly, the condensed operator equals an
+independent scipy T^T A T, is symmetric, annihilates constants and is exact
+on quadratics; a ball holding the whole domain reproduces the uniform
 refinement entry for entry; the oracle's HNStructureQ23d solve converges on
 them.
 """
@@ -217,16 +223,53 @@ def test_adaptive_with_every_patch_marked_is_uniform():
         assert np.array_equal(np.sort(la.dirichlet), np.sort(lu.dirichlet))
 
 
-def test_adaptive_step_that_marks_nothing_repeats_the_level():
-    """a ball outside the domain marks no patch: each adaptive level repeats the
-    previous one with an identity transfer"""
+def test_adaptive_step_that_marks_nothing_keeps_the_uniform_levels():
+    """a ball outside the domain marks no node: the mesh is not refined, so the
+    hierarchy is the uniform one (the reference keeps its levels)"""
     h = q2_hierarchy(2, adaptive=2, radius=0.25, centre=(5.0, 5.0, 5.0))
-    for l in (2, 3):
-        lev, prev = h.levels[l], h.levels[l - 1]
-        c2f, z, v, a = lev.transfer
-        assert lev.n == prev.n and np.array_equal(c2f, np.arange(prev.n))
-        assert len(z) == len(v) == len(a) == 0
-        assert np.array_equal(lev.coords, prev.coords)
+    u = q2_hierarchy(2)
+    assert h.nlevels == u.nlevels
+    for lh, lu in zip(h.levels, u.levels):
+        assert np.array_equal(lh.coords, lu.coords) and np.array_equal(lh.col, lu.col)
+
+
+def test_adaptive_levels_equal_reference(golden):
+    """the adaptive hierarchy of the fixture's refinement (cube.inp, prerefine
+    3, two refine_nodes steps of the nodes in the sphere of radius 0.08 around
+    (0.41, 0.53, 0.47)) against the compiled reference's MeshAgent"""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import digests as D
+    from gascoignelib_amd.structure import Mesh
+    z = golden("q2_3d_8_ada")
+    nl = int(z["nlevels"])
+    M = Mesh.adaptive("cube.inp", 3, 1, 2, 0.08, (0.41, 0.53, 0.47))
+    assert M.nlevels == nl
+    w1 = np.array([0.375, 0.75, -0.125])
+    wf = np.outer(w1, w1).ravel()
+    ranks_r, ranks_m = [], []
+    for s in range(nl):
+        m = nl - 1 - s
+        xr, xm = z[f"L{m}_coords"], M.coords(s)
+        rr, rm = D.rank_of(xr), D.rank_of(xm)
+        ranks_r.append(rr)
+        ranks_m.append(rm)
+        assert len(xr) == len(xm)
+        assert np.array_equal(xr[np.argsort(rr)], xm[np.argsort(rm)])
+        cr = D.canonical_level(xr, cells=z[f"L{m}_cells"])[0]
+        cm = D.canonical_level(xm, cells=M.cells(s))[0]
+        assert cr["cells_sha256"] == cm["cells_sha256"]
+        em, fm = M.hn(s)
+        er, fr = z[f"L{m}_hn_edge"], z[f"L{m}_hn_face"]
+        assert sorted(map(tuple, rr[er])) == sorted(map(tuple, rm[em]))
+        pr = {rr[r[0]]: sorted(zip(rr[r[1:]], wf)) for r in fr}
+        pm = {rm[r[0]]: sorted(zip(rm[r[1:]], wf)) for r in fm}
+        assert pr == pm
+    for s in range(1, nl):
+        m = nl - 1 - s
+        tr = D.canonical_transfer(*(z[f"L{m}_{k}"] for k in ("c2f", "zweier", "vierer", "achter")),
+                                  ranks_r[s], ranks_r[s - 1])
+        assert tr == D.canonical_transfer(*M.transfer(s), ranks_m[s], ranks_m[s - 1])
 
 
 @pytest.mark.parametrize("kw", [dict(adaptive=3), dict(adaptive=2, radius=-1.0)])
