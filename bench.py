@@ -331,12 +331,18 @@ def main():
             "ms_per_solve": float(np.median(ts22)) * 1e3,
             "gmres": rep22.gmres_iterations,
             "speedup_vs_fp64": t64 / float(np.median(ts22))},
-        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": traffic,
+        # achieved: the algorithmic bytes of a CSR residual (nnz (8 + 4) +
+        # (n + 1) 4 + n (8 + 8 + 8) for the FP64 residual); the layout itself
+        # moves fewer (no stored zeros, shared column patterns): its own bytes
+        # and the measured HBM traffic are reported beside
+        "roofline": {"bound": "hbm", "achieved": csr64 / t_spmv / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": csr64 / t_spmv / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic,
                      "kernel": "FP64 residual SpMV r = b - A x, finest level",
-                     "bytes_per_launch": b64, "us_per_launch": t_spmv * 1e6,
-                     "stored_entries": lay["stored"], "pattern_entries": fine.nnz,
-                     "csr_equivalent_gbs": csr64 / t_spmv / 1e9},
+                     "bytes_per_launch": csr64, "us_per_launch": t_spmv * 1e6,
+                     "pattern_entries": fine.nnz, "stored_entries": lay["stored"],
+                     "format_bytes_per_launch": b64, "format_gbs": gbs64,
+                     "format_frac": gbs64 / HBM_PEAK_GBS},
         "spmv_mixed": {"gbs": bmx / t_spmv_mx / 1e9, "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS,
                        "bytes_per_launch": bmx, "us_per_launch": t_spmv_mx * 1e6},
         "setup_s": dict({"structure": t_struct, "device_upload": t_setup}, **t_asm),
