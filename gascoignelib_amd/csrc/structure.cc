@@ -1165,9 +1165,7 @@ void coarser_level(const std::unordered_map<uint64_t, int>& S, std::vector<uint6
 // level is the previous one with the cells refined that the next finer cell
 // set no longer holds, so nodes, transfer maps and patch grouping come from
 // refine().
-int adaptive_levels(const Mesh& one, const Mesh& fine, std::vector<Mesh>& out) {
-  std::vector<uint64_t> K;
-  if (cell_keys(fine, K)) return -1;
+int adaptive_levels(const Mesh& one, const std::vector<uint64_t>& K, std::vector<Mesh>& out) {
   int top = 0;
   for (uint64_t c : K) top = std::max(top, ck_level(c));
   std::vector<std::vector<uint64_t>> sets(1, K);
@@ -1206,6 +1204,117 @@ int adaptive_levels(const Mesh& one, const Mesh& fine, std::vector<Mesh>& out) {
     if (KF != sets[l]) return fail("adaptive levels: level %d is not reached by refinement", l);
     if (q2_hanging(F, F.hn_e, F.hn_f)) return -1;
     out.push_back(std::move(F));
+  }
+  return 0;
+}
+
+// Gascoigne's hierarchical mesh file (HierarchicalMesh3d::write_gup /
+// read_gup, src/Mesh/hierarchicalmesh3d.cc:1534-1700; cell and boundary-cell
+// text forms of src/Mesh/cell.h:182-192, boundarycell.h:96-108, hang lists of
+// hanglistio.cc): the level-0 hexes and boundary quads become the macro
+// mesh, the hexes without children the finest cells (as lattice keys of the
+// unit box; coordinates are written with 6 significant digits and snapped).
+int read_gup(const char* path, Mesh& base, std::vector<uint64_t>& leaves) {
+  std::ifstream in(path);
+  if (!in) return fail("cannot open %s", path);
+  std::string sym;
+  int dim = 0;
+  int64_t nv = 0;
+  in >> dim >> sym;
+  if (!in || dim != 3 || sym != "dimension") return fail("%s: not a 3D .gup file", path);
+  in >> nv >> sym;
+  if (!in || nv <= 0 || sym != "vertexs") return fail("%s: bad vertex header", path);
+  std::vector<double> X(nv * 3);
+  for (int64_t i = 0; i < nv * 3; ++i) in >> X[i];
+  int64_t nh = 0;
+  in >> nh >> sym;
+  if (!in || nh <= 0 || sym != "hexs") return fail("%s: bad hex header", path);
+  std::vector<std::array<int32_t, 8>> hv(nh);
+  std::vector<int32_t> hlev(nh), hnch(nh);
+  for (int64_t h = 0; h < nh; ++h) {
+    int64_t father, mat, nch;
+    for (int m = 0; m < 8; ++m) in >> hv[h][m];
+    in >> hlev[h] >> father >> mat >> sym >> nch;
+    if (!in || sym != "@" || nch < 0 || nch > 8) return fail("%s: bad hex %lld", path, (long long)h);
+    hnch[h] = (int32_t)nch;
+    int64_t c;
+    for (int64_t k = 0; k < nch; ++k) in >> c;
+    in >> sym;
+    if (sym != ":") return fail("%s: bad hex %lld", path, (long long)h);
+    for (int k = 0; k < 6; ++k) in >> c;
+    for (int m = 0; m < 8; ++m)
+      if (hv[h][m] < 0 || hv[h][m] >= nv) return fail("%s: hex %lld: bad vertex", path, (long long)h);
+  }
+  // the face and edge hang lists: N vertices, "->", hanging, ":", 2 neighbours
+  for (int n : {4, 2}) {
+    int64_t cnt;
+    in >> cnt >> sym;
+    if (!in || sym != "hangs") return fail("%s: bad hang list", path);
+    int64_t t;
+    for (int64_t k = 0; k < cnt; ++k) {
+      for (int a = 0; a < n; ++a) in >> t;
+      in >> sym >> t >> sym >> t >> t;
+    }
+  }
+  int64_t nq = 0;
+  in >> nq >> sym;
+  if (!in || sym != "boundaryquads") return fail("%s: bad boundary header", path);
+  std::unordered_map<int32_t, int32_t> vmap;
+  auto local = [&](int32_t v) {
+    auto it = vmap.find(v);
+    if (it != vmap.end()) return it->second;
+    const int32_t id = (int32_t)vmap.size();
+    vmap.emplace(v, id);
+    for (int d = 0; d < 3; ++d) base.X.push_back(X[3 * (int64_t)v + d]);
+    return id;
+  };
+  base = Mesh();
+  base.dim = 3;
+  base.npc = 8;
+  for (int64_t h = 0; h < nh; ++h)
+    if (hlev[h] == 0)
+      for (int m = 0; m < 8; ++m) base.cells.push_back(local(hv[h][m]));
+  for (int64_t k = 0; k < nq; ++k) {
+    int64_t mat, lev, father, ofq, eiq, nch, c;
+    int32_t q[4];
+    in >> mat >> sym;
+    for (int a = 0; a < 4; ++a) in >> q[a];
+    in >> lev >> father >> ofq >> eiq >> sym >> nch;
+    if (!in || sym != "@") return fail("%s: bad boundary quad %lld", path, (long long)k);
+    for (int64_t a = 0; a < nch; ++a) in >> c;
+    if (lev != 0) continue;
+    for (int a = 0; a < 4; ++a) {
+      if (q[a] < 0 || q[a] >= nv) return fail("%s: boundary quad %lld: bad vertex", path, (long long)k);
+      base.bnd.push_back(local(q[a]));
+    }
+    base.bcol.push_back((int32_t)mat);
+  }
+  if (base.ncells() != 1) return fail("%s: the .gup reader takes the unit box (one macro hex)", path);
+  for (int m = 0; m < 8; ++m)
+    for (int d = 0; d < 3; ++d)
+      if (base.X[3 * base.cells[m] + d] != (double)kHexCorner[m][d])
+        return fail("%s: the macro hex is not the unit box in Gascoigne's corner order", path);
+  // finest cells: snapped lattice keys
+  leaves.clear();
+  for (int64_t h = 0; h < nh; ++h) {
+    if (hnch[h] != 0) continue;
+    const double* o = &X[3 * (int64_t)hv[h][0]];
+    const double* x1 = &X[3 * (int64_t)hv[h][6]];
+    const double sz = x1[0] - o[0];
+    if (!(sz > 0)) return fail("%s: hex %lld is not a box", path, (long long)h);
+    const int L = (int)std::lround(-std::log2(sz));
+    if (L < 1 || L > 18 || hlev[h] != L)
+      return fail("%s: hex %lld: level %d does not match its size", path, (long long)h, hlev[h]);
+    const double s = std::ldexp(1.0, -L);
+    int64_t ijk[3];
+    for (int d = 0; d < 3; ++d) ijk[d] = std::llround(o[d] / s);
+    for (int m = 0; m < 8; ++m)
+      for (int d = 0; d < 3; ++d) {
+        const double want = (ijk[d] + kHexCorner[m][d]) * s;
+        if (std::fabs(X[3 * (int64_t)hv[h][m] + d] - want) > 1e-4 * s)
+          return fail("%s: hex %lld is not a cell of the dyadic unit box", path, (long long)h);
+      }
+    leaves.push_back(ckey(L, ijk[0], ijk[1], ijk[2]));
   }
   return 0;
 }
@@ -1252,10 +1361,30 @@ int ghs_build_adaptive(const char* inp_path, int prerefine, int drop_coarse, int
     // the multigrid levels GascoigneMeshConstructor derives from the final
     // mesh (not the meshes after each step)
     std::vector<Mesh> lv;
-    if (adaptive_levels(all[1], all.back(), lv)) return -1;
+    std::vector<uint64_t> K;
+    if (cell_keys(all.back(), K) || adaptive_levels(all[1], K, lv)) return -1;
     all.resize(1);
     for (Mesh& m : lv) all.push_back(std::move(m));
   }
+  ghs_hier* h = new ghs_hier();
+  h->levels.assign(std::make_move_iterator(all.begin() + drop_coarse),
+                   std::make_move_iterator(all.end()));
+  *out = h;
+  return 0;
+}
+
+int ghs_build_gup(const char* gup_path, int drop_coarse, ghs_hier** out) {
+  if (!out || !gup_path || drop_coarse < 0) return fail("ghs_build_gup: bad arguments");
+  *out = nullptr;
+  std::vector<Mesh> all(2);
+  std::vector<uint64_t> K;
+  if (read_gup(gup_path, all[0], K)) return -1;
+  if (refine(all[0], all[1])) return -1;
+  std::vector<Mesh> lv;
+  if (adaptive_levels(all[1], K, lv)) return -1;
+  all.resize(1);
+  for (Mesh& m : lv) all.push_back(std::move(m));
+  if (drop_coarse >= (int)all.size()) return fail("ghs_build_gup: drop_coarse leaves no level");
   ghs_hier* h = new ghs_hier();
   h->levels.assign(std::make_move_iterator(all.begin() + drop_coarse),
                    std::make_move_iterator(all.end()));

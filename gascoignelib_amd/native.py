@@ -175,6 +175,7 @@ STRUCT_SIGNATURES = {
     "ghs_level_transfer": (C.c_int, [vp, C.c_int, i32p, i32p, i32p, i32p]),
     "ghs_structure_from_cells": (C.c_int, [C.c_int64, C.c_int64, C.c_int, i32p, i32p, i32p]),
     "ghs_colour_order": (C.c_int, [C.c_int64, i32p, i32p, i32p, i32p]),
+    "ghs_build_gup": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
     "ghs_assemble_on": (C.c_int, [C.c_int, C.c_int64, f64p, C.c_int64, i32p, f64p, i32p, i32p,
                                   C.c_int64, i32p, f64p]),
     "ghs_level_patch_sizes": (C.c_int, [vp, C.c_int, i64p, i64p]),

@@ -42,6 +42,18 @@ class Mesh:
         m.nlevels = N.structure().ghs_nlevels(m.h)
         return m
 
+    @classmethod
+    def from_gup(cls, path, drop_coarse=1):
+        """3D hierarchy of a Gascoigne hierarchical mesh file (ghs_build_gup):
+        the macro mesh and finest cells it holds, with the multigrid levels
+        Gascoigne derives from them"""
+        m = cls.__new__(cls)
+        m.dim = 3
+        m.h = C.c_void_p()
+        check_s(N.structure().ghs_build_gup(os.fspath(path).encode(), drop_coarse, C.byref(m.h)))
+        m.nlevels = N.structure().ghs_nlevels(m.h)
+        return m
+
     def sizes(self, l, with_nnz=False):
         n, c, z = C.c_int64(), C.c_int64(), C.c_int64()
         check_s(N.structure().ghs_level_sizes(self.h, l, C.byref(n), C.byref(c),
@@ -309,7 +321,7 @@ def stokes_hierarchy(prerefine, seed=0, visc=1.0, alpha=0.1, inp="channel.inp", 
 
 
 def q2_hierarchy(prerefine, inp="cube.inp", drop_coarse=1, colours=(1,), name=None,
-                 keep_coords=True, adaptive=0, radius=0.25, centre=(0.5, 0.5, 0.5)):
+                 keep_coords=True, adaptive=0, radius=0.25, centre=(0.5, 0.5, 0.5), gup=None):
     """Poisson with the triquadratic patch element (CGDiscQ23d, Laplace<3>) on a
     refined hex mesh: one unknown per mesh node, 2x2x2-cell patches as
     elements (up to 125 couplings per row), nodal transfers between levels
@@ -320,8 +332,12 @@ def q2_hierarchy(prerefine, inp="cube.inp", drop_coarse=1, colours=(1,), name=No
     centre), and the multigrid levels are those Gascoigne's mesh constructor
     derives from the result (coarser levels coarsen the whole mesh where they
     can), with Q2 hanging-node constraints and the operator condensed over
-    them."""
-    if adaptive:
+    them.  With gup, the mesh hierarchy comes from that Gascoigne hierarchical
+    mesh file (prerefine, inp and the ball are then unused)."""
+    if gup is not None:
+        M = Mesh.from_gup(gup, drop_coarse)
+        adaptive = 1
+    elif adaptive:
         M = Mesh.adaptive(inp, prerefine, drop_coarse, adaptive, radius, centre)
     else:
         M = Mesh(inp, 3, prerefine, drop_coarse)

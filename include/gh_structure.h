@@ -122,6 +122,15 @@ int ghs_level_assemble_q2(const ghs_hier* h, int l, int ncolours, const int32_t*
  * children where the level jumps allow.  The unit box only (node coordinates
  * on the dyadic lattice).  Levels otherwise as in ghs_build; levels with
  * hanging nodes carry the Q2 constraints. */
+/* The hierarchy of a Gascoigne hierarchical mesh file (.gup, written by
+ * MeshAgent::write_gup, src/Mesh/meshagent.cc; read as
+ * HierarchicalMesh3d::read_gup, src/Mesh/hierarchicalmesh3d.cc:1615-1700
+ * does): its macro mesh (level-0 hexes and boundary quads with their
+ * colours) and its finest cells, with the multigrid levels Gascoigne derives
+ * from them (as ghs_build_adaptive; Q2 hanging-node constraints where cells
+ * of different levels meet).  The unit box only: one macro hex, cells on the
+ * dyadic lattice. */
+int ghs_build_gup(const char* gup_path, int drop_coarse, ghs_hier** out);
 int ghs_build_adaptive(const char* inp_path, int prerefine, int drop_coarse, int nadapt,
                        double radius, const double* centre /* 3 */, ghs_hier** out);
 /* Q2 hanging-node constraints of level l (HNStructureQ23d layout):

@@ -2,7 +2,8 @@
 // reference MeshAgent (compiled from the reference sources by
 // oracle/ref/Makefile) and writes, per level, the node coordinates, the cell
 // and patch connectivity, the Dirichlet colour lists, the hanging-node maps
-// and the mesh-transfer maps (c2f / 2er / 4er / 8er) as .npy files.
+// and the mesh-transfer maps (c2f / 2er / 4er / 8er) as .npy files, and the
+// hierarchical mesh itself as mesh.gup (MeshAgent::write_gup).
 //
 // Test infrastructure: the output is the bit-exact numbering that the
 // structure producer in gascoignelib_amd/ must reproduce.
@@ -149,6 +150,8 @@ int main(int argc, char** argv) {
       MA.refine_nodes(ref);
     }
   }
+  // the hierarchical mesh in Gascoigne's own text format (mesh.gup)
+  MA.write_gup(out + "/mesh");
   std::FILE* f = std::fopen((out + "/meta.txt").c_str(), "w");
   std::fprintf(f, "nlevels %d\ndim %d\n", MA.nlevels(), MA.GetDimension());
   for (int l = 0; l < MA.nlevels(); ++l)

@@ -42,6 +42,9 @@ FIXTURES = {
     "stokes_3d_8": (3, "channel.inp", 3, 1, "Q13dLps", "1,2", None),
 }
 
+# fixtures whose hierarchical mesh file (.gup) is kept too, gzipped
+GUP = ("q2_3d_8_ada",)
+
 # Equation block of the param file per discretisation (the Stokes workload's)
 EQUATION = {"Q13dLps": "//Block Equation\nvisc 1.0\nalpha 0.1\n"}
 
@@ -68,6 +71,13 @@ def run(name):
                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         subprocess.check_call([os.path.join(REF_BIN, "la_dump"), param, ldir, disc, cols] + extra,
                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        if name in GUP:
+            # the hierarchical mesh as Gascoigne writes it (MeshAgent::write_gup)
+            import gzip
+            import shutil
+            with open(os.path.join(mdir, "mesh.gup"), "rb") as src, \
+                    gzip.open(os.path.join(OUT, name + ".gup.gz"), "wb", 9) as dst:
+                shutil.copyfileobj(src, dst)
         arrays = {}
         meta = open(os.path.join(mdir, "meta.txt")).read().split()
         nlevels = int(meta[1])

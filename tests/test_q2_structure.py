@@ -233,17 +233,28 @@ def test_adaptive_step_that_marks_nothing_keeps_the_uniform_levels():
         assert np.array_equal(lh.coords, lu.coords) and np.array_equal(lh.col, lu.col)
 
 
-def test_adaptive_levels_equal_reference(golden):
+@pytest.mark.parametrize("source", ["refine", "gup"])
+def test_adaptive_levels_equal_reference(golden, tmp_path, source):
     """the adaptive hierarchy of the fixture's refinement (cube.inp, prerefine
     3, two refine_nodes steps of the nodes in the sphere of radius 0.08 around
-    (0.41, 0.53, 0.47)) against the compiled reference's MeshAgent"""
+    (0.41, 0.53, 0.47)) against the compiled reference's MeshAgent: made by
+    the producer's own refinement, and read from the .gup file the
+    reference's MeshAgent::write_gup wrote for that mesh"""
+    import gzip
+    import shutil
     import sys
     sys.path.insert(0, GOLDEN)
     import digests as D
     from gascoignelib_amd.structure import Mesh
     z = golden("q2_3d_8_ada")
     nl = int(z["nlevels"])
-    M = Mesh.adaptive("cube.inp", 3, 1, 2, 0.08, (0.41, 0.53, 0.47))
+    if source == "refine":
+        M = Mesh.adaptive("cube.inp", 3, 1, 2, 0.08, (0.41, 0.53, 0.47))
+    else:
+        gup = tmp_path / "mesh.gup"
+        with gzip.open(os.path.join(GOLDEN, "q2_3d_8_ada.gup.gz"), "rb") as src, open(gup, "wb") as dst:
+            shutil.copyfileobj(src, dst)
+        M = Mesh.from_gup(gup, 1)
     assert M.nlevels == nl
     w1 = np.array([0.375, 0.75, -0.125])
     wf = np.outer(w1, w1).ravel()
@@ -276,3 +287,13 @@ def test_adaptive_levels_equal_reference(golden):
 def test_adaptive_bad_arguments_raise(kw):
     with pytest.raises(RuntimeError, match="ghs_build_adaptive"):
         q2_hierarchy(2, **kw)
+
+
+def test_gup_errors(tmp_path):
+    from gascoignelib_amd.structure import Mesh
+    bad = tmp_path / "bad.gup"
+    bad.write_text("2 dimension\n")
+    with pytest.raises(RuntimeError, match="not a 3D .gup"):
+        Mesh.from_gup(bad)
+    with pytest.raises(RuntimeError, match="cannot open"):
+        Mesh.from_gup(tmp_path / "missing.gup")
