@@ -2,8 +2,8 @@
 // node-block matrix, LU with partial pivoting on the device, and the solve
 //   x <- (1/count_i) sum_p R_p^T A_p^-1 R_p x.
 // Factoring: one workgroup of 1024 threads per patch, the m x m patch matrix
-// (m <= 128) in LDS, the LU's operations uncontracted so that the factors
-// equal the oracle's; the recorded row exchanges are composed on the host into
+// (m <= 128) in the threads' registers, the LU's operations uncontracted so
+// that the factors equal the oracle's; the recorded row exchanges are composed on the host into
 // one gather per factor row.  Solving: one wavefront per patch, the local vector
 // gathered in pivoted order into the lanes' registers (rows lane and
 // lane + 64), column-oriented triangular solves (each pass reads only its
@@ -23,9 +23,19 @@ namespace {
 constexpr int kVankaMaxM = 128;       // local unknowns per patch
 constexpr int kVankaMaxNodes = 128;   // nodes per patch
 
-// Factoring one patch: the m x m matrix in LDS (93 KB at m = 108, so one
-// patch per CU), 1024 threads, two barriers per column (below).
+// Factoring one patch: 1024 threads, lane tr = t % 64 of wavefront tc = t / 64
+// keeps the entries (tr + 64 i, tc + 16 j), i < 2, j < 8, of the m x m patch
+// matrix (m <= 128) in registers, so the 64 owners of a column form one
+// wavefront and its pivot search is a wavefront reduction.  Row exchanges are
+// not carried out: every thread keeps the position of each of its rows in the
+// exchanged order, and the factors are written in that order at the end.
+// LDS holds only the patch's CSR positions and, per column, the multipliers
+// and the pivot (double-buffered), a few KB, so two patches share a CU.
 constexpr int kFactorThreads = 1024;
+constexpr int kFactorLanes = 64;                          // rows tr + 64 i
+constexpr int kFactorWaves = kFactorThreads / kFactorLanes;  // columns tc + 16 j
+constexpr int kRowsPer = kVankaMaxM / kFactorLanes;       // 2
+constexpr int kColsPer = kVankaMaxM / kFactorWaves;       // 8
 
 // DPP move of a double (both halves); lanes the masks leave out read 0.
 template <int CTRL, int ROWMASK, int BANKMASK>
@@ -56,150 +66,190 @@ __device__ __forceinline__ double wave_max_nonneg(double x) {
   return lane_read(x, 63);
 }
 
-__global__ __launch_bounds__(kFactorThreads) void k_vanka_factor(
+// Minimum over the 64 lanes of a wavefront, the same pattern; lanes the masks
+// leave out keep INT_MAX.
+template <int CTRL, int ROWMASK, int BANKMASK>
+__device__ __forceinline__ int dpp_min(int x) {
+  return min(x, __builtin_amdgcn_update_dpp(INT_MAX, x, CTRL, ROWMASK, BANKMASK, false));
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+  x = dpp_min<0x111, 0xf, 0xf>(x);
+  x = dpp_min<0x112, 0xf, 0xf>(x);
+  x = dpp_min<0x114, 0xf, 0xe>(x);
+  x = dpp_min<0x118, 0xf, 0xc>(x);
+  x = dpp_min<0x142, 0xa, 0xf>(x);
+  x = dpp_min<0x143, 0xc, 0xf>(x);
+  return __builtin_amdgcn_readlane(x, 63);
+}
+
+struct FactorShared {
+  double l[2][kVankaMaxM];  // the multipliers of the column, by row
+  int32_t pr[2], rp[2];     // the pivot's position before the exchange, and its row
+  double d[2];              // the pivot
+};
+
+// Columns 16 KB .. 16 KB + 15 (KB a constant, so that the column's registers
+// are known): per column k, one barrier.
+//   A: the wavefront owning column k (already updated by itself) finds the
+//      pivot by a maximum of |a_ik| and a minimum of position * 256 + row,
+//      forms the multipliers, puts them and the pivot into LDS, records
+//      piv[k];
+//   B: every thread exchanges the positions k and piv[k] among its rows and
+//      updates its entries below and right of (k, k): the pivot row's entries
+//      in a wavefront's columns are held by one of its own lanes (rp % 64),
+//      which broadcasts them.
+// The buffers alternate between columns, so the owner of column k + 1 may
+// start before the others have finished column k.
+template <int KB>
+__device__ __forceinline__ void factor_columns(int m, int tr, int tc,
+                                               double (&a)[kRowsPer][kColsPer],
+                                               int (&pos)[kRowsPer], FactorShared& S,
+                                               int32_t* __restrict__ pv) {
+  for (int kk = 0; kk < kFactorWaves; ++kk) {
+    const int k = KB * kFactorWaves + kk;
+    if (k >= m) return;
+    const int buf = k & 1;
+    if (tc == kk) {
+      double best = -1.0;  // fmax passes NaN over
+#pragma unroll
+      for (int i = 0; i < kRowsPer; ++i)
+        if (pos[i] >= k) best = fmax(best, fabs(a[i][KB]));
+      best = wave_max_nonneg(best);
+      // the smallest position holding the maximum (a NaN a_kk keeps row k, as
+      // the oracle's strict comparisons do); a key always exists, since a
+      // non-NaN a_kk makes best >= |a_kk| attained
+      int key = INT_MAX;
+#pragma unroll
+      for (int i = 0; i < kRowsPer; ++i) {
+        const double v = a[i][KB];
+        const bool cand = pos[i] > k ? fabs(v) == best
+                                     : pos[i] == k && (fabs(v) == best || v != v);
+        if (cand) key = min(key, pos[i] * 256 + tr + kFactorLanes * i);
+      }
+      key = wave_min(key);
+      const int pr = key >> 8, rp = key & 255;
+      const double d = lane_read(rp < kFactorLanes ? a[0][KB] : a[1][KB], rp % kFactorLanes);
+      if (d != 0.0) {
+#pragma unroll
+        for (int i = 0; i < kRowsPer; ++i) {
+          const int r = tr + kFactorLanes * i;
+          if (pos[i] >= k && r != rp) {
+            const double l = a[i][KB] / d;
+            a[i][KB] = l;
+            S.l[buf][r] = l;
+          }
+        }
+      }
+      if (tr == 0) {
+        S.pr[buf] = pr;
+        S.rp[buf] = rp;
+        S.d[buf] = d;
+        pv[k] = pr;
+      }
+    }
+    __syncthreads();
+    const int pr = __builtin_amdgcn_readfirstlane(S.pr[buf]);
+    const int rp = __builtin_amdgcn_readfirstlane(S.rp[buf]);
+    const double d = S.d[buf];
+#pragma unroll
+    for (int i = 0; i < kRowsPer; ++i) {
+      const int r = tr + kFactorLanes * i;
+      pos[i] = (r == rp) ? k : (pos[i] == k ? pr : pos[i]);
+    }
+    if (d != 0.0) {
+      double l[kRowsPer];
+#pragma unroll
+      for (int i = 0; i < kRowsPer; ++i) l[i] = pos[i] > k ? S.l[buf][tr + kFactorLanes * i] : 0.0;
+#pragma unroll
+      for (int j = KB; j < kColsPer; ++j) {
+        // column tc + 16 j > k: every j > KB, and j = KB right of the pivot
+        if (!((j > KB || tc > kk) && tc + kFactorWaves * j < m)) continue;
+        // the pivot row's entry in this column, from the lane holding it
+        const double u = lane_read(rp < kFactorLanes ? a[0][j] : a[1][j], rp % kFactorLanes);
+#pragma unroll
+        for (int i = 0; i < kRowsPer; ++i)
+          if (pos[i] > k) a[i][j] = add_rn(a[i][j], -mul_rn(l[i], u));
+      }
+    }
+  }
+}
+
+// LU with partial pivoting of every patch matrix, the oracle's operations in
+// the oracle's order per entry: pivot = the first row (in the exchanged order)
+// of largest |a_ik|, i >= k; no elimination when the pivot is 0;
+// l_i = a_ik / d and a_ij -= l_i a_kj, product and difference rounded
+// separately.  (8 waves per SIMD: two patches per CU, at most 64 VGPRs.)
+__global__ __launch_bounds__(kFactorThreads, 8) void k_vanka_factor(
     int64_t npatches, int psize, int B, const int32_t* __restrict__ nodes,
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const double* __restrict__ val, double* __restrict__ lu64, float* __restrict__ lu32,
     int32_t* __restrict__ piv) {
-  extern __shared__ double a[];  // m * m, row-major
+  extern __shared__ int32_t tab[];  // psize * psize CSR positions of the node blocks, -1: none
   __shared__ int32_t pn[kVankaMaxNodes];
-  __shared__ int s_piv;
-  __shared__ double s_pv, s_akk;
+  __shared__ FactorShared S;
   constexpr int NT = kFactorThreads;
   const int64_t p = blockIdx.x;
   const int m = psize * B;
-  const int t = threadIdx.x;
-  for (int e = t; e < m * m; e += NT) a[e] = 0.0;
+  const int t = threadIdx.x, tr = t % kFactorLanes, tc = t / kFactorLanes;
+  for (int e = t; e < psize * psize; e += NT) tab[e] = -1;
   for (int r = t; r < psize; r += NT) pn[r] = nodes[p * psize + r];
   __syncthreads();
-  // copy_entries_sparseblockmatrix: entries whose column is a patch node.
-  // The patch's rows share the threads: NT / psize threads walk each row's
-  // entries (every entry lands in its own place, so the order is free).
+  // copy_entries_sparseblockmatrix: the entries whose column is a patch node.
+  // NT / psize threads (at least 8) walk each row's entries.
   {
-    const int per = NT / psize;  // psize <= 128, so at least 8 threads per row
+    const int per = NT / psize;
     const int r = t / per, u0 = t - r * per;
     if (r < psize) {
       const int32_t row = pn[r];
       for (int32_t q = rowptr[row] + u0; q < rowptr[row + 1]; q += per) {
         const int32_t c0 = col[q];
-        int c = -1;
         for (int u = 0; u < psize; ++u)
-          if (pn[u] == c0) c = u;
-        if (c < 0) continue;
-        for (int cr = 0; cr < B; ++cr)
-          for (int cc = 0; cc < B; ++cc)
-            a[(B * r + cr) * m + B * c + cc] = val[((int64_t)q * B + cr) * B + cc];
+          if (pn[u] == c0) tab[r * psize + u] = q;
       }
     }
   }
   __syncthreads();
-  // LU with partial pivoting (first largest |a_ik|), two barriers per column:
-  //   phase 1: swap rows k and pr (every column but k), divide column k by the
-  //            pivot d (the division of row pr takes the old a_kk, saved by
-  //            the search, since the swap moves that row's column-k entry);
-  //   phase 2: a_ij -= l_i a_kj; wavefront 0 updates column k + 1 itself and
-  //            searches it for the next pivot in the same phase, the others
-  //            update columns k + 2, ... (every product and difference
-  //            rounded separately, as the oracle computes them).
-  // The search (wavefront 0; lanes hold rows k + lane and k + lane + 64): the
-  // largest |a_ik| by a DPP maximum, then the first row holding it (rows
-  // k..k+63 before k+64..) by ballots.
-  auto search = [&](int k, double v0, double v1) {
-    const bool in0 = k + t < m, in1 = k + t + 64 < m;
-    const double best = wave_max_nonneg(fmax(in0 ? fabs(v0) : -1.0, in1 ? fabs(v1) : -1.0));
-    const uint64_t b0 = __ballot(in0 && fabs(v0) == best);
-    int bi;
-    double bv;
-    if (b0) {
-      const int l = __builtin_ctzll(b0);
-      bi = k + l;
-      bv = lane_read(v0, l);
-    } else {
-      const uint64_t b1 = __ballot(in1 && fabs(v1) == best);
-      const int l = __builtin_ctzll(b1);
-      bi = k + 64 + l;
-      bv = lane_read(v1, l);
-    }
-    const double akk = lane_read(v0, 0);  // row k itself
-    if (t == 0) {
-      s_piv = bi;
-      s_pv = bv;
-      s_akk = akk;
-    }
-  };
-  if (t < 64) search(0, t < m ? a[t * m] : 0.0, t + 64 < m ? a[(t + 64) * m] : 0.0);
-  __syncthreads();
-  constexpr int kUpd = NT - 64;            // threads updating columns k + 2 ...
-  constexpr int kGroups = kUpd / 128;      // 7 row groups of 128 columns
-  const int tu = t - 64, tj = tu & 127, tg = tu >> 7;
-  for (int k = 0; k < m; ++k) {
-    const int pr = s_piv;
-    const double d = s_pv, akk = s_akk;
-    if (pr != k)
-      for (int j = t; j < m; j += NT) {
-        if (j == k) continue;
-        const double x = a[k * m + j];
-        a[k * m + j] = a[pr * m + j];
-        a[pr * m + j] = x;
+  double a[kRowsPer][kColsPer];
+  int pos[kRowsPer];  // position of row tr + 64 i in the exchanged order; -1: no row
+#pragma unroll
+  for (int i = 0; i < kRowsPer; ++i) {
+    const int r = tr + kFactorLanes * i;
+    pos[i] = r < m ? r : -1;
+#pragma unroll
+    for (int j = 0; j < kColsPer; ++j) {
+      const int c = tc + kFactorWaves * j;
+      double x = 0.0;
+      if (r < m && c < m) {
+        const int32_t q = tab[(r / B) * psize + c / B];
+        if (q >= 0) x = val[((int64_t)q * B + r % B) * B + c % B];
       }
-    if (t == 0) {
-      a[k * m + k] = (pr != k) ? d : akk;
-      piv[p * m + k] = pr;
+      a[i][j] = x;
     }
-    if (d != 0.0) {
-      for (int i = k + 1 + t; i < m; i += NT) {
-        const double x = (i == pr) ? akk : a[i * m + k];
-        a[i * m + k] = x / d;
-      }
-    } else if (pr != k && t == 0) {
-      a[pr * m + k] = akk;
-    }
-    __syncthreads();
-    if (t < 64) {
-      // column k + 1: update (when d != 0), then search it
-      const int j = k + 1;
-      if (j < m) {
-        double v0 = 0.0, v1 = 0.0;
-        const double akj = a[k * m + j];
-        const int i0 = j + t, i1 = j + t + 64;
-        if (i0 < m) {
-          v0 = a[i0 * m + j];
-          if (d != 0.0) {
-            v0 = add_rn(v0, -mul_rn(a[i0 * m + k], akj));
-            a[i0 * m + j] = v0;
-          }
-        }
-        if (i1 < m) {
-          v1 = a[i1 * m + j];
-          if (d != 0.0) {
-            v1 = add_rn(v1, -mul_rn(a[i1 * m + k], akj));
-            a[i1 * m + j] = v1;
-          }
-        }
-        search(j, v0, v1);
-      }
-    } else if (d != 0.0 && tg < kGroups) {
-      const int j = k + 2 + tj;
-      if (j < m) {
-        const double akj = a[k * m + j];
-        int i = k + 1 + tg;
-        for (; i + kGroups < m; i += 2 * kGroups) {
-          const double l0 = a[i * m + k], l1 = a[(i + kGroups) * m + k];
-          const double x0 = a[i * m + j], x1 = a[(i + kGroups) * m + j];
-          a[i * m + j] = add_rn(x0, -mul_rn(l0, akj));
-          a[(i + kGroups) * m + j] = add_rn(x1, -mul_rn(l1, akj));
-        }
-        if (i < m) a[i * m + j] = add_rn(a[i * m + j], -mul_rn(a[i * m + k], akj));
-      }
-    }
-    __syncthreads();
   }
-  // column-major factors for coalesced column reads in the solve
+  int32_t* pv = piv + p * m;
+  factor_columns<0>(m, tr, tc, a, pos, S, pv);
+  factor_columns<1>(m, tr, tc, a, pos, S, pv);
+  factor_columns<2>(m, tr, tc, a, pos, S, pv);
+  factor_columns<3>(m, tr, tc, a, pos, S, pv);
+  factor_columns<4>(m, tr, tc, a, pos, S, pv);
+  factor_columns<5>(m, tr, tc, a, pos, S, pv);
+  factor_columns<6>(m, tr, tc, a, pos, S, pv);
+  factor_columns<7>(m, tr, tc, a, pos, S, pv);
+  // column-major factors (coalesced column reads in the solve), rows in the
+  // exchanged order
   const int64_t base = p * (int64_t)m * m;
-  for (int e = t; e < m * m; e += NT) {
-    const int k = e / m, i = e - k * m;
-    if (lu64) lu64[base + e] = a[i * m + k];
-    else lu32[base + e] = (float)a[i * m + k];
+#pragma unroll
+  for (int i = 0; i < kRowsPer; ++i) {
+    if (pos[i] < 0) continue;
+#pragma unroll
+    for (int j = 0; j < kColsPer; ++j) {
+      const int c = tc + kFactorWaves * j;
+      if (c >= m) continue;
+      const int64_t e = base + (int64_t)c * m + pos[i];
+      if (lu64) lu64[e] = a[i][j];
+      else lu32[e] = (float)a[i][j];
+    }
   }
 }
 
@@ -350,6 +400,7 @@ int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, cons
   const int B = A->ncomp, m = psize * B;
   GH_ARG(psize <= kVankaMaxNodes && m <= kVankaMaxM,
          "gh_vanka_create: at most 128 unknowns per patch");
+  GH_ARG(A->nnz > 0 || npatches == 0, "gh_vanka_create: patches of a matrix without entries");
   const int64_t n = A->n;
   // node -> (patch, local) lists, ascending patch; every node must be covered
   std::vector<int32_t> nptr(n + 1, 0), nent((size_t)npatches * psize);
@@ -399,7 +450,7 @@ int vanka_create(gh_ctx* ctx, const gh_csr* A, int64_t npatches, int psize, cons
       hipMemcpy(V->nent, nent.data(), sizeof(int32_t) * npatches * psize, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(GH_EHIP, "gh_vanka_create: upload failed"));
   if (npatches > 0) {
-    const size_t lds = sizeof(double) * (size_t)m * m;
+    const size_t lds = sizeof(int32_t) * (size_t)psize * psize;
     if (hipFuncSetAttribute((const void*)k_vanka_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
       return bail(fail(GH_EHIP, "gh_vanka_create: %zu bytes of LDS per patch not available", lds));
