@@ -197,20 +197,30 @@ def main():
         return kernel_only(ctx, hier, f, args)
     common = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
                   ndirect=1000, order=None if args.order == "none" else args.order)
-    t0 = time.perf_counter()
-    mg64 = H.CudaMultiLevelSolver(ctx, hier, prec="f64", **common)
+    t_solvers = {}
+
+    def build(name, **kw):
+        t0 = time.perf_counter()
+        mg = H.CudaMultiLevelSolver(ctx, hier, **kw)
+        t_solvers[name] = time.perf_counter() - t0
+        return mg
+
+    mg64 = build("fp64", prec="f64", **common)
     mpre, mpost = args.mixed_smooth
     common = dict(common, iterpre=args.fp64_smooth[0], iterpost=args.fp64_smooth[1])
     mixed_common = dict(common, iterpre=mpre, iterpost=mpost)
-    mgmx = H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed, krylov=args.mixed_krylov,
-                                  **mixed_common)
+    mgmx = build("mixed", prec=args.mixed, krylov=args.mixed_krylov, **mixed_common)
     same = (mpre, mpost) == args.fp64_smooth
-    mgmx22 = None if same else H.CudaMultiLevelSolver(ctx, hier, prec=args.mixed,
-                                                       krylov=args.mixed_krylov, **common)
+    mgmx22 = None if same else build("mixed_same_cycle", prec=args.mixed,
+                                     krylov=args.mixed_krylov, **common)
     # the plain FP64 path run like the mixed one: the same flexible GMRES, restart
     # length, cycle and stopping rule, FP64 V-cycle (the like-for-like baseline)
-    mg64f = H.CudaMultiLevelSolver(ctx, hier, prec="f64", krylov="fgmres", **mixed_common)
-    t_setup = time.perf_counter() - t0
+    mg64f = build("fp64_same_stop", prec="f64", krylov="fgmres", **mixed_common)
+    # the FP32-matrix / FP32-vector V-cycle (FP32-stored values), run like the
+    # mixed one
+    mgf32 = None if args.mixed == "f32" else build("mixed_fp32_matrix", prec="f32",
+                                                   krylov=args.mixed_krylov, **mixed_common)
+    t_setup = sum(t_solvers.values())
     F = H.CudaVectorInterface.from_global(ctx, f)
     U64, UMX = mg64.new_vector(), mgmx.new_vector()
 
@@ -247,6 +257,9 @@ def main():
         ts22, rep22 = timed(mgmx22, U22, f"mixed-{args.mixed} V(2,2)", args.mixed_linear_tol)
     U64F = mg64f.new_vector()
     ts64f, rep64f = timed(mg64f, U64F, f"fp64 fgmres V({mpre},{mpost})", 0.9e-10)
+    if mgf32 is not None:
+        UF32 = mgf32.new_vector()
+        tsf32, repf32 = timed(mgf32, UF32, f"mixed-f32 V({mpre},{mpost})", args.mixed_linear_tol)
     u64, umx = U64.copy_back(), UMX.copy_back()
 
     # true FP64 residuals and the mixed path's error vs the FP64 solve
@@ -259,6 +272,18 @@ def main():
         return R.norm() / F.norm()
     rr64, rrmx, rr64f = true_rel_res(U64), true_rel_res(UMX), true_rel_res(U64F)
     err_mx = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+    f32_line = None
+    if mgf32 is not None:
+        tf32 = float(np.median(tsf32))
+        f32_line = {"dtype": "f64 outer / f32 V-cycle (f32-stored matrix values)",
+                    "cycle": "V(%d,%d)" % (mpre, mpost), "krylov": args.mixed_krylov,
+                    "linear_tol": args.mixed_linear_tol, "ms_per_solve": tf32 * 1e3,
+                    "dof_per_s": ndof / tf32, "gmres": repf32.gmres_iterations,
+                    "true_rel_residual": true_rel_res(UF32),
+                    "rel_error_vs_fp64": float(np.linalg.norm(UF32.copy_back() - u64)
+                                               / np.linalg.norm(u64)),
+                    "speedup_vs_fp64": float(np.median(ts64)) / tf32,
+                    "speedup_vs_fp64_same_stop": float(np.median(ts64f)) / tf32}
 
     # roofline of the dominant kernel: the FP64 residual SpMV on the finest
     # level, on the very matrix the FP64 solve runs (the solver's internal
@@ -326,6 +351,7 @@ def main():
                            "dof_per_s": ndof / t64f, "gmres": rep64f.gmres_iterations,
                            "true_rel_residual": rr64f,
                            "speedup_mixed_vs_this": t64f / tmx},
+        "mixed_fp32_matrix": f32_line,
         "mixed_same_cycle": None if mgmx22 is None else {
             "cycle": "V(%d,%d), the FP64 path's" % args.fp64_smooth,
             "ms_per_solve": float(np.median(ts22)) * 1e3,
@@ -345,7 +371,8 @@ def main():
                      "format_frac": gbs64 / HBM_PEAK_GBS},
         "spmv_mixed": {"gbs": bmx / t_spmv_mx / 1e9, "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS,
                        "bytes_per_launch": bmx, "us_per_launch": t_spmv_mx * 1e6},
-        "setup_s": dict({"structure": t_struct, "device_upload": t_setup}, **t_asm),
+        "setup_s": dict({"structure": t_struct, "device_upload": t_setup,
+                         "solvers": t_solvers}, **t_asm),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
