@@ -332,10 +332,11 @@ def main():
         "data": "synthetic: seeded uniform [-1,1] nodal f keyed by lattice position, Dirichlet 0",
         "config": {"workload": f"3D Q1 Poisson, unit cube, {args.cells}^3 cells",
                    "dof": ndof, "nnz": fine.nnz, "levels": hier.nlevels,
-                   "solver": "FP64: Newton/IR + GMRES(10) left-preconditioned, V(2,2); mixed: "
-                             + (f"flexible GMRES({args.mixed_vmax})"
-                                if args.mixed_krylov == "fgmres" else "Newton/IR + GMRES(10)")
-                             + f", V({mpre},{mpost}); Jacobi omega=0.8, ndirect=1000",
+                   # kept short: the record's readers truncate long strings
+                   "solver": "FP64 Newton+GMRES(10) V(2,2); mixed "
+                             + (f"FGMRES({args.mixed_vmax})"
+                                if args.mixed_krylov == "fgmres" else "Newton+GMRES(10)")
+                             + f" V({mpre},{mpost}); Jacobi 0.8; ndirect 1000",
                    "order": args.order},
         "fp64": {"dof_per_s": dofs_64, "ms_per_solve": t64 * 1e3, "linear_tol": 1e-10,
                  "newton": rep64.newton_iterations, "gmres": rep64.gmres_iterations,
@@ -1146,9 +1147,8 @@ def cpu_baseline_ilu(hier, f, io, sm, reps=2):
     dt = float(np.median(tsol))
     return {"value": len(f) / dt, "unit": "DOF/s", "cores": int(O.lib().or_num_threads()),
             "kind": "port",
-            "sample": f"full FP64 solves with the ILU smoother V({sm[0]},{sm[1]}) of the same "
-                      f"{len(f)}-DOF problem, median of {reps} ({dt:.1f} s each, gmres "
-                      f"{rep.gmres_iterations}); reference threading",
+            "sample": f"median of {reps} full FP64 solves, ILU V({sm[0]},{sm[1]}), {len(f)} DOF, "
+                      f"{dt:.1f} s each, gmres {rep.gmres_iterations}; reference threading",
             "ilu_factor_finest_s": t_factor, "ilu_solve_finest_ms": float(np.median(ts)) * 1e3,
             "mg_setup_s": t_setup, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
@@ -1191,9 +1191,8 @@ def cpu_baseline(hier, f, reps=3):
     ref = out["reference_threading"]
     return {"value": ref["dof_per_s"], "unit": "DOF/s", "cores": int(O.lib().or_num_threads()),
             "kind": "port",
-            "sample": f"full FP64 solves of the same {len(f)}-DOF problem, median of {reps} "
-                      f"({ref['s_per_solve']:.1f} s each, newton {ref['newton']}, "
-                      f"gmres {ref['gmres']}); reference threading",
+            "sample": f"median of {reps} full FP64 solves, {len(f)} DOF, "
+                      f"{ref['s_per_solve']:.1f} s each; reference threading",
             "variants": out, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
