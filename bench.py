@@ -971,9 +971,48 @@ def bench_q2(args, adaptive=0):
         "residual_spmv_mixed": {"us_per_launch": t_spmv_mx * 1e6, "bytes_per_launch": bmx,
                                 "gbs": bmx / t_spmv_mx / 1e9,
                                 "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS, "layout": lay_mx},
-        "setup_s": {"structure": t_struct, "device": t_setup},
+        "setup_s": dict({"structure": t_struct, "device": t_setup},
+                        **({} if adaptive else q2_assembly_times(ctx, hier))),
     }
     print(json.dumps(line), flush=True)
+
+
+def q2_assembly_times(ctx, hier, reps=3):
+    """Seconds to assemble the finest level's Q2 operator (Laplace on the
+    triquadratic patch elements, Dirichlet rows): on the host producer (its
+    threads) and on the device (gh_assemble_q2) into a matrix of the same
+    pattern in the caller's numbering; each the median of `reps` after one
+    warm-up.  The device values equal the host's bit for bit (checked)."""
+    from gascoignelib_amd import hip as H
+    M = hier.mesh
+    l = M.nlevels - 1
+    rp, col = M.patch_structure(l)
+    d = M.dirichlet(l, (1,))
+
+    def host():
+        return M.assemble_q2(l, len(col), (1,))
+
+    val = host()
+    A = H.CudaCSRMatrixInterface(ctx, rp, col, np.where(val != 0.0, val, 1.0))
+    P = H.DwrPatches(ctx, 3, M.coords(l), M.patches(l))
+    mask = H.DirichletMask(ctx, len(rp) - 1, 1, d)
+
+    def device():
+        P.assemble_q2(A, mask=mask)
+        ctx.sync()
+
+    out = {}
+    for name, fn in (("assembly_q2_host_fine", host), ("assembly_q2_device_fine", device)):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        out[name] = float(np.median(ts))
+    if not np.array_equal(A.values(), val):
+        raise RuntimeError("device Q2 assembly differs from the host producer")
+    return out
 
 
 def ilu_bytes(n, nnz, sv, sx):

@@ -19,7 +19,8 @@
 //
 // CGDisc::Form (cgdisc.h:415-448) is the residual counterpart: y += d * sum
 // over cells of the element residual.  gh_form_q1 forms r = f - A(u) cell by
-// cell from u without any matrix.
+// cell from u without any matrix.  gh_assemble_q2 / gh_form_q2 do the same
+// for the triquadratic patch elements of CGDiscQ23d.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -175,6 +176,107 @@ __global__ __launch_bounds__(kBlock) void k_form_q1(
   r[i] = acc;
 }
 
+// ------------------------------------------------ triquadratic patch elements
+// The Q2 element of CGDiscQ23d (src/DofHandler/cgdisc.h:1292) on a patch: the
+// 27 patch nodes carry the triquadratic Lagrange basis (BaseQ23d), and for the
+// Laplacian on an axis-aligned box the element matrix is a sum of tensor
+// products of the 1D quadratic stiffness S = (7,-8,1;-8,16,-8;1,-8,7)/(3L) and
+// mass M = (4,2,-1;2,16,2;-1,2,4) L/30 on [0, L], L = 2h (the reference's
+// 3x3x3 Gauss rule is exact for these products):
+//   E_ab = S_x M_y M_z + M_x S_y M_z + M_x M_y S_z,
+// each factor, product and sum rounded separately in that order, as the host
+// producer forms it.
+__device__ __forceinline__ double q2s(int p, int q, double L) {
+  const double s = (p == q) ? (p == 1 ? 16.0 : 7.0) : ((p == 1 || q == 1) ? -8.0 : 1.0);
+  return s / mul_rn(3.0, L);
+}
+__device__ __forceinline__ double q2m(int p, int q, double L) {
+  const double m = (p == q) ? (p == 1 ? 16.0 : 4.0) : ((p == 1 || q == 1) ? 2.0 : -1.0);
+  return mul_rn(m, L) / 30.0;
+}
+__device__ __forceinline__ double q2_entry(int a, int b, const double* __restrict__ hs) {
+  const double lx = mul_rn(2.0, hs[0]), ly = mul_rn(2.0, hs[1]), lz = mul_rn(2.0, hs[2]);
+  const int ax = a % 3, ay = (a / 3) % 3, az = a / 9, bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+  const double sx = q2s(ax, bx, lx), sy = q2s(ay, by, ly), sz = q2s(az, bz, lz);
+  const double mx = q2m(ax, bx, lx), my = q2m(ay, by, ly), mz = q2m(az, bz, lz);
+  const double t1 = mul_rn(mul_rn(sx, my), mz);
+  const double t2 = mul_rn(mul_rn(mx, sy), mz);
+  const double t3 = mul_rn(mul_rn(mx, my), sz);
+  return add_rn(add_rn(t1, t2), t3);
+}
+
+// One thread per row i: the row's entries start at 0 and receive, patch by
+// patch in ascending order and node by node, k_p E_ab at the entry of node b
+// (found by bisection: the row's columns ascend), as the host producer adds
+// them; then the Dirichlet rows become unit rows and the Dirichlet columns
+// zero.  A patch node without an entry in the row sets flags[2].
+__global__ __launch_bounds__(kBlock) void k_assemble_q2(
+    int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const int32_t* __restrict__ pptr, const int32_t* __restrict__ plist,
+    const int32_t* __restrict__ nodes, const double* __restrict__ hs,
+    const double* __restrict__ kp, const uint8_t* __restrict__ isdir, double* __restrict__ val,
+    int* flags) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t q0 = rowptr[i], q1 = rowptr[i + 1];
+  for (int32_t q = q0; q < q1; ++q) val[q] = 0.0;
+  bool bad = false;
+  for (int32_t e = pptr[i]; e < pptr[i + 1]; ++e) {
+    const int64_t p = plist[e] >> 5;
+    const int a = plist[e] & 31;
+    const int32_t* v = nodes + p * 27;
+    const double k = kp ? kp[p] : 1.0;
+    for (int b = 0; b < 27; ++b) {
+      const int32_t j = v[b];
+      int32_t lo = q0, hi = q1;  // first entry with col >= j
+      while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        if (col[mid] < j) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo == q1 || col[lo] != j) {
+        bad = true;
+        continue;
+      }
+      val[lo] = add_rn(val[lo], mul_rn(k, q2_entry(a, b, hs + 3 * p)));
+    }
+  }
+  if (isdir) {
+    const bool drow = isdir[i];
+    for (int32_t q = q0; q < q1; ++q) {
+      const int32_t j = col[q];
+      if (drow) val[q] = (j == i) ? 1.0 : 0.0;
+      else if (isdir[j]) val[q] = 0.0;
+    }
+  }
+  if (bad) flags[2] = 1;
+}
+
+// One thread per node: r_i = f_i - sum over the patches p of node i
+// (ascending) of k_p sum_b E_ab u(v_b); Dirichlet rows zero.
+__global__ __launch_bounds__(kBlock) void k_form_q2(
+    int64_t n, const int32_t* __restrict__ pptr, const int32_t* __restrict__ plist,
+    const int32_t* __restrict__ nodes, const double* __restrict__ hs,
+    const double* __restrict__ kp, const uint8_t* __restrict__ isdir,
+    const double* __restrict__ u, const double* __restrict__ f, double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (isdir && isdir[i]) {
+    r[i] = 0.0;
+    return;
+  }
+  double acc = f[i];
+  for (int32_t e = pptr[i]; e < pptr[i + 1]; ++e) {
+    const int64_t p = plist[e] >> 5;
+    const int a = plist[e] & 31;
+    const int32_t* v = nodes + p * 27;
+    double s = 0.0;
+    for (int b = 0; b < 27; ++b) s = add_rn(s, mul_rn(q2_entry(a, b, hs + 3 * p), u[v[b]]));
+    acc = add_rn(acc, -mul_rn(kp ? kp[p] : 1.0, s));
+  }
+  r[i] = acc;
+}
+
 }  // namespace
 
 namespace gh {
@@ -246,6 +348,71 @@ int gh_form_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* d
     hipLaunchKernelGGL(k_form_q1<2>, g, dim3(kBlock), 0, ctx->stream, C->n, C->nptr, C->nlist,
                        C->cells, C->h, kc, isdir, (const double*)u->d, (const double*)f->d,
                        (double*)r->d);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+int gh_assemble_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask* dirichlet,
+                   gh_csr* A) {
+  GH_ARG(ctx && P && A, "gh_assemble_q2: NULL argument");
+  GH_ARG(P->dim == 3, "gh_assemble_q2: triquadratic patch elements are three-dimensional");
+  GH_ARG(!A->host_only && A->ncomp == 1,
+         "gh_assemble_q2: needs a scalar matrix with a device layout");
+  GH_ARG(A->n == P->nnodes, "gh_assemble_q2: the matrix and the patches have different node counts");
+  GH_ARG(!k || (k->prec == GH_F64 && k->size == P->npatches),
+         "gh_assemble_q2: k must be an FP64 vector with one entry per patch");
+  GH_ARG(!dirichlet || (dirichlet->n == A->n && dirichlet->ncomp == 1),
+         "gh_assemble_q2: the Dirichlet mask does not match the matrix");
+  // the entries of a row are found by bisection: its columns must ascend
+  if (A->cols_ascend < 0) {
+    A->cols_ascend = 1;
+    for (int64_t i = 0; i < A->n && A->cols_ascend; ++i)
+      for (int32_t q = A->h_rowptr[i] + 1; q < A->h_rowptr[i + 1]; ++q)
+        if (A->h_col[q] <= A->h_col[q - 1]) {
+          A->cols_ascend = 0;
+          break;
+        }
+  }
+  GH_ARG(A->cols_ascend == 1, "gh_assemble_q2: the matrix's columns do not ascend in every row");
+  const uint8_t* isdir = nullptr;
+  if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
+  double* v64 = nullptr;
+  GH_TRY(csr_asm_values(ctx, A, &v64));
+  if (!A->d_flags) {
+    if (hipMalloc((void**)&A->d_flags, sizeof(int) * 4) != hipSuccess)
+      return fail(GH_ENOMEM, "gh_assemble_q2: flags");
+    GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
+  }
+  if (A->n > 0) {
+    hipLaunchKernelGGL(k_assemble_q2, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
+                       A->rowptr, A->col, P->pptr, P->plist, P->nodes, P->hs,
+                       k ? (const double*)k->d : nullptr, isdir, v64, A->d_flags);
+    GH_HIP(hipGetLastError());
+  }
+  return csr_asm_commit(ctx, A, "gh_assemble_q2");
+}
+
+int gh_form_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask* dirichlet,
+               const gh_vec* u, const gh_vec* f, gh_vec* r) {
+  GH_ARG(ctx && P && u && f && r, "gh_form_q2: NULL argument");
+  GH_ARG(P->dim == 3, "gh_form_q2: triquadratic patch elements are three-dimensional");
+  GH_ARG(u->prec == GH_F64 && f->prec == GH_F64 && r->prec == GH_F64,
+         "gh_form_q2: vectors must be FP64");
+  GH_ARG(u->ncomp == 1 && f->ncomp == 1 && r->ncomp == 1 && u->n == P->nnodes &&
+             f->n == P->nnodes && r->n == P->nnodes,
+         "gh_form_q2: u, f and r must be scalar nodal vectors of the patches' mesh");
+  GH_ARG(r->d != u->d, "gh_form_q2: r and u must not alias");
+  GH_ARG(!k || (k->prec == GH_F64 && k->size == P->npatches),
+         "gh_form_q2: k must be an FP64 vector with one entry per patch");
+  GH_ARG(!dirichlet || (dirichlet->n == P->nnodes && dirichlet->ncomp == 1),
+         "gh_form_q2: the Dirichlet mask does not match the mesh");
+  const uint8_t* isdir = nullptr;
+  if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
+  if (P->nnodes == 0) return GH_OK;
+  hipLaunchKernelGGL(k_form_q2, dim3(grid_for(P->nnodes)), dim3(kBlock), 0, ctx->stream,
+                     P->nnodes, P->pptr, P->plist, P->nodes, P->hs,
+                     k ? (const double*)k->d : nullptr, isdir, (const double*)u->d,
+                     (const double*)f->d, (double*)r->d);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }

@@ -1424,7 +1424,7 @@ int csr_asm_commit(gh_ctx* ctx, gh_csr* A, const char* who) {
     GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
     GH_HIP(hipStreamSynchronize(ctx->stream));
     if (flags[2])
-      return fail(GH_EINVAL, "%s: a cell coupling has no entry in the matrix pattern", who);
+      return fail(GH_EINVAL, "%s: an element coupling has no entry in the matrix pattern", who);
     if (flags[1])
       return fail(GH_EINVAL,
                   "%s: an entry that was zero at creation (left out of the device layout) "

@@ -14,20 +14,13 @@
 // Acta Numerica 10 (2001), sections 3-4).  One thread per (patch, cell)
 // integrates with the 3^d-point Gauss rule on the cell, exact for these
 // integrands (degree <= 4 per direction).
+#include <climits>
 #include <cmath>
 #include <vector>
 
 #include "gh_kernels.h"
 
 using namespace gh;
-
-struct gh_patches {
-  gh_ctx* ctx = nullptr;
-  int dim = 3;
-  int64_t nnodes = 0, npatches = 0;
-  int32_t* nodes = nullptr;  // npatches * 3^dim, lexicographic (x fastest)
-  double* h = nullptr;       // npatches * dim cell sizes (half the patch extent)
-};
 
 namespace {
 
@@ -185,26 +178,42 @@ int gh_patches_create(gh_ctx* ctx, int dim, int64_t nnodes, const double* xyz, i
       }
     }
   }
+  // the steps from node 0 to its neighbours along x, y, z (nodes 1, 3, 9),
+  // the patch element's box as the host producer measures it
+  std::vector<double> hs((size_t)npatches * 3, 1.0);
+  for (int64_t p = 0; p < npatches; ++p)
+    for (int d = 0; d < dim; ++d) {
+      const int32_t* nd = nodes + p * nn;
+      hs[p * 3 + d] = xyz[(int64_t)nd[d == 0 ? 1 : (d == 1 ? 3 : 9)] * 3 + d] - xyz[(int64_t)nd[0] * 3 + d];
+    }
+  // node -> (patch, local node), ascending patch
+  if (npatches > (int64_t)(INT_MAX / 32))
+    return fail(GH_EINVAL, "gh_patches_create: too many patches");
+  std::vector<int32_t> pptr(nnodes + 1, 0), plist((size_t)npatches * nn);
+  for (int64_t k = 0; k < npatches * nn; ++k) pptr[nodes[k] + 1]++;
+  for (int64_t i = 0; i < nnodes; ++i) pptr[i + 1] += pptr[i];
+  {
+    std::vector<int32_t> fill(pptr.begin(), pptr.end() - 1);
+    for (int64_t p = 0; p < npatches; ++p)
+      for (int a = 0; a < nn; ++a) plist[fill[nodes[p * nn + a]]++] = (int32_t)(p * 32 + a);
+  }
   gh_patches* P = new gh_patches();
   P->ctx = ctx;
   P->dim = dim;
   P->nnodes = nnodes;
   P->npatches = npatches;
-  const size_t nb = sizeof(int32_t) * std::max<int64_t>(npatches * nn, 1);
-  const size_t hb = sizeof(double) * std::max<int64_t>(npatches * dim, 1);
-  if (hipMalloc((void**)&P->nodes, nb) != hipSuccess || hipMalloc((void**)&P->h, hb) != hipSuccess) {
-    if (P->nodes) (void)hipFree(P->nodes);
-    delete P;
-    return fail(GH_ENOMEM, "gh_patches_create: allocation failed");
-  }
   ctx_retain(ctx);  // balanced by gh_patches_destroy
-  if (npatches > 0 &&
-      (hipMemcpy(P->nodes, nodes, sizeof(int32_t) * npatches * nn, hipMemcpyHostToDevice) !=
-           hipSuccess ||
-       hipMemcpy(P->h, h.data(), sizeof(double) * npatches * dim, hipMemcpyHostToDevice) !=
-           hipSuccess)) {
+  auto up = [&](void** d, const void* src, size_t bytes) {
+    if (hipMalloc(d, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    return bytes == 0 || hipMemcpy(*d, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up((void**)&P->nodes, nodes, sizeof(int32_t) * npatches * nn) ||
+      !up((void**)&P->h, h.data(), sizeof(double) * npatches * dim) ||
+      !up((void**)&P->hs, hs.data(), sizeof(double) * npatches * 3) ||
+      !up((void**)&P->pptr, pptr.data(), sizeof(int32_t) * (nnodes + 1)) ||
+      !up((void**)&P->plist, plist.data(), sizeof(int32_t) * npatches * nn)) {
     gh_patches_destroy(P);
-    return fail(GH_EHIP, "gh_patches_create: upload failed");
+    return fail(GH_ENOMEM, "gh_patches_create: allocation or upload failed");
   }
   *out = P;
   return GH_OK;
@@ -214,8 +223,8 @@ int gh_patches_destroy(gh_patches* P) {
   if (!P) return GH_OK;
   gh_ctx* ctx = P->ctx;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (P->nodes) (void)hipFree(P->nodes);
-  if (P->h) (void)hipFree(P->h);
+  for (void* d : {(void*)P->nodes, (void*)P->h, (void*)P->hs, (void*)P->pptr, (void*)P->plist})
+    if (d) (void)hipFree(d);
   delete P;
   if (ctx) ctx_release(ctx);
   return GH_OK;

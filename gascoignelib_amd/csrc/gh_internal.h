@@ -42,6 +42,23 @@ struct gh_cells {
   int32_t* nlist = nullptr;
 };
 
+// Patches of a refined mesh: the 2^dim children of one coarser cell, 3^dim
+// nodes each in lexicographic order (x fastest), axis-aligned boxes with
+// midpoint middle nodes.  Used by the DWR indicators (the Q2 interpolation of
+// the adjoint) and by the device assembly of triquadratic patch elements.
+struct gh_patches {
+  gh_ctx* ctx = nullptr;
+  int dim = 3;
+  int64_t nnodes = 0, npatches = 0;
+  int32_t* nodes = nullptr;  // npatches * 3^dim
+  double* h = nullptr;       // npatches * dim: half the patch extent per axis
+  double* hs = nullptr;      // npatches * 3: node 0 to nodes 1, 3, 9 along x, y, z
+  // node -> patches: the patches of node i are plist[pptr[i] .. pptr[i+1]),
+  // each entry patch * 32 + local node, in ascending patch order
+  int32_t* pptr = nullptr;
+  int32_t* plist = nullptr;
+};
+
 namespace gh {
 
 // ---------------------------------------------------------------- errors
@@ -210,7 +227,8 @@ struct gh_csr {
   uint32_t* d_kept = nullptr;  // bit q: CSR entry q is stored in the SELL layout
   double* d_val64 = nullptr;   // FP64 CSR values of a reduced-precision matrix
   int* d_flags = nullptr;      // [0] singular diagonal, [1] a left-out entry became nonzero,
-                               // [2] a cell coupling has no entry in the pattern
+                               // [2] an element coupling has no entry in the pattern
+  int cols_ascend = -1;        // 1: every row's columns ascend (found on first use)
 };
 
 struct gh_transfer {

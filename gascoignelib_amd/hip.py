@@ -562,8 +562,10 @@ class CoefficientCells:
 
 
 class DwrPatches:
-    """Patches of a refined mesh level for dual-weighted residual indicators
-    (gh_patches_create / gh_dwr_estimate): estimate(u, z, f, k) returns the
+    """Patches of a refined mesh level (gh_patches_create), for dual-weighted
+    residual indicators (gh_dwr_estimate) and for the device assembly of
+    triquadratic patch elements (gh_assemble_q2, gh_form_q2).
+    estimate(u, z, f, k) returns the
     per-cell eta_K = int_K f w - k_K int_K grad u_h . grad w, w = I2 z_h - z_h,
     as a device vector of npatches * 2^dim values (each patch's cells in
     lexicographic order); their sum estimates J(u) - J(u_h)."""
@@ -580,6 +582,19 @@ class DwrPatches:
         self.h = C.c_void_p()
         check(N.hip().gh_patches_create(ctx.h, dim, len(xyz), ptr(self._xyz), self.npatches,
                                         ptr(self._nodes, N.i32p), C.byref(self.h)))
+
+    def assemble_q2(self, A, k=None, mask=None):
+        """A <- the operator of -div(k grad u) on the triquadratic patch
+        elements of these patches (CGDiscQ23d), formed on the device (k:
+        CudaVectorInterface with one entry per patch, or None)"""
+        check(N.hip().gh_assemble_q2(self.ctx.h, self.h, k.h if k is not None else None,
+                                     mask.h if mask is not None else None, A.h))
+
+    def form_q2(self, r, u, f, k=None, mask=None):
+        """r = f - A(u), patch element by patch element (CGDisc::Form), mask
+        rows zero"""
+        check(N.hip().gh_form_q2(self.ctx.h, self.h, k.h if k is not None else None,
+                                 mask.h if mask is not None else None, u.h, f.h, r.h))
 
     def estimate(self, u, z, f, k=None, eta=None):
         if eta is None:

@@ -156,6 +156,8 @@ HIP_SIGNATURES = {
     "gh_assemble_q1": (C.c_int, [vp, vp, vp, vp, vp]),
     "gh_csr_download_values": (C.c_int, [vp, vp, f64p]),
     "gh_form_q1": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "gh_assemble_q2": (C.c_int, [vp, vp, vp, vp, vp]),
+    "gh_form_q2": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "gh_mg_assemble_q1": (C.c_int, [vp, vp, P(vp), P(vp)]),
 }
 

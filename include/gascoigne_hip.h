@@ -471,6 +471,23 @@ int gh_assemble_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mas
  * u enters at every node, Dirichlet nodes included. */
 int gh_form_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* dirichlet,
                const gh_vec* u, const gh_vec* f, gh_vec* r);
+/* The same for the triquadratic patch elements of CGDiscQ23d
+ * (src/DofHandler/cgdisc.h:1292; the Q2 Lagrange basis of BaseQ23d on the 27
+ * nodes of a patch, src/Discretization/Q2/baseq23d.h): the operator of
+ * -div(k grad u) with one coefficient per patch element (k: FP64 vector of
+ * P's patch count, NULL: k = 1) on the patches P of gh_patches_create (dim 3),
+ * written into the scalar matrix A like gh_assemble_q1.  Each entry sums its
+ * patches' contributions in ascending patch order (the host producer's
+ * order), and A's columns must ascend in every row (GH_EINVAL otherwise).
+ * Unconstrained: on meshes with hanging nodes the condensation is the
+ * caller's.  Synchronises. */
+int gh_assemble_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask* dirichlet,
+                   gh_csr* A);
+/* CGDisc::Form (src/DofHandler/cgdisc.h:415-448) for the same elements:
+ * r = f - A(u) patch by patch from u, each node summing its patches in
+ * ascending order; the mask's rows set to zero. */
+int gh_form_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask* dirichlet,
+               const gh_vec* u, const gh_vec* f, gh_vec* r);
 /* StdMultiLevelSolver::AssembleMatrix + ComputeIlu
  * (src/Solver/stdmultilevelsolver.cc:763-790): gh_assemble_q1 on every level
  * of a solver, into the level operators it runs on (the FP64 operator, the
