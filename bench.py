@@ -972,16 +972,19 @@ def bench_q2(args, adaptive=0):
                                 "gbs": bmx / t_spmv_mx / 1e9,
                                 "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS, "layout": lay_mx},
         "setup_s": dict({"structure": t_struct, "device": t_setup},
-                        **({} if adaptive else q2_assembly_times(ctx, hier))),
+                        **({} if adaptive else q2_assembly_times(
+                            ctx, hier, {t: sol[t][0] for t in ("fp64", "mixed")}))),
     }
     print(json.dumps(line), flush=True)
 
 
-def q2_assembly_times(ctx, hier, reps=3):
+def q2_assembly_times(ctx, hier, solvers=None, reps=3):
     """Seconds to assemble the finest level's Q2 operator (Laplace on the
     triquadratic patch elements, Dirichlet rows): on the host producer (its
     threads) and on the device (gh_assemble_q2) into a matrix of the same
-    pattern in the caller's numbering; each the median of `reps` after one
+    pattern in the caller's numbering; and the device re-assembly of every
+    level of each given solver (gh_mg_assemble_q2, the same coefficients, so
+    the solvers keep their values).  Each the median of `reps` after one
     warm-up.  The device values equal the host's bit for bit (checked)."""
     from gascoignelib_amd import hip as H
     M = hier.mesh
@@ -1001,8 +1004,13 @@ def q2_assembly_times(ctx, hier, reps=3):
         P.assemble_q2(A, mask=mask)
         ctx.sync()
 
+    pts = [H.DwrPatches(ctx, 3, M.coords(m), M.patches(m)) for m in range(M.nlevels)]
+    runs = [("assembly_q2_host_fine", host), ("assembly_q2_device_fine", device)]
+    for tag, mg in (solvers or {}).items():
+        runs.append((f"reassembly_q2_device_{tag}_solver",
+                     lambda mg=mg: (mg.assemble_q2(pts), ctx.sync())))
     out = {}
-    for name, fn in (("assembly_q2_host_fine", host), ("assembly_q2_device_fine", device)):
+    for name, fn in runs:
         fn()
         ts = []
         for _ in range(reps):

@@ -205,40 +205,55 @@ __device__ __forceinline__ double q2_entry(int a, int b, const double* __restric
   return add_rn(add_rn(t1, t2), t3);
 }
 
-// One thread per row i: the row's entries start at 0 and receive, patch by
-// patch in ascending order and node by node, k_p E_ab at the entry of node b
-// (found by bisection: the row's columns ascend), as the host producer adds
+// One thread per row i (the matrix's numbering; its node is perm[i], or i):
+// the row's entries start at 0 and receive, patch by patch in ascending order
+// and node by node, k_p E_ab at the entry of node b, as the host producer adds
 // them; then the Dirichlet rows become unit rows and the Dirichlet columns
-// zero.  A patch node without an entry in the row sets flags[2].
+// zero.  An entry is found by bisection where the row's columns ascend (mode
+// 0: as stored; mode 1: mapped through perm, the renumbered operators that
+// keep the caller's entry order), else by a scan (mode 2).  A patch node
+// without an entry in the row sets flags[2].
 __global__ __launch_bounds__(kBlock) void k_assemble_q2(
     int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-    const int32_t* __restrict__ pptr, const int32_t* __restrict__ plist,
-    const int32_t* __restrict__ nodes, const double* __restrict__ hs,
-    const double* __restrict__ kp, const uint8_t* __restrict__ isdir, double* __restrict__ val,
-    int* flags) {
+    const int32_t* __restrict__ perm, int mode, const int32_t* __restrict__ pptr,
+    const int32_t* __restrict__ plist, const int32_t* __restrict__ nodes,
+    const double* __restrict__ hs, const double* __restrict__ kp,
+    const uint8_t* __restrict__ isdir, double* __restrict__ val, int* flags) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
+  const int32_t ic = perm ? perm[i] : (int32_t)i;
   const int32_t q0 = rowptr[i], q1 = rowptr[i + 1];
   for (int32_t q = q0; q < q1; ++q) val[q] = 0.0;
+  auto key = [&](int32_t q) { return perm ? perm[col[q]] : col[q]; };
   bool bad = false;
-  for (int32_t e = pptr[i]; e < pptr[i + 1]; ++e) {
+  for (int32_t e = pptr[ic]; e < pptr[ic + 1]; ++e) {
     const int64_t p = plist[e] >> 5;
     const int a = plist[e] & 31;
     const int32_t* v = nodes + p * 27;
     const double k = kp ? kp[p] : 1.0;
     for (int b = 0; b < 27; ++b) {
       const int32_t j = v[b];
-      int32_t lo = q0, hi = q1;  // first entry with col >= j
-      while (lo < hi) {
-        const int32_t mid = lo + ((hi - lo) >> 1);
-        if (col[mid] < j) lo = mid + 1;
-        else hi = mid;
+      int32_t at = q1;
+      if (mode < 2) {
+        int32_t lo = q0, hi = q1;  // first entry whose key is >= j
+        while (lo < hi) {
+          const int32_t mid = lo + ((hi - lo) >> 1);
+          if (key(mid) < j) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < q1 && key(lo) == j) at = lo;
+      } else {
+        for (int32_t q = q0; q < q1; ++q)
+          if (key(q) == j) {
+            at = q;
+            break;
+          }
       }
-      if (lo == q1 || col[lo] != j) {
+      if (at == q1) {
         bad = true;
         continue;
       }
-      val[lo] = add_rn(val[lo], mul_rn(k, q2_entry(a, b, hs + 3 * p)));
+      val[at] = add_rn(val[at], mul_rn(k, q2_entry(a, b, hs + 3 * p)));
     }
   }
   if (isdir) {
@@ -296,6 +311,39 @@ int assemble_q1_launch(gh_ctx* ctx, const gh_cells* C, const double* kcell, cons
   else
     hipLaunchKernelGGL(k_assemble_q1<2>, g, dim3(kBlock), 0, ctx->stream, A->n, A->rowptr, A->col,
                        perm, C->nptr, C->nlist, C->cells, C->h, kcell, isdir, val64, A->d_flags);
+  GH_HIP(hipGetLastError());
+  return GH_OK;
+}
+
+// How k_assemble_q2 finds a row's entries in A (perm: A's row i is node
+// perm_h[i]): 0 bisection on the stored columns, 1 bisection on the columns
+// mapped through perm, 2 a scan.  Cached in A (one numbering per matrix).
+int assemble_q2_mode(gh_csr* A, const std::vector<int32_t>* perm_h) {
+  if (A->asm_mode >= 0) return A->asm_mode;
+  auto ascend = [&](bool mapped) {
+    for (int64_t i = 0; i < A->n; ++i)
+      for (int32_t q = A->h_rowptr[i] + 1; q < A->h_rowptr[i + 1]; ++q) {
+        const int32_t c0 = A->h_col[q - 1], c1 = A->h_col[q];
+        if (mapped ? (*perm_h)[c1] <= (*perm_h)[c0] : c1 <= c0) return false;
+      }
+    return true;
+  };
+  if (!perm_h) A->asm_mode = ascend(false) ? 0 : 2;
+  else A->asm_mode = ascend(true) ? 1 : 2;
+  return A->asm_mode;
+}
+
+int assemble_q2_launch(gh_ctx* ctx, const gh_patches* P, const double* kp, const uint8_t* isdir,
+                       const int32_t* perm, int mode, gh_csr* A, double* val64) {
+  if (!A->d_flags) {
+    if (hipMalloc((void**)&A->d_flags, sizeof(int) * 4) != hipSuccess)
+      return fail(GH_ENOMEM, "device assembly: flags");
+    GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
+  }
+  if (A->n == 0) return GH_OK;
+  hipLaunchKernelGGL(k_assemble_q2, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
+                     A->rowptr, A->col, perm, mode, P->pptr, P->plist, P->nodes, P->hs, kp,
+                     isdir, val64, A->d_flags);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
@@ -363,32 +411,12 @@ int gh_assemble_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_m
          "gh_assemble_q2: k must be an FP64 vector with one entry per patch");
   GH_ARG(!dirichlet || (dirichlet->n == A->n && dirichlet->ncomp == 1),
          "gh_assemble_q2: the Dirichlet mask does not match the matrix");
-  // the entries of a row are found by bisection: its columns must ascend
-  if (A->cols_ascend < 0) {
-    A->cols_ascend = 1;
-    for (int64_t i = 0; i < A->n && A->cols_ascend; ++i)
-      for (int32_t q = A->h_rowptr[i] + 1; q < A->h_rowptr[i + 1]; ++q)
-        if (A->h_col[q] <= A->h_col[q - 1]) {
-          A->cols_ascend = 0;
-          break;
-        }
-  }
-  GH_ARG(A->cols_ascend == 1, "gh_assemble_q2: the matrix's columns do not ascend in every row");
   const uint8_t* isdir = nullptr;
   if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
   double* v64 = nullptr;
   GH_TRY(csr_asm_values(ctx, A, &v64));
-  if (!A->d_flags) {
-    if (hipMalloc((void**)&A->d_flags, sizeof(int) * 4) != hipSuccess)
-      return fail(GH_ENOMEM, "gh_assemble_q2: flags");
-    GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
-  }
-  if (A->n > 0) {
-    hipLaunchKernelGGL(k_assemble_q2, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
-                       A->rowptr, A->col, P->pptr, P->plist, P->nodes, P->hs,
-                       k ? (const double*)k->d : nullptr, isdir, v64, A->d_flags);
-    GH_HIP(hipGetLastError());
-  }
+  GH_TRY(assemble_q2_launch(ctx, P, k ? (const double*)k->d : nullptr, isdir, nullptr,
+                            assemble_q2_mode(A, nullptr), A, v64));
   return csr_asm_commit(ctx, A, "gh_assemble_q2");
 }
 

@@ -228,7 +228,7 @@ struct gh_csr {
   double* d_val64 = nullptr;   // FP64 CSR values of a reduced-precision matrix
   int* d_flags = nullptr;      // [0] singular diagonal, [1] a left-out entry became nonzero,
                                // [2] an element coupling has no entry in the pattern
-  int cols_ascend = -1;        // 1: every row's columns ascend (found on first use)
+  int asm_mode = -1;           // how the Q2 device assembly finds entries (found on first use)
 };
 
 struct gh_transfer {
@@ -342,6 +342,11 @@ int mask_node_flags(gh_ctx* ctx, gh_mask* M, const uint8_t** out);
 // nullptr: the same numbering); isdir per row of A (nullptr: none)
 int assemble_q1_launch(gh_ctx* ctx, const gh_cells* C, const double* kcell, const uint8_t* isdir,
                        const int32_t* perm, gh_csr* A, double* val64);
+// the same for the triquadratic patch elements P (one coefficient per patch);
+// mode from assemble_q2_mode (perm_h: the host copy of perm, nullptr: none)
+int assemble_q2_mode(gh_csr* A, const std::vector<int32_t>* perm_h);
+int assemble_q2_launch(gh_ctx* ctx, const gh_patches* P, const double* kpatch, const uint8_t* isdir,
+                       const int32_t* perm, int mode, gh_csr* A, double* val64);
 int transfer_from_csr(gh_ctx* ctx, int64_t ncoarse, int64_t nfine, std::vector<int32_t> rrow,
                       std::vector<uint32_t> rcol, std::vector<int32_t> prow,
                       std::vector<uint32_t> pcol, gh_transfer** out);

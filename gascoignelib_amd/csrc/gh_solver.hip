@@ -946,6 +946,44 @@ int gh_mg_assemble_q1(gh_ctx* ctx, gh_mg* mg, gh_cells* const* C, gh_vec* const*
   return GH_OK;
 }
 
+int gh_mg_assemble_q2(gh_ctx* ctx, gh_mg* mg, gh_patches* const* P, gh_vec* const* k) {
+  GH_ARG(ctx && mg && P, "gh_mg_assemble_q2: NULL argument");
+  GH_ARG(!mg->P.adjoint, "gh_mg_assemble_q2: the adjoint solver's transposed levels are not assembled");
+  GH_ARG(mg->P.smoother < 2, "gh_mg_assemble_q2: Vanka and ILU factors are not re-formed");
+  for (int l = 0; l < mg->nlevels; ++l) {
+    const MgLevel& L = mg->L[l];
+    if (!P[l] || P[l]->dim != 3 || P[l]->nnodes != L.n || L.ncomp != 1)
+      return fail(GH_EINVAL, "gh_mg_assemble_q2: patches of level %d do not match a scalar level", l);
+    if (k && k[l] && (k[l]->prec != GH_F64 || k[l]->size != P[l]->npatches))
+      return fail(GH_EINVAL, "gh_mg_assemble_q2: k of level %d needs one FP64 entry per patch", l);
+    if (L.H)
+      return fail(GH_EINVAL, "gh_mg_assemble_q2: level %d has hanging nodes (not assembled here)", l);
+  }
+  for (int l = 0; l < mg->nlevels; ++l) {
+    MgLevel& L = mg->L[l];
+    const uint8_t* isdir = nullptr;
+    if (L.M) GH_TRY(mask_node_flags(ctx, L.M, &isdir));
+    std::vector<int32_t> perm_h;
+    if (L.d_perm) {
+      perm_h.resize(L.n);
+      GH_HIP(hipMemcpy(perm_h.data(), L.d_perm, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost));
+    }
+    const double* kp = (k && k[l]) ? (const double*)k[l]->d : nullptr;
+    gh_csr* mats[3] = {L.A64, L.A != L.A64 ? L.A : nullptr, L.A64s};
+    for (gh_csr* M : mats) {
+      if (!M) continue;
+      double* v64 = nullptr;
+      GH_TRY(csr_asm_values(ctx, M, &v64));
+      const int mode = assemble_q2_mode(M, L.d_perm ? &perm_h : nullptr);
+      GH_TRY(assemble_q2_launch(ctx, P[l], kp, isdir, L.d_perm, mode, M, v64));
+      GH_TRY(csr_asm_commit(ctx, M, "gh_mg_assemble_q2"));
+    }
+    if (L.direct) GH_TRY(dense_inverse_device(ctx, L.A64, L.Ainv));
+  }
+  GH_HIP(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
 int gh_mg_set_vanka(gh_mg* mg, int level, int64_t npatches, int psize, const int32_t* nodes) {
   GH_ARG(mg && level >= 0 && level < mg->nlevels && (nodes || npatches == 0) && npatches >= 0 &&
              psize >= 1,

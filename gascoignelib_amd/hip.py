@@ -510,6 +510,18 @@ class CudaMultiLevelSolver:
             karr = (C.c_void_p * nl)(*[(v.h if v is not None else None) for v in k])
         check(N.hip().gh_mg_assemble_q1(self.ctx.h, self.h, carr, karr))
 
+    def assemble_q2(self, patches, k=None):
+        """The same for a hierarchy of triquadratic patch elements without
+        hanging nodes: every level's operator re-formed from its patches
+        (DwrPatches per level, caller numbering) and per-patch coefficients
+        (CudaVectorInterface per level, or None)"""
+        nl = self.hier.nlevels
+        parr = (C.c_void_p * nl)(*[p.h for p in patches])
+        karr = None
+        if k is not None:
+            karr = (C.c_void_p * nl)(*[(v.h if v is not None else None) for v in k])
+        check(N.hip().gh_mg_assemble_q2(self.ctx.h, self.h, parr, karr))
+
     def free(self):
         if self.h:
             N.hip().gh_mg_destroy(self.h)

@@ -159,6 +159,7 @@ HIP_SIGNATURES = {
     "gh_assemble_q2": (C.c_int, [vp, vp, vp, vp, vp]),
     "gh_form_q2": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "gh_mg_assemble_q1": (C.c_int, [vp, vp, P(vp), P(vp)]),
+    "gh_mg_assemble_q2": (C.c_int, [vp, vp, P(vp), P(vp)]),
 }
 
 STRUCT_SIGNATURES = {

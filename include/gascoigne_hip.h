@@ -499,6 +499,12 @@ int gh_form_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask*
  * masks apply.  Levels with hanging nodes, the Vanka smoother and adjoint
  * solvers are refused (GH_EINVAL).  Synchronises. */
 int gh_mg_assemble_q1(gh_ctx* ctx, gh_mg* mg, gh_cells* const* C, gh_vec* const* k);
+/* The same for a hierarchy of triquadratic patch elements (gh_assemble_q2 on
+ * every level; P[l] and k[l], one coefficient per patch, in the caller's
+ * numbering of level l).  Levels with hanging nodes (whose operators are
+ * condensed), the Vanka and ILU smoothers and adjoint solvers are refused
+ * (GH_EINVAL).  Synchronises. */
+int gh_mg_assemble_q2(gh_ctx* ctx, gh_mg* mg, gh_patches* const* P, gh_vec* const* k);
 
 #ifdef __cplusplus
 }

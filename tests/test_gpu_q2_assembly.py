@@ -178,3 +178,55 @@ def test_q2_assembly_errors(ctx, level):
     # the refused calls left the matrix usable
     Pt.assemble_q2(A, mask=H.DirichletMask(ctx, n, 1, L["d"]))
     assert np.array_equal(A.values(), L["host"])
+
+
+def q2_levels_with_k(M, ks, colours=(1,)):
+    """a uniform Q2 hierarchy whose level operators carry per-patch
+    coefficients ks[l] (numpy restatement of the device assembly)"""
+    from gascoignelib_amd.problem import Hierarchy, Level
+    levels = []
+    for l in range(M.nlevels):
+        rp, col = M.patch_structure(l)
+        L = dict(P=M.patches(l), coords=M.coords(l), rp=rp, col=col, n=len(rp) - 1,
+                 d=M.dirichlet(l, colours))
+        val = numpy_assembly(L, ks[l])
+        levels.append(Level(n=L["n"], ncomp=1, rowptr=rp, col=col, val=val,
+                            transfer=M.transfer(l) if l > 0 else None, hn_kind=23,
+                            hn_edges=np.zeros((0, 4), np.int32),
+                            hn_faces=np.zeros((0, 10), np.int32), dirichlet=L["d"],
+                            dirichlet_comps=np.zeros(1, np.int32), coords=L["coords"]))
+    h = Hierarchy(levels, 3, "q2k")
+    h.mesh = M
+    return h
+
+
+@pytest.mark.parametrize("prec,krylov", [("f64", "gmres"), ("f16", "fgmres")])
+def test_q2_solver_reassembly_matches_host_built_solver(ctx, prec, krylov):
+    print(f"[mg re-assembly q2 {prec} {krylov}]", flush=True)
+    M = Mesh("cube.inp", 3, 4, 1)
+    rng = np.random.default_rng(9)
+    k1 = [np.exp(rng.normal(size=len(M.patches(l)))) for l in range(M.nlevels)]
+    k2 = [np.exp(0.5 * rng.normal(size=len(M.patches(l)))) for l in range(M.nlevels)]
+    h1, h2 = q2_levels_with_k(M, k1), q2_levels_with_k(M, k2)
+    params = dict(mgtype="V", iterpre=2, iterpost=2, iterexact=10, omega=0.8, mgomega=1.0,
+                  ndirect=200, prec=prec, krylov=krylov, order="lexicographic")
+    mg1 = H.CudaMultiLevelSolver(ctx, h1, **params)
+    mg2 = H.CudaMultiLevelSolver(ctx, h2, **params)
+    pts = [H.DwrPatches(ctx, 3, M.coords(l), M.patches(l)) for l in range(M.nlevels)]
+    mg1.assemble_q2(pts, [H.CudaVectorInterface.from_global(ctx, k) for k in k2])
+    f = np.random.default_rng(4).uniform(-1, 1, h2.fine.n)
+    f[h2.fine.dirichlet] = 0.0
+    F = H.CudaVectorInterface.from_global(ctx, f)
+    sols, reps = [], []
+    for mg in (mg1, mg2):
+        U = mg.new_vector()
+        rep = mg.Solve(U, F, H.CGInfo(1e-10, 1e-30, 60), H.NLInfo(1e-10, 1e-30, 20),
+                       vmax=10 if krylov == "gmres" else 20, norm_kind=1)
+        sols.append(U.copy_back())
+        reps.append((rep.newton_iterations, rep.gmres_iterations, H.N.NL_STATUS[rep.status]))
+    assert reps[0] == reps[1] and reps[0][2] == "converged"
+    assert np.array_equal(sols[0], sols[1])
+    for l in range(h1.nlevels):
+        for fp64 in (True, False):
+            assert np.array_equal(mg1.level_matrix(l, fp64).values(),
+                                  mg2.level_matrix(l, fp64).values())
