@@ -210,12 +210,14 @@ __device__ __forceinline__ double q2_entry(int a, int b, const double* __restric
 // and node by node, k_p E_ab at the entry of node b, as the host producer adds
 // them; then the Dirichlet rows become unit rows and the Dirichlet columns
 // zero.  An entry is found by bisection where the row's columns ascend (mode
-// 0: as stored; mode 1: mapped through perm, the renumbered operators that
-// keep the caller's entry order), else by a scan (mode 2).  A patch node
-// without an entry in the row sets flags[2].
+// 0: as stored, the node's position found through iperm; mode 1: mapped
+// through perm, the renumbered operators that keep the caller's entry
+// order), else by a scan (mode 2).  A patch node without an entry in the row
+// sets flags[2].
 __global__ __launch_bounds__(kBlock) void k_assemble_q2(
     int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-    const int32_t* __restrict__ perm, int mode, const int32_t* __restrict__ pptr,
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm, int mode,
+    const int32_t* __restrict__ pptr,
     const int32_t* __restrict__ plist, const int32_t* __restrict__ nodes,
     const double* __restrict__ hs, const double* __restrict__ kp,
     const uint8_t* __restrict__ isdir, double* __restrict__ val, int* flags) {
@@ -224,7 +226,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble_q2(
   const int32_t ic = perm ? perm[i] : (int32_t)i;
   const int32_t q0 = rowptr[i], q1 = rowptr[i + 1];
   for (int32_t q = q0; q < q1; ++q) val[q] = 0.0;
-  auto key = [&](int32_t q) { return perm ? perm[col[q]] : col[q]; };
+  // mode 0 compares stored columns with the node's position in A, modes 1
+  // and 2 columns mapped to nodes with the node
+  auto key = [&](int32_t q) { return (perm && mode != 0) ? perm[col[q]] : col[q]; };
   bool bad = false;
   for (int32_t e = pptr[ic]; e < pptr[ic + 1]; ++e) {
     const int64_t p = plist[e] >> 5;
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_q2(
     const int32_t* v = nodes + p * 27;
     const double k = kp ? kp[p] : 1.0;
     for (int b = 0; b < 27; ++b) {
-      const int32_t j = v[b];
+      const int32_t j = (perm && mode == 0) ? iperm[v[b]] : v[b];
       int32_t at = q1;
       if (mode < 2) {
         int32_t lo = q0, hi = q1;  // first entry whose key is >= j
@@ -328,13 +332,14 @@ int assemble_q2_mode(gh_csr* A, const std::vector<int32_t>* perm_h) {
       }
     return true;
   };
-  if (!perm_h) A->asm_mode = ascend(false) ? 0 : 2;
-  else A->asm_mode = ascend(true) ? 1 : 2;
+  if (ascend(false)) A->asm_mode = 0;
+  else A->asm_mode = (perm_h && ascend(true)) ? 1 : 2;
   return A->asm_mode;
 }
 
 int assemble_q2_launch(gh_ctx* ctx, const gh_patches* P, const double* kp, const uint8_t* isdir,
-                       const int32_t* perm, int mode, gh_csr* A, double* val64) {
+                       const int32_t* perm, const int32_t* iperm, int mode, gh_csr* A,
+                       double* val64) {
   if (!A->d_flags) {
     if (hipMalloc((void**)&A->d_flags, sizeof(int) * 4) != hipSuccess)
       return fail(GH_ENOMEM, "device assembly: flags");
@@ -342,8 +347,8 @@ int assemble_q2_launch(gh_ctx* ctx, const gh_patches* P, const double* kp, const
   }
   if (A->n == 0) return GH_OK;
   hipLaunchKernelGGL(k_assemble_q2, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
-                     A->rowptr, A->col, perm, mode, P->pptr, P->plist, P->nodes, P->hs, kp,
-                     isdir, val64, A->d_flags);
+                     A->rowptr, A->col, perm, iperm, mode, P->pptr, P->plist, P->nodes, P->hs,
+                     kp, isdir, val64, A->d_flags);
   GH_HIP(hipGetLastError());
   return GH_OK;
 }
@@ -415,7 +420,7 @@ int gh_assemble_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_m
   if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
   double* v64 = nullptr;
   GH_TRY(csr_asm_values(ctx, A, &v64));
-  GH_TRY(assemble_q2_launch(ctx, P, k ? (const double*)k->d : nullptr, isdir, nullptr,
+  GH_TRY(assemble_q2_launch(ctx, P, k ? (const double*)k->d : nullptr, isdir, nullptr, nullptr,
                             assemble_q2_mode(A, nullptr), A, v64));
   return csr_asm_commit(ctx, A, "gh_assemble_q2");
 }

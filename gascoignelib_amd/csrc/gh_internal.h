@@ -346,7 +346,8 @@ int assemble_q1_launch(gh_ctx* ctx, const gh_cells* C, const double* kcell, cons
 // mode from assemble_q2_mode (perm_h: the host copy of perm, nullptr: none)
 int assemble_q2_mode(gh_csr* A, const std::vector<int32_t>* perm_h);
 int assemble_q2_launch(gh_ctx* ctx, const gh_patches* P, const double* kpatch, const uint8_t* isdir,
-                       const int32_t* perm, int mode, gh_csr* A, double* val64);
+                       const int32_t* perm, const int32_t* iperm, int mode, gh_csr* A,
+                       double* val64);
 int transfer_from_csr(gh_ctx* ctx, int64_t ncoarse, int64_t nfine, std::vector<int32_t> rrow,
                       std::vector<uint32_t> rcol, std::vector<int32_t> prow,
                       std::vector<uint32_t> pcol, gh_transfer** out);

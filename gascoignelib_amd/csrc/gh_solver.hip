@@ -963,21 +963,32 @@ int gh_mg_assemble_q2(gh_ctx* ctx, gh_mg* mg, gh_patches* const* P, gh_vec* cons
     MgLevel& L = mg->L[l];
     const uint8_t* isdir = nullptr;
     if (L.M) GH_TRY(mask_node_flags(ctx, L.M, &isdir));
-    std::vector<int32_t> perm_h;
+    // renumbered level: the order and its inverse (node -> position)
+    std::vector<int32_t> perm_h, iperm_h;
+    int32_t* d_iperm = nullptr;
     if (L.d_perm) {
       perm_h.resize(L.n);
+      iperm_h.resize(L.n);
       GH_HIP(hipMemcpy(perm_h.data(), L.d_perm, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < L.n; ++i) iperm_h[perm_h[i]] = (int32_t)i;
+      GH_TRY(dupload(ctx, &d_iperm, iperm_h));
     }
     const double* kp = (k && k[l]) ? (const double*)k[l]->d : nullptr;
     gh_csr* mats[3] = {L.A64, L.A != L.A64 ? L.A : nullptr, L.A64s};
+    int rc = GH_OK;
     for (gh_csr* M : mats) {
-      if (!M) continue;
+      if (!M || rc) continue;
       double* v64 = nullptr;
-      GH_TRY(csr_asm_values(ctx, M, &v64));
-      const int mode = assemble_q2_mode(M, L.d_perm ? &perm_h : nullptr);
-      GH_TRY(assemble_q2_launch(ctx, P[l], kp, isdir, L.d_perm, mode, M, v64));
-      GH_TRY(csr_asm_commit(ctx, M, "gh_mg_assemble_q2"));
+      rc = csr_asm_values(ctx, M, &v64);
+      const int mode = rc ? 0 : assemble_q2_mode(M, L.d_perm ? &perm_h : nullptr);
+      if (!rc) rc = assemble_q2_launch(ctx, P[l], kp, isdir, L.d_perm, d_iperm, mode, M, v64);
+      if (!rc) rc = csr_asm_commit(ctx, M, "gh_mg_assemble_q2");
     }
+    if (d_iperm) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(d_iperm);
+    }
+    GH_TRY(rc);
     if (L.direct) GH_TRY(dense_inverse_device(ctx, L.A64, L.Ainv));
   }
   GH_HIP(hipStreamSynchronize(ctx->stream));
