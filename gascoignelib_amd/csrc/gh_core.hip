@@ -555,29 +555,75 @@ constexpr int kMaxSlicePatterns = 16;
 
 // orig: original row of each slot (nullptr: slot = row); rowptr/col/vals are
 // already in slot order.
+// A table from 64-bit keys (the row-pattern hashes) to counters or row
+// indices: open addressing with linear probing, sized for n keys; it stands
+// in for std::unordered_map where millions of rows are looked up.
+struct KeyTable {
+  std::vector<uint64_t> key;
+  std::vector<int64_t> val;
+  std::vector<uint8_t> used;
+  uint64_t mask = 0;
+  KeyTable(int64_t n, int64_t init) {
+    uint64_t cap = 16;
+    while (cap < 2 * (uint64_t)std::max<int64_t>(n, 1)) cap <<= 1;
+    key.assign(cap, 0);
+    val.assign(cap, init);
+    used.assign(cap, 0);
+    mask = cap - 1;
+    fill = init;
+  }
+  // the value of k, inserted with the initial value if absent
+  int64_t& operator[](uint64_t k) {
+    uint64_t h = (k * 0x9E3779B97F4A7C15ull) >> 17;
+    for (;; ++h) {
+      const uint64_t b = h & mask;
+      if (!used[b]) {
+        used[b] = 1;
+        key[b] = k;
+        val[b] = fill;
+        return val[b];
+      }
+      if (key[b] == k) return val[b];
+    }
+  }
+  int64_t fill = 0;
+};
+
 static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* col,
                            const double* vals, int prec, const int32_t* orig, Sell* S) {
   S->n = n;
   S->prec = prec;
   S->maxlen = 0;
   S->nslices = (n + kSlice - 1) / kSlice;
+  const int64_t ns = S->nslices;
   const bool valued = vals != nullptr;
   // entries kept per row: a valued operator drops its exact zeros
   auto keep = [&](int64_t q) { return !valued || vals[q] != 0.0; };
-  std::vector<uint32_t> sptr(S->nslices + 1, 0);
+  // The loops over rows and slices run on host threads; each writes its own
+  // rows or slices, and the passes that accumulate (offsets, counters, the
+  // pattern candidates) run in order, so the layout does not depend on the
+  // thread count.
+  std::vector<int32_t> lens(n);
+  parallel_rows(n, [&](int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) {
+      int32_t len = 0;
+      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) len += keep(q);
+      lens[i] = len;
+    }
+  });
+  std::vector<uint32_t> sptr(ns + 1, 0);
   std::vector<uint16_t> rlen(n);
   int64_t stored = 0;
   for (int64_t i = 0; i < n; ++i) {
-    int64_t len = 0;
-    for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) len += keep(q);
-    if (len > 65535) return fail(GH_EINVAL, "row %lld has more than 65535 entries", (long long)i);
-    rlen[i] = (uint16_t)len;
-    stored += len;
-    S->maxlen = std::max<int>(S->maxlen, (int)len);
+    if (lens[i] > 65535) return fail(GH_EINVAL, "row %lld has more than 65535 entries", (long long)i);
+    rlen[i] = (uint16_t)lens[i];
+    stored += lens[i];
+    S->maxlen = std::max<int>(S->maxlen, lens[i]);
   }
+  std::vector<int32_t>().swap(lens);
   S->stored = stored;
   uint64_t total = 0;
-  for (int64_t s = 0; s < S->nslices; ++s) {
+  for (int64_t s = 0; s < ns; ++s) {
     int w = 0;
     for (int64_t i = s * kSlice; i < std::min<int64_t>(n, (s + 1) * kSlice); ++i)
       w = std::max<int>(w, rlen[i]);
@@ -585,23 +631,25 @@ static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const 
     total += (uint64_t)w * kSlice;
     if (total > 0xFFFFFFFFull) return fail(GH_EINVAL, "operator too large for 32-bit slot offsets");
   }
-  sptr[S->nslices] = (uint32_t)total;
+  sptr[ns] = (uint32_t)total;
   S->slots = (int64_t)total;
   // per slot: column (global) and value, rows' kept entries in order
   std::vector<uint32_t> scol(total, 0);
   std::vector<double> sval(valued ? total : 0, 0.0);
-  for (int64_t s = 0; s < S->nslices; ++s)
-    for (int lane = 0; lane < kSlice; ++lane) {
-      const int64_t i = s * kSlice + lane;
-      if (i >= n) break;
-      uint64_t slot = sptr[s] + lane;
-      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-        if (!keep(q)) continue;
-        scol[slot] = col[q];
-        if (valued) sval[slot] = vals[q];
-        slot += kSlice;
+  parallel_for(ns, 512, [&](int64_t s0, int64_t s1) {
+    for (int64_t s = s0; s < s1; ++s)
+      for (int lane = 0; lane < kSlice; ++lane) {
+        const int64_t i = s * kSlice + lane;
+        if (i >= n) break;
+        uint64_t slot = sptr[s] + lane;
+        for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+          if (!keep(q)) continue;
+          scol[slot] = col[q];
+          if (valued) sval[slot] = vals[q];
+          slot += kSlice;
+        }
       }
-    }
+  });
   GH_TRY(upload(ctx, &S->sptr, sptr.data(), sptr.size()));
   if (!valued) {
     GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
@@ -616,143 +664,195 @@ static int sell_build_impl(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const 
   {
     constexpr size_t kMaxCandidates = 1 << 20;
     constexpr int64_t kMaxTable = 1 << 22;  // offsets in the table
-    auto row_hash = [&](int64_t i) {
-      uint64_t hsh = 1469598103934665603ull;
-      const int64_t orow = orig ? orig[i] : i;
-      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
-        if (keep(q)) hsh = (hsh ^ (uint32_t)(int32_t)((int64_t)col[q] - orow)) * 1099511628211ull;
-      return hsh;
-    };
-    // rows per hash: only patterns frequent enough become candidates
+    auto orow = [&](int64_t i) { return orig ? (int64_t)orig[i] : i; };
     std::vector<uint64_t> rh(n);
-    std::unordered_map<uint64_t, int64_t> hcount;
-    hcount.reserve(n / 4 + 16);
-    for (int64_t i = 0; i < n; ++i) hcount[rh[i] = row_hash(i)]++;
-    std::unordered_map<uint64_t, std::vector<int32_t>> byhash;
-    std::vector<std::vector<int32_t>> cand;
-    std::vector<int64_t> count;
-    std::vector<int32_t> rcand(n, -1), offs;
-    for (int64_t i = 0; i < n; ++i) {
-      const uint64_t hsh = rh[i];
-      if (hcount[hsh] < kMinPatternRows) continue;
-      offs.clear();
-      const int64_t orow = orig ? orig[i] : i;
-      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
-        if (keep(q)) offs.push_back((int32_t)((int64_t)col[q] - orow));
-      std::vector<int32_t>& ids = byhash[hsh];
-      int32_t id = -1;
-      for (int32_t c : ids)
-        if (cand[c] == offs) id = c;
-      if (id < 0 && cand.size() < kMaxCandidates) {
-        id = (int32_t)cand.size();
-        cand.push_back(offs);
-        count.push_back(0);
-        ids.push_back(id);
+    parallel_rows(n, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i) {
+        uint64_t hsh = 1469598103934665603ull;
+        const int64_t o = orow(i);
+        for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
+          if (keep(q)) hsh = (hsh ^ (uint32_t)(int32_t)((int64_t)col[q] - o)) * 1099511628211ull;
+        rh[i] = hsh;
       }
+    });
+    // equal relative patterns of rows i and r
+    auto same_pattern = [&](int64_t i, int64_t r) {
+      int64_t q = rowptr[i], t = rowptr[r];
+      const int64_t qe = rowptr[i + 1], te = rowptr[r + 1], oi = orow(i), orr = orow(r);
+      for (;;) {
+        while (q < qe && !keep(q)) ++q;
+        while (t < te && !keep(t)) ++t;
+        if (q == qe || t == te) return q == qe && t == te;
+        if ((int64_t)col[q] - oi != (int64_t)col[t] - orr) return false;
+        ++q;
+        ++t;
+      }
+    };
+    // rows per hash: only patterns frequent enough become candidates; the
+    // first row of each such hash represents it
+    std::vector<int32_t> rrep(n, -1);
+    {
+      KeyTable hcount(n, 0), first(n, -1);
+      for (int64_t i = 0; i < n; ++i) hcount[rh[i]]++;
+      for (int64_t i = 0; i < n; ++i)
+        if (hcount[rh[i]] >= kMinPatternRows) {
+          int64_t& f = first[rh[i]];
+          if (f < 0) f = i;
+          rrep[i] = (int32_t)f;
+        }
+    }
+    std::vector<uint8_t> as_rep(n, 0);
+    parallel_rows(n, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i)
+        if (rrep[i] >= 0) as_rep[i] = rrep[i] == i || same_pattern(i, rrep[i]);
+    });
+    // candidates in order of first appearance (the row that first shows each
+    // pattern); a row like its hash's first row takes that row's candidate,
+    // others (hash collisions) search the hash's candidates
+    std::unordered_map<uint64_t, std::vector<int32_t>> byhash;
+    std::vector<int32_t> cand_row, rcand(n, -1), cand_of(n, -1);
+    std::vector<int64_t> count;
+    for (int64_t i = 0; i < n; ++i) {
+      if (rrep[i] < 0) continue;
+      int32_t id = -1;
+      if (as_rep[i] && rrep[i] != i) {
+        id = cand_of[rrep[i]];
+      } else {
+        std::vector<int32_t>& ids = byhash[rh[i]];
+        for (int32_t c : ids)
+          if (same_pattern(i, cand_row[c])) id = c;
+        if (id < 0 && cand_row.size() < kMaxCandidates) {
+          id = (int32_t)cand_row.size();
+          cand_row.push_back((int32_t)i);
+          count.push_back(0);
+          ids.push_back(id);
+        }
+      }
+      cand_of[i] = id;
       if (id >= 0) {
         count[id]++;
         rcand[i] = id;
       }
     }
     // table in order of first appearance: patterns used by >= kMinPatternRows rows
-    std::vector<int32_t> pid_of(cand.size(), -1);
-    for (size_t c = 0; c < cand.size(); ++c) {
-      if (count[c] < kMinPatternRows || (int64_t)(ptab.size() + cand[c].size()) > kMaxTable ||
+    std::vector<int32_t> pid_of(cand_row.size(), -1);
+    for (size_t c = 0; c < cand_row.size(); ++c) {
+      const int64_t r = cand_row[c];
+      if (count[c] < kMinPatternRows || (int64_t)ptab.size() + rlen[r] > kMaxTable ||
           pstart.size() > 65535)
         continue;
       pid_of[c] = (int32_t)pstart.size() - 1;
-      ptab.insert(ptab.end(), cand[c].begin(), cand[c].end());
+      for (int64_t q = rowptr[r]; q < rowptr[r + 1]; ++q)
+        if (keep(q)) ptab.push_back((int32_t)((int64_t)col[q] - orow(r)));
       pstart.push_back((int32_t)ptab.size());
     }
-    for (int64_t i = 0; i < n; ++i)
-      if (rcand[i] >= 0) rpid[i] = pid_of[rcand[i]];
+    parallel_rows(n, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i)
+        if (rcand[i] >= 0) rpid[i] = pid_of[rcand[i]];
+    });
   }
   S->npat = (int64_t)pstart.size() - 1;
   S->ntab = (int64_t)ptab.size();
   // column encoding per slice: pattern; else 16-bit offsets when the slice's
-  // columns span fewer than 2^16 indices; else 32-bit
-  std::vector<uint32_t> cptr(S->nslices), cbase(S->nslices, 0);
+  // columns span fewer than 2^16 indices; else 32-bit.  Per slice first (on
+  // the threads), then the offsets in slice order.
+  std::vector<uint32_t> cptr(ns), cbase(ns, 0), lo_s(ns, 0), hi_s(ns, 0);
+  std::vector<int64_t> kept_s(ns, 0);
+  std::vector<uint8_t> wide(ns, 0), pat(ns, 0), one(ns, 0);
+  parallel_for(ns, 512, [&](int64_t s0, int64_t s1) {
+    std::vector<int32_t> ids(kSlice);
+    for (int64_t s = s0; s < s1; ++s) {
+      bool all = true;
+      int64_t kept = 0;
+      int m = 0;
+      for (int lane = 0; lane < kSlice; ++lane) {
+        const int64_t i = s * kSlice + lane;
+        if (i >= n) break;
+        all = all && rpid[i] >= 0;
+        kept += rlen[i];
+        ids[m++] = rpid[i];
+      }
+      kept_s[s] = kept;
+      if (all) {
+        bool same = true;
+        for (int l = 1; l < m; ++l) same = same && ids[l] == ids[0];
+        std::sort(ids.begin(), ids.begin() + m);
+        all = (std::unique(ids.begin(), ids.begin() + m) - ids.begin()) <= kMaxSlicePatterns;
+        one[s] = same;
+      }
+      if (all) {
+        pat[s] = 1;
+        continue;
+      }
+      uint32_t lo = 0xFFFFFFFFu, hi = 0;
+      for (int lane = 0; lane < kSlice; ++lane) {
+        const int64_t i = s * kSlice + lane;
+        if (i >= n) break;
+        for (int k = 0; k < rlen[i]; ++k) {
+          const uint32_t c = scol[sptr[s] + (uint64_t)k * kSlice + lane];
+          lo = std::min(lo, c);
+          hi = std::max(hi, c);
+        }
+      }
+      if (lo > hi) lo = hi = 0;  // no entries
+      lo_s[s] = lo;
+      hi_s[s] = hi;
+    }
+  });
   uint64_t units = 0;
-  std::vector<uint8_t> wide(S->nslices, 0), pat(S->nslices, 0);
   S->narrow = 0;
   S->patterned = 0;
   S->col_bytes = 0;
-  std::vector<int32_t> ids(kSlice);
-  for (int64_t s = 0; s < S->nslices; ++s) {
-    bool all = true;
-    int64_t kept = 0;
-    int m = 0;
-    for (int lane = 0; lane < kSlice; ++lane) {
-      const int64_t i = s * kSlice + lane;
-      if (i >= n) break;
-      all = all && rpid[i] >= 0;
-      kept += rlen[i];
-      ids[m++] = rpid[i];
-    }
-    if (all) {
-      std::sort(ids.begin(), ids.begin() + m);
-      all = (std::unique(ids.begin(), ids.begin() + m) - ids.begin()) <= kMaxSlicePatterns;
-    }
-    if (all) {
-      pat[s] = 1;
+  for (int64_t s = 0; s < ns; ++s) {
+    if (pat[s]) {
       cptr[s] = 0x40000000u;
       S->patterned++;
-      bool same = true;
-      for (int lane = 0; lane < kSlice && s * kSlice + lane < n; ++lane) {
-        rlen[s * kSlice + lane] = (uint16_t)rpid[s * kSlice + lane];
-        same = same && rpid[s * kSlice + lane] == rpid[s * kSlice];
-      }
-      if (same) {  // one pattern for the whole wavefront: its id in cbase
+      if (one[s]) {  // one pattern for the whole wavefront: its id in cbase
         cptr[s] |= 0x20000000u;
         cbase[s] = (uint32_t)rpid[s * kSlice];
       }
       continue;
     }
-    uint32_t lo = 0xFFFFFFFFu, hi = 0;
-    for (int lane = 0; lane < kSlice; ++lane) {
-      const int64_t i = s * kSlice + lane;
-      if (i >= n) break;
-      for (int k = 0; k < rlen[i]; ++k) {
-        const uint32_t c = scol[sptr[s] + (uint64_t)k * kSlice + lane];
-        lo = std::min(lo, c);
-        hi = std::max(hi, c);
-      }
-    }
     const uint64_t w = (sptr[s + 1] - sptr[s]);
-    if (lo > hi) lo = hi = 0;  // no entries
-    wide[s] = (hi - lo) > 0xFFFFu;
+    wide[s] = (hi_s[s] - lo_s[s]) > 0xFFFFu;
     if (wide[s]) {
       units += units & 1;  // 4-byte alignment
       cptr[s] = (uint32_t)units | 0x80000000u;
       units += 2 * w;
-      S->col_bytes += 4 * kept;
+      S->col_bytes += 4 * kept_s[s];
     } else {
       cptr[s] = (uint32_t)units;
-      cbase[s] = lo;
+      cbase[s] = lo_s[s];
       units += w;
       S->narrow++;
-      S->col_bytes += 2 * kept;
+      S->col_bytes += 2 * kept_s[s];
     }
     if (units >= 0x20000000ull) return fail(GH_EINVAL, "operator too large for its column index");
   }
   std::vector<uint16_t> cidx(std::max<uint64_t>(units, 1), 0);
-  for (int64_t s = 0; s < S->nslices; ++s) {
-    if (pat[s]) continue;
-    const uint64_t w = (sptr[s + 1] - sptr[s]);
-    const uint64_t at = cptr[s] & 0x1FFFFFFFu;
-    for (uint64_t j = 0; j < w; ++j) {
-      const int lane = (int)(j % kSlice);
-      const int64_t i = s * kSlice + lane;
-      const bool used = i < n && (int64_t)(j / kSlice) < rlen[i];
-      const uint32_t c = used ? scol[sptr[s] + j] : cbase[s];  // padding: never read
-      if (wide[s]) {
-        cidx[at + 2 * j] = (uint16_t)(c & 0xFFFFu);
-        cidx[at + 2 * j + 1] = (uint16_t)(c >> 16);
-      } else {
-        cidx[at + j] = (uint16_t)(c - cbase[s]);
+  parallel_for(ns, 512, [&](int64_t s0, int64_t s1) {
+    for (int64_t s = s0; s < s1; ++s) {
+      if (pat[s]) {  // rlen holds the pattern ids
+        for (int lane = 0; lane < kSlice && s * kSlice + lane < n; ++lane)
+          rlen[s * kSlice + lane] = (uint16_t)rpid[s * kSlice + lane];
+        continue;
+      }
+      const uint64_t w = (sptr[s + 1] - sptr[s]);
+      const uint64_t at = cptr[s] & 0x1FFFFFFFu;
+      for (uint64_t j = 0; j < w; ++j) {
+        const int lane = (int)(j % kSlice);
+        const int64_t i = s * kSlice + lane;
+        const bool used = i < n && (int64_t)(j / kSlice) < rlen[i];
+        const uint32_t c = used ? scol[sptr[s] + j] : cbase[s];  // padding: never read
+        if (wide[s]) {
+          cidx[at + 2 * j] = (uint16_t)(c & 0xFFFFu);
+          cidx[at + 2 * j + 1] = (uint16_t)(c >> 16);
+        } else {
+          cidx[at + j] = (uint16_t)(c - cbase[s]);
+        }
       }
     }
-  }
+  });
   GH_TRY(upload(ctx, &S->rlen, rlen.data(), rlen.size()));
   GH_TRY(upload(ctx, &S->cidx, cidx.data(), cidx.size()));
   GH_TRY(upload(ctx, &S->cptr, cptr.data(), cptr.size()));
@@ -800,22 +900,23 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
   constexpr int64_t kSellWindow = 4096;
   std::vector<uint64_t> key(n);
   std::vector<int32_t> len(n);
-  for (int64_t i = 0; i < n; ++i) {
-    uint64_t h = 1469598103934665603ull;
-    int32_t k = 0;
-    for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-      if (vals[q] == 0.0) continue;
-      h = (h ^ (uint32_t)(int32_t)((int64_t)col[q] - i)) * 1099511628211ull;
-      ++k;
+  parallel_rows(n, [&](int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) {
+      uint64_t h = 1469598103934665603ull;
+      int32_t k = 0;
+      for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        if (vals[q] == 0.0) continue;
+        h = (h ^ (uint32_t)(int32_t)((int64_t)col[q] - i)) * 1099511628211ull;
+        ++k;
+      }
+      key[i] = h;
+      len[i] = k;
     }
-    key[i] = h;
-    len[i] = k;
-  }
+  });
   // rows of patterns frequent enough for the pattern table (by hash)
   std::vector<uint8_t> frequent(n, 0);
   {
-    std::unordered_map<uint64_t, int64_t> hcount;
-    hcount.reserve(n / 4 + 16);
+    KeyTable hcount(n, 0);
     for (int64_t i = 0; i < n; ++i) hcount[key[i]]++;
     for (int64_t i = 0; i < n; ++i) frequent[i] = hcount[key[i]] >= kMinPatternRows;
   }
@@ -824,51 +925,61 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
   auto window_sort = [&](bool by_pattern) {
     std::vector<int32_t> p(n);
     for (int64_t i = 0; i < n; ++i) p[i] = (int32_t)i;
-    for (int64_t w0 = 0; w0 < n; w0 += kSellWindow) {
-      const int64_t w1 = std::min<int64_t>(n, w0 + kSellWindow);
-      std::stable_sort(p.begin() + w0, p.begin() + w1, [&](int32_t a, int32_t b) {
-        if (len[a] != len[b]) return len[a] > len[b];
-        if (!by_pattern) return false;
-        if (frequent[a] != frequent[b]) return frequent[a] > frequent[b];
-        return key[a] < key[b];
-      });
-    }
+    const int64_t nw = (n + kSellWindow - 1) / kSellWindow;
+    parallel_for(nw, 16, [&](int64_t wa, int64_t wb) {
+      for (int64_t w = wa; w < wb; ++w) {
+        const int64_t w0 = w * kSellWindow, w1 = std::min<int64_t>(n, w0 + kSellWindow);
+        std::stable_sort(p.begin() + w0, p.begin() + w1, [&](int32_t a, int32_t b) {
+          if (len[a] != len[b]) return len[a] > len[b];
+          if (!by_pattern) return false;
+          if (frequent[a] != frequent[b]) return frequent[a] > frequent[b];
+          return key[a] < key[b];
+        });
+      }
+    });
     return p;
   };
   // slices whose 64 rows share one pattern
+  const int64_t nsl = (n + kSlice - 1) / kSlice;
+  // per-slice figures on the threads, summed in slice order
+  auto per_slice = [&](auto&& fn) {
+    std::vector<int64_t> v(nsl);
+    parallel_for(nsl, 1024, [&](int64_t a, int64_t b) {
+      for (int64_t s = a; s < b; ++s) v[s] = fn(s * kSlice);
+    });
+    int64_t t = 0;
+    for (int64_t x : v) t += x;
+    return t;
+  };
   auto uniform = [&](const std::vector<int32_t>* p) {
-    int64_t u = 0;
-    for (int64_t s0 = 0; s0 + kSlice <= n; s0 += kSlice) {
+    return per_slice([&](int64_t s0) -> int64_t {
+      if (s0 + kSlice > n) return 0;
       const uint64_t k0 = key[p ? (*p)[s0] : s0];
       bool same = true;
       for (int l = 1; l < kSlice && same; ++l) same = key[p ? (*p)[s0 + l] : s0 + l] == k0;
-      u += same;
-    }
-    return u;
+      return same;
+    });
   };
   // slots a sweep runs: 64 times the longest row of each slice
   auto slots = [&](const std::vector<int32_t>* p) {
-    int64_t t = 0;
-    for (int64_t s0 = 0; s0 < n; s0 += kSlice) {
+    return per_slice([&](int64_t s0) -> int64_t {
       int32_t w = 0;
       for (int64_t t0 = s0; t0 < std::min<int64_t>(n, s0 + kSlice); ++t0)
         w = std::max(w, len[p ? (*p)[t0] : t0]);
-      t += (int64_t)kSlice * w;
-    }
-    return t;
+      return (int64_t)kSlice * w;
+    });
   };
   // rows a slice's lanes touch, in groups of 16 (the vectors' 128-byte lines
   // in FP64): 4 or 5 for 64 consecutive rows
   auto groups = [&](const std::vector<int32_t>& p) {
-    int64_t g = 0;
-    std::vector<int32_t> b(kSlice);
-    for (int64_t s0 = 0; s0 < n; s0 += kSlice) {
+    const int64_t g = per_slice([&](int64_t s0) -> int64_t {
+      int32_t b[kSlice];
       const int m = (int)std::min<int64_t>(kSlice, n - s0);
       for (int l = 0; l < m; ++l) b[l] = p[s0 + l] >> 4;
-      std::sort(b.begin(), b.begin() + m);
-      g += std::unique(b.begin(), b.begin() + m) - b.begin();
-    }
-    return (double)g / (double)((n + kSlice - 1) / kSlice);
+      std::sort(b, b + m);
+      return std::unique(b, b + m) - b;
+    });
+    return (double)g / (double)nsl;
   };
   const int64_t u0 = uniform(nullptr), t0 = slots(nullptr);
   const int64_t du = std::max<int64_t>(1, (n / kSlice) / 20);
@@ -892,11 +1003,13 @@ int sell_build(gh_ctx* ctx, int64_t n, const int32_t* rowptr, const uint32_t* co
   for (int64_t t = 0; t < n; ++t) rp2[t + 1] = rp2[t] + (rowptr[perm[t] + 1] - rowptr[perm[t]]);
   std::vector<uint32_t> c2(rp2[n]);
   std::vector<double> v2(rp2[n]);
-  for (int64_t t = 0; t < n; ++t) {
-    const int32_t i = perm[t];
-    std::copy(col + rowptr[i], col + rowptr[i + 1], c2.begin() + rp2[t]);
-    std::copy(vals + rowptr[i], vals + rowptr[i + 1], v2.begin() + rp2[t]);
-  }
+  parallel_rows(n, [&](int64_t a, int64_t b) {
+    for (int64_t t = a; t < b; ++t) {
+      const int32_t i = perm[t];
+      std::copy(col + rowptr[i], col + rowptr[i + 1], c2.begin() + rp2[t]);
+      std::copy(vals + rowptr[i], vals + rowptr[i + 1], v2.begin() + rp2[t]);
+    }
+  });
   GH_TRY(sell_build_impl(ctx, n, rp2.data(), c2.data(), v2.data(), prec, perm.data(), S));
   GH_TRY(upload(ctx, &S->perm, perm.data(), perm.size()));
   GH_HIP(hipStreamSynchronize(ctx->stream));
@@ -1457,19 +1570,32 @@ int csr_create_impl(gh_ctx* ctx, int64_t n, int ncomp, const int32_t* rowptr, co
   GH_ARG(rowptr[0] == 0 && nnz >= 0 && (nnz == 0 || (col && val)), "gh_csr_create: bad rowptr");
   const int BB = ncomp * ncomp;
   std::vector<int32_t> diag(n);
-  for (int64_t i = 0; i < n; ++i) {
+  for (int64_t i = 0; i < n; ++i)
     if (rowptr[i + 1] < rowptr[i]) return fail(GH_EINVAL, "gh_csr_create: rowptr not monotone");
-    diag[i] = -1;
-    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
-      if (col[p] < 0 || col[p] >= n)
-        return fail(GH_EINVAL, "gh_csr_create: column %d out of range in row %lld", col[p],
-                    (long long)i);
-      if (check_sorted && p > rowptr[i] && col[p] <= col[p - 1])
-        return fail(GH_EINVAL, "gh_csr_create: columns of row %lld not strictly ascending",
-                    (long long)i);
-      if (col[p] == i && diag[i] < 0) diag[i] = p;
+  // per row (on the host threads): 1 column out of range, 2 columns not
+  // ascending, 3 no diagonal; the first faulty row is reported
+  std::vector<uint8_t> bad(n, 0);
+  parallel_rows(n, [&](int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) {
+      diag[i] = -1;
+      for (int32_t p = rowptr[i]; p < rowptr[i + 1] && !bad[i]; ++p) {
+        if (col[p] < 0 || col[p] >= n) bad[i] = 1;
+        else if (check_sorted && p > rowptr[i] && col[p] <= col[p - 1]) bad[i] = 2;
+        else if (col[p] == i && diag[i] < 0) diag[i] = p;
+      }
+      if (!bad[i] && diag[i] < 0) bad[i] = 3;
     }
-    if (diag[i] < 0) return fail(GH_EINVAL, "gh_csr_create: row %lld has no diagonal", (long long)i);
+  });
+  for (int64_t i = 0; i < n; ++i) {
+    if (bad[i] == 1)
+      for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p)
+        if (col[p] < 0 || col[p] >= n)
+          return fail(GH_EINVAL, "gh_csr_create: column %d out of range in row %lld", col[p],
+                      (long long)i);
+    if (bad[i] == 2)
+      return fail(GH_EINVAL, "gh_csr_create: columns of row %lld not strictly ascending",
+                  (long long)i);
+    if (bad[i] == 3) return fail(GH_EINVAL, "gh_csr_create: row %lld has no diagonal", (long long)i);
   }
   gh_csr* A = new gh_csr();
   A->ctx = ctx;

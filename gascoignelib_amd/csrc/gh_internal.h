@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gascoigne_hip.h"
@@ -275,6 +277,34 @@ struct gh_mask {
 };
 
 namespace gh {
+// fn(begin, end) over [0, n) in contiguous chunks of at least min_chunk
+// items on up to 16 host threads (the calling thread takes the first); the
+// chunks write disjoint ranges, so the result does not depend on the thread
+// count.
+template <class F>
+void parallel_for(int64_t n, int64_t min_chunk, F&& fn) {
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  const int T = (int)std::min<int64_t>(std::min<int64_t>(16, hw),
+                                       std::max<int64_t>(1, n / std::max<int64_t>(1, min_chunk)));
+  if (T <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  const int64_t step = (n + T - 1) / T;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) {
+    const int64_t a = t * step, b = std::min(n, a + step);
+    if (a < b) pool.emplace_back([&fn, a, b] { fn(a, b); });
+  }
+  fn((int64_t)0, std::min(n, step));
+  for (std::thread& th : pool) th.join();
+}
+// rows of an operator: chunks of at least 32768 rows
+template <class F>
+void parallel_rows(int64_t n, F&& fn) {
+  parallel_for(n, (int64_t)1 << 15, fn);
+}
+
 void ctx_retain(gh_ctx* ctx);
 void ctx_release(gh_ctx* ctx);
 // vector helpers shared by the solver translation unit

@@ -557,27 +557,6 @@ namespace {
 // the caller's node perm[k].  Every row keeps its entries in the caller's order,
 // so sums are formed in the same order as without the renumbering.
 
-// fn(begin, end) over [0, n) in contiguous chunks on up to 16 host threads
-// (the calling thread takes the first); the chunks write disjoint ranges, so
-// the result does not depend on the thread count.
-template <class F>
-void parallel_rows(int64_t n, F&& fn) {
-  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
-  const int T = (int)std::min<int64_t>(std::min<int64_t>(16, hw), std::max<int64_t>(1, n >> 15));
-  if (T <= 1) {
-    fn((int64_t)0, n);
-    return;
-  }
-  const int64_t step = (n + T - 1) / T;
-  std::vector<std::thread> pool;
-  for (int t = 1; t < T; ++t) {
-    const int64_t a = t * step, b = std::min(n, a + step);
-    if (a < b) pool.emplace_back([&fn, a, b] { fn(a, b); });
-  }
-  fn((int64_t)0, std::min(n, step));
-  for (std::thread& th : pool) th.join();
-}
-
 int build_ordered_level(gh_ctx* ctx, const gh_mg_params* params, int l, gh_csr* const* A,
                         gh_transfer* const* T, gh_hn* const* H, gh_mask* const* M,
                         const int32_t* const* perm, bool sorted64, MgLevel& Lv) {
