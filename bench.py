@@ -972,7 +972,7 @@ def bench_q2(args, adaptive=0):
                                 "gbs": bmx / t_spmv_mx / 1e9,
                                 "frac": bmx / t_spmv_mx / 1e9 / HBM_PEAK_GBS, "layout": lay_mx},
         "setup_s": dict({"structure": t_struct, "device": t_setup},
-                        **({} if adaptive else q2_assembly_times(
+                        **(q2_condensed_times(ctx, hier) if adaptive else q2_assembly_times(
                             ctx, hier, {t: sol[t][0] for t in ("fp64", "mixed")}))),
     }
     print(json.dumps(line), flush=True)
@@ -1020,6 +1020,49 @@ def q2_assembly_times(ctx, hier, solvers=None, reps=3):
         out[name] = float(np.median(ts))
     if not np.array_equal(A.values(), val):
         raise RuntimeError("device Q2 assembly differs from the host producer")
+    return out
+
+
+def q2_condensed_times(ctx, hier, reps=3):
+    """Seconds to form the finest level's Q2 operator condensed over its
+    hanging nodes (Dirichlet rows): the host producer's assembly and
+    condensation, against the device's (gh_assemble_q2 of the unconstrained
+    operator, then gh_condense_q2) into matrices of the same patterns; each
+    the median of `reps` after one warm-up.  The values are checked equal."""
+    from gascoignelib_amd import hip as H
+    from gascoignelib_amd import native as N
+    M = hier.mesh
+    l = M.nlevels - 1
+    rp, col = M.patch_structure(l)
+    free = M.assemble_q2(l, len(col), ())
+    rpc, colc, want = M.q2_condensed(l, (1,))
+    e, f = (np.ascontiguousarray(a, dtype=np.int32) for a in M.hn(l))
+    A = H.CudaCSRMatrixInterface(ctx, rp, col, np.where(free != 0.0, free, 1.0))
+    Ac = H.CudaCSRMatrixInterface(ctx, rpc, colc, np.where(want != 0.0, want, 1.0))
+    P = H.DwrPatches(ctx, 3, M.coords(l), M.patches(l))
+    mask = H.DirichletMask(ctx, len(rpc) - 1, 1, M.dirichlet(l, (1,)))
+
+    def host():
+        M.q2_condensed(l, (1,))
+
+    def device():
+        P.assemble_q2(A)
+        N.check(N.hip().gh_condense_q2(ctx.h, len(e), N.ptr(e, N.i32p), len(f), N.ptr(f, N.i32p),
+                                       A.h, mask.h, Ac.h))
+        ctx.sync()
+
+    out = {}
+    for name, fn in (("assembly_q2_condensed_host_fine", host),
+                     ("assembly_q2_condensed_device_fine", device)):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        out[name] = float(np.median(ts))
+    if not np.array_equal(Ac.values(), want):
+        raise RuntimeError("device Q2 condensation differs from the host producer")
     return out
 
 

@@ -296,6 +296,68 @@ __global__ __launch_bounds__(kBlock) void k_form_q2(
   r[i] = acc;
 }
 
+// Q2 operator condensed over the hanging nodes: A_c = T^T A T with T setting
+// each hanging value from its parents (CGDisc::LocalToGlobal with
+// CondenseHanging, src/DofHandler/cgdisc.h:353-366, in global form, as the
+// host producer forms it).  One thread per row i of A_c: a hanging row is an
+// identity row; any other row receives, in this order, the entries of row i
+// of A and then those of the rows h of the hanging nodes with parent i
+// (weighted w_hi, ascending h), each entry (r, j) going to column j, or, for a
+// hanging j, to its parents p with the weight w_jp: s * (a_rj * w_jp) and
+// s * a_rj with s = 1 or w_hi.  The contributions to one entry are summed in
+// that order (the host merges a stable sort by column), found by bisection
+// in the row's ascending columns; a missing column sets flags[2].  Then the
+// Dirichlet rows and columns.
+__global__ __launch_bounds__(kBlock) void k_condense_q2(
+    int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const double* __restrict__ val, const int32_t* __restrict__ hstart,
+    const int32_t* __restrict__ hpar, const double* __restrict__ hw,
+    const int32_t* __restrict__ fstart, const int32_t* __restrict__ flist,
+    const double* __restrict__ fw, const int32_t* __restrict__ rpc,
+    const int32_t* __restrict__ colc, const uint8_t* __restrict__ isdir,
+    double* __restrict__ valc, int* flags) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t q0 = rpc[i], q1 = rpc[i + 1];
+  if (hstart[i + 1] > hstart[i]) {
+    for (int32_t q = q0; q < q1; ++q) valc[q] = (colc[q] == i) ? 1.0 : 0.0;
+    return;
+  }
+  for (int32_t q = q0; q < q1; ++q) valc[q] = 0.0;
+  bool bad = false;
+  auto add = [&](int32_t c, double x) {
+    int32_t lo = q0, hi = q1;
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if (colc[mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < q1 && colc[lo] == c) valc[lo] = add_rn(valc[lo], x);
+    else bad = true;
+  };
+  auto row = [&](int64_t r, double s) {
+    for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+      const int32_t j = col[q];
+      const double v = val[q];
+      if (hstart[j + 1] > hstart[j]) {
+        for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k) add(hpar[k], mul_rn(s, mul_rn(v, hw[k])));
+      } else {
+        add(j, mul_rn(s, v));
+      }
+    }
+  };
+  row(i, 1.0);
+  for (int32_t k = fstart[i]; k < fstart[i + 1]; ++k) row(flist[k], fw[k]);
+  if (isdir) {
+    const bool drow = isdir[i];
+    for (int32_t q = q0; q < q1; ++q) {
+      if (drow) valc[q] = (colc[q] == i) ? 1.0 : 0.0;
+      else if (isdir[colc[q]]) valc[q] = 0.0;
+    }
+  }
+  if (bad) flags[2] = 1;
+}
+
 }  // namespace
 
 namespace gh {
@@ -423,6 +485,104 @@ int gh_assemble_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_m
   GH_TRY(assemble_q2_launch(ctx, P, k ? (const double*)k->d : nullptr, isdir, nullptr, nullptr,
                             assemble_q2_mode(A, nullptr), A, v64));
   return csr_asm_commit(ctx, A, "gh_assemble_q2");
+}
+
+int gh_condense_q2(gh_ctx* ctx, int64_t nedges, const int32_t* edges, int64_t nfaces,
+                   const int32_t* faces, const gh_csr* A, const gh_mask* dirichlet, gh_csr* Ac) {
+  GH_ARG(ctx && A && Ac && nedges >= 0 && nfaces >= 0 && (nedges == 0 || edges) &&
+             (nfaces == 0 || faces),
+         "gh_condense_q2: bad arguments");
+  GH_ARG(!A->host_only && A->ncomp == 1 && A->prec == GH_F64,
+         "gh_condense_q2: A must be a scalar FP64 matrix with a device layout");
+  GH_ARG(!Ac->host_only && Ac->ncomp == 1 && Ac->n == A->n,
+         "gh_condense_q2: Ac must be a scalar matrix of A's size with a device layout");
+  GH_ARG(!dirichlet || (dirichlet->n == A->n && dirichlet->ncomp == 1),
+         "gh_condense_q2: the Dirichlet mask does not match the matrix");
+  GH_ARG(assemble_q2_mode(Ac, nullptr) == 0,
+         "gh_condense_q2: Ac's columns must ascend in every row");
+  const int64_t n = A->n;
+  // hanging node -> (parent, weight): the edge constraints' parents with
+  // (3/8, 3/4, -1/8), then the face constraints' with the products
+  const double w1[3] = {0.375, 0.75, -0.125};
+  std::vector<int32_t> hstart(n + 1, 0);
+  for (int64_t k = 0; k < nedges; ++k) {
+    const int32_t h = edges[4 * k];
+    GH_ARG(h >= 0 && h < n, "gh_condense_q2: edge constraint node out of range");
+    hstart[h + 1] += 3;
+  }
+  for (int64_t k = 0; k < nfaces; ++k) {
+    const int32_t h = faces[10 * k];
+    GH_ARG(h >= 0 && h < n, "gh_condense_q2: face constraint node out of range");
+    hstart[h + 1] += 9;
+  }
+  for (int64_t i = 0; i < n; ++i) hstart[i + 1] += hstart[i];
+  std::vector<int32_t> hpar(hstart[n]);
+  std::vector<double> hw(hstart[n]);
+  {
+    std::vector<int32_t> fill(hstart.begin(), hstart.end() - 1);
+    for (int64_t k = 0; k < nedges; ++k)
+      for (int t = 0; t < 3; ++t) {
+        const int32_t q = fill[edges[4 * k]]++;
+        hpar[q] = edges[4 * k + 1 + t];
+        hw[q] = w1[t];
+      }
+    for (int64_t k = 0; k < nfaces; ++k)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const int32_t q = fill[faces[10 * k]]++;
+          hpar[q] = faces[10 * k + 1 + 3 * a + b];
+          hw[q] = w1[a] * w1[b];
+        }
+  }
+  for (int32_t p : hpar) GH_ARG(p >= 0 && p < n, "gh_condense_q2: constraint parent out of range");
+  // parent -> hanging nodes that feed it, ascending hanging node
+  std::vector<int32_t> fstart(n + 1, 0), flist(hpar.size());
+  std::vector<double> fw(hpar.size());
+  for (int32_t p : hpar) fstart[p + 1]++;
+  for (int64_t i = 0; i < n; ++i) fstart[i + 1] += fstart[i];
+  {
+    std::vector<int32_t> fill(fstart.begin(), fstart.end() - 1);
+    for (int64_t h = 0; h < n; ++h)
+      for (int32_t k = hstart[h]; k < hstart[h + 1]; ++k) {
+        const int32_t q = fill[hpar[k]]++;
+        flist[q] = (int32_t)h;
+        fw[q] = hw[k];
+      }
+  }
+  const uint8_t* isdir = nullptr;
+  if (dirichlet) GH_TRY(mask_node_flags(ctx, const_cast<gh_mask*>(dirichlet), &isdir));
+  double* vc = nullptr;
+  GH_TRY(csr_asm_values(ctx, Ac, &vc));
+  if (!Ac->d_flags) {
+    if (hipMalloc((void**)&Ac->d_flags, sizeof(int) * 4) != hipSuccess)
+      return fail(GH_ENOMEM, "gh_condense_q2: flags");
+    GH_HIP(hipMemsetAsync(Ac->d_flags, 0, sizeof(int) * 4, ctx->stream));
+  }
+  int32_t *d_hs = nullptr, *d_hp = nullptr, *d_fs = nullptr, *d_fl = nullptr;
+  double *d_hw = nullptr, *d_fw = nullptr;
+  auto up = [&](void** d, const void* h, size_t bytes) {
+    if (hipMalloc(d, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    return bytes == 0 || hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  int rc = GH_OK;
+  if (!up((void**)&d_hs, hstart.data(), 4 * hstart.size()) ||
+      !up((void**)&d_hp, hpar.data(), 4 * hpar.size()) ||
+      !up((void**)&d_hw, hw.data(), 8 * hw.size()) ||
+      !up((void**)&d_fs, fstart.data(), 4 * fstart.size()) ||
+      !up((void**)&d_fl, flist.data(), 4 * flist.size()) ||
+      !up((void**)&d_fw, fw.data(), 8 * fw.size()))
+    rc = fail(GH_ENOMEM, "gh_condense_q2: constraint tables");
+  if (!rc && n > 0) {
+    hipLaunchKernelGGL(k_condense_q2, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n,
+                       A->rowptr, A->col, (const double*)A->val, d_hs, d_hp, d_hw, d_fs, d_fl,
+                       d_fw, Ac->rowptr, Ac->col, isdir, vc, Ac->d_flags);
+    if (hipGetLastError() != hipSuccess) rc = fail(GH_EHIP, "gh_condense_q2: launch failed");
+  }
+  if (!rc) rc = csr_asm_commit(ctx, Ac, "gh_condense_q2");  // synchronises
+  else (void)hipStreamSynchronize(ctx->stream);
+  for (void* d : {(void*)d_hs, (void*)d_hp, (void*)d_hw, (void*)d_fs, (void*)d_fl, (void*)d_fw})
+    if (d) (void)hipFree(d);
+  return rc;
 }
 
 int gh_form_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask* dirichlet,

@@ -158,6 +158,7 @@ HIP_SIGNATURES = {
     "gh_form_q1": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "gh_assemble_q2": (C.c_int, [vp, vp, vp, vp, vp]),
     "gh_form_q2": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "gh_condense_q2": (C.c_int, [vp, C.c_int64, i32p, C.c_int64, i32p, vp, vp, vp]),
     "gh_mg_assemble_q1": (C.c_int, [vp, vp, P(vp), P(vp)]),
     "gh_mg_assemble_q2": (C.c_int, [vp, vp, P(vp), P(vp)]),
 }

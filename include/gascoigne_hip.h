@@ -483,6 +483,19 @@ int gh_form_q1(gh_ctx* ctx, const gh_cells* C, const gh_vec* k, const gh_mask* d
  * caller's.  Synchronises. */
 int gh_assemble_q2(gh_ctx* ctx, const gh_patches* P, const gh_vec* k, const gh_mask* dirichlet,
                    gh_csr* A);
+/* The Q2 operator condensed over the hanging nodes (CGDisc::LocalToGlobal
+ * with HNStructureQ23d::CondenseHanging, src/DofHandler/cgdisc.h:353-366,
+ * src/Discretization/Q2/hnstructureq23d.cc:120-156, in global form):
+ * Ac = T^T A T from the unconstrained FP64 operator A (gh_assemble_q2), T the
+ * interpolation of the edge constraints (nedges x [h, near, middle, far],
+ * weights 3/8, 3/4, -1/8) and the face constraints (nfaces x [h, 9 parents],
+ * products of those weights); hanging rows are identity rows, then the
+ * mask's rows and columns.  Ac: a scalar matrix of the condensed pattern with
+ * ascending columns (any precision), its values written like
+ * gh_assemble_q2's; the values equal the host producer's condensation (the
+ * sign of a zero sum aside).  Synchronises. */
+int gh_condense_q2(gh_ctx* ctx, int64_t nedges, const int32_t* edges, int64_t nfaces,
+                   const int32_t* faces, const gh_csr* A, const gh_mask* dirichlet, gh_csr* Ac);
 /* CGDisc::Form (src/DofHandler/cgdisc.h:415-448) for the same elements:
  * r = f - A(u) patch by patch from u, each node summing its patches in
  * ascending order; the mask's rows set to zero. */

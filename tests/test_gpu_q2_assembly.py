@@ -230,3 +230,37 @@ def test_q2_solver_reassembly_matches_host_built_solver(ctx, prec, krylov):
         for fp64 in (True, False):
             assert np.array_equal(mg1.level_matrix(l, fp64).values(),
                                   mg2.level_matrix(l, fp64).values())
+
+
+def test_q2_condensed_over_hanging_nodes_equals_host(ctx):
+    """gh_assemble_q2 (unconstrained) then gh_condense_q2 on every level of an
+    adaptive Q2 hierarchy (8^3 + 2 refinements in a ball): the condensed
+    values equal the host producer's (M.q2_condensed) bit for bit, the sign
+    of an exactly-zero sum aside"""
+    print("[condense q2]", flush=True)
+    lib = N.hip()
+    M = Mesh.adaptive("cube.inp", 3, 1, 2, 0.25, (0.6, 0.45, 0.4))
+    checked = 0
+    for l in range(M.nlevels):
+        e, f = M.hn(l)
+        rp, col = M.patch_structure(l)
+        free = M.assemble_q2(l, len(col), ())
+        A = H.CudaCSRMatrixInterface(ctx, rp, col, np.where(free != 0.0, free, 1.0))
+        Pt = H.DwrPatches(ctx, 3, M.coords(l), M.patches(l))
+        Pt.assemble_q2(A)
+        assert np.array_equal(A.values(), free)
+        rpc, colc, want = M.q2_condensed(l, (1,))
+        n = len(rpc) - 1
+        Ac = H.CudaCSRMatrixInterface(ctx, rpc, colc, np.where(want != 0.0, want, 1.0))
+        mask = H.DirichletMask(ctx, n, 1, M.dirichlet(l, (1,)))
+        e = np.ascontiguousarray(e, dtype=np.int32)
+        f = np.ascontiguousarray(f, dtype=np.int32)
+        N.check(lib.gh_condense_q2(ctx.h, len(e), N.ptr(e, N.i32p), len(f), N.ptr(f, N.i32p),
+                                   A.h, mask.h, Ac.h))
+        assert np.array_equal(Ac.values(), want), f"level {l}"
+        checked += len(e) + len(f) > 0
+    assert checked >= 1  # levels with hanging nodes were among them
+    # a constraint naming a node outside the matrix is refused
+    bad = np.array([[len(rpc) + 5, 0, 1, 2]], dtype=np.int32)
+    rc = lib.gh_condense_q2(ctx.h, 1, N.ptr(bad, N.i32p), 0, None, A.h, None, Ac.h)
+    assert rc == N.GH_EINVAL
