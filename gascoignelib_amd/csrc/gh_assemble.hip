@@ -205,15 +205,20 @@ __device__ __forceinline__ double q2_entry(int a, int b, const double* __restric
   return add_rn(add_rn(t1, t2), t3);
 }
 
-// One thread per row i (the matrix's numbering; its node is perm[i], or i):
-// the row's entries start at 0 and receive, patch by patch in ascending order
-// and node by node, k_p E_ab at the entry of node b, as the host producer adds
-// them; then the Dirichlet rows become unit rows and the Dirichlet columns
-// zero.  An entry is found by bisection where the row's columns ascend (mode
-// 0: as stored, the node's position found through iperm; mode 1: mapped
-// through perm, the renumbered operators that keep the caller's entry
-// order), else by a scan (mode 2).  A patch node without an entry in the row
-// sets flags[2].
+// One wavefront per row i (the matrix's numbering; its node is perm[i], or
+// i): the row's column keys and its running values sit in LDS; for each patch
+// of the node, in ascending order, lane b takes the patch's node b, finds its
+// entry by bisection among the keys and adds k_p E_ab there, as the host
+// producer adds them (one patch after the other, so each entry receives its
+// patches in ascending order); then the Dirichlet rows become unit rows and
+// the Dirichlet columns zero.  The keys: the stored columns, with the node's
+// position found through iperm (mode 0), or the columns mapped through perm
+// (mode 1, the renumbered operators that keep the caller's entry order);
+// either must ascend along the row.  A patch node without an entry in the row
+// sets flags[2].  Rows hold at most kQ2RowMax entries (checked by the host).
+constexpr int kQ2RowMax = 256;
+constexpr int kQ2Waves = kBlock / 64;
+
 __global__ __launch_bounds__(kBlock) void k_assemble_q2(
     int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm, int mode,
@@ -221,52 +226,49 @@ __global__ __launch_bounds__(kBlock) void k_assemble_q2(
     const int32_t* __restrict__ plist, const int32_t* __restrict__ nodes,
     const double* __restrict__ hs, const double* __restrict__ kp,
     const uint8_t* __restrict__ isdir, double* __restrict__ val, int* flags) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  __shared__ int32_t s_key[kQ2Waves][kQ2RowMax];
+  __shared__ double s_acc[kQ2Waves][kQ2RowMax];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * kQ2Waves + w;
+  if (i >= n) return;  // the whole wavefront: no block barriers below
+  int32_t* key = s_key[w];
+  double* acc = s_acc[w];
   const int32_t ic = perm ? perm[i] : (int32_t)i;
-  const int32_t q0 = rowptr[i], q1 = rowptr[i + 1];
-  for (int32_t q = q0; q < q1; ++q) val[q] = 0.0;
-  // mode 0 compares stored columns with the node's position in A, modes 1
-  // and 2 columns mapped to nodes with the node
-  auto key = [&](int32_t q) { return (perm && mode != 0) ? perm[col[q]] : col[q]; };
+  const int32_t q0 = rowptr[i], len = rowptr[i + 1] - q0;
+  for (int t = lane; t < len; t += 64) {
+    key[t] = (perm && mode != 0) ? perm[col[q0 + t]] : col[q0 + t];
+    acc[t] = 0.0;
+  }
+  // the LDS operations of one wavefront complete in issue order, so the lanes
+  // see each other's keys and sums from here on
+  __builtin_amdgcn_wave_barrier();
   bool bad = false;
   for (int32_t e = pptr[ic]; e < pptr[ic + 1]; ++e) {
     const int64_t p = plist[e] >> 5;
     const int a = plist[e] & 31;
-    const int32_t* v = nodes + p * 27;
-    const double k = kp ? kp[p] : 1.0;
-    for (int b = 0; b < 27; ++b) {
-      const int32_t j = (perm && mode == 0) ? iperm[v[b]] : v[b];
-      int32_t at = q1;
-      if (mode < 2) {
-        int32_t lo = q0, hi = q1;  // first entry whose key is >= j
-        while (lo < hi) {
-          const int32_t mid = lo + ((hi - lo) >> 1);
-          if (key(mid) < j) lo = mid + 1;
-          else hi = mid;
-        }
-        if (lo < q1 && key(lo) == j) at = lo;
-      } else {
-        for (int32_t q = q0; q < q1; ++q)
-          if (key(q) == j) {
-            at = q;
-            break;
-          }
+    if (lane < 27) {
+      const int32_t v = nodes[p * 27 + lane];
+      const int32_t j = (perm && mode == 0) ? iperm[v] : v;
+      int32_t lo = 0, hi = len;  // first key >= j
+      while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        if (key[mid] < j) lo = mid + 1;
+        else hi = mid;
       }
-      if (at == q1) {
+      if (lo < len && key[lo] == j)
+        acc[lo] = add_rn(acc[lo], mul_rn(kp ? kp[p] : 1.0, q2_entry(a, lane, hs + 3 * p)));
+      else
         bad = true;
-        continue;
-      }
-      val[at] = add_rn(val[at], mul_rn(k, q2_entry(a, b, hs + 3 * p)));
     }
+    __builtin_amdgcn_wave_barrier();
   }
-  if (isdir) {
-    const bool drow = isdir[i];
-    for (int32_t q = q0; q < q1; ++q) {
-      const int32_t j = col[q];
-      if (drow) val[q] = (j == i) ? 1.0 : 0.0;
-      else if (isdir[j]) val[q] = 0.0;
-    }
+  const bool drow = isdir && isdir[i];
+  for (int t = lane; t < len; t += 64) {
+    const int32_t c = col[q0 + t];
+    double x = acc[t];
+    if (drow) x = (c == i) ? 1.0 : 0.0;
+    else if (isdir && isdir[c]) x = 0.0;
+    val[q0 + t] = x;
   }
   if (bad) flags[2] = 1;
 }
@@ -381,11 +383,14 @@ int assemble_q1_launch(gh_ctx* ctx, const gh_cells* C, const double* kcell, cons
   return GH_OK;
 }
 
-// How k_assemble_q2 finds a row's entries in A (perm: A's row i is node
-// perm_h[i]): 0 bisection on the stored columns, 1 bisection on the columns
-// mapped through perm, 2 a scan.  Cached in A (one numbering per matrix).
+// How k_assemble_q2 finds a row's entries in A (perm_h: A's row i is node
+// perm_h[i]): 0 bisection on the stored columns, 1 on the columns mapped
+// through perm, -2 neither ascends (refused), -3 a row longer than
+// kQ2RowMax.  Cached in A (one numbering per matrix).
 int assemble_q2_mode(gh_csr* A, const std::vector<int32_t>* perm_h) {
-  if (A->asm_mode >= 0) return A->asm_mode;
+  if (A->asm_mode != -1) return A->asm_mode;
+  for (int64_t i = 0; i < A->n; ++i)
+    if (A->h_rowptr[i + 1] - A->h_rowptr[i] > kQ2RowMax) return A->asm_mode = -3;
   auto ascend = [&](bool mapped) {
     for (int64_t i = 0; i < A->n; ++i)
       for (int32_t q = A->h_rowptr[i] + 1; q < A->h_rowptr[i + 1]; ++q) {
@@ -395,7 +400,7 @@ int assemble_q2_mode(gh_csr* A, const std::vector<int32_t>* perm_h) {
     return true;
   };
   if (ascend(false)) A->asm_mode = 0;
-  else A->asm_mode = (perm_h && ascend(true)) ? 1 : 2;
+  else A->asm_mode = (perm_h && ascend(true)) ? 1 : -2;
   return A->asm_mode;
 }
 
@@ -407,8 +412,12 @@ int assemble_q2_launch(gh_ctx* ctx, const gh_patches* P, const double* kp, const
       return fail(GH_ENOMEM, "device assembly: flags");
     GH_HIP(hipMemsetAsync(A->d_flags, 0, sizeof(int) * 4, ctx->stream));
   }
+  if (mode < 0)
+    return fail(GH_EINVAL, mode == -3 ? "Q2 assembly: a row has more than 256 entries"
+                                      : "Q2 assembly: the matrix's columns do not ascend in every row");
   if (A->n == 0) return GH_OK;
-  hipLaunchKernelGGL(k_assemble_q2, dim3(grid_for(A->n)), dim3(kBlock), 0, ctx->stream, A->n,
+  const int64_t blocks = (A->n + kQ2Waves - 1) / kQ2Waves;
+  hipLaunchKernelGGL(k_assemble_q2, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, A->n,
                      A->rowptr, A->col, perm, iperm, mode, P->pptr, P->plist, P->nodes, P->hs,
                      kp, isdir, val64, A->d_flags);
   GH_HIP(hipGetLastError());
@@ -498,8 +507,16 @@ int gh_condense_q2(gh_ctx* ctx, int64_t nedges, const int32_t* edges, int64_t nf
          "gh_condense_q2: Ac must be a scalar matrix of A's size with a device layout");
   GH_ARG(!dirichlet || (dirichlet->n == A->n && dirichlet->ncomp == 1),
          "gh_condense_q2: the Dirichlet mask does not match the matrix");
-  GH_ARG(assemble_q2_mode(Ac, nullptr) == 0,
-         "gh_condense_q2: Ac's columns must ascend in every row");
+  if (Ac->cols_ascend < 0) {
+    Ac->cols_ascend = 1;
+    for (int64_t i = 0; i < Ac->n && Ac->cols_ascend; ++i)
+      for (int32_t q = Ac->h_rowptr[i] + 1; q < Ac->h_rowptr[i + 1]; ++q)
+        if (Ac->h_col[q] <= Ac->h_col[q - 1]) {
+          Ac->cols_ascend = 0;
+          break;
+        }
+  }
+  GH_ARG(Ac->cols_ascend == 1, "gh_condense_q2: Ac's columns must ascend in every row");
   const int64_t n = A->n;
   // hanging node -> (parent, weight): the edge constraints' parents with
   // (3/8, 3/4, -1/8), then the face constraints' with the products

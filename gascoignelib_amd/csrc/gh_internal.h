@@ -231,6 +231,7 @@ struct gh_csr {
   int* d_flags = nullptr;      // [0] singular diagonal, [1] a left-out entry became nonzero,
                                // [2] an element coupling has no entry in the pattern
   int asm_mode = -1;           // how the Q2 device assembly finds entries (found on first use)
+  int cols_ascend = -1;        // 1: every row's columns ascend (found on first use)
 };
 
 struct gh_transfer {
