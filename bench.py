@@ -1030,7 +1030,6 @@ def q2_condensed_times(ctx, hier, reps=3):
     operator, then gh_condense_q2) into matrices of the same patterns; each
     the median of `reps` after one warm-up.  The values are checked equal."""
     from gascoignelib_amd import hip as H
-    from gascoignelib_amd import native as N
     M = hier.mesh
     l = M.nlevels - 1
     rp, col = M.patch_structure(l)
@@ -1047,8 +1046,7 @@ def q2_condensed_times(ctx, hier, reps=3):
 
     def device():
         P.assemble_q2(A)
-        N.check(N.hip().gh_condense_q2(ctx.h, len(e), N.ptr(e, N.i32p), len(f), N.ptr(f, N.i32p),
-                                       A.h, mask.h, Ac.h))
+        H.condense_q2(ctx, e, f, A, Ac, mask)
         ctx.sync()
 
     out = {}

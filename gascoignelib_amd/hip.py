@@ -623,6 +623,17 @@ class DwrPatches:
             pass
 
 
+def condense_q2(ctx, edges, faces, A, Ac, mask=None):
+    """Ac <- T^T A T: the unconstrained FP64 Q2 operator A (DwrPatches.assemble_q2)
+    condensed over the Q2 hanging nodes (edges (ne, 4) [h, near, middle, far],
+    faces (nf, 10) [h, 9 parents]) on the device (gh_condense_q2); hanging rows
+    identity rows, then the mask's rows and columns"""
+    e = np.ascontiguousarray(edges, dtype=np.int32).reshape(-1, 4)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 10)
+    check(N.hip().gh_condense_q2(ctx.h, len(e), ptr(e, N.i32p), len(f), ptr(f, N.i32p), A.h,
+                                 mask.h if mask is not None else None, Ac.h))
+
+
 class SparseBlockIlu:
     """SparseBlockIlu: ILU(0) factors of an FP64 matrix, ILU row i = node
     perm[i]; solve(x) in place (FP64 or FP32 vector)."""

@@ -253,10 +253,7 @@ def test_q2_condensed_over_hanging_nodes_equals_host(ctx):
         n = len(rpc) - 1
         Ac = H.CudaCSRMatrixInterface(ctx, rpc, colc, np.where(want != 0.0, want, 1.0))
         mask = H.DirichletMask(ctx, n, 1, M.dirichlet(l, (1,)))
-        e = np.ascontiguousarray(e, dtype=np.int32)
-        f = np.ascontiguousarray(f, dtype=np.int32)
-        N.check(lib.gh_condense_q2(ctx.h, len(e), N.ptr(e, N.i32p), len(f), N.ptr(f, N.i32p),
-                                   A.h, mask.h, Ac.h))
+        H.condense_q2(ctx, e, f, A, Ac, mask)
         assert np.array_equal(Ac.values(), want), f"level {l}"
         checked += len(e) + len(f) > 0
     assert checked >= 1  # levels with hanging nodes were among them
