@@ -301,15 +301,23 @@ __global__ __launch_bounds__(kBlock) void k_form_q2(
 // Q2 operator condensed over the hanging nodes: A_c = T^T A T with T setting
 // each hanging value from its parents (CGDisc::LocalToGlobal with
 // CondenseHanging, src/DofHandler/cgdisc.h:353-366, in global form, as the
-// host producer forms it).  One thread per row i of A_c: a hanging row is an
-// identity row; any other row receives, in this order, the entries of row i
-// of A and then those of the rows h of the hanging nodes with parent i
-// (weighted w_hi, ascending h), each entry (r, j) going to column j, or, for a
-// hanging j, to its parents p with the weight w_jp: s * (a_rj * w_jp) and
-// s * a_rj with s = 1 or w_hi.  The contributions to one entry are summed in
-// that order (the host merges a stable sort by column), found by bisection
-// in the row's ascending columns; a missing column sets flags[2].  Then the
-// Dirichlet rows and columns.
+// host producer forms it).  Row i of A_c: a hanging row is an identity row;
+// any other row receives, in this order, the entries of row i of A and then
+// those of the rows h of the hanging nodes with parent i (weighted w_hi,
+// ascending h), each entry (r, j) going to column j, or, for a hanging j, to
+// its parents p with the weight w_jp: s * (a_rj * w_jp) and s * a_rj with
+// s = 1 or w_hi.  The contributions to one entry are summed in that order
+// (the host merges a stable sort by column, which is the same order).  Then
+// the Dirichlet rows and columns.  A missing column sets flags[2].
+//
+// One wavefront per row, the row's columns and running sums in LDS (rows of
+// at most kCondMax entries; longer ones, rare, take one lane and global
+// memory).  A row that neither holds a hanging column nor has hanging nodes
+// attached takes row i of A alone, whose columns are distinct: the lanes add
+// its entries at once; the other rows (near the refinement interface) walk
+// their sequence in one lane, in order.
+constexpr int kCondMax = 512;
+
 __global__ __launch_bounds__(kBlock) void k_condense_q2(
     int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
     const double* __restrict__ val, const int32_t* __restrict__ hstart,
@@ -318,44 +326,80 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
     const double* __restrict__ fw, const int32_t* __restrict__ rpc,
     const int32_t* __restrict__ colc, const uint8_t* __restrict__ isdir,
     double* __restrict__ valc, int* flags) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int32_t q0 = rpc[i], q1 = rpc[i + 1];
+  __shared__ int32_t s_key[kQ2Waves][kCondMax];
+  __shared__ double s_acc[kQ2Waves][kCondMax];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * kQ2Waves + w;
+  if (i >= n) return;  // the whole wavefront: no block barriers below
+  const int32_t q0 = rpc[i], len = rpc[i + 1] - q0;
   if (hstart[i + 1] > hstart[i]) {
-    for (int32_t q = q0; q < q1; ++q) valc[q] = (colc[q] == i) ? 1.0 : 0.0;
+    for (int t = lane; t < len; t += 64) valc[q0 + t] = (colc[q0 + t] == i) ? 1.0 : 0.0;
     return;
   }
-  for (int32_t q = q0; q < q1; ++q) valc[q] = 0.0;
+  const bool lds = len <= kCondMax;
+  const int32_t* keys = lds ? s_key[w] : colc + q0;
+  double* acc = lds ? s_acc[w] : valc + q0;
+  for (int t = lane; t < len; t += 64) {
+    if (lds) s_key[w][t] = colc[q0 + t];
+    acc[t] = 0.0;
+  }
+  // LDS operations of one wavefront complete in issue order; the sums of a
+  // long row live in global memory, where each lane's stores must complete
+  // before other lanes read them
+  if (!lds) __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
   bool bad = false;
-  auto add = [&](int32_t c, double x) {
-    int32_t lo = q0, hi = q1;
+  auto find = [&](int32_t c) {
+    int32_t lo = 0, hi = len;
     while (lo < hi) {
       const int32_t mid = lo + ((hi - lo) >> 1);
-      if (colc[mid] < c) lo = mid + 1;
+      if (keys[mid] < c) lo = mid + 1;
       else hi = mid;
     }
-    if (lo < q1 && colc[lo] == c) valc[lo] = add_rn(valc[lo], x);
-    else bad = true;
+    return (lo < len && keys[lo] == c) ? lo : -1;
   };
-  auto row = [&](int64_t r, double s) {
-    for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
-      const int32_t j = col[q];
-      const double v = val[q];
-      if (hstart[j + 1] > hstart[j]) {
-        for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k) add(hpar[k], mul_rn(s, mul_rn(v, hw[k])));
-      } else {
-        add(j, mul_rn(s, v));
+  // does row i of A hold a hanging column?
+  bool hang_col = false;
+  for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += 64) {
+    const int32_t j = col[q];
+    hang_col = hang_col || hstart[j + 1] > hstart[j];
+  }
+  const bool simple = !__any(hang_col) && fstart[i + 1] == fstart[i];
+  if (simple) {
+    for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += 64) {
+      const int32_t at = find(col[q]);
+      if (at >= 0) acc[at] = add_rn(acc[at], mul_rn(1.0, val[q]));
+      else bad = true;
+    }
+  } else if (lane == 0) {
+    auto add = [&](int32_t c, double x) {
+      const int32_t at = find(c);
+      if (at >= 0) acc[at] = add_rn(acc[at], x);
+      else bad = true;
+    };
+    auto row = [&](int64_t r, double sc) {
+      for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+        const int32_t j = col[q];
+        const double v = val[q];
+        if (hstart[j + 1] > hstart[j]) {
+          for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k) add(hpar[k], mul_rn(sc, mul_rn(v, hw[k])));
+        } else {
+          add(j, mul_rn(sc, v));
+        }
       }
-    }
-  };
-  row(i, 1.0);
-  for (int32_t k = fstart[i]; k < fstart[i + 1]; ++k) row(flist[k], fw[k]);
-  if (isdir) {
-    const bool drow = isdir[i];
-    for (int32_t q = q0; q < q1; ++q) {
-      if (drow) valc[q] = (colc[q] == i) ? 1.0 : 0.0;
-      else if (isdir[colc[q]]) valc[q] = 0.0;
-    }
+    };
+    row(i, 1.0);
+    for (int32_t k = fstart[i]; k < fstart[i + 1]; ++k) row(flist[k], fw[k]);
+  }
+  if (!lds) __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+  const bool drow = isdir && isdir[i];
+  for (int t = lane; t < len; t += 64) {
+    const int32_t c = colc[q0 + t];
+    double x = acc[t];
+    if (drow) x = (c == i) ? 1.0 : 0.0;
+    else if (isdir && isdir[c]) x = 0.0;
+    valc[q0 + t] = x;
   }
   if (bad) flags[2] = 1;
 }
@@ -590,7 +634,8 @@ int gh_condense_q2(gh_ctx* ctx, int64_t nedges, const int32_t* edges, int64_t nf
       !up((void**)&d_fw, fw.data(), 8 * fw.size()))
     rc = fail(GH_ENOMEM, "gh_condense_q2: constraint tables");
   if (!rc && n > 0) {
-    hipLaunchKernelGGL(k_condense_q2, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n,
+    const int64_t blocks = (n + kQ2Waves - 1) / kQ2Waves;
+    hipLaunchKernelGGL(k_condense_q2, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, n,
                        A->rowptr, A->col, (const double*)A->val, d_hs, d_hp, d_hw, d_fs, d_fl,
                        d_fw, Ac->rowptr, Ac->col, isdir, vc, Ac->d_flags);
     if (hipGetLastError() != hipSuccess) rc = fail(GH_EHIP, "gh_condense_q2: launch failed");
