@@ -314,8 +314,8 @@ __global__ __launch_bounds__(kBlock) void k_form_q2(
 // at most kCondMax entries; longer ones, rare, take one lane and global
 // memory).  A row that neither holds a hanging column nor has hanging nodes
 // attached takes row i of A alone, whose columns are distinct: the lanes add
-// its entries at once; the other rows (near the refinement interface) walk
-// their sequence in one lane, in order.
+// its entries at once.  For the other rows (near the refinement interface)
+// the lanes find the contributions' targets and one lane adds them in order.
 constexpr int kCondMax = 512;
 
 __global__ __launch_bounds__(kBlock) void k_condense_q2(
@@ -328,6 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
     double* __restrict__ valc, int* flags) {
   __shared__ int32_t s_key[kQ2Waves][kCondMax];
   __shared__ double s_acc[kQ2Waves][kCondMax];
+  // staged contributions of a chunk: per lane up to 9 (target, value); the
+  // host admits one constraint, of at most 9 parents, per hanging node
+  __shared__ int32_t s_pos[kQ2Waves][64][9];
+  __shared__ double s_x[kQ2Waves][64][9];
+  __shared__ int32_t s_cnt[kQ2Waves][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * kQ2Waves + w;
   if (i >= n) return;  // the whole wavefront: no block barriers below
@@ -371,25 +376,48 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
       if (at >= 0) acc[at] = add_rn(acc[at], mul_rn(1.0, val[q]));
       else bad = true;
     }
-  } else if (lane == 0) {
-    auto add = [&](int32_t c, double x) {
-      const int32_t at = find(c);
-      if (at >= 0) acc[at] = add_rn(acc[at], x);
-      else bad = true;
-    };
-    auto row = [&](int64_t r, double sc) {
-      for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
-        const int32_t j = col[q];
-        const double v = val[q];
-        if (hstart[j + 1] > hstart[j]) {
-          for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k) add(hpar[k], mul_rn(sc, mul_rn(v, hw[k])));
-        } else {
-          add(j, mul_rn(sc, v));
+  } else {
+    // in chunks of 64 entries of each contributing row: the lanes find the
+    // targets of their entry's contributions (one, or the 3 or 9 parents of a
+    // hanging column) and stage them; one lane then adds the chunk's
+    // contributions in sequence order
+    int32_t* st_pos = s_pos[w][lane];
+    double* st_x = s_x[w][lane];
+    auto segment = [&](int64_t r, double sc) {
+      for (int32_t base = rp[r]; base < rp[r + 1]; base += 64) {
+        const int32_t q = base + lane;
+        int m = 0;
+        if (q < rp[r + 1]) {
+          const int32_t j = col[q];
+          const double v = val[q];
+          if (hstart[j + 1] > hstart[j]) {
+            for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k, ++m) {
+              st_pos[m] = find(hpar[k]);
+              st_x[m] = mul_rn(sc, mul_rn(v, hw[k]));
+            }
+          } else {
+            st_pos[m] = find(j);
+            st_x[m] = mul_rn(sc, v);
+            ++m;
+          }
         }
+        s_cnt[w][lane] = m;
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+          const int nl = (int)min<int32_t>(64, rp[r + 1] - base);
+          for (int l = 0; l < nl; ++l)
+            for (int k = 0; k < s_cnt[w][l]; ++k) {
+              const int32_t at = s_pos[w][l][k];
+              if (at >= 0) acc[at] = add_rn(acc[at], s_x[w][l][k]);
+              else bad = true;
+            }
+        }
+        if (!lds) __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
       }
     };
-    row(i, 1.0);
-    for (int32_t k = fstart[i]; k < fstart[i + 1]; ++k) row(flist[k], fw[k]);
+    segment(i, 1.0);
+    for (int32_t k = fstart[i]; k < fstart[i + 1]; ++k) segment(flist[k], fw[k]);
   }
   if (!lds) __threadfence_block();
   __builtin_amdgcn_wave_barrier();
@@ -569,11 +597,13 @@ int gh_condense_q2(gh_ctx* ctx, int64_t nedges, const int32_t* edges, int64_t nf
   for (int64_t k = 0; k < nedges; ++k) {
     const int32_t h = edges[4 * k];
     GH_ARG(h >= 0 && h < n, "gh_condense_q2: edge constraint node out of range");
+    GH_ARG(hstart[h + 1] == 0, "gh_condense_q2: a node is hanging in two constraints");
     hstart[h + 1] += 3;
   }
   for (int64_t k = 0; k < nfaces; ++k) {
     const int32_t h = faces[10 * k];
     GH_ARG(h >= 0 && h < n, "gh_condense_q2: face constraint node out of range");
+    GH_ARG(hstart[h + 1] == 0, "gh_condense_q2: a node is hanging in two constraints");
     hstart[h + 1] += 9;
   }
   for (int64_t i = 0; i < n; ++i) hstart[i + 1] += hstart[i];

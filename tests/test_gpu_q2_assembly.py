@@ -261,3 +261,7 @@ def test_q2_condensed_over_hanging_nodes_equals_host(ctx):
     bad = np.array([[len(rpc) + 5, 0, 1, 2]], dtype=np.int32)
     rc = lib.gh_condense_q2(ctx.h, 1, N.ptr(bad, N.i32p), 0, None, A.h, None, Ac.h)
     assert rc == N.GH_EINVAL
+    # and so is a node that two constraints name
+    twice = np.array([[3, 0, 1, 2], [3, 4, 5, 6]], dtype=np.int32)
+    rc = lib.gh_condense_q2(ctx.h, 2, N.ptr(twice, N.i32p), 0, None, A.h, None, Ac.h)
+    assert rc == N.GH_EINVAL
