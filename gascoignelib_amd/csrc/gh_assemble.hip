@@ -315,7 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_form_q2(
 // memory).  A row that neither holds a hanging column nor has hanging nodes
 // attached takes row i of A alone, whose columns are distinct: the lanes add
 // its entries at once.  For the other rows (near the refinement interface)
-// the lanes find the contributions' targets and one lane adds them in order.
+// the lanes find the contributions' targets, and each lane adds, in order,
+// the contributions to the sums it owns.
 constexpr int kCondMax = 512;
 
 __global__ __launch_bounds__(kBlock) void k_condense_q2(
@@ -328,11 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
     double* __restrict__ valc, int* flags) {
   __shared__ int32_t s_key[kQ2Waves][kCondMax];
   __shared__ double s_acc[kQ2Waves][kCondMax];
-  // staged contributions of a chunk: per lane up to 9 (target, value); the
-  // host admits one constraint, of at most 9 parents, per hanging node
-  __shared__ int32_t s_pos[kQ2Waves][64][9];
-  __shared__ double s_x[kQ2Waves][64][9];
-  __shared__ int32_t s_cnt[kQ2Waves][64];
+  // a chunk's contributions (target, value): up to 9 per entry, as the host
+  // admits one constraint, of at most 9 parents, per hanging node
+  __shared__ int32_t s_pos[kQ2Waves][64 * 9];
+  __shared__ double s_x[kQ2Waves][64 * 9];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * kQ2Waves + w;
   if (i >= n) return;  // the whole wavefront: no block barriers below
@@ -377,42 +377,48 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
       else bad = true;
     }
   } else {
-    // in chunks of 64 entries of each contributing row: the lanes find the
-    // targets of their entry's contributions (one, or the 3 or 9 parents of a
-    // hanging column) and stage them; one lane then adds the chunk's
-    // contributions in sequence order
-    int32_t* st_pos = s_pos[w][lane];
-    double* st_x = s_x[w][lane];
+    // in chunks of 64 entries of each contributing row: each lane finds the
+    // targets of its entry's contributions (one, or the 3 or 9 parents of a
+    // hanging column); the chunk's contributions go to a list in sequence
+    // order, and each lane adds those whose target it owns (target mod 64),
+    // so the contributions to every sum are added in sequence order
+    int32_t* f_pos = s_pos[w];
+    double* f_x = s_x[w];
     auto segment = [&](int64_t r, double sc) {
       for (int32_t base = rp[r]; base < rp[r + 1]; base += 64) {
         const int32_t q = base + lane;
-        int m = 0;
+        int32_t j = -1, m = 0;
+        double v = 0.0;
         if (q < rp[r + 1]) {
-          const int32_t j = col[q];
-          const double v = val[q];
+          j = col[q];
+          v = val[q];
+          m = hstart[j + 1] > hstart[j] ? hstart[j + 1] - hstart[j] : 1;
+        }
+        // exclusive prefix of m over the lanes
+        int32_t inc = m;
+        for (int d = 1; d < 64; d <<= 1) {
+          const int32_t t = __shfl_up(inc, d, 64);
+          if (lane >= d) inc += t;
+        }
+        const int32_t total = __shfl(inc, 63, 64);
+        int32_t off = inc - m;
+        if (m > 0) {
           if (hstart[j + 1] > hstart[j]) {
-            for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k, ++m) {
-              st_pos[m] = find(hpar[k]);
-              st_x[m] = mul_rn(sc, mul_rn(v, hw[k]));
+            for (int32_t k = hstart[j]; k < hstart[j + 1]; ++k, ++off) {
+              f_pos[off] = find(hpar[k]);
+              f_x[off] = mul_rn(sc, mul_rn(v, hw[k]));
             }
           } else {
-            st_pos[m] = find(j);
-            st_x[m] = mul_rn(sc, v);
-            ++m;
+            f_pos[off] = find(j);
+            f_x[off] = mul_rn(sc, v);
           }
         }
-        s_cnt[w][lane] = m;
         __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-          const int nl = (int)min<int32_t>(64, rp[r + 1] - base);
-          for (int l = 0; l < nl; ++l)
-            for (int k = 0; k < s_cnt[w][l]; ++k) {
-              const int32_t at = s_pos[w][l][k];
-              if (at >= 0) acc[at] = add_rn(acc[at], s_x[w][l][k]);
-              else bad = true;
-            }
+        for (int32_t f = 0; f < total; ++f) {
+          const int32_t at = f_pos[f];
+          if (at < 0) bad = true;
+          else if ((at & 63) == lane) acc[at] = add_rn(acc[at], f_x[f]);
         }
-        if (!lds) __threadfence_block();
         __builtin_amdgcn_wave_barrier();
       }
     };
