@@ -1440,7 +1440,10 @@ static int launch_block_inverse(gh_ctx* ctx, int64_t n, int B, const int32_t* di
 // The SELL value stream of a scalar matrix from its FP64 CSR values: slot t
 // (row perm[t]) stores the entries of its row that the layout kept, in CSR
 // order, at sptr[s] + lane + 64 k; an entry left out at creation (an exact
-// zero then) must still be zero, else flag[1] is set.
+// zero then) must still be zero, else flag[1] is set.  One wavefront per
+// slice: the lanes read 64 consecutive CSR entries of one row at a time
+// (coalesced), rank the kept ones by a ballot, and store each at its slot's
+// k-th place; the slice's rows are taken 4 at a time.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_sell_refresh(int64_t n, const uint32_t* __restrict__ sptr,
                                                          const int32_t* __restrict__ perm,
@@ -1448,20 +1451,53 @@ __global__ __launch_bounds__(kBlock) void k_sell_refresh(int64_t n, const uint32
                                                          const uint32_t* __restrict__ kept,
                                                          const double* __restrict__ val64,
                                                          VT* __restrict__ sval, int* flags) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n) return;
-  const int64_t row = perm ? (int64_t)perm[t] : t;
-  VT* vp = sval + sptr[t >> 6] + (t & (kSlice - 1));
-  int k = 0;
-  for (int32_t q = rowptr[row]; q < rowptr[row + 1]; ++q) {
-    const double v = val64[q];
-    if ((kept[q >> 5] >> (q & 31)) & 1u) {
-      vp[(int64_t)k * kSlice] = (VT)v;
-      ++k;
-    } else if (v != 0.0) {
-      flags[1] = 1;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int64_t s = (int64_t)blockIdx.x * (kBlock / kSlice) + (threadIdx.x >> 6);
+  const int64_t t0 = s * kSlice;
+  if (t0 >= n) return;  // the whole wavefront
+  const int rows = (int)min<int64_t>(kSlice, n - t0);
+  // the lane's slot's row bounds, handed to the wavefront row by row below
+  int32_t b0 = 0, b1 = 0;
+  if (lane < rows) {
+    const int64_t row = perm ? (int64_t)perm[t0 + lane] : t0 + lane;
+    b0 = rowptr[row];
+    b1 = rowptr[row + 1];
+  }
+  VT* base = sval + sptr[s];
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  bool bad = false;
+  // 4 rows at a time, their loads issued together (lanes past the last row
+  // hold empty ranges, so every j below 64 is safe)
+  for (int j0 = 0; j0 < rows; j0 += 4) {
+    int32_t q0[4], q1[4], k[4];
+    int32_t longest = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      q0[u] = __shfl(b0, j0 + u, 64);
+      q1[u] = __shfl(b1, j0 + u, 64);
+      k[u] = 0;
+      longest = max(longest, q1[u] - q0[u]);
+    }
+    for (int32_t c = 0; c < longest; c += 64) {
+      double v[4];
+      bool in[4], keep[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t q = q0[u] + c + lane;
+        in[u] = q < q1[u];
+        v[u] = in[u] ? val64[q] : 0.0;
+        keep[u] = in[u] && ((kept[q >> 5] >> (q & 31)) & 1u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t m = __ballot(keep[u]);
+        if (keep[u]) base[j0 + u + (int64_t)(k[u] + __popcll(m & below)) * kSlice] = (VT)v[u];
+        else if (in[u] && v[u] != 0.0) bad = true;
+        k[u] += __popcll(m);
+      }
     }
   }
+  if (bad) flags[1] = 1;
 }
 
 int csr_asm_values(gh_ctx* ctx, gh_csr* A, double** val64) {
@@ -1518,7 +1554,8 @@ int csr_asm_commit(gh_ctx* ctx, gh_csr* A, const char* who) {
                        ctx->stream, nnz, (__half*)A->val, (const double*)v64);
   GH_HIP(hipGetLastError());
   const Sell& S = A->sell;
-  const dim3 g(grid_for(n));
+  const int64_t nslices = (n + kSlice - 1) / kSlice;
+  const dim3 g((unsigned)std::max<int64_t>((nslices + kBlock / kSlice - 1) / (kBlock / kSlice), 1));
   if (A->prec == GH_F64)
     hipLaunchKernelGGL(k_sell_refresh<double>, g, dim3(kBlock), 0, ctx->stream, n, S.sptr, S.perm,
                        A->rowptr, A->d_kept, v64, (double*)S.val, A->d_flags);
