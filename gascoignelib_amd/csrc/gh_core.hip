@@ -1441,9 +1441,11 @@ static int launch_block_inverse(gh_ctx* ctx, int64_t n, int B, const int32_t* di
 // (row perm[t]) stores the entries of its row that the layout kept, in CSR
 // order, at sptr[s] + lane + 64 k; an entry left out at creation (an exact
 // zero then) must still be zero, else flag[1] is set.  One wavefront per
-// slice: the lanes read 64 consecutive CSR entries of one row at a time
-// (coalesced), rank the kept ones by a ballot, and store each at its slot's
-// k-th place; the slice's rows are taken 4 at a time.
+// slice, in groups of 8 rows: the lanes read 64 consecutive CSR entries of a
+// row at a time (coalesced; 4 rows in flight), rank the kept ones by a
+// ballot and stage them in LDS, up to 128 per row (further ones are stored
+// directly); the group then goes out k by k, the 8 rows of one k being 8
+// adjacent slots.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_sell_refresh(int64_t n, const uint32_t* __restrict__ sptr,
                                                          const int32_t* __restrict__ perm,
@@ -1451,12 +1453,17 @@ __global__ __launch_bounds__(kBlock) void k_sell_refresh(int64_t n, const uint32
                                                          const uint32_t* __restrict__ kept,
                                                          const double* __restrict__ val64,
                                                          VT* __restrict__ sval, int* flags) {
+  constexpr int G = 8, KT = 128;
+  __shared__ double tile[kBlock / kSlice][G][KT + 1];
   const int lane = threadIdx.x & (kSlice - 1);
-  const int64_t s = (int64_t)blockIdx.x * (kBlock / kSlice) + (threadIdx.x >> 6);
+  const int w = threadIdx.x >> 6;
+  const int64_t s = (int64_t)blockIdx.x * (kBlock / kSlice) + w;
   const int64_t t0 = s * kSlice;
   if (t0 >= n) return;  // the whole wavefront
   const int rows = (int)min<int64_t>(kSlice, n - t0);
-  // the lane's slot's row bounds, handed to the wavefront row by row below
+  // the lane's slot's row bounds, handed to the wavefront row by row below;
+  // lanes past the last row hold empty ranges, so every row index below 64
+  // is safe
   int32_t b0 = 0, b1 = 0;
   if (lane < rows) {
     const int64_t row = perm ? (int64_t)perm[t0 + lane] : t0 + lane;
@@ -1465,37 +1472,59 @@ __global__ __launch_bounds__(kBlock) void k_sell_refresh(int64_t n, const uint32
   }
   VT* base = sval + sptr[s];
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  double (*tl)[KT + 1] = tile[w];
   bool bad = false;
-  // 4 rows at a time, their loads issued together (lanes past the last row
-  // hold empty ranges, so every j below 64 is safe)
-  for (int j0 = 0; j0 < rows; j0 += 4) {
-    int32_t q0[4], q1[4], k[4];
-    int32_t longest = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      q0[u] = __shfl(b0, j0 + u, 64);
-      q1[u] = __shfl(b1, j0 + u, 64);
-      k[u] = 0;
-      longest = max(longest, q1[u] - q0[u]);
-    }
-    for (int32_t c = 0; c < longest; c += 64) {
-      double v[4];
-      bool in[4], keep[4];
+  for (int g0 = 0; g0 < rows; g0 += G) {
+    int32_t len = 0;  // lane r < G: kept entries of row g0 + r
+    for (int j0 = g0; j0 < g0 + G; j0 += 4) {
+      int32_t q0[4], q1[4], k[4];
+      int32_t longest = 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int32_t q = q0[u] + c + lane;
-        in[u] = q < q1[u];
-        v[u] = in[u] ? val64[q] : 0.0;
-        keep[u] = in[u] && ((kept[q >> 5] >> (q & 31)) & 1u);
+        q0[u] = __shfl(b0, j0 + u, 64);
+        q1[u] = __shfl(b1, j0 + u, 64);
+        k[u] = 0;
+        longest = max(longest, q1[u] - q0[u]);
+      }
+      for (int32_t c = 0; c < longest; c += 64) {
+        double v[4];
+        bool in[4], keep[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int32_t q = q0[u] + c + lane;
+          in[u] = q < q1[u];
+          v[u] = in[u] ? val64[q] : 0.0;
+          keep[u] = in[u] && ((kept[q >> 5] >> (q & 31)) & 1u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint64_t m = __ballot(keep[u]);
+          if (keep[u]) {
+            const int32_t at = k[u] + __popcll(m & below);
+            if (at < KT) tl[j0 - g0 + u][at] = v[u];
+            else base[j0 + u + (int64_t)at * kSlice] = (VT)v[u];
+          } else if (in[u] && v[u] != 0.0) {
+            bad = true;
+          }
+          k[u] += __popcll(m);
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint64_t m = __ballot(keep[u]);
-        if (keep[u]) base[j0 + u + (int64_t)(k[u] + __popcll(m & below)) * kSlice] = (VT)v[u];
-        else if (in[u] && v[u] != 0.0) bad = true;
-        k[u] += __popcll(m);
-      }
+      for (int u = 0; u < 4; ++u)
+        if (lane == j0 - g0 + u) len = k[u];
     }
+    // LDS operations of one wavefront complete in issue order
+    __builtin_amdgcn_wave_barrier();
+    int32_t top = min(len, KT);
+    for (int d = 1; d < G; d <<= 1) top = max(top, __shfl_xor(top, d, 64));
+    top = __shfl(top, 0, 64);
+    const int r = lane & (G - 1);
+    const int32_t mine = min(__shfl(len, r, 64), KT);
+    for (int32_t kb = 0; kb < top; kb += kSlice / G) {
+      const int32_t at = kb + (lane >> 3);
+      if (at < mine) base[g0 + r + (int64_t)at * kSlice] = (VT)tl[r][at];
+    }
+    __builtin_amdgcn_wave_barrier();
   }
   if (bad) flags[1] = 1;
 }
