@@ -311,13 +311,15 @@ __global__ __launch_bounds__(kBlock) void k_form_q2(
 // the Dirichlet rows and columns.  A missing column sets flags[2].
 //
 // One wavefront per row, the row's columns and running sums in LDS (rows of
-// at most kCondMax entries; longer ones, rare, take one lane and global
-// memory).  A row that neither holds a hanging column nor has hanging nodes
+// at most kCondMax entries; longer ones, rare, keep them in global memory).  A row that neither holds a hanging column nor has hanging nodes
 // attached takes row i of A alone, whose columns are distinct: the lanes add
 // its entries at once.  For the other rows (near the refinement interface)
 // the lanes find the contributions' targets, and each lane adds, in order,
 // the contributions to the sums it owns.
-constexpr int kCondMax = 512;
+constexpr int kCondMax = 256;
+// entries per chunk of a contributing row: 16 keeps the LDS of a wavefront
+// at 4.7 KB, so 8 wavefronts per SIMD fit (the kernel waits on memory)
+constexpr int kCondChunk = 16;
 
 __global__ __launch_bounds__(kBlock) void k_condense_q2(
     int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
@@ -331,8 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
   __shared__ double s_acc[kQ2Waves][kCondMax];
   // a chunk's contributions (target, value): up to 9 per entry, as the host
   // admits one constraint, of at most 9 parents, per hanging node
-  __shared__ int32_t s_pos[kQ2Waves][64 * 9];
-  __shared__ double s_x[kQ2Waves][64 * 9];
+  __shared__ int32_t s_pos[kQ2Waves][kCondChunk * 9];
+  __shared__ double s_x[kQ2Waves][kCondChunk * 9];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * kQ2Waves + w;
   if (i >= n) return;  // the whole wavefront: no block barriers below
@@ -377,19 +379,19 @@ __global__ __launch_bounds__(kBlock) void k_condense_q2(
       else bad = true;
     }
   } else {
-    // in chunks of 64 entries of each contributing row: each lane finds the
-    // targets of its entry's contributions (one, or the 3 or 9 parents of a
+    // in chunks of kCondChunk entries of each contributing row: a lane finds
+    // the targets of its entry's contributions (one, or the 3 or 9 parents of a
     // hanging column); the chunk's contributions go to a list in sequence
     // order, and each lane adds those whose target it owns (target mod 64),
     // so the contributions to every sum are added in sequence order
     int32_t* f_pos = s_pos[w];
     double* f_x = s_x[w];
     auto segment = [&](int64_t r, double sc) {
-      for (int32_t base = rp[r]; base < rp[r + 1]; base += 64) {
+      for (int32_t base = rp[r]; base < rp[r + 1]; base += kCondChunk) {
         const int32_t q = base + lane;
         int32_t j = -1, m = 0;
         double v = 0.0;
-        if (q < rp[r + 1]) {
+        if (lane < kCondChunk && q < rp[r + 1]) {
           j = col[q];
           v = val[q];
           m = hstart[j + 1] > hstart[j] ? hstart[j + 1] - hstart[j] : 1;
